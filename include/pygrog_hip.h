@@ -1,0 +1,347 @@
+/*
+ * pygrog_hip.h -- C ABI of libpygrog_hip.so, the MI355X (gfx950) GROG
+ * gridding library.
+ *
+ * Conventions
+ *   - Every entry point returns 0 on success and a nonzero code on error;
+ *     pg_last_error() then returns a message for the calling thread.
+ *   - Pointers are device pointers unless stated otherwise.  Sizes are
+ *     element counts.  Complex arrays are interleaved (re, im) pairs:
+ *     "c64" = float2, "c128" = double2.
+ *   - `stream` is a hipStream_t (NULL = legacy default stream).  All work is
+ *     enqueued asynchronously on it; no entry point synchronises the host,
+ *     and no entry point allocates device memory: outputs and workspaces
+ *     are caller-provided.
+ *
+ * The first block mirrors the seven ops of the reference's pybind module
+ * `_pygrog_torch` (csrc/torch/module.cpp:54-87), one entry per dtype.  The
+ * remaining blocks are the device-resident plan compiler, the fused gridding
+ * kernels and the helpers they need; their Python callers live in
+ * pygrog_amd/_lib.py and pygrog_amd/calib/_grog.py.
+ */
+#ifndef PYGROG_HIP_H
+#define PYGROG_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *pg_stream_t; /* hipStream_t */
+
+/* ------------------------------------------------------------------------
+ * Library status
+ * ---------------------------------------------------------------------- */
+/* Message of the last failed call on this thread ("" if none). */
+const char *pg_last_error(void);
+/* ABI version of this header (bumped on any signature change). */
+int pg_abi_version(void);
+
+/* ------------------------------------------------------------------------
+ * Reference native ops (csrc/torch/module.cpp:54-87)
+ * ---------------------------------------------------------------------- */
+
+/* out[n,b,:] = kernel[indexes[n]] @ data[n,b,:]
+ * data, out: (N, B, C); indexes: (N,) int64; kernel: (K, C, C) row-major.
+ * An index outside [0, K) yields a zero output row.
+ * Replaces grog_interpolate (csrc/torch/grog_interp.cpp:112-149;
+ * CUDA grog_interp_cuda.cu:28-106). */
+int pg_grog_interpolate_c64(const void *data, const int64_t *indexes,
+                            const void *kernel, void *out, int64_t N,
+                            int64_t B, int64_t C, int64_t K,
+                            pg_stream_t stream);
+int pg_grog_interpolate_c128(const void *data, const int64_t *indexes,
+                             const void *kernel, void *out, int64_t N,
+                             int64_t B, int64_t C, int64_t K,
+                             pg_stream_t stream);
+
+/* grid[indices[i]] += weights[i] * data[i], in place; indices outside
+ * [0, grid_size) are skipped.  Correct for any index order; equal indices
+ * that are adjacent are reduced in registers first, so sorted input costs
+ * one atomic per run per wavefront.
+ * Replaces scatter_add (csrc/torch/sparse_ops.cpp:233-267). */
+int pg_scatter_add_c64(void *grid, int64_t grid_size, const void *data,
+                       const int64_t *indices, const float *weights,
+                       int64_t n, pg_stream_t stream);
+int pg_scatter_add_c128(void *grid, int64_t grid_size, const void *data,
+                        const int64_t *indices, const double *weights,
+                        int64_t n, pg_stream_t stream);
+
+/* Same contract as pg_scatter_add_*; `bin_starts` (n_bin_starts int64, the
+ * searchsorted bin edges of the reference) is validated but not needed by
+ * the wavefront-run reduction.
+ * Replaces scatter_add_binned (csrc/torch/sparse_ops.cpp:299-340). */
+int pg_scatter_add_binned_c64(void *grid, int64_t grid_size, const void *data,
+                              const int64_t *indices, const float *weights,
+                              int64_t n, const int64_t *bin_starts,
+                              int64_t n_bin_starts, int64_t bin_size,
+                              pg_stream_t stream);
+int pg_scatter_add_binned_c128(void *grid, int64_t grid_size,
+                               const void *data, const int64_t *indices,
+                               const double *weights, int64_t n,
+                               const int64_t *bin_starts, int64_t n_bin_starts,
+                               int64_t bin_size, pg_stream_t stream);
+
+/* out[i] = weights[i] * grid[indices[i]] (0 where the index is out of range).
+ * Replaces gather (csrc/torch/sparse_ops.cpp:269-297). */
+int pg_gather_c64(const void *grid, int64_t grid_size, const int64_t *indices,
+                  const float *weights, int64_t n, void *out,
+                  pg_stream_t stream);
+int pg_gather_c128(const void *grid, int64_t grid_size,
+                   const int64_t *indices, const double *weights, int64_t n,
+                   void *out, pg_stream_t stream);
+
+/* psf[indices[i]] += w_sq[i], in place (real PSF).
+ * Replaces psf_scatter_scalar (csrc/torch/toep_psf.cpp:242-274). */
+int pg_psf_scatter_scalar_f32(float *psf, int64_t grid_size,
+                              const int64_t *indices, const float *w_sq,
+                              int64_t n, pg_stream_t stream);
+int pg_psf_scatter_scalar_f64(double *psf, int64_t grid_size,
+                              const int64_t *indices, const double *w_sq,
+                              int64_t n, pg_stream_t stream);
+
+/* psf[indices[i], l, l'] += w_sq[i] * conj(b[i,l]) * b[i,l'];
+ * psf: (grid_size, M, M) complex; basis_per_sample: (n, M) complex.
+ * Replaces psf_scatter_outer (csrc/torch/toep_psf.cpp:276-320). */
+int pg_psf_scatter_outer_c64(void *psf, int64_t grid_size, int64_t M,
+                             const int64_t *indices, const float *w_sq,
+                             const void *basis_per_sample, int64_t n,
+                             pg_stream_t stream);
+int pg_psf_scatter_outer_c128(void *psf, int64_t grid_size, int64_t M,
+                              const int64_t *indices, const double *w_sq,
+                              const void *basis_per_sample, int64_t n,
+                              pg_stream_t stream);
+
+/* psf[indices[i], k, k'] += w_sq[i] * conj(B[k,t_i]) * B[k',t_i];
+ * psf: (grid_size, K, K) complex; basis: (K, T) complex; time_index: (n,).
+ * Samples with t_i outside [0, T) are skipped.
+ * Replaces psf_scatter_outer_basis (csrc/torch/toep_psf.cpp:322-367). */
+int pg_psf_scatter_outer_basis_c64(void *psf, int64_t grid_size, int64_t K,
+                                   const int64_t *indices, const float *w_sq,
+                                   const void *basis, int64_t T,
+                                   const int64_t *time_index, int64_t n,
+                                   pg_stream_t stream);
+int pg_psf_scatter_outer_basis_c128(void *psf, int64_t grid_size, int64_t K,
+                                    const int64_t *indices,
+                                    const double *w_sq, const void *basis,
+                                    int64_t T, const int64_t *time_index,
+                                    int64_t n, pg_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Plan compiler (device-resident GrogPlan).  Replaces the host-side
+ * _create_plan_single / _attach_fft_plan / calc_interp_table quantisation
+ * (src/pygrog/calib/_grog.py:1081-1234, 918-998, 382-422).
+ * ---------------------------------------------------------------------- */
+
+/* Geometry shared by the plan kernels (host struct, passed by value). */
+typedef struct pg_plan_geom {
+  int32_t ndim;           /* 2 or 3 */
+  int32_t kw;             /* kernel offsets per source */
+  int64_t grid_shape[3];  /* oversampled grid, row-major (axis 0 slowest) */
+  int64_t strides[3];     /* flat-index strides of grid_shape */
+  float half_amp[3];      /* 0.5 * (2 * (s - 1 - s // 2)), f32 */
+  float origins[3];       /* -(s // 2) */
+  float steps[3];         /* (s - 1) / (gs - 1), f32 */
+  int32_t offsets[128][3]; /* kernel offsets, kw rows (kw <= 128) */
+} pg_plan_geom;
+
+/* cmax[s, d] = max_n |coords[s, n, d]|;  coords (S, npts, ndim) f32. */
+int pg_plan_absmax(const float *coords, int64_t S, int64_t npts, int32_t ndim,
+                   float *cmax, pg_stream_t stream);
+
+/* Per (s, n, k) candidate replica (natural order, nper = npts * kw):
+ *   target_idx (S, nper) int64   flat grid index or -1 (invalid);
+ *   distances  (S, nper, ndim) f32  target - source in image-grid units;
+ *   gauss      (S, nper) f32     exp(-0.5 |d/steps|^2), 0 when invalid;
+ *   sort_key   (S*nper) uint64   s*(G+1) + (valid ? cell + 1 : 0);
+ *   sort_val   (S*nper) uint32   s*nper + n*kw + k;
+ *   n_invalid  (S) int64         per-element invalid counts (must be zeroed
+ *                                 by the caller; accumulated atomically).
+ * Float work follows the reference's f32 operation order without
+ * contraction, so target_idx and distances are bit-exact. */
+int pg_plan_targets(const float *coords, const float *cmax, int64_t S,
+                    int64_t npts, pg_plan_geom geom, int64_t *target_idx,
+                    float *distances, float *gauss, uint64_t *sort_key,
+                    uint32_t *sort_val, int64_t *n_invalid,
+                    pg_stream_t stream);
+
+/* Stable LSD radix sort of (uint64 key, uint32 value) pairs on bits
+ * [0, end_bit).  Call with workspace == NULL to query *ws_bytes. */
+int pg_sort_pairs_u64(void *workspace, size_t *ws_bytes,
+                      const uint64_t *keys_in, uint64_t *keys_out,
+                      const uint32_t *vals_in, uint32_t *vals_out, int64_t n,
+                      int32_t end_bit, pg_stream_t stream);
+/* Stable radix sort of (uint32 key, uint32 value) pairs on bits [0, end_bit). */
+int pg_sort_pairs_u32(void *workspace, size_t *ws_bytes,
+                      const uint32_t *keys_in, uint32_t *keys_out,
+                      const uint32_t *vals_in, uint32_t *vals_out, int64_t n,
+                      int32_t end_bit, pg_stream_t stream);
+/* out[i] = sum_{j<i} in[j] (uint32); out has n + 1 entries (out[n] = total).
+ * Call with workspace == NULL to query *ws_bytes. */
+int pg_exclusive_scan_u32(void *workspace, size_t *ws_bytes,
+                          const uint32_t *in, uint32_t *out, int64_t n,
+                          pg_stream_t stream);
+
+/* From the sorted (key, value) arrays of pg_plan_targets:
+ *   indices (S, nper) int64 sorted grid index (invalid -> 0);
+ *   sort_perm, inv_perm (S, nper) int64;
+ *   gauss_sorted (S*nper) f32 = gauss[value];
+ *   cell_ptr (S, G+1) uint32 CSR of the valid replicas of each element
+ *   (replica r of element s = sorted position n_invalid[s] + r). */
+int pg_plan_post(const uint64_t *keys_sorted, const uint32_t *vals_sorted,
+                 const float *gauss, const int64_t *n_invalid, int64_t S,
+                 int64_t nper, int64_t G, int64_t *indices, int64_t *sort_perm,
+                 int64_t *inv_perm, float *gauss_sorted, uint32_t *cell_ptr,
+                 pg_stream_t stream);
+
+/* EGROG weights (reference _grog.py:1182-1212, 958-975) per cell:
+ *   gsum = sum (f64, sorted order) of gauss; w = gauss / f32(max(gsum,1e-12));
+ *   weights (S, nper) f32 natural order (0 for invalid);
+ *   sqrt_weights (S, nper) f32 sorted order (0 for invalid);
+ *   rep_src (S, Rmax) uint32, rep_s (S, Rmax) f32 per valid replica;
+ *   density (S, G) f32 = sum of sqrt_w^2 in sorted order;
+ *   unit_count (S*G) uint32 = ceil(replicas / chunk). */
+int pg_plan_weights(const uint32_t *vals_sorted, const float *gauss_sorted,
+                    const int64_t *n_invalid, const uint32_t *cell_ptr,
+                    int64_t S, int64_t nper, int64_t G, int32_t kw,
+                    int32_t chunk, float *weights, float *sqrt_weights,
+                    uint32_t *rep_src, float *rep_s, float *density,
+                    uint32_t *unit_count, int64_t rep_stride,
+                    pg_stream_t stream);
+
+/* Unit arrays from the exclusive scan of unit_count (cell_unit, S*G+1):
+ *   unit_cell (U) uint32 cell of each unit (element-local);
+ *   unit_rep (U + S) uint32 first replica of each unit, element-local,
+ *   with a terminating entry per element: unit_rep[cell_unit[(s+1)G] + s]. */
+int pg_plan_units(const uint32_t *cell_ptr, const uint32_t *cell_unit,
+                  int64_t S, int64_t G, int32_t chunk, uint32_t *unit_cell,
+                  uint32_t *unit_rep, pg_stream_t stream);
+
+/* Quantised table entry of every candidate replica (reference
+ * calc_interp_table, _grog.py:406-422, generalised from 10**precision to an
+ * integer quantisation q):  pfac = q, stepsize = f32(step), nsteps = q + 1,
+ *   idx_d = rint((radius + rint(d_d * pfac) / pfac) / stepsize),
+ *   entry = clamp(sum_d rint(idx_d * nsteps^d), 0, n_entries - 1).
+ * interp_idx (S, nper) int64 natural order; rep_ent (S, Rmax) uint32 for
+ * valid replicas in cell-sorted order (via inv_perm).
+ * axis_order: 0 = reference pairing (entry axis d <- distances[..., d]). */
+int pg_plan_quantize(const float *distances, const int64_t *inv_perm,
+                     const int64_t *n_invalid, const int64_t *target_idx,
+                     int64_t S, int64_t nper, int32_t ndim, float pfac,
+                     float stepsize, float radius, int64_t nsteps,
+                     int64_t n_entries, int64_t *interp_idx, uint32_t *rep_ent,
+                     int64_t rep_stride, pg_stream_t stream);
+
+/* ------------------------------------------------------------------------
+ * Fused gridding (reference interpolate(grid=True) -> _grid_kspace,
+ * src/pygrog/calib/_grog.py:736-902, on top of grog_interp.cpp:39-73 and
+ * sparse_ops_cpu_impl.inl:38-120).
+ * ---------------------------------------------------------------------- */
+
+/* One frame = one (source block, plan, table, output grid) job. */
+typedef struct pg_frame {
+  const void *y;            /* (C, n_src) c64, coil-major */
+  void *grid;               /* (C, n_cell) c64, fully written */
+  const void *table;        /* packed table, pg_table_pack layout */
+  const uint32_t *rep_src;  /* (R) source of each replica, cell-sorted */
+  const uint32_t *rep_ent;  /* (R) table entry of each replica */
+  const float *rep_s;       /* (R) sqrt weight of each replica */
+  const uint32_t *unit_cell;/* (U) */
+  const uint32_t *unit_rep; /* (U + 1) */
+  const uint32_t *cell_unit;/* (n_cell + 1) first unit of each cell */
+  const void *vbuf;         /* (R, C) c64 applied replicas (two-pass modes) */
+  int64_t n_src;
+  int64_t n_cell;
+} pg_frame;
+
+/* Pack a (K, C, C) c64 row-major table into the layout read by the fused
+ * kernels: packed[e][p][co] = (K[co][2p], K[co][2p+1]) as float4,
+ * p < ceil(C/2) (zero padded).  packed holds K * ceil(C/2) * C float4. */
+int pg_table_pack_c64(const void *table, int64_t K, int64_t C, void *packed,
+                      pg_stream_t stream);
+
+/* Plain FP32 fused gridding: per replica v = (K[e] y[:, src]) * s * s,
+ * reduced per cell in plan order (chunked sequential folds), written as a
+ * dense coil-major grid (zeros in empty cells).  frames: device array of F
+ * pg_frame; tile_ptr: device array (F + 1) of cumulative tile counts with
+ * tile_cells cells per tile.  Deterministic: the result does not depend on
+ * tile size, block count or the other frames in the launch. */
+int pg_grid_fp32(const pg_frame *frames, const uint32_t *tile_ptr, int64_t F,
+                 int64_t n_tiles, int64_t C, int64_t tile_cells,
+                 int64_t n_blocks, pg_stream_t stream);
+
+/* Second pass of the grouped modes: same reduction as pg_grid_fp32 but v is
+ * read from frame.vbuf[r] (cell-sorted rows) instead of being computed. */
+int pg_grid_reduce_v(const pg_frame *frames, const uint32_t *tile_ptr,
+                     int64_t F, int64_t n_tiles, int64_t C,
+                     int64_t tile_cells, int64_t n_blocks, pg_stream_t stream);
+
+/* Third mode, used by SparseFFT (reference _scatter_ifft_crop,
+ * src/pygrog/operator/_sparse_fft.py:396-435, 1154-1165): identity operator,
+ * v = y[:, rep_src[r]] * rep_s[r] with rep_src = sort_perm and rep_s =
+ * sqrt_weights, reduced per cell exactly like pg_grid_fp32. */
+int pg_grid_scatter(const pg_frame *frames, const uint32_t *tile_ptr,
+                    int64_t F, int64_t n_tiles, int64_t C, int64_t tile_cells,
+                    int64_t n_blocks, pg_stream_t stream);
+
+/* Cells per tile the gridding kernels use for a given coil count (0 if
+ * C > 64); tile_ptr must be built with this value. */
+int pg_grid_tile_cells(int64_t C);
+
+/* out[b, m] = weights[inv_perm[m]] * grid[b, indices[inv_perm[m]]]:
+ * SparseFFT gather fused with the unsort (reference _fft_pad_gather,
+ * src/pygrog/operator/_sparse_fft.py:1167-1177).  grid (B, G), out (B, n). */
+int pg_gather_perm_c64(const void *grid, int64_t G, const int64_t *indices,
+                       const float *weights, const int64_t *inv_perm,
+                       int64_t n, int64_t B, void *out, pg_stream_t stream);
+
+/* unit_count[s*G + c] = ceil((cell_ptr[s][c+1] - cell_ptr[s][c]) / chunk). */
+int pg_unit_count(const uint32_t *cell_ptr, int64_t S, int64_t G,
+                  int32_t chunk, uint32_t *unit_count, pg_stream_t stream);
+
+/* y (C, N) c64 -> y_t (N, C) c64 sample-major. */
+int pg_transpose_cn_c64(const void *y, int64_t C, int64_t N, void *y_t,
+                        pg_stream_t stream);
+
+/* Pack a (K, C, C) c64 table for the BF16 MFMA grouped GEMM (real 2C x 2C
+ * form, hi and optionally lo bf16 parts).  n_parts = 1 or 2; packed holds
+ * K * n_parts * (2C)^2 bf16. */
+int pg_table_pack_bf16(const void *table, int64_t K, int64_t C, int32_t n_parts,
+                       void *packed, pg_stream_t stream);
+
+/* Grouped complex GEMM on BF16 MFMA with FP32 accumulation.  Replicas are
+ * grouped by table entry (ent_ptr (K+1), ent_rep (R) = cell-sorted replica
+ * rank, both from pg_sort_pairs_u32 on rep_ent); for every replica r,
+ * vbuf[r] = (K[e] y_t[rep_src[r]]) * s * s.  n_terms = 1 (hi*hi) or 3
+ * (hi*hi + hi*lo + lo*hi split-BF16 products). */
+int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
+                          const uint32_t *ent_ptr, const uint32_t *ent_rep,
+                          const uint32_t *rep_src, const float *rep_s,
+                          int64_t K, int64_t C, int64_t max_group,
+                          int32_t n_terms, void *vbuf, pg_stream_t stream);
+/* Element count (bf16) of the pg_table_pack_bf16 layout, -1 if C > 64. */
+int64_t pg_table_bf16_elems(int64_t K, int64_t C, int32_t n_parts);
+
+/* interpolate(ret_image=False) fused: out[b, co, n*kw + k] =
+ *   K[interp_idx[n, k]] @ y[b, :, n]  for every candidate replica
+ * (including invalid ones, as the reference does).  y (B, C, N) c64,
+ * interp_idx (N*kw) int64, table (K, C, C) c64 row-major, out (B, C, N*kw).
+ * Replaces _sparse_full_single + _interpolate (_grog.py:668-695, 1283-1295). */
+int pg_sparse_apply_c64(const void *y, const int64_t *interp_idx,
+                        const void *table, void *out, int64_t B, int64_t C,
+                        int64_t N, int64_t kw, int64_t K, pg_stream_t stream);
+
+/* cell_ptr (G + 1) uint32 from sorted int64 indices (n of them); indices
+ * outside [0, G) are not allowed (the caller checks sortedness/range). */
+int pg_csr_from_sorted(const int64_t *indices, int64_t n, int64_t G,
+                       uint32_t *cell_ptr, pg_stream_t stream);
+/* Same for sorted uint32 keys (table-entry groups of the grouped modes). */
+int pg_csr_from_sorted_u32(const uint32_t *keys, int64_t n, int64_t G,
+                           uint32_t *cell_ptr, pg_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PYGROG_HIP_H */

@@ -1,0 +1,162 @@
+"""CG and LSMR on duck-typed operators (reference src/pygrog/_solve/_solvers.py).
+
+PyGROG convention: ``forward = A`` (image -> k-space), ``adjoint = A^H``.
+CG solves ``(A^H A + damp I) x = A^H b``.  Its scalars (<r, z>, <p, Ap>,
+alpha, beta, residual norms) are accumulated in float64 by default
+(``scalar_dtype``), while the vectors and operator applications stay in the
+data precision; ``scalar_dtype=torch.float32`` reproduces the reference's
+complex64 ``vdot`` scalars (:117-134).
+"""
+
+from __future__ import annotations
+
+__all__ = ["cg", "lsmr", "solve"]
+
+import math
+
+import torch
+
+
+def _normalize_kspace_shape(op, b: torch.Tensor) -> torch.Tensor:
+    """Reshape flat ``(..., n_samples)`` into ``(..., *natural_shape)`` so
+    ``b - op.forward(x)`` aligns (reference :24-43)."""
+    nat = getattr(op, "natural_shape", None)
+    if nat is None or len(nat) <= 1:
+        return b
+    nat = tuple(int(s) for s in nat)
+    n = 1
+    for s in nat:
+        n *= s
+    if b.shape[-1] != n:
+        return b
+    return b.reshape(*b.shape[:-1], *nat)
+
+
+def _dot(a: torch.Tensor, b: torch.Tensor, dt) -> torch.Tensor:
+    """Re <a, b> accumulated in ``dt`` (a real dtype)."""
+    ct = torch.complex128 if dt == torch.float64 else torch.complex64
+    a = a.reshape(-1)
+    b = b.reshape(-1)
+    if a.is_complex() or b.is_complex():
+        return torch.vdot(a.to(ct), b.to(ct)).real
+    return torch.dot(a.to(dt), b.to(dt))
+
+
+def _norm(a: torch.Tensor, dt) -> torch.Tensor:
+    ct = torch.complex128 if dt == torch.float64 else torch.complex64
+    return torch.linalg.vector_norm(a.to(ct) if a.is_complex() else a.to(dt))
+
+
+def cg(op, b, *, damp: float = 0.0, x0=None, max_iter: int = 20, tol: float = 1e-6, preconditioner=None,
+       callback=None, scalar_dtype=torch.float64):
+    """Conjugate gradients on the normal equations (reference :49-139)."""
+    b = _normalize_kspace_shape(op, b)
+    AHb = op.adjoint(b)
+    AHA = getattr(op, "normal", None)
+    if AHA is None:
+        def AHA(v):
+            return op.adjoint(op.forward(v))
+
+    dt = scalar_dtype
+    x = torch.zeros_like(AHb) if x0 is None else x0.clone()
+    r = AHb - AHA(x) - damp * x
+    z = preconditioner.apply(r) if preconditioner is not None else r
+    p = z.clone()
+    rz_old = _dot(r, z, dt)
+    r0 = _norm(r, dt) + 1e-30
+    for k in range(max_iter):
+        Ap = AHA(p) + damp * p
+        alpha = rz_old / (_dot(p, Ap, dt) + 1e-30)
+        a = alpha.to(x.real.dtype if x.is_complex() else x.dtype)
+        x = x + a * p
+        r = r - a * Ap
+        rel = (_norm(r, dt) / r0).item()
+        if callback is not None:
+            callback(k, x, rel)
+        if rel < tol:
+            break
+        z = preconditioner.apply(r) if preconditioner is not None else r
+        rz_new = _dot(r, z, dt)
+        beta = rz_new / (rz_old + 1e-30)
+        p = z + beta.to(a.dtype) * p
+        rz_old = rz_new
+    return x
+
+
+def lsmr(op, b, *, damp: float = 0.0, x0=None, max_iter: int = 20, atol: float = 1e-6, btol: float = 1e-6,
+         callback=None):
+    """LSMR for ``min ||A x - b||^2 + damp^2 ||x||^2`` (Fong & Saunders,
+    SIAM J. Sci. Comput. 33(5), 2011, Algorithm 1; reference :145-262).
+
+    Stops when ``|zetabar| / ||b|| < atol + btol`` (the reference's test).
+    """
+    b = _normalize_kspace_shape(op, b)
+    if x0 is not None:
+        b = b - op.forward(x0)
+    beta = float(torch.linalg.vector_norm(b).item())
+    if beta == 0.0:
+        return x0.clone() if x0 is not None else op.adjoint(b)
+    u = b / beta
+    v = op.adjoint(u)
+    alpha = float(torch.linalg.vector_norm(v).item())
+    if alpha == 0.0:
+        return x0.clone() if x0 is not None else torch.zeros_like(v)
+    v = v / alpha
+
+    zetabar = alpha * beta
+    alphabar = alpha
+    rho = 1.0
+    rhobar = 1.0
+    cbar = 1.0
+    sbar = 0.0
+    h = v.clone()
+    hbar = torch.zeros_like(v)
+    x = torch.zeros_like(v) if x0 is None else x0.clone()
+    normb = beta
+    for k in range(max_iter):
+        # bidiagonalisation
+        u = op.forward(v) - alpha * u
+        beta = float(torch.linalg.vector_norm(u).item())
+        if beta > 0:
+            u = u / beta
+            v = op.adjoint(u) - beta * v
+            alpha = float(torch.linalg.vector_norm(v).item())
+            if alpha > 0:
+                v = v / alpha
+        # rotation P_k (damping), then Q_k
+        alphahat = math.hypot(alphabar, damp)
+        rhoold = rho
+        rho = math.hypot(alphahat, beta)
+        c = alphahat / rho
+        s = beta / rho
+        thetanew = s * alpha
+        alphabar = c * alpha
+        # rotation Qbar_k
+        rhobarold = rhobar
+        thetabar = sbar * rho
+        rhotemp = cbar * rho
+        rhobar = math.hypot(rhotemp, thetanew)
+        cbar = rhotemp / rhobar
+        sbar = thetanew / rhobar
+        zeta = cbar * zetabar
+        zetabar = -sbar * zetabar
+        # updates
+        hbar = h - (thetabar * rho / (rhoold * rhobarold)) * hbar
+        x = x + (zeta / (rho * rhobar)) * hbar
+        h = v - (thetanew / rho) * h
+        rel = abs(zetabar) / (normb + 1e-30)
+        if callback is not None:
+            callback(k, x, rel)
+        if rel < atol + btol:
+            break
+    return x
+
+
+def solve(op, b, *, method: str = "cg", **kwargs):
+    """Dispatch to :func:`cg` or :func:`lsmr` (reference :268-302)."""
+    m = method.lower()
+    if m == "cg":
+        return cg(op, b, **kwargs)
+    if m == "lsmr":
+        return lsmr(op, b, **kwargs)
+    raise ValueError(f"Unknown solver method: {method!r}. Use 'cg' or 'lsmr'.")

@@ -1,0 +1,7 @@
+"""Calibration: GRAPPA operator table and GROG interpolation (reference
+src/pygrog/calib/__init__.py)."""
+
+from ._grappa import KernelTable, KernelTableResult, build_table, train_grappa
+from ._grog import GrogInterpolator, GrogPlan
+
+__all__ = ["GrogInterpolator", "GrogPlan", "KernelTable", "KernelTableResult", "build_table", "train_grappa"]

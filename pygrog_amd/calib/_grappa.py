@@ -1,0 +1,218 @@
+"""GRAPPA operator table for GROG, built in FP64 on the GPU.
+
+Follows the reference algorithm (src/pygrog/calib/_grappa.py):
+  * calibration normalised by its Frobenius norm                 (:275-287)
+  * unit-shift operators G = X^T from the Tikhonov least squares
+    min ||S X - T||^2 + lamda ||X||^2, S/T = shifted ACR rows     (:16-59, 326-368)
+  * fractional powers G^p = expm(p * logm(G)), logm = V diag(log w) V^-1
+                                                                 (:62-72, 184-208)
+  * table[i1*n + i0] = Gx[i0] @ Gy[i1]            (2D)
+    table[i2*n^2 + i1*n + i0] = Gx[i0] @ Gy[i1] @ Gz[i2]  (3D)   (:131-181)
+with the exponents the reference uses: p_i = (i - (n-1)/2) * stepsize in
+float32 (:100-103).
+
+Where the reference computes in complex64 (LAPACK gelsd on the augmented
+system, complex128 eig cast back to complex64, complex64 expm and products),
+everything here runs in complex128 on the GPU and is cast to complex64 only
+at the end.  The least squares is solved by QR of the augmented system
+``[S; sqrt(lamda) I] X = [T; 0]`` (same minimiser as gelsd).  If this torch
+build has no GPU kernel for a dense factorisation (eig needs MAGMA on ROCm),
+that one factorisation of the C x C (or M x C) matrix runs in FP64 on the host
+and ``KernelTableResult.host_steps`` records it.
+
+3D axis pairing (reference quirk): ``_grappa_op_3d`` trains ``Gx`` along the
+calibration's last axis while the quantised shift ``d0`` is grid axis 0, so
+the reference table pairs ``Gx`` (axis-2 operator) with ``d0``.  The default
+``axis_order="reference"`` reproduces that; ``"corrected"`` builds
+``G_ax0^{d0} @ G_ax1^{d1} @ G_ax2^{d2}`` instead.
+"""
+
+from __future__ import annotations
+
+__all__ = ["KernelTable", "KernelTableResult", "grappa_power", "lstsq", "prepare_grappa_table", "train_grappa"]
+
+import math
+from dataclasses import dataclass, field
+from types import SimpleNamespace
+
+import numpy as np
+import torch
+
+
+@dataclass
+class KernelTableResult:
+    table: torch.Tensor  # (K, C, C) complex64, on the compute device
+    nsteps: int
+    ndim: int
+    stepsize: float
+    host_steps: list = field(default_factory=list)
+    table64: torch.Tensor | None = None  # complex128 table (kept on request)
+
+
+def _to_dev(x, device, dtype):
+    t = torch.as_tensor(np.asarray(x) if not isinstance(x, torch.Tensor) else x)
+    return t.to(device=device, dtype=dtype)
+
+
+def lstsq(A, b, lamda: float = 0.0, host_steps: list | None = None):
+    """Tikhonov least squares via QR of ``[A; sqrt(lamda) I] X = [b; 0]``
+    (reference _grappa.py:16-59).  Batched over leading dims."""
+    A_t = torch.as_tensor(A)
+    b_t = torch.as_tensor(b)
+    N = A_t.shape[-1]
+    if lamda > 0:
+        I_reg = math.sqrt(lamda) * torch.eye(N, dtype=A_t.dtype, device=A_t.device)
+        if A_t.dim() > 2:
+            I_reg = I_reg.expand(*A_t.shape[:-2], N, N)
+        zeros = torch.zeros(*A_t.shape[:-2], N, b_t.shape[-1], dtype=b_t.dtype, device=b_t.device)
+        A_aug = torch.cat([A_t, I_reg], dim=-2)
+        b_aug = torch.cat([b_t, zeros], dim=-2)
+    else:
+        A_aug, b_aug = A_t, b_t
+    try:
+        Q, R = torch.linalg.qr(A_aug, mode="reduced")
+        rhs = Q.mH @ b_aug
+        return torch.linalg.solve_triangular(R, rhs, upper=True)
+    except RuntimeError:
+        if host_steps is not None:
+            host_steps.append("lstsq")
+        Q, R = torch.linalg.qr(A_aug.cpu(), mode="reduced")
+        sol = torch.linalg.solve_triangular(R, Q.mH @ b_aug.cpu(), upper=True)
+        return sol.to(A_t.device)
+
+
+def _matrix_logm(A: torch.Tensor, host_steps: list | None = None) -> torch.Tensor:
+    """logm(A) = V diag(log w) V^-1 in complex128 (reference _grappa.py:62-72,
+    without the cast back to complex64)."""
+    A128 = A.to(torch.complex128)
+    try:
+        vals, vecs = torch.linalg.eig(A128)
+    except (RuntimeError, NotImplementedError):
+        if host_steps is not None:
+            host_steps.append("eig")
+        vals, vecs = torch.linalg.eig(A128.cpu())
+        vals, vecs = vals.to(A.device), vecs.to(A.device)
+    return vecs @ torch.diag_embed(torch.log(vals)) @ torch.linalg.inv(vecs)
+
+
+def grappa_power(G_unit: torch.Tensor, exponents, host_steps: list | None = None) -> torch.Tensor:
+    """Batched ``G^p = expm(p logm(G))`` in complex128 (reference :184-208)."""
+    G_unit = torch.as_tensor(G_unit)
+    exps = torch.as_tensor(exponents, dtype=torch.float32).to(G_unit.device)
+    lG = _matrix_logm(G_unit, host_steps)
+    lG_batch = exps.to(torch.float64).to(lG.dtype).view(-1, 1, 1) * lG.unsqueeze(0)
+    return torch.linalg.matrix_exp(lG_batch)
+
+
+def prepare_grappa_table(Gx, Gy, Gz, nsteps: int, ndim: int, axis_order: str = "reference") -> torch.Tensor:  # noqa: ARG001
+    """Outer-product operator table (reference _grappa.py:131-181)."""
+    Gx = torch.as_tensor(Gx)
+    Gy = torch.as_tensor(Gy)
+    if ndim == 2:
+        # table[i * nsteps + j] = Gx[j] @ Gy[i]
+        return (Gx[None] @ Gy[:, None]).reshape(-1, *Gx.shape[-2:])
+    if ndim == 3:
+        if Gz is None:
+            raise ValueError("3D interpolation requires Z operator")
+        Gz = torch.as_tensor(Gz)
+        if axis_order == "corrected":
+            # shift along grid axis 0 uses the axis-0 operator (Gz here)
+            first, last = Gz, Gx
+        else:
+            first, last = Gx, Gz
+        # table[i * nsteps^2 + j * nsteps + k] = first[k] @ Gy[j] @ last[i]
+        return (first[None, None] @ Gy[None, :, None] @ last[:, None, None]).reshape(-1, *Gx.shape[-2:])
+    raise ValueError(f"GROG interpolation only supports 2D or 3D data, got {ndim}D")
+
+
+def _grappa_op_2d(calib, lamda, host_steps):
+    calib = calib.movedim(0, -1)
+    nc = calib.shape[-1]
+    Sy = calib[:, :-1, :].reshape(-1, nc)
+    Sx = calib[:-1, ...].reshape(-1, nc)
+    Ty = calib[:, 1:, :].reshape(-1, nc)
+    Tx = calib[1:, ...].reshape(-1, nc)
+    Gy = lstsq(Sy, Ty, lamda, host_steps).mT
+    Gx = lstsq(Sx, Tx, lamda, host_steps).mT
+    return Gy, Gx
+
+
+def _grappa_op_3d(calib, lamda, host_steps):
+    calib = calib.movedim(0, -1)
+    nc = calib.shape[-1]
+    Sz = calib[:-1, :, :, :].reshape(-1, nc)
+    Sy = calib[:, :-1, :, :].reshape(-1, nc)
+    Sx = calib[:, :, :-1, :].reshape(-1, nc)
+    Tz = calib[1:, :, :, :].reshape(-1, nc)
+    Ty = calib[:, 1:, :, :].reshape(-1, nc)
+    Tx = calib[:, :, 1:, :].reshape(-1, nc)
+    Gz = lstsq(Sz, Tz, lamda, host_steps).mT
+    Gy = lstsq(Sy, Ty, lamda, host_steps).mT
+    Gx = lstsq(Sx, Tx, lamda, host_steps).mT
+    return Gz, Gy, Gx
+
+
+def train_grappa(train_data, lamda: float | None = None, coords=None, *, device=None, host_steps: list | None = None) -> SimpleNamespace:
+    """Unit-shift GROG operators (reference _grappa.py:211-287), FP64."""
+    if coords is not None:
+        raise NotImplementedError("radial self-calibration (coords=...) is not part of this build")
+    if lamda is None:
+        lamda = 0.01
+    dev = torch.device(device) if device is not None else torch.device("cpu")
+    calib = _to_dev(train_data, dev, torch.complex128)
+    calib = calib / torch.linalg.norm(calib)
+    ndim = calib.ndim - 1
+    if ndim == 2:
+        gy, gx = _grappa_op_2d(calib, lamda, host_steps)
+        return SimpleNamespace(x=gx, y=gy, z=None)
+    if ndim == 3:
+        gz, gy, gx = _grappa_op_3d(calib, lamda, host_steps)
+        return SimpleNamespace(x=gx, y=gy, z=gz)
+    raise ValueError(f"calibration must be (coils, *2 or 3 spatial dims), got {tuple(calib.shape)}")
+
+
+def quant_steps(radius: float, precision: int | None = None, quant: int | None = None):
+    """(pfac, stepsize, nsteps) of the shift quantisation.
+
+    ``precision`` p (reference, _grog.py:406-408 and _grappa.py:100-101):
+    pfac = 10**p, stepsize = 10**-p, nsteps = int(2 radius / stepsize + 1)
+    in Python double arithmetic.  ``quant`` q (extension for 1/q-grid
+    steps): pfac = q, stepsize = 1/q, nsteps = q + 1.
+    """
+    if quant is not None:
+        q = int(quant)
+        if q < 1:
+            raise ValueError("quant must be a positive integer")
+        return float(q), 1.0 / q, q + 1
+    p = int(precision if precision is not None else 1)
+    stepsize = 10 ** (-p)
+    return 10.0**p, stepsize, int(2 * radius / stepsize + 1)
+
+
+def build_table(train_data, radius: float, precision: int | None = 1, lamda: float = 0.01, *, quant=None,
+                device=None, axis_order: str = "reference", keep_fp64: bool = False) -> KernelTableResult:
+    """Operator table on ``device`` (complex64 result of an FP64 build)."""
+    host_steps: list = []
+    kern = train_grappa(train_data, lamda, device=device, host_steps=host_steps)
+    pfac, stepsize, nsteps = quant_steps(radius, precision, quant)
+    deltas = torch.arange(nsteps).float()
+    deltas = (deltas - (nsteps - 1) / 2) * stepsize  # float32, as the reference
+    Gx = grappa_power(kern.x, deltas, host_steps)
+    Gy = grappa_power(kern.y, deltas, host_steps)
+    if kern.z is not None:
+        Gz = grappa_power(kern.z, deltas, host_steps)
+        ndim = 3
+    else:
+        Gz = None
+        ndim = 2
+    t64 = prepare_grappa_table(Gx, Gy, Gz, nsteps, ndim, axis_order=axis_order)
+    t32 = t64.to(torch.complex64).contiguous()
+    return KernelTableResult(t32, nsteps, ndim, stepsize, host_steps, t64 if keep_fp64 else None)
+
+
+def KernelTable(train_data, radius: float, precision: int = 1, lamda: float = 0.01, *, quant=None, device=None,
+                axis_order: str = "reference"):
+    """Reference-compatible entry (``_grappa.py:75-128``): returns
+    ``(table (K, C, C) complex64 numpy array, nsteps, ndim)``."""
+    res = build_table(train_data, radius, precision, lamda, quant=quant, device=device, axis_order=axis_order)
+    return res.table.cpu().numpy(), res.nsteps, res.ndim

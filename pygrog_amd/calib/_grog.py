@@ -1,0 +1,734 @@
+"""GROG interpolator and plan (drop-in for ``pygrog.calib``), device-resident.
+
+Public surface and semantics follow the reference src/pygrog/calib/_grog.py:
+``GrogPlan`` (:27-181), ``GrogInterpolator`` (:270-912) with
+``calc_interp_table`` (:382-422), ``interpolate`` (:437-555), shot-by-shot
+gridding (:697-731, 904-912), and the plan construction of
+``_create_plan`` / ``_create_plan_single`` / ``_attach_fft_plan``
+(:918-1234).
+
+Where the reference builds the plan with float32 torch ops on the host and
+grids with per-coil Python loops over C++ scatter calls, here:
+  * the plan (targets, shifts, EGROG weights, sort permutations, the per-cell
+    CSR and the chunked reduction units) is compiled by HIP kernels
+    (``pygrog_amd/csrc/plan.hip``) with the reference's float32 operation
+    order, so target indices and shifts are bit-exact;
+  * ``interpolate(grid=True)`` / ``ret_image=True`` run one fused kernel
+    (operator apply + deterministic cell-ordered reduction) for all batch,
+    stack and coil entries (``pygrog_amd/csrc/grid.hip``), or the BF16-MFMA
+    grouped-GEMM path (``grid_mfma.hip``) with ``mode="bf16"``/``"bf16x3"``;
+  * ``interpolate()`` (sparse output) runs one fused apply kernel.
+
+Public tensor fields of the plan (``target_idx``, ``indices``, ...) live on
+the GPU and are copied to the plan's home device (the device of the input
+coordinates, CPU for numpy input) on first access.
+"""
+
+from __future__ import annotations
+
+__all__ = ["GrogInterpolator", "GrogPlan"]
+
+import copy
+import ctypes
+import gc
+from types import SimpleNamespace
+
+import numpy as np
+import torch
+
+from .. import _lib
+from ._grappa import build_table, quant_steps
+
+# Replicas folded sequentially per reduction unit; fixes the summation order
+# of every cell independently of tiling (see grid.hip).
+UNIT_CHUNK = 32
+
+_LAZY_FIELDS = ("target_idx", "distances", "weights", "indices", "sqrt_weights", "sort_perm", "inv_perm", "time_map")
+
+
+# ---------------------------------------------------------------------------
+# Plan container
+# ---------------------------------------------------------------------------
+class GrogPlan(SimpleNamespace):
+    """GROG / FFT plan (reference ``GrogPlan``, _grog.py:27-181).
+
+    Scalar fields (``grid_shape``, ``image_shape``, ``n_samples``, ...) are
+    plain attributes.  Trajectory-bound tensors are held on the GPU in
+    ``_dev`` and materialised on the home device on first attribute access.
+    """
+
+    def __getattr__(self, name):
+        d = self.__dict__
+        if name in _LAZY_FIELDS:
+            dev = d.get("_dev")
+            if dev is not None and name in dev and dev[name] is not None:
+                t = dev[name]
+                home = d.get("_home")
+                v = t if (home is None or t.device == home) else t.to(home)
+                d[name] = v
+                return v
+            if name == "time_map":
+                return None
+        raise AttributeError(name)
+
+    @property
+    def stack_shape(self) -> tuple:
+        return tuple(getattr(self, "_stack_shape", ()))
+
+    @stack_shape.setter
+    def stack_shape(self, value):
+        self._stack_shape = tuple(int(s) for s in (value or ()))
+
+    @property
+    def is_stacked(self) -> bool:
+        return len(self.stack_shape) > 0
+
+    @property
+    def pre_weights(self) -> torch.Tensor:
+        """``sqrt_weights`` in interpolate-output order: ``sqrt_w[inv_perm]``
+        (reference _grog.py:60-76)."""
+        sw = self.sqrt_weights
+        ip = self.inv_perm
+        if not self.is_stacked:
+            return sw[ip]
+        return torch.gather(sw, -1, ip)
+
+    @classmethod
+    def _stack(cls, plans: list) -> "GrogPlan":
+        """Stack unstacked per-element plans (reference _grog.py:80-181)."""
+        if not plans:
+            raise ValueError("Cannot stack an empty list of plans.")
+        for p in plans:
+            if getattr(p, "stack_shape", ()) != ():
+                raise ValueError("GrogPlan.stack requires unstacked input plans.")
+        ref = plans[0]
+        global_fields = ("shape", "oversamp", "grid_shape", "kernel_width", "kernel_shape", "radius",
+                         "image_shape", "grid_size", "n_samples", "natural_shape")
+        for f in global_fields:
+            ref_v = getattr(ref, f, None)
+            for p in plans[1:]:
+                if getattr(p, f, None) != ref_v:
+                    raise ValueError(f"GrogPlan.stack: field '{f}' differs across plans")
+        out = cls(**{f: getattr(ref, f) for f in global_fields})
+        out.stack_shape = (len(plans),)
+        home = ref.__dict__.get("_home")
+        out._home = home
+
+        def dev_or_attr(p, name):
+            d = p.__dict__.get("_dev") or {}
+            t = d.get(name)
+            return t if t is not None else torch.as_tensor(getattr(p, name))
+
+        dev = {}
+        for name in ("target_idx", "weights", "indices", "sqrt_weights", "sort_perm", "inv_perm"):
+            dev[name] = torch.stack([dev_or_attr(p, name) for p in plans], dim=0)
+        if all(getattr(p, "distances", None) is not None for p in plans):
+            dev["distances"] = torch.stack([dev_or_attr(p, "distances") for p in plans], dim=0)
+        else:
+            out.distances = None
+        tms = [getattr(p, "time_map", None) for p in plans]
+        if any(t is not None for t in tms):
+            if any(t is None for t in tms):
+                raise ValueError("GrogPlan.stack: all plans must agree on time_map presence")
+            dev["time_map"] = torch.stack([torch.as_tensor(t) for t in tms], dim=0)
+        out._dev = dev
+        return out
+
+
+# ---------------------------------------------------------------------------
+# Helpers (reference _grog.py:1240-1322)
+# ---------------------------------------------------------------------------
+def _kernel_offsets(ndim: int, width: int, shape: str) -> np.ndarray:
+    """Integer neighbourhood offsets, (kw, ndim) int32 (reference :1240-1277).
+
+    Axis values ``arange(width) - floor((width - 1) / 2)``, enumerated with the
+    last axis fastest; 'circle'/'sphere' keeps radius <= (width-1)/2 + 0.5,
+    'cross' keeps offsets with at most one nonzero axis.
+    """
+    half = (width - 1) / 2.0
+    ax = np.arange(width) - int(np.floor(half))
+    mesh = np.meshgrid(*([ax] * ndim), indexing="ij")
+    offs = np.stack([m.ravel() for m in mesh], axis=-1).astype(np.int32)
+    key = shape.lower()
+    if key in ("square", "cube"):
+        return offs
+    if key in ("circle", "sphere"):
+        r = np.sqrt((offs.astype(np.float32) ** 2).sum(axis=-1))
+        return offs[r <= half + 0.5]
+    if key == "cross":
+        return offs[(offs != 0).sum(axis=-1) <= 1]
+    raise ValueError(
+        f"Unknown kernel_shape '{shape}'. Use 'circle'/'sphere', 'square'/'cube', or 'cross'."
+    )
+
+
+def _default_shape(ndim, shape):
+    if np.isscalar(shape):
+        shape = ndim * [shape]
+    return tuple(int(s) for s in shape)
+
+
+def _default_oversamp(ndim, oversamp):
+    if oversamp is None:
+        oversamp = (1.0, 1.0) if ndim == 2 else (1.0, 1.0, 1.2)
+    if np.isscalar(oversamp):
+        oversamp = ndim * [oversamp]
+    oversamp = tuple(oversamp)
+    if len(oversamp) != ndim:
+        raise ValueError(f"Oversampling {oversamp} does not match ndim={ndim}")
+    return oversamp
+
+
+def _u32(t: torch.Tensor) -> int:
+    return _lib.ptr(t)
+
+
+def _exclusive_scan(counts: torch.Tensor, device) -> torch.Tensor:
+    """counts: (n + 1,) int32 with counts[n] == 0 -> (n + 1,) exclusive sums."""
+    n = counts.numel() - 1
+    out = torch.empty_like(counts)
+    _lib.run_with_workspace("pg_exclusive_scan_u32", device, _u32(counts), _u32(out), n, _lib.stream(device))
+    return out
+
+
+def compile_units(cell_ptr: torch.Tensor, S: int, G: int, device, chunk: int = UNIT_CHUNK,
+                  unit_count: torch.Tensor | None = None):
+    """Chunked reduction units from a per-element CSR (S, G + 1) uint32.
+
+    Returns (cell_unit (S*G + 1), unit_cell (U), unit_rep (U + S)) int32
+    tensors holding uint32 values; unit u of element s spans replicas
+    [unit_rep[u + s], unit_rep[u + s + 1]).
+    """
+    st = _lib.stream(device)
+    if unit_count is None:
+        unit_count = torch.zeros(S * G + 1, dtype=torch.int32, device=device)
+        _lib.call("pg_unit_count", _u32(cell_ptr), S, G, chunk, _u32(unit_count), st)
+    cell_unit = _exclusive_scan(unit_count, device)
+    U = int(cell_unit[-1].item())
+    unit_cell = torch.empty(max(U, 1), dtype=torch.int32, device=device)
+    unit_rep = torch.empty(U + S, dtype=torch.int32, device=device)
+    _lib.call("pg_plan_units", _u32(cell_ptr), _u32(cell_unit), S, G, chunk, _u32(unit_cell), _u32(unit_rep), st)
+    return cell_unit, unit_cell, unit_rep
+
+
+class GridProgram:
+    """A ready-to-launch fused gridding job: device frame table + tile map.
+
+    Frames are (batch b, stack element s) pairs of one compiled plan; built
+    once per (input, output) buffer pair so repeated launches only enqueue
+    the kernel (no host work beyond the ctypes call).
+    """
+
+    def __init__(self, frames: list, C: int, G: int, device, tile_cells: int, n_blocks: int | None = None):
+        self.device = device
+        self.C = int(C)
+        self.F = len(frames)
+        self.tile_cells = int(tile_cells)
+        arr = (_lib.Frame * self.F)(*frames)
+        raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
+        self.frames = raw.to(device)
+        tiles_per = (G + self.tile_cells - 1) // self.tile_cells
+        self.n_tiles = tiles_per * self.F
+        self.tile_ptr = (torch.arange(self.F + 1, dtype=torch.int64) * tiles_per).to(torch.int32).to(device)
+        if n_blocks is None:
+            n_blocks = min(self.n_tiles, 256 * 12)
+        self.n_blocks = int(n_blocks)
+
+    def launch(self, entry: str = "pg_grid_fp32") -> None:
+        _lib.call(entry, _lib.ptr(self.frames), _lib.ptr(self.tile_ptr), self.F, self.n_tiles, self.C,
+                  self.tile_cells, self.n_blocks, _lib.stream(self.device))
+
+
+# ---------------------------------------------------------------------------
+# Plan creation on the GPU
+# ---------------------------------------------------------------------------
+def _geom(shape, grid_shape, offsets) -> _lib.PlanGeom:
+    ndim = len(shape)
+    kw = offsets.shape[0]
+    if kw > 128:
+        raise ValueError(f"kernel has {kw} offsets; at most 128 are supported")
+    g = _lib.PlanGeom()
+    g.ndim = ndim
+    g.kw = kw
+    for d in range(3):
+        if d < ndim:
+            s, gs = shape[d], grid_shape[d]
+            g.grid_shape[d] = gs
+            g.strides[d] = int(np.prod(grid_shape[d + 1:])) if d + 1 < ndim else 1
+            amp = 2 * (s - 1 - s // 2)
+            g.half_amp[d] = float(np.float32(0.5) * np.float32(amp))
+            g.origins[d] = float(np.float32(-(s // 2)))
+            g.steps[d] = float(np.float32((s - 1) / (gs - 1) if gs > 1 else 1.0))
+        else:
+            g.grid_shape[d] = 1
+            g.strides[d] = 0
+            g.half_amp[d] = 0.0
+            g.origins[d] = 0.0
+            g.steps[d] = 1.0
+    for k in range(kw):
+        for d in range(ndim):
+            g.offsets[k][d] = int(offsets[k, d])
+    return g
+
+
+def _create_plan(shape, coords, oversamp, kernel_width, kernel_shape, time_map, stack_shape=(), device=None):
+    """Build the plan of every stack element in one set of kernel launches
+    (reference _create_plan + _create_plan_single + _attach_fft_plan)."""
+    dev = _lib.hip_device(device)
+    st = _lib.stream(dev)
+    home = coords.device if isinstance(coords, torch.Tensor) else torch.device("cpu")
+    c = torch.as_tensor(coords)
+    if c.dtype != torch.float32:
+        c = c.to(torch.float32)
+    ndim = int(c.shape[-1])
+    s_shape = tuple(int(s) for s in (stack_shape or ()))
+    if s_shape and tuple(c.shape[: len(s_shape)]) != s_shape:
+        raise ValueError(f"coords leading shape {tuple(c.shape[:len(s_shape)])} does not match stack_shape {s_shape}")
+    spatial = tuple(int(s) for s in c.shape[len(s_shape):-1])
+    S = int(np.prod(s_shape)) if s_shape else 1
+    npts = int(np.prod(spatial)) if spatial else 1
+
+    shape = _default_shape(ndim, shape)
+    oversamp = _default_oversamp(ndim, oversamp)
+    grid_shape = tuple(int(np.ceil(s * o)) for s, o in zip(shape, oversamp))
+    G = int(np.prod(grid_shape))
+    offsets = _kernel_offsets(ndim, kernel_width, kernel_shape)
+    kw = int(offsets.shape[0])
+    nper = npts * kw
+    geom = _geom(shape, grid_shape, offsets)
+
+    cd = c.reshape(S, npts, ndim).to(dev).contiguous()
+    cmax = torch.zeros(S, ndim, dtype=torch.float32, device=dev)
+    _lib.call("pg_plan_absmax", _lib.ptr(cd), S, npts, ndim, _lib.ptr(cmax), st)
+
+    total = S * nper
+    target_idx = torch.empty(S, nper, dtype=torch.int64, device=dev)
+    distances = torch.empty(S, nper, ndim, dtype=torch.float32, device=dev)
+    gauss = torch.empty(S, nper, dtype=torch.float32, device=dev)
+    keys = torch.empty(total, dtype=torch.int64, device=dev)
+    vals = torch.empty(total, dtype=torch.int32, device=dev)
+    n_invalid = torch.zeros(S, dtype=torch.int64, device=dev)
+    _lib.call("pg_plan_targets", _lib.ptr(cd), _lib.ptr(cmax), S, npts, geom, _lib.ptr(target_idx), _lib.ptr(distances),
+              _lib.ptr(gauss), _lib.ptr(keys), _lib.ptr(vals), _lib.ptr(n_invalid), st)
+
+    keys_s = torch.empty_like(keys)
+    vals_s = torch.empty_like(vals)
+    end_bit = max(1, int(S * (G + 1) - 1).bit_length())
+    _lib.run_with_workspace("pg_sort_pairs_u64", dev, _lib.ptr(keys), _lib.ptr(keys_s), _lib.ptr(vals), _lib.ptr(vals_s),
+                            total, end_bit, st)
+    del keys, vals
+
+    indices = torch.empty(S, nper, dtype=torch.int64, device=dev)
+    sort_perm = torch.empty(S, nper, dtype=torch.int64, device=dev)
+    inv_perm = torch.empty(S, nper, dtype=torch.int64, device=dev)
+    gauss_sorted = torch.empty(total, dtype=torch.float32, device=dev)
+    cell_ptr = torch.empty(S, G + 1, dtype=torch.int32, device=dev)
+    if nper == 0:
+        cell_ptr.zero_()
+    _lib.call("pg_plan_post", _lib.ptr(keys_s), _lib.ptr(vals_s), _lib.ptr(gauss), _lib.ptr(n_invalid), S, nper, G,
+              _lib.ptr(indices), _lib.ptr(sort_perm), _lib.ptr(inv_perm), _lib.ptr(gauss_sorted), _lib.ptr(cell_ptr), st)
+    del keys_s
+
+    weights = torch.zeros(S, nper, dtype=torch.float32, device=dev)
+    sqrt_weights = torch.zeros(S, nper, dtype=torch.float32, device=dev)
+    rep_src = torch.zeros(S, max(nper, 1), dtype=torch.int32, device=dev)
+    rep_s = torch.zeros(S, max(nper, 1), dtype=torch.float32, device=dev)
+    density = torch.empty(S, G, dtype=torch.float32, device=dev)
+    unit_count = torch.zeros(S * G + 1, dtype=torch.int32, device=dev)
+    _lib.call("pg_plan_weights", _lib.ptr(vals_s), _lib.ptr(gauss_sorted), _lib.ptr(n_invalid), _lib.ptr(cell_ptr), S,
+              nper, G, kw, UNIT_CHUNK, _lib.ptr(weights), _lib.ptr(sqrt_weights), _lib.ptr(rep_src), _lib.ptr(rep_s),
+              _lib.ptr(density), _lib.ptr(unit_count), max(nper, 1), st)
+    del vals_s, gauss_sorted, gauss
+    cell_unit, unit_cell, unit_rep = compile_units(cell_ptr, S, G, dev, UNIT_CHUNK, unit_count)
+    n_inv = n_invalid.cpu().tolist()
+
+    if time_map is not None:
+        tm = torch.as_tensor(np.asarray(time_map, dtype=np.float32) if not isinstance(time_map, torch.Tensor) else time_map)
+        tm = tm.to(torch.float32).reshape(S, npts, 1).expand(S, npts, kw).reshape(S, nper).to(dev).contiguous()
+    else:
+        tm = None
+
+    full = tuple(s_shape) + spatial + (kw,)
+    dev_fields = {
+        "target_idx": target_idx.reshape(full),
+        "distances": distances.reshape(full + (ndim,)),
+        "weights": weights.reshape(full),
+        "time_map": tm.reshape(full) if tm is not None else None,
+        "indices": indices.reshape(s_shape + (nper,)) if s_shape else indices.reshape(nper),
+        "sqrt_weights": sqrt_weights.reshape(s_shape + (nper,)) if s_shape else sqrt_weights.reshape(nper),
+        "sort_perm": sort_perm.reshape(s_shape + (nper,)) if s_shape else sort_perm.reshape(nper),
+        "inv_perm": inv_perm.reshape(s_shape + (nper,)) if s_shape else inv_perm.reshape(nper),
+    }
+    plan = GrogPlan(shape=shape, oversamp=oversamp, grid_shape=grid_shape, kernel_width=kernel_width,
+                    kernel_shape=kernel_shape, radius=0.5)
+    plan._home = home
+    plan._dev = dev_fields
+    plan._compiled = SimpleNamespace(
+        device=dev, S=S, npts=npts, kw=kw, nper=nper, G=G, ndim=ndim, spatial=spatial,
+        cell_ptr=cell_ptr, rep_src=rep_src, rep_s=rep_s, rep_ent=None, cell_unit=cell_unit, unit_cell=unit_cell,
+        unit_rep=unit_rep, density=density, n_invalid=n_invalid, n_inv=n_inv,
+        n_valid=[nper - v for v in n_inv], target_flat=target_idx, distances=distances, inv_perm_flat=inv_perm,
+        interp_idx=None, entry_groups=None,
+    )
+    if s_shape:
+        plan.stack_shape = s_shape
+    return plan
+
+
+def _attach_fft_plan(plan, image_shape=None):
+    """FFT-plan scalars (reference _attach_fft_plan, _grog.py:918-998); the
+    sorted arrays were produced by the plan kernels."""
+    grid_shape = tuple(plan.grid_shape)
+    if image_shape is None:
+        image_shape = tuple(plan.shape)
+    elif np.isscalar(image_shape):
+        image_shape = len(grid_shape) * (int(image_shape),)
+    plan.image_shape = tuple(int(s) for s in image_shape)
+    plan.grid_size = int(np.prod(grid_shape))
+    comp = plan._compiled
+    plan.natural_shape = tuple(comp.spatial) + (comp.kw,)
+    plan.n_samples = int(comp.nper)
+
+
+# ---------------------------------------------------------------------------
+# GrogInterpolator
+# ---------------------------------------------------------------------------
+class GrogInterpolator:
+    """Data-driven GROG interpolator (reference ``GrogInterpolator``).
+
+    Parameters are the reference's (_grog.py:302-312) plus ``device``, the
+    HIP device that holds the plan and runs the kernels (default: current).
+    """
+
+    _interpolator_set = False
+    _dataset_shape_set = False
+    _data: list | None = None
+
+    def __init__(self, shape, coords, oversamp=None, kernel_width: int = 2, kernel_shape: str = "circle",
+                 time_map=None, image_shape=None, stack_shape: tuple = (), *, device=None):
+        coords_arr = coords if isinstance(coords, torch.Tensor) else np.asarray(coords)
+        stack_eff = tuple(int(s) for s in (stack_shape or ()))
+        if not stack_eff and coords_arr.ndim == 4:
+            stack_eff = (int(coords_arr.shape[0]),)
+        self.plan = _create_plan(shape, coords_arr, oversamp, kernel_width, kernel_shape, time_map,
+                                 stack_shape=stack_eff, device=device)
+        self.device = self.plan._compiled.device
+        _attach_fft_plan(self.plan, image_shape)
+        self._programs = {}
+
+    # -- metadata / convenience (reference :337-376) -------------------------
+    def metadata(self):
+        oshape = np.ceil(np.asarray(self.plan.shape) * np.asarray(self.plan.oversamp)).astype(int).tolist()
+        return SimpleNamespace(shape=tuple(self.plan.shape), oshape=tuple(oshape), target_idx=self.plan.target_idx,
+                               weights=self.plan.weights, time_map=self.plan.time_map)
+
+    def fft_plan(self, image_shape=None):
+        if image_shape is not None and tuple(np.atleast_1d(image_shape)) != tuple(self.plan.image_shape):
+            plan = copy.copy(self.plan)
+            _attach_fft_plan(plan, image_shape)
+            return plan
+        return self.plan
+
+    # -- table (reference :382-422) ------------------------------------------
+    def calc_interp_table(self, train_data, lamda: float = 0.01, precision: int = 1, *, quant: int | None = None,
+                          axis_order: str = "reference"):
+        """Build the operator table (FP64 on the GPU) and quantise every
+        replica's shift to a table entry (bit-exact, in the plan kernels).
+
+        ``quant=q`` quantises shifts to 1/q grid steps (table of (q+1)^ndim
+        entries) instead of ``precision`` decimal digits.
+        """
+        comp = self.plan._compiled
+        if comp.distances is None:
+            raise RuntimeError("calc_interp_table: shifts were already released by a previous call")
+        dev = comp.device
+        res = build_table(train_data, self.plan.radius, precision, lamda, quant=quant, device=dev,
+                          axis_order=axis_order)
+        pfac, stepsize, nsteps = quant_steps(self.plan.radius, precision, quant)
+        if res.table.shape[0] != nsteps ** comp.ndim:
+            raise RuntimeError("table size does not match the quantisation")
+        K = int(res.table.shape[0])
+        interp_idx = torch.empty(comp.S, comp.nper, dtype=torch.int64, device=dev)
+        rep_ent = torch.zeros(comp.S, max(comp.nper, 1), dtype=torch.int32, device=dev)
+        _lib.call("pg_plan_quantize", _lib.ptr(comp.distances), _lib.ptr(comp.inv_perm_flat), _lib.ptr(comp.n_invalid),
+                  _lib.ptr(comp.target_flat), comp.S, comp.nper, comp.ndim, float(np.float32(pfac)),
+                  float(np.float32(stepsize)), float(np.float32(self.plan.radius)), int(nsteps), K,
+                  _lib.ptr(interp_idx), _lib.ptr(rep_ent), max(comp.nper, 1), _lib.stream(dev))
+        comp.rep_ent = rep_ent
+        comp.interp_idx = interp_idx
+        comp.entry_groups = None
+        C = int(res.table.shape[1])
+        comp.C = C
+        comp.table = res.table
+        comp.table_host_steps = res.host_steps
+        if C <= 64:
+            packed = torch.empty(K * ((C + 1) // 2) * C * 4, dtype=torch.float32, device=dev)
+            _lib.call("pg_table_pack_c64", _lib.ptr(res.table), K, C, _lib.ptr(packed), _lib.stream(dev))
+            comp.table_packed = packed
+        else:
+            comp.table_packed = None
+        comp.table_bf16 = {}
+        # release the shifts, as the reference does
+        comp.distances = None
+        self.plan._dev["distances"] = None
+        self.plan.distances = None
+        self._interp_kernel = res.table
+        full = self.plan._dev["target_idx"].shape
+        self._interp_idx = interp_idx.reshape(full)
+        self._interpolator_set = True
+        self._programs = {}
+
+    def set_dataset_shape(self, shape):
+        if self._dataset_shape_set:
+            return
+        self.dataset_shape = tuple(shape)
+        self._dataset_shape_set = True
+
+    # -- data layout helpers --------------------------------------------------
+    def _split(self, data: torch.Tensor):
+        """(*B, *S, C, *spatial) -> (batch_shape, C)."""
+        comp = self.plan._compiled
+        s_shape = tuple(self.plan.stack_shape)
+        spatial = tuple(comp.spatial)
+        ncoils_pos = data.ndim - len(spatial) - 1
+        if ncoils_pos < len(s_shape):
+            raise ValueError(f"data.ndim={data.ndim} too small for stack_shape={s_shape} + spatial={spatial} (+coils)")
+        if s_shape:
+            pref = tuple(int(s) for s in data.shape[ncoils_pos - len(s_shape):ncoils_pos])
+            if pref != s_shape:
+                raise ValueError(f"data stack prefix {pref} != plan.stack_shape {s_shape}")
+        if tuple(int(s) for s in data.shape[ncoils_pos + 1:]) != spatial:
+            raise ValueError(f"data spatial dims {tuple(data.shape[ncoils_pos + 1:])} != plan {spatial}")
+        batch_shape = tuple(int(s) for s in data.shape[: ncoils_pos - len(s_shape)])
+        return batch_shape, int(data.shape[ncoils_pos])
+
+    def _to_device_c64(self, data: torch.Tensor, B: int, S: int, C: int) -> torch.Tensor:
+        comp = self.plan._compiled
+        return data.to(comp.device).to(torch.complex64).reshape(B, S, C, comp.npts).contiguous()
+
+    def _frames(self, y: torch.Tensor, out: torch.Tensor, B: int, vbuf: torch.Tensor | None = None):
+        comp = self.plan._compiled
+        S, C, N, G = comp.S, comp.C, comp.npts, comp.G
+        frames = []
+        for b in range(B):
+            for s in range(S):
+                f = _lib.Frame()
+                f.y = y.data_ptr() + ((b * S + s) * C * N) * 8
+                f.grid = out.data_ptr() + ((b * S + s) * C * G) * 8
+                f.table = _lib.ptr(comp.table_packed) if comp.table_packed is not None else None
+                stride = max(comp.nper, 1) * 4
+                f.rep_src = comp.rep_src.data_ptr() + s * stride
+                f.rep_ent = comp.rep_ent.data_ptr() + s * stride
+                f.rep_s = comp.rep_s.data_ptr() + s * stride
+                f.unit_cell = comp.unit_cell.data_ptr()
+                f.unit_rep = comp.unit_rep.data_ptr() + s * 4
+                f.cell_unit = comp.cell_unit.data_ptr() + s * G * 4
+                if vbuf is not None:
+                    off = (b * S + s) * max(comp.nper, 1) * C * 8
+                    f.vbuf = vbuf.data_ptr() + off
+                f.n_src = N
+                f.n_cell = G
+                frames.append(f)
+        return frames
+
+    def grid_program(self, y: torch.Tensor, out: torch.Tensor, B: int = 1, vbuf=None, n_blocks=None) -> GridProgram:
+        comp = self.plan._compiled
+        tc = _lib.load().pg_grid_tile_cells(comp.C)
+        return GridProgram(self._frames(y, out, B, vbuf), comp.C, comp.G, comp.device, tc, n_blocks)
+
+    # -- grouped (BF16 MFMA) structures ---------------------------------------
+    def _entry_groups(self):
+        """Per stack element: replicas of each table entry (stable radix sort of
+        the entries), as (ent_ptr (K+1), ent_rep (R), max group size)."""
+        comp = self.plan._compiled
+        if comp.entry_groups is not None:
+            return comp.entry_groups
+        dev = comp.device
+        st = _lib.stream(dev)
+        K = int(comp.table.shape[0])
+        groups = []
+        for s in range(comp.S):
+            R = comp.n_valid[s]
+            keys = comp.rep_ent[s, :max(R, 1)].contiguous()
+            vals = torch.arange(max(R, 1), dtype=torch.int32, device=dev)
+            keys_o = torch.empty_like(keys)
+            vals_o = torch.empty_like(vals)
+            if R > 0:
+                _lib.run_with_workspace("pg_sort_pairs_u32", dev, _lib.ptr(keys), _lib.ptr(keys_o), _lib.ptr(vals),
+                                        _lib.ptr(vals_o), R, max(1, (K - 1).bit_length()), st)
+            ent_ptr = torch.empty(K + 1, dtype=torch.int32, device=dev)
+            _lib.call("pg_csr_from_sorted_u32", _lib.ptr(keys_o), R, K, _lib.ptr(ent_ptr), st)
+            mx = int((ent_ptr[1:] - ent_ptr[:-1]).max().item()) if K > 0 else 0
+            groups.append((ent_ptr, vals_o, mx))
+        comp.entry_groups = groups
+        return groups
+
+    def _table_bf16(self, n_parts: int):
+        comp = self.plan._compiled
+        if n_parts not in comp.table_bf16:
+            K, C = int(comp.table.shape[0]), comp.C
+            n = int(_lib.load().pg_table_bf16_elems(K, C, n_parts))
+            packed = torch.empty(n, dtype=torch.int16, device=comp.device)
+            _lib.call("pg_table_pack_bf16", _lib.ptr(comp.table), K, C, n_parts, _lib.ptr(packed),
+                      _lib.stream(comp.device))
+            comp.table_bf16[n_parts] = packed
+        return comp.table_bf16[n_parts]
+
+    # -- gridding on device (no host transfer) --------------------------------
+    def grid_device(self, y: torch.Tensor, mode: str = "fp32", out: torch.Tensor | None = None) -> torch.Tensor:
+        """Grid device-resident ``y`` (B, S, C, N) complex64 -> (B, S, C, G).
+
+        mode "fp32": plain fused kernel.  "bf16" / "bf16x3": grouped complex
+        GEMM on BF16 MFMA (1 or 3 split products, FP32 accumulate), then the
+        same deterministic reduction.
+        """
+        if not self._interpolator_set:
+            raise RuntimeError("GRAPPA kernels not set. Call calc_interp_table() first.")
+        comp = self.plan._compiled
+        if comp.C > 64:
+            raise RuntimeError(f"fused gridding supports up to 64 coils, got {comp.C}")
+        B, S, C, N = y.shape
+        if (S, C, N) != (comp.S, comp.C, comp.npts):
+            raise ValueError(f"y shape {tuple(y.shape)} does not match plan (B, {comp.S}, {comp.C}, {comp.npts})")
+        if out is None:
+            out = torch.empty(B, S, C, comp.G, dtype=torch.complex64, device=comp.device)
+        if mode == "fp32":
+            self.grid_program(y, out, B).launch("pg_grid_fp32")
+            return out
+        if mode not in ("bf16", "bf16x3"):
+            raise ValueError(f"unknown gridding mode {mode!r}")
+        n_terms = 1 if mode == "bf16" else 3
+        tbl = self._table_bf16(2 if n_terms == 3 else 1)
+        groups = self._entry_groups()
+        vbuf = torch.empty(B * S * max(comp.nper, 1) * C, dtype=torch.complex64, device=comp.device)
+        yt = torch.empty(N * C, dtype=torch.complex64, device=comp.device)
+        st = _lib.stream(comp.device)
+        K = int(comp.table.shape[0])
+        for b in range(B):
+            for s in range(S):
+                ent_ptr, ent_rep, mx = groups[s]
+                if comp.n_valid[s] == 0:
+                    continue
+                _lib.call("pg_transpose_cn_c64", y[b, s].data_ptr(), C, N, _lib.ptr(yt), st)
+                vb = vbuf.data_ptr() + (b * S + s) * max(comp.nper, 1) * C * 8
+                _lib.call("pg_apply_grouped_bf16", _lib.ptr(yt), _lib.ptr(tbl), _lib.ptr(ent_ptr), _lib.ptr(ent_rep),
+                          comp.rep_src[s].data_ptr(), comp.rep_s[s].data_ptr(), K, C, mx, n_terms, vb, st)
+        self.grid_program(y, out, B, vbuf=vbuf).launch("pg_grid_reduce_v")
+        return out
+
+    # -- forward interpolation (reference :437-555) ----------------------------
+    def interpolate(self, data, shot_index=None, ret_image: bool = False, grid: bool = False, *, mode: str = "fp32"):
+        if not self._interpolator_set:
+            raise RuntimeError("GRAPPA kernels not set. Call calc_interp_table() first.")
+        if grid and ret_image:
+            raise ValueError("'grid' and 'ret_image' are mutually exclusive.")
+        is_numpy = isinstance(data, np.ndarray)
+        data_t = torch.as_tensor(data)
+        src_device = data_t.device
+        out_dtype = data_t.dtype if data_t.is_complex() else torch.complex64
+
+        if shot_index is not None:
+            if np.isscalar(shot_index):
+                shot_index = (shot_index,)
+            gridded = self._grid_shot(data_t, tuple(shot_index))
+            if self._data is None:
+                self._data = []
+            self._data.append(gridded)
+            return None
+
+        plan = self.plan
+        comp = plan._compiled
+        batch_shape, C = self._split(data_t)
+        if C != comp.C:
+            raise ValueError(f"data has {C} coils but the table was trained for {comp.C}")
+        B = int(np.prod(batch_shape)) if batch_shape else 1
+        s_shape = tuple(plan.stack_shape)
+        y = self._to_device_c64(data_t, B, comp.S, C)
+
+        if ret_image or grid:
+            g = self.grid_device(y, mode=mode)  # (B, S, C, G)
+            gshape = tuple(plan.grid_shape)
+            if ret_image:
+                from .._base._fftc import ifft
+                from .._utils import resize
+
+                img = ifft(g.reshape(B, comp.S, C, *gshape), axes=tuple(range(-len(gshape), 0)))
+                img = resize(img, (B, comp.S, C, *plan.image_shape))
+                image = img.abs().square().sum(2).sqrt()
+                image = image.reshape(*batch_shape, *s_shape, *plan.image_shape).to(src_device)
+                return image.numpy() if is_numpy else image
+            kspace = g.reshape(*batch_shape, *s_shape, C, *gshape).to(out_dtype).to(src_device)
+            from ..operator._masked_fft import MaskedFFTPlan
+
+            density = comp.density.reshape(*s_shape, *gshape)
+            mplan = MaskedFFTPlan(grid_shape=gshape, image_shape=tuple(plan.image_shape), stack_shape=s_shape,
+                                  mask=(density > 0).float().to(src_device), density=density.to(src_device))
+            if is_numpy:
+                numpy_plan = MaskedFFTPlan(grid_shape=gshape, image_shape=tuple(plan.image_shape), stack_shape=s_shape,
+                                           mask=np.asarray(mplan.mask.cpu()), density=np.asarray(mplan.density.cpu()))
+                return np.asarray(kspace.cpu()), numpy_plan
+            return kspace, mplan
+
+        sparse = torch.empty(B, comp.S, C, comp.nper, dtype=torch.complex64, device=comp.device)
+        K = int(comp.table.shape[0])
+        st = _lib.stream(comp.device)
+        for s in range(comp.S):
+            ys = y[:, s].contiguous() if comp.S > 1 else y[:, 0]
+            os_ = torch.empty(B, C, comp.nper, dtype=torch.complex64, device=comp.device)
+            _lib.call("pg_sparse_apply_c64", _lib.ptr(ys), comp.interp_idx[s].data_ptr(), _lib.ptr(comp.table),
+                      _lib.ptr(os_), B, C, comp.npts, comp.kw, K, st)
+            sparse[:, s] = os_
+        out = sparse.reshape(*batch_shape, *s_shape, C, comp.nper).to(out_dtype).to(src_device)
+        return out.numpy() if is_numpy else out
+
+    def __call__(self, data, shot_index=None, ret_image=False, grid=False):
+        return self.interpolate(data, shot_index, ret_image, grid)
+
+    # -- shot-by-shot (reference :697-731, 904-912) ---------------------------
+    def _grid_shot(self, data: torch.Tensor, shot_index: tuple) -> torch.Tensor:
+        comp = self.plan._compiled
+        dev = comp.device
+        tidx = self.plan._dev["target_idx"]
+        w_all = self.plan._dev["weights"]
+        iidx = self._interp_idx
+        t_shot = tidx[shot_index].reshape(-1).contiguous()
+        w_shot = w_all[shot_index].reshape(-1).contiguous()
+        e_shot = iidx[shot_index].reshape(-1).contiguous()
+        kw = comp.kw
+        npts = t_shot.numel() // kw
+        C = int(data.shape[0])
+        y = data.to(dev).to(torch.complex64).reshape(1, C, npts).contiguous()
+        applied = torch.empty(1, C, npts * kw, dtype=torch.complex64, device=dev)
+        st = _lib.stream(dev)
+        _lib.call("pg_sparse_apply_c64", _lib.ptr(y), _lib.ptr(e_shot), _lib.ptr(comp.table), _lib.ptr(applied), 1, C,
+                  npts, kw, int(comp.table.shape[0]), st)
+        out = torch.zeros(C, comp.G, dtype=torch.complex64, device=dev)
+        for c in range(C):
+            _lib.call("pg_scatter_add_c64", out[c].data_ptr(), comp.G, applied[0, c].data_ptr(), _lib.ptr(t_shot),
+                      _lib.ptr(w_shot), npts * kw, st)
+        return out.to(data.device)
+
+    def collect_shots(self) -> torch.Tensor:
+        if self._data is None or len(self._data) == 0:
+            raise RuntimeError("No shots accumulated.")
+        output = torch.stack(self._data, dim=0).sum(dim=0)
+        self._data = None
+        return output.reshape(output.shape[0], *self.plan.grid_shape)
+
+    # -- statistics used by the benchmark ---------------------------------------
+    def work_stats(self) -> dict:
+        comp = self.plan._compiled
+        return {
+            "S": comp.S, "npts": comp.npts, "kw": comp.kw, "G": comp.G, "C": getattr(comp, "C", None),
+            "replicas_valid": int(sum(comp.n_valid)), "units": int(comp.cell_unit[-1].item()),
+            "table_entries": int(comp.table.shape[0]) if getattr(comp, "table", None) is not None else None,
+        }
+
+
+def _free():
+    gc.collect()
+    if torch.cuda.is_available():
+        torch.cuda.empty_cache()
+
+
+_ = ctypes  # ctypes structures are built through _lib

@@ -1,0 +1,451 @@
+// Device-resident GROG plan compiler.
+//
+// This translation unit is compiled with -ffp-contract=off: every float
+// operation below is a separately rounded IEEE op (division and rint are the
+// correctly rounded gfx950 sequences), in the operation order of the
+// reference's float32 torch code, so target indices, shifts and quantised
+// table entries are bit-exact with it:
+//   _rescale_coords            src/pygrog/calib/_grog.py:1318-1322
+//   _create_plan_single        src/pygrog/calib/_grog.py:1081-1234
+//   _attach_fft_plan           src/pygrog/calib/_grog.py:918-998
+//   calc_interp_table (quant.) src/pygrog/calib/_grog.py:406-422
+//
+// Pipeline (driven from pygrog_amd/calib/_grog.py):
+//   absmax -> targets -> radix sort (key = element, cell) -> post (perms,
+//   CSR) -> weights (EGROG per cell, f64 sums in reference order) -> scan ->
+//   units; later quantize (table entries) once the table is known.
+#include <hipcub/hipcub.hpp>
+
+#include "common.h"
+
+namespace pg {
+
+// cmax[s, d] = max |coords[s, :, d]| via unsigned-bit atomicMax (values are
+// non-negative, so the IEEE bit order is the numeric order).  cmax zeroed by
+// the caller.
+__global__ void k_absmax(const float *__restrict__ coords, int64_t S,
+                         int64_t npts, int ndim, unsigned *__restrict__ cmax) {
+  const int64_t total = npts * ndim;
+  const int s = blockIdx.y;
+  const float *c = coords + s * total;
+  unsigned m[3] = {0u, 0u, 0u};
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int d = (int)(t % ndim);
+    const unsigned b = __float_as_uint(fabsf(c[t]));
+    if (b > m[d]) m[d] = b;
+  }
+  for (int d = 0; d < ndim; ++d) {
+    unsigned v = m[d];
+    for (int off = 32; off > 0; off >>= 1) {
+      unsigned o = __shfl_down(v, off, kWave);
+      v = o > v ? o : v;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(cmax + s * ndim + d, v);
+  }
+}
+
+struct GeomDev {
+  int ndim, kw;
+  int64_t gs[3], st[3];
+  float ha[3], org[3], stp[3];
+  int off[128][3];
+};
+
+__global__ void k_plan_targets(const float *__restrict__ coords,
+                               const float *__restrict__ cmax, int64_t S,
+                               int64_t npts, GeomDev g,
+                               int64_t *__restrict__ target_idx,
+                               float *__restrict__ distances,
+                               float *__restrict__ gauss,
+                               uint64_t *__restrict__ sort_key,
+                               uint32_t *__restrict__ sort_val,
+                               unsigned long long *__restrict__ n_invalid) {
+  const int64_t nper = npts * g.kw;
+  const int64_t total = S * nper;
+  const int64_t G = g.gs[0] * (g.ndim > 1 ? g.gs[1] : 1) *
+                    (g.ndim > 2 ? g.gs[2] : 1);
+  for (int64_t gi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+       gi - threadIdx.x < total; gi += (int64_t)gridDim.x * blockDim.x) {
+    const bool active = gi < total;
+    int64_t s = 0;
+    bool valid = false;
+    if (active) {
+      s = gi / nper;
+      const int64_t loc = gi - s * nper;
+      const int64_t n = loc / g.kw;
+      const int k = (int)(loc - n * g.kw);
+      const float *c = coords + (s * npts + n) * g.ndim;
+      bool inb = true;
+      int64_t flat = 0;
+      float dsq = 0.0f;
+      float dist[3];
+      for (int d = 0; d < g.ndim; ++d) {
+        // _rescale_coords: (0.5 * amp) * coords / cmax
+        const float cs = (g.ha[d] * c[d]) / cmax[s * g.ndim + d];
+        // frac_idx = (coords - origins) / grid_steps; nearest = round()
+        const float frac = (cs - g.org[d]) / g.stp[d];
+        const int nearest = (int)__builtin_rintf(frac);
+        const int t = nearest + g.off[k][d];
+        // target_coords_space = origins + target_nd.float() * grid_steps
+        const float tc = g.org[d] + (float)t * g.stp[d];
+        const float dd = tc - cs;
+        dist[d] = dd;
+        if (t < 0 || (int64_t)t >= g.gs[d]) inb = false;
+        if (!(fabsf(dd) <= 0.5f)) inb = false;
+        flat += (int64_t)t * g.st[d];
+        // dist_grid = distances / grid_steps; dist_sq = sum(dist_grid**2)
+        const float dg = dd / g.stp[d];
+        dsq = (d == 0) ? dg * dg : dsq + dg * dg;
+      }
+      valid = inb;
+      for (int d = 0; d < g.ndim; ++d) distances[gi * g.ndim + d] = dist[d];
+      target_idx[gi] = valid ? flat : (int64_t)-1;
+      gauss[gi] = valid ? expf(-0.5f * dsq) : 0.0f;
+      sort_key[gi] = (uint64_t)s * (uint64_t)(G + 1) +
+                     (valid ? (uint64_t)(flat + 1) : 0ull);
+      sort_val[gi] = (uint32_t)gi;
+    }
+    // Invalid count per element: one atomic per (wave, element) run.
+    const bool inv = active && !valid;
+    const int64_t s0 = __shfl(s, 0, kWave);
+    const unsigned long long same = __ballot(inv && s == s0);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (lane == 0 && same) atomicAdd(n_invalid + s0, (unsigned long long)__popcll(same));
+    if (inv && s != s0) atomicAdd(n_invalid + s, 1ull);
+  }
+}
+
+__global__ void k_plan_post(const uint64_t *__restrict__ keys,
+                            const uint32_t *__restrict__ vals,
+                            const float *__restrict__ gauss,
+                            const int64_t *__restrict__ n_invalid, int64_t S,
+                            int64_t nper, int64_t G,
+                            int64_t *__restrict__ indices,
+                            int64_t *__restrict__ sort_perm,
+                            int64_t *__restrict__ inv_perm,
+                            float *__restrict__ gauss_sorted,
+                            uint32_t *__restrict__ cell_ptr) {
+  const int64_t total = S * nper;
+  for (int64_t gi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; gi < total;
+       gi += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t key = keys[gi];
+    const int64_t s = gi / nper;
+    const int64_t iloc = gi - s * nper;
+    const int64_t kloc = (int64_t)(key - (uint64_t)s * (uint64_t)(G + 1));
+    const uint32_t val = vals[gi];
+    const int64_t vloc = (int64_t)val - s * nper;
+    indices[gi] = kloc > 0 ? kloc - 1 : 0;
+    sort_perm[gi] = vloc;
+    inv_perm[s * nper + vloc] = iloc;
+    gauss_sorted[gi] = gauss[val];
+    uint32_t *cp = cell_ptr + s * (G + 1);
+    const int64_t ninv = n_invalid[s];
+    if (iloc >= ninv) {
+      const int64_t r = iloc - ninv;
+      const int64_t cell = kloc - 1;
+      int64_t prev = -1;
+      if (r > 0) {
+        const int64_t pk = (int64_t)(keys[gi - 1] - (uint64_t)s * (uint64_t)(G + 1));
+        prev = pk - 1;
+      }
+      for (int64_t c = prev + 1; c <= cell; ++c) cp[c] = (uint32_t)r;
+      if (iloc == nper - 1)
+        for (int64_t c = cell + 1; c <= G; ++c) cp[c] = (uint32_t)(r + 1);
+    } else if (iloc == nper - 1) {
+      // No valid replica in this element.
+      for (int64_t c = 0; c <= G; ++c) cp[c] = 0u;
+    }
+  }
+}
+
+__global__ void k_plan_weights(const uint32_t *__restrict__ vals,
+                               const float *__restrict__ gauss_sorted,
+                               const int64_t *__restrict__ n_invalid,
+                               const uint32_t *__restrict__ cell_ptr,
+                               int64_t S, int64_t nper, int64_t G, int kw,
+                               int chunk, float *__restrict__ weights,
+                               float *__restrict__ sqrt_weights,
+                               uint32_t *__restrict__ rep_src,
+                               float *__restrict__ rep_s,
+                               float *__restrict__ density,
+                               uint32_t *__restrict__ unit_count,
+                               int64_t rep_stride) {
+  const int64_t total = S * G;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = t / G;
+    const int64_t c = t - s * G;
+    const uint32_t *cp = cell_ptr + s * (G + 1);
+    const int64_t r0 = cp[c], r1 = cp[c + 1];
+    const int64_t base = s * nper + n_invalid[s];
+    // gauss_sum.scatter_add_ in float64, valid replicas in natural order
+    // (the stable sort keeps that order inside a cell).
+    double gsum = 0.0;
+    for (int64_t r = r0; r < r1; ++r) gsum += (double)gauss_sorted[base + r];
+    // gauss_sum[...].clamp(min=1e-12).float()
+    const float denom = (float)(gsum > 1e-12 ? gsum : 1e-12);
+    float dens = 0.0f;
+    for (int64_t r = r0; r < r1; ++r) {
+      const float w = gauss_sorted[base + r] / denom;
+      const float sw = sqrtf(w > 0.0f ? w : 0.0f);
+      const uint32_t v = vals[base + r];
+      weights[v] = w;
+      sqrt_weights[base + r] = sw;
+      rep_s[s * rep_stride + r] = sw;
+      rep_src[s * rep_stride + r] = (uint32_t)(((int64_t)v - s * nper) / kw);
+      dens = dens + sw * sw;
+    }
+    density[t] = dens;
+    unit_count[t] = (uint32_t)((r1 - r0 + chunk - 1) / chunk);
+  }
+}
+
+__global__ void k_plan_units(const uint32_t *__restrict__ cell_ptr,
+                             const uint32_t *__restrict__ cell_unit, int64_t S,
+                             int64_t G, int chunk,
+                             uint32_t *__restrict__ unit_cell,
+                             uint32_t *__restrict__ unit_rep) {
+  const int64_t total = S * G;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = t / G;
+    const int64_t c = t - s * G;
+    const uint32_t *cp = cell_ptr + s * (G + 1);
+    const uint32_t u0 = cell_unit[t];
+    const uint32_t nu = cell_unit[t + 1] - u0;
+    for (uint32_t k = 0; k < nu; ++k) {
+      unit_cell[u0 + k] = (uint32_t)c;
+      unit_rep[u0 + k + s] = cp[c] + k * (uint32_t)chunk;
+    }
+    if (c == G - 1) unit_rep[cell_unit[t + 1] + s] = cp[G];
+  }
+}
+
+__global__ void k_plan_quantize(const float *__restrict__ distances,
+                                const int64_t *__restrict__ inv_perm,
+                                const int64_t *__restrict__ n_invalid,
+                                const int64_t *__restrict__ target_idx,
+                                int64_t S, int64_t nper, int ndim, float pfac,
+                                float stepsize, float radius, int64_t nsteps,
+                                int64_t n_entries,
+                                int64_t *__restrict__ interp_idx,
+                                uint32_t *__restrict__ rep_ent,
+                                int64_t rep_stride) {
+  const int64_t total = S * nper;
+  for (int64_t gi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; gi < total;
+       gi += (int64_t)gridDim.x * blockDim.x) {
+    // (radius + round(d * pfac) / pfac) / stepsize -> round -> * nsteps^d ->
+    // round -> int32 -> sum
+    int64_t e = 0;
+    float stride_f = 1.0f;
+    for (int d = 0; d < ndim; ++d) {
+      const float dd = distances[gi * ndim + d];
+      const float a = __builtin_rintf(dd * pfac) / pfac;
+      const float q = __builtin_rintf((radius + a) / stepsize);
+      e += (int64_t)(int32_t)__builtin_rintf(q * stride_f);
+      stride_f = (float)(stride_f * (float)nsteps);
+    }
+    if (e < 0) e = 0;
+    if (e > n_entries - 1) e = n_entries - 1;
+    interp_idx[gi] = e;
+    if (target_idx[gi] >= 0) {
+      const int64_t s = gi / nper;
+      const int64_t r = inv_perm[gi] - n_invalid[s];
+      rep_ent[s * rep_stride + r] = (uint32_t)e;
+    }
+  }
+}
+
+// cell_ptr from sorted indices (all inside [0, G)).
+template <class I>
+__global__ void k_csr_from_sorted(const I *__restrict__ idx, int64_t n,
+                                  int64_t G, uint32_t *__restrict__ cp) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t cur = (i < n) ? (int64_t)idx[i] : G;
+    const int64_t prev = (i > 0) ? (int64_t)idx[i - 1] : -1;
+    for (int64_t c = prev + 1; c <= cur; ++c) cp[c] = (uint32_t)i;
+  }
+}
+
+}  // namespace pg
+
+using namespace pg;
+
+extern "C" {
+
+int pg_plan_absmax(const float *coords, int64_t S, int64_t npts, int32_t ndim,
+                   float *cmax, pg_stream_t stream) {
+  PG_CHECK(ndim >= 1 && ndim <= 3, "plan: ndim must be 1..3");
+  PG_CHECK(S >= 1 && npts >= 0 && S <= 65535, "plan: bad stack/points");
+  if (npts == 0) return 0;
+  const int threads = 256;
+  dim3 grid(grid_for(npts * ndim, threads, 1024), (unsigned)S);
+  hipLaunchKernelGGL(k_absmax, grid, dim3(threads), 0, as_stream(stream),
+                     coords, S, npts, ndim, reinterpret_cast<unsigned *>(cmax));
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_plan_targets(const float *coords, const float *cmax, int64_t S,
+                    int64_t npts, pg_plan_geom geom, int64_t *target_idx,
+                    float *distances, float *gauss, uint64_t *sort_key,
+                    uint32_t *sort_val, int64_t *n_invalid,
+                    pg_stream_t stream) {
+  PG_CHECK(geom.ndim >= 1 && geom.ndim <= 3, "plan: ndim must be 1..3");
+  PG_CHECK(geom.kw >= 1 && geom.kw <= 128, "plan: kernel size must be 1..128");
+  const int64_t total = S * npts * geom.kw;
+  PG_CHECK(total < (int64_t)UINT32_MAX, "plan: too many candidate replicas");
+  if (total == 0) return 0;
+  GeomDev g;
+  g.ndim = geom.ndim;
+  g.kw = geom.kw;
+  for (int d = 0; d < 3; ++d) {
+    g.gs[d] = geom.grid_shape[d];
+    g.st[d] = geom.strides[d];
+    g.ha[d] = geom.half_amp[d];
+    g.org[d] = geom.origins[d];
+    g.stp[d] = geom.steps[d];
+  }
+  for (int k = 0; k < 128; ++k)
+    for (int d = 0; d < 3; ++d) g.off[k][d] = geom.offsets[k][d];
+  const int threads = 256;
+  hipLaunchKernelGGL(k_plan_targets, dim3(grid_for(total, threads)),
+                     dim3(threads), 0, as_stream(stream), coords, cmax, S,
+                     npts, g, target_idx, distances, gauss, sort_key, sort_val,
+                     reinterpret_cast<unsigned long long *>(n_invalid));
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_sort_pairs_u64(void *workspace, size_t *ws_bytes,
+                      const uint64_t *keys_in, uint64_t *keys_out,
+                      const uint32_t *vals_in, uint32_t *vals_out, int64_t n,
+                      int32_t end_bit, pg_stream_t stream) {
+  PG_CHECK(ws_bytes != nullptr, "sort: ws_bytes is null");
+  PG_CHECK(n >= 0 && n < (int64_t)INT32_MAX, "sort: bad n");
+  PG_CHECK(end_bit >= 1 && end_bit <= 64, "sort: bad end_bit");
+  PG_HIP(hipcub::DeviceRadixSort::SortPairs(
+      workspace, *ws_bytes, keys_in, keys_out, vals_in, vals_out, (int)n, 0,
+      end_bit, as_stream(stream)));
+  return 0;
+}
+
+int pg_sort_pairs_u32(void *workspace, size_t *ws_bytes,
+                      const uint32_t *keys_in, uint32_t *keys_out,
+                      const uint32_t *vals_in, uint32_t *vals_out, int64_t n,
+                      int32_t end_bit, pg_stream_t stream) {
+  PG_CHECK(ws_bytes != nullptr, "sort: ws_bytes is null");
+  PG_CHECK(n >= 0 && n < (int64_t)INT32_MAX, "sort: bad n");
+  PG_CHECK(end_bit >= 1 && end_bit <= 32, "sort: bad end_bit");
+  PG_HIP(hipcub::DeviceRadixSort::SortPairs(
+      workspace, *ws_bytes, keys_in, keys_out, vals_in, vals_out, (int)n, 0,
+      end_bit, as_stream(stream)));
+  return 0;
+}
+
+int pg_exclusive_scan_u32(void *workspace, size_t *ws_bytes,
+                          const uint32_t *in, uint32_t *out, int64_t n,
+                          pg_stream_t stream) {
+  PG_CHECK(ws_bytes != nullptr, "scan: ws_bytes is null");
+  PG_CHECK(n >= 0 && n < (int64_t)INT32_MAX, "scan: bad n");
+  // Scan n + 1 items where the last input is read as 0: the caller passes an
+  // input array of n + 1 entries with in[n] == 0.
+  PG_HIP(hipcub::DeviceScan::ExclusiveSum(workspace, *ws_bytes, in, out,
+                                          (int)(n + 1), as_stream(stream)));
+  return 0;
+}
+
+int pg_plan_post(const uint64_t *keys_sorted, const uint32_t *vals_sorted,
+                 const float *gauss, const int64_t *n_invalid, int64_t S,
+                 int64_t nper, int64_t G, int64_t *indices, int64_t *sort_perm,
+                 int64_t *inv_perm, float *gauss_sorted, uint32_t *cell_ptr,
+                 pg_stream_t stream) {
+  const int64_t total = S * nper;
+  if (total == 0) return 0;
+  const int threads = 256;
+  hipLaunchKernelGGL(k_plan_post, dim3(grid_for(total, threads)),
+                     dim3(threads), 0, as_stream(stream), keys_sorted,
+                     vals_sorted, gauss, n_invalid, S, nper, G, indices,
+                     sort_perm, inv_perm, gauss_sorted, cell_ptr);
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_plan_weights(const uint32_t *vals_sorted, const float *gauss_sorted,
+                    const int64_t *n_invalid, const uint32_t *cell_ptr,
+                    int64_t S, int64_t nper, int64_t G, int32_t kw,
+                    int32_t chunk, float *weights, float *sqrt_weights,
+                    uint32_t *rep_src, float *rep_s, float *density,
+                    uint32_t *unit_count, int64_t rep_stride,
+                    pg_stream_t stream) {
+  PG_CHECK(chunk >= 1, "plan: chunk must be positive");
+  const int64_t total = S * G;
+  if (total == 0) return 0;
+  const int threads = 256;
+  hipLaunchKernelGGL(k_plan_weights, dim3(grid_for(total, threads)),
+                     dim3(threads), 0, as_stream(stream), vals_sorted,
+                     gauss_sorted, n_invalid, cell_ptr, S, nper, G, kw, chunk,
+                     weights, sqrt_weights, rep_src, rep_s, density,
+                     unit_count, rep_stride);
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_plan_units(const uint32_t *cell_ptr, const uint32_t *cell_unit,
+                  int64_t S, int64_t G, int32_t chunk, uint32_t *unit_cell,
+                  uint32_t *unit_rep, pg_stream_t stream) {
+  const int64_t total = S * G;
+  if (total == 0) return 0;
+  const int threads = 256;
+  hipLaunchKernelGGL(k_plan_units, dim3(grid_for(total, threads)),
+                     dim3(threads), 0, as_stream(stream), cell_ptr, cell_unit,
+                     S, G, chunk, unit_cell, unit_rep);
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_plan_quantize(const float *distances, const int64_t *inv_perm,
+                     const int64_t *n_invalid, const int64_t *target_idx,
+                     int64_t S, int64_t nper, int32_t ndim, float pfac,
+                     float stepsize, float radius, int64_t nsteps,
+                     int64_t n_entries, int64_t *interp_idx, uint32_t *rep_ent,
+                     int64_t rep_stride, pg_stream_t stream) {
+  PG_CHECK(ndim >= 1 && ndim <= 3, "quantize: ndim must be 1..3");
+  PG_CHECK(n_entries >= 1, "quantize: empty table");
+  const int64_t total = S * nper;
+  if (total == 0) return 0;
+  const int threads = 256;
+  hipLaunchKernelGGL(k_plan_quantize, dim3(grid_for(total, threads)),
+                     dim3(threads), 0, as_stream(stream), distances, inv_perm,
+                     n_invalid, target_idx, S, nper, ndim, pfac, stepsize,
+                     radius, nsteps, n_entries, interp_idx, rep_ent,
+                     rep_stride);
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_csr_from_sorted(const int64_t *indices, int64_t n, int64_t G,
+                       uint32_t *cell_ptr, pg_stream_t stream) {
+  PG_CHECK(n >= 0 && G >= 0 && n < (int64_t)UINT32_MAX, "csr: bad sizes");
+  const int threads = 256;
+  hipLaunchKernelGGL(k_csr_from_sorted<int64_t>,
+                     dim3(grid_for(n + 1, threads)), dim3(threads), 0,
+                     as_stream(stream), indices, n, G, cell_ptr);
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_csr_from_sorted_u32(const uint32_t *keys, int64_t n, int64_t G,
+                           uint32_t *cell_ptr, pg_stream_t stream) {
+  PG_CHECK(n >= 0 && G >= 0 && n < (int64_t)UINT32_MAX, "csr: bad sizes");
+  const int threads = 256;
+  hipLaunchKernelGGL(k_csr_from_sorted<uint32_t>,
+                     dim3(grid_for(n + 1, threads)), dim3(threads), 0,
+                     as_stream(stream), keys, n, G, cell_ptr);
+  PG_LAUNCHED();
+  return 0;
+}
+
+}  // extern "C"

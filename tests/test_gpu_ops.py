@@ -1,0 +1,193 @@
+"""The seven reference native ops on the GPU vs the C oracle and the
+reference's own known-answer tests (tests/test_fast_binning.py,
+tests/test_sparse_fft.py:45-106, tests/test_toeplitz_kernels.py)."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ops as orc
+from pygrog_amd.operator import gather, scatter_add
+from pygrog_amd.operator._sparse_fft import _get_torch_ext, _scatter_add
+
+pytestmark = pytest.mark.gpu
+
+
+def _sorted(n, G, device, seed=0):
+    g = torch.Generator().manual_seed(seed)
+    idx = torch.randint(0, G, (n,), generator=g).sort().values
+    w = torch.rand(n, generator=g)
+    d = torch.randn(n, dtype=torch.complex64, generator=g)
+    return idx.to(device), w.to(device), d.to(device)
+
+
+def test_extension_loads():
+    ext = _get_torch_ext()
+    for name in ("grog_interpolate", "scatter_add", "scatter_add_binned", "gather", "psf_scatter_scalar",
+                 "psf_scatter_outer", "psf_scatter_outer_basis"):
+        assert hasattr(ext, name)
+
+
+@pytest.mark.parametrize("n,G", [(512, 64), (20000, 300), (100000, 7)])
+def test_scatter_add_vs_oracle(device, n, G):
+    idx, w, d = _sorted(n, G, device, seed=n)
+    grid = torch.zeros(G, dtype=torch.complex64, device=device)
+    scatter_add(grid, d, idx, w)
+    ref = np.zeros(G, np.complex64)
+    orc.scatter_add(ref, d.cpu().numpy(), idx.cpu().numpy(), w.cpu().numpy())
+    np.testing.assert_allclose(grid.cpu().numpy(), ref, rtol=1e-5, atol=1e-4)
+
+
+def test_scatter_add_unsorted_and_out_of_range(device):
+    g = torch.Generator().manual_seed(3)
+    idx = torch.randint(-5, 70, (3000,), generator=g)
+    w = torch.rand(3000, generator=g)
+    d = torch.randn(3000, dtype=torch.complex64, generator=g)
+    grid = torch.zeros(64, dtype=torch.complex64, device=device)
+    scatter_add(grid, d.to(device), idx.to(device), w.to(device))
+    ref = np.zeros(64, np.complex64)
+    orc.scatter_add(ref, d.numpy(), idx.numpy(), w.numpy())
+    np.testing.assert_allclose(grid.cpu().numpy(), ref, rtol=1e-5, atol=1e-5)
+
+
+def test_scatter_add_accumulates_into_existing(device):
+    idx, w, d = _sorted(128, 32, device, seed=1)
+    grid = torch.ones(32, dtype=torch.complex64, device=device)
+    ref = grid.clone()
+    scatter_add(grid, d, idx, w)
+    ref.index_add_(0, idx, d * w.to(torch.complex64))
+    torch.testing.assert_close(grid, ref, rtol=1e-4, atol=1e-4)
+
+
+def test_scatter_add_known_answers(device):
+    # reference tests/test_fast_binning.py:101-131, test_sparse_fft.py:45-75
+    grid = torch.zeros(10, dtype=torch.complex64, device=device)
+    scatter_add(grid, torch.tensor([1 + 2j], dtype=torch.complex64, device=device),
+                torch.tensor([3], device=device), torch.tensor([0.5], device=device))
+    assert abs(grid[3].item() - (0.5 + 1j)) < 1e-5
+    assert grid[:3].abs().max() == 0 and grid[4:].abs().max() == 0
+    grid = torch.zeros(16, dtype=torch.complex64, device=device)
+    scatter_add(grid, torch.ones(256, dtype=torch.complex64, device=device), torch.full((256,), 7, device=device),
+                torch.ones(256, device=device))
+    assert abs(grid[7].real.item() - 256) < 1e-3 and grid[7].imag.abs().item() < 1e-5
+    grid = torch.zeros(8, dtype=torch.complex64, device=device)
+    scatter_add(grid, torch.ones(3, dtype=torch.complex64, device=device), torch.tensor([0, 0, 4], device=device),
+                torch.ones(3, device=device))
+    assert grid[0].real.item() == pytest.approx(2.0) and grid[4].real.item() == pytest.approx(1.0)
+    grid = torch.zeros(16, dtype=torch.complex64, device=device)
+    scatter_add(grid, torch.empty(0, dtype=torch.complex64, device=device),
+                torch.empty(0, dtype=torch.int64, device=device), torch.empty(0, device=device))
+    assert grid.abs().max() == 0
+
+
+def test_scatter_binned_matches_simple(device):
+    idx, w, d = _sorted(4096, 1024, device, seed=5)
+    g1 = torch.zeros(1024, dtype=torch.complex64, device=device)
+    _scatter_add(g1, d, idx, w)
+    edges = torch.arange(5, device=device) * 256
+    bin_starts = torch.searchsorted(idx, edges)
+    g2 = torch.zeros(1024, dtype=torch.complex64, device=device)
+    _scatter_add(g2, d, idx, w, bin_starts=bin_starts, bin_size=256)
+    torch.testing.assert_close(g1, g2, rtol=1e-4, atol=1e-4)
+
+
+def test_scatter_add_c128(device):
+    idx, w, d = _sorted(5000, 100, device, seed=9)
+    grid = torch.zeros(100, dtype=torch.complex128, device=device)
+    scatter_add(grid, d.to(torch.complex128), idx, w.double())
+    ref = np.zeros(100, np.complex128)
+    orc.scatter_add(ref, d.cpu().numpy().astype(np.complex128), idx.cpu().numpy(), w.cpu().numpy().astype(np.float64))
+    np.testing.assert_allclose(grid.cpu().numpy(), ref, rtol=1e-12, atol=1e-12)
+
+
+def test_gather_vs_oracle_and_edges(device):
+    idx, w, _ = _sorted(2000, 64, device, seed=2)
+    grid = torch.randn(64, dtype=torch.complex64, device=device)
+    out = gather(grid, idx, w)
+    ref = orc.gather(grid.cpu().numpy(), idx.cpu().numpy(), w.cpu().numpy())
+    assert np.array_equal(out.cpu().numpy(), ref)
+    assert gather(grid, torch.empty(0, dtype=torch.int64, device=device), torch.empty(0, device=device)).numel() == 0
+    z = gather(torch.ones(8, dtype=torch.complex64, device=device) * 99, torch.arange(8, device=device),
+               torch.zeros(8, device=device))
+    assert z.abs().max() < 1e-6
+    # out of range reads as 0 (reference sparse_ops_cpu_impl.inl:158-163)
+    o = gather(grid, torch.tensor([-1, 64, 3], device=device), torch.ones(3, device=device))
+    assert o[0] == 0 and o[1] == 0 and o[2] == grid[3]
+
+
+def test_scatter_gather_round_trip(device):
+    data = torch.randn(32, dtype=torch.complex64, device=device)
+    idx = torch.arange(32, device=device)
+    w = torch.ones(32, device=device)
+    grid = torch.zeros(32, dtype=torch.complex64, device=device)
+    scatter_add(grid, data, idx, w)
+    torch.testing.assert_close(gather(grid, idx, w), data, rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("N,B,C,K", [(1000, 1, 8, 50), (333, 3, 5, 7), (64, 2, 32, 9)])
+def test_grog_interpolate_vs_oracle(device, N, B, C, K):
+    rng = np.random.default_rng(N + C)
+    d = (rng.standard_normal((N, B, C)) + 1j * rng.standard_normal((N, B, C))).astype(np.complex64)
+    k = (rng.standard_normal((K, C, C)) + 1j * rng.standard_normal((K, C, C))).astype(np.complex64)
+    idx = rng.integers(0, K, N)
+    out = _get_torch_ext().grog_interpolate(torch.from_numpy(d).to(device), torch.from_numpy(idx).to(device),
+                                            torch.from_numpy(k).to(device))
+    ref = orc.interpolate(d, idx, k)
+    np.testing.assert_allclose(out.cpu().numpy(), ref, rtol=1e-5, atol=1e-5 * np.abs(ref).max())
+
+
+def test_grog_interpolate_checks(device):
+    ext = _get_torch_ext()
+    with pytest.raises(RuntimeError, match="3-D"):
+        ext.grog_interpolate(torch.zeros(3, 2, dtype=torch.complex64, device=device),
+                             torch.zeros(3, dtype=torch.int64, device=device),
+                             torch.zeros(1, 2, 2, dtype=torch.complex64, device=device))
+    with pytest.raises(RuntimeError, match="int64"):
+        ext.grog_interpolate(torch.zeros(3, 1, 2, dtype=torch.complex64, device=device),
+                             torch.zeros(3, dtype=torch.int32, device=device),
+                             torch.zeros(1, 2, 2, dtype=torch.complex64, device=device))
+
+
+def test_psf_scatter_scalar(device):
+    idx, w, _ = _sorted(200, 64, device, seed=0)
+    psf = torch.zeros(64, device=device)
+    _get_torch_ext().psf_scatter_scalar(psf, idx, w + 0.1)
+    ref = np.zeros(64, np.float32)
+    orc.psf_scatter_scalar(ref, idx.cpu().numpy(), (w + 0.1).cpu().numpy())
+    np.testing.assert_allclose(psf.cpu().numpy(), ref, rtol=1e-6, atol=1e-6)
+    psf64 = torch.zeros(64, dtype=torch.float64, device=device)
+    _get_torch_ext().psf_scatter_scalar(psf64, idx, (w + 0.1).double())
+    ref64 = torch.zeros(64, dtype=torch.float64).index_add_(0, idx.cpu(), (w + 0.1).double().cpu())
+    np.testing.assert_allclose(psf64.cpu().numpy(), ref64.numpy(), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("M", [1, 3, 5])
+def test_psf_scatter_outer(device, M):
+    idx, w, _ = _sorted(150, 32, device, seed=M)
+    b = torch.randn(150, M, dtype=torch.complex64, device=device)
+    psf = torch.zeros(32, M, M, dtype=torch.complex64, device=device)
+    _get_torch_ext().psf_scatter_outer(psf, idx, w + 0.1, b)
+    ref = np.zeros((32, M, M), np.complex64)
+    orc.psf_scatter_outer(ref, idx.cpu().numpy(), (w + 0.1).cpu().numpy(), b.cpu().numpy())
+    np.testing.assert_allclose(psf.cpu().numpy(), ref, rtol=1e-5, atol=1e-5)
+    torch.testing.assert_close(psf, psf.conj().transpose(-2, -1), rtol=1e-5, atol=1e-5)
+
+
+@pytest.mark.parametrize("K,T", [(1, 8), (3, 16), (5, 24)])
+def test_psf_scatter_outer_basis(device, K, T):
+    idx, w, _ = _sorted(200, 32, device, seed=K * 100 + T)
+    basis = torch.randn(K, T, dtype=torch.complex64, device=device)
+    ti = torch.randint(0, T, (200,), device=device)
+    psf = torch.zeros(32, K, K, dtype=torch.complex64, device=device)
+    _get_torch_ext().psf_scatter_outer_basis(psf, idx, w + 0.1, basis, ti)
+    ref = np.zeros((32, K, K), np.complex64)
+    orc.psf_scatter_outer_basis(ref, idx.cpu().numpy(), (w + 0.1).cpu().numpy(), basis.cpu().numpy(), ti.cpu().numpy())
+    np.testing.assert_allclose(psf.cpu().numpy(), ref, rtol=1e-5, atol=1e-5)
+
+
+def test_cpu_tensor_rejected():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    with pytest.raises(RuntimeError, match="GPU only"):
+        scatter_add(torch.zeros(4, dtype=torch.complex64), torch.ones(1, dtype=torch.complex64),
+                    torch.tensor([0]), torch.ones(1))
