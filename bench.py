@@ -1,0 +1,326 @@
+"""GROG gridding benchmark on one MI355X.
+
+Workload (default ``kooshball3d``): 3D golden-means kooshball on a 192^3 grid,
+32 coils, 20,000 spokes x 384 samples (7.68 M samples), complex64 k-space
+i.i.d. complex Gaussian (sigma 1 per component), operator table fit from a
+20^3 calibration region of a seeded ellipsoid phantom times 32 Gaussian coil
+maps, shifts quantised to 1/40 grid step (41^3 = 68,921 table entries),
+2-wide sphere kernel (4 candidate neighbours), lamda = 1e-3, oversampling 1.
+
+A step grids the device-resident k-space (1, 32, 7.68 M) into a dense
+(32, 192^3) complex64 grid with the compiled plan and table already resident
+(``GrogInterpolator.grid_device``).  Modes:
+  fp32    plain fused kernel (per-replica operator apply, FP32);
+  bf16    grouped complex GEMM on BF16 MFMA, FP32 accumulate (1 product);
+  bf16x3  grouped, split-BF16 (3 products, ~FP32 accuracy).
+``value`` is the fastest BF16-MFMA mode whose rel-L2 vs the FP64 reference is
+<= 1e-3, in coil-samples/s = samples * coils / step time.
+
+The JSON line also carries: every mode's throughput and rel-L2, the roofline
+of the headline mode's dominant kernel (algorithmic bytes / HIP-event kernel
+time vs 8 TB/s), and the CPU baseline: the reference's own CPU gridding flow
+(``interpolate(grid=True)``: csrc/torch ops compiled in ``oracle/_ref``) on a
+bounded spoke sample, timed on this host's cores.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ...]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "coil-samples/s gridded, BF16-MFMA vs FP32 @relL2≤1e-3; p99 serve latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------------------
+# workloads
+# ---------------------------------------------------------------------------
+def make_workload(name: str, dev):
+    from pygrog_amd import synth
+    from pygrog_amd.calib import GrogInterpolator
+
+    if name == "kooshball3d":
+        shape, C, n_sp, n_rd = (192, 192, 192), 32, 20000, 384
+        coords = synth.kooshball_3d(n_sp, n_rd)
+        g = GrogInterpolator(shape, coords, oversamp=(1, 1, 1), kernel_width=2, kernel_shape="sphere", device=dev)
+        calib = synth.calibration(shape, C, (20, 20, 20), seed=7, device=dev)
+        g.calc_interp_table(calib, lamda=1e-3, quant=40, keep_fp64=True)
+        desc = ("3D golden-means kooshball 192^3, 32 coils, 20000 spokes x 384 samples (7.68M), "
+                "q=1/40 (68921-entry table), 2-wide sphere kernel, lamda=1e-3, oversamp 1")
+        y = synth.complex_gaussian((1, 1, C, n_sp * n_rd), seed=11, device=dev)
+        spokes = n_sp
+    elif name == "radial2d":
+        shape, C, n_sp, n_rd = (256, 256), 8, 128, 512
+        coords = synth.radial_2d(n_sp, n_rd)
+        g = GrogInterpolator(shape, coords, kernel_width=3, device=dev)
+        calib = synth.calibration(shape, C, (24, 24), seed=3, device=dev)
+        g.calc_interp_table(calib, lamda=1e-3, quant=32, keep_fp64=True)
+        desc = "2D golden-angle radial 256^2, 8 coils, 128x512, q=1/32, 3x3 kernel, lamda=1e-3"
+        y = synth.complex_gaussian((1, 1, C, n_sp * n_rd), seed=11, device=dev)
+        spokes = n_sp
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    return {"name": name, "g": g, "y": y, "C": C, "N": y.shape[-1], "shape": shape, "desc": desc,
+            "spokes": spokes, "n_read": n_rd, "coords": coords}
+
+
+# ---------------------------------------------------------------------------
+# FP64 reference on the device (rel-L2 of every mode)
+# ---------------------------------------------------------------------------
+def grid_fp64_device(g, y: torch.Tensor, chunk: int = 1 << 16) -> torch.Tensor:
+    """sum_r w_r K64[e_r] y[:, src_r] in float64 (y: (C, N))."""
+    comp = g.plan._compiled
+    T = comp.table64
+    C, N = y.shape
+    t = comp.target_flat.reshape(-1)
+    e = comp.interp_idx.reshape(-1)
+    w = g.plan._dev["weights"].reshape(-1)
+    valid = torch.nonzero(t >= 0).squeeze(1)
+    out = torch.zeros(C, comp.G, 2, dtype=torch.float64, device=y.device)
+    y64 = y.to(torch.complex128)
+    for i in range(0, valid.numel(), chunk):
+        v = valid[i:i + chunk]
+        Kg = T[e[v]].contiguous()
+        Y = y64[:, v // comp.kw].T.contiguous().unsqueeze(-1)
+        V = torch.bmm(Kg, Y).squeeze(-1) * w[v].to(torch.float64)[:, None]
+        out.index_add_(1, t[v], torch.view_as_real(V.T.contiguous()))
+    return torch.view_as_complex(out)
+
+
+def rel_l2(a: torch.Tensor, b: torch.Tensor) -> float:
+    a = a.to(torch.complex128)
+    return float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b))
+
+
+# ---------------------------------------------------------------------------
+# algorithmic bytes per kernel launch (the roofline numerators)
+# ---------------------------------------------------------------------------
+def alg_bytes(kernel: str, st: dict) -> float:
+    C, N, G, R, U, K = st["C"], st["N"], st["G"], st["R"], st["U"], st["K"]
+    if kernel == "pg_grid_fp32":
+        # y read + grid write + replica records (src, entry, sqrt w) + units + cell map + table once
+        return 8 * C * N + 8 * C * G + 12 * R + 8 * U + 4 * G + 8 * C * C * K
+    if kernel == "pg_apply_grouped_bf16":
+        parts = st["n_parts"]
+        # y_t rows of every replica + table (bf16, real 2C x 2C per entry) + V write + records
+        return 8 * C * R + 2 * parts * 4 * C * C * K + 8 * C * R + 12 * R + 4 * K
+    if kernel == "pg_grid_reduce_v":
+        return 8 * C * R + 8 * C * G + 8 * U + 4 * G
+    if kernel == "pg_transpose_cn_c64":
+        return 16 * C * N
+    raise KeyError(kernel)
+
+
+# ---------------------------------------------------------------------------
+# reference CPU baseline (bounded sample)
+# ---------------------------------------------------------------------------
+def cpu_baseline(wl, n_spokes: int, threads: int) -> dict:
+    """Time the reference's CPU flow of interpolate(grid=True)
+    (src/pygrog/calib/_grog.py:668-695, 736-821 on csrc/torch CPU ops)
+    on the first ``n_spokes`` spokes."""
+    import oracle
+
+    ref = oracle.ref_ext()
+    kind = "reference"
+    if ref is None:
+        kind = "port"
+    g = wl["g"]
+    comp = g.plan._compiled
+    kw, C = comp.kw, wl["C"]
+    n = n_spokes * wl["n_read"]
+    nper = n * kw
+    tgt = comp.target_flat.reshape(-1)[:nper].cpu()
+    w = g.plan._dev["weights"].reshape(-1)[:nper].cpu()
+    e = comp.interp_idx.reshape(-1)[:nper].cpu()
+    kernel = comp.table.cpu()
+    data = wl["y"][0, 0, :, :n].cpu().contiguous()
+    # the sample's FFT plan, as _attach_fft_plan builds it (untimed setup)
+    idx = tgt.clone()
+    sqrt_w = torch.sqrt(w.clamp(min=0.0))
+    bad = idx < 0
+    idx[bad] = 0
+    sqrt_w[bad] = 0.0
+    sort_perm = torch.argsort(idx)
+    sidx = idx[sort_perm].contiguous()
+    sqw = sqrt_w[sort_perm].contiguous()
+    inv_perm = torch.empty_like(sort_perm)
+    inv_perm[sort_perm] = torch.arange(nper)
+    pre_w = sqw[inv_perm]
+    G = comp.G
+    torch.set_num_threads(threads)
+
+    def run():
+        # _sparse_full_single: replicate, transpose, apply, permute back
+        data_rep = data.reshape(C, n).unsqueeze(-1).expand(-1, -1, kw).reshape(C, n * kw)
+        data_interp = data_rep.T.reshape(n * kw, 1, C).contiguous()
+        if ref is not None:
+            res = ref.grog_interpolate(data_interp, e, kernel)
+        else:
+            from oracle import ops
+
+            res = torch.from_numpy(ops.interpolate(data_interp.numpy(), e.numpy(), kernel.numpy()))
+        data_interp.copy_(res)
+        sparse = data_interp.permute(1, 2, 0).contiguous().reshape(C, nper)
+        # _grid_kspace: pre-weights, density, per-coil sort + scatter
+        sparse_dc = sparse * pre_w.to(sparse.dtype).unsqueeze(0)
+        grid = torch.zeros(C, G, dtype=torch.complex64)
+        dens = torch.zeros(G, dtype=torch.float32)
+        dens.index_add_(0, sidx, (sqw * sqw).contiguous())
+        for c in range(C):
+            sc = sparse_dc[c][sort_perm]
+            if ref is not None:
+                ref.scatter_add(grid[c], sc, sidx, sqw)
+            else:
+                from oracle import ops
+
+                gc_ = grid[c].numpy()
+                ops.scatter_add(gc_, sc.numpy(), sidx.numpy(), sqw.numpy())
+        return grid
+
+    t0 = time.perf_counter()
+    run()
+    t1 = time.perf_counter()
+    return {"seconds": t1 - t0, "coil_samples": n * C, "kind": kind, "spokes": n_spokes}
+
+
+# ---------------------------------------------------------------------------
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="kooshball3d", choices=["kooshball3d", "radial2d"])
+    ap.add_argument("--modes", default="fp32,bf16,bf16x3")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work (s)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 rel-L2 check")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus N>1 is not supported (see DESIGN.md, "
+                                   "'How the path would shard')", "n_gpus": args.gpus}))
+        return 2
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        return 2
+    from pygrog_amd.calib._grog import KernelTimer
+
+    dev = torch.device("cuda:0")
+    t_setup = time.perf_counter()
+    wl = make_workload(args.workload, dev)
+    g = wl["g"]
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t_setup
+    log(f"setup {setup_s:.1f}s: {g.work_stats()}")
+    st = g.work_stats()
+    comp = g.plan._compiled
+    stats = {"C": wl["C"], "N": wl["N"], "G": comp.G, "R": st["replicas_valid"], "U": st["units"],
+             "K": st["table_entries"]}
+    y = wl["y"]
+    out = torch.empty(1, 1, wl["C"], comp.G, dtype=torch.complex64, device=dev)
+    ref64 = None
+    if not args.no_check:
+        ref64 = grid_fp64_device(g, y[0, 0])
+        torch.cuda.synchronize()
+
+    modes = {}
+    cache = {}
+    for mode in [m for m in args.modes.split(",") if m]:
+        for _ in range(args.warmup):
+            g.grid_device(y, mode, out, program_cache=cache)
+        torch.cuda.synchronize()
+        rl = rel_l2(out[0, 0], ref64) if ref64 is not None else None
+        timer = KernelTimer(dev)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.steps):
+            g.grid_device(y, mode, out, timer=timer, program_cache=cache)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.steps
+        kt = {k: float(np.mean(v)) for k, v in timer.durations().items()}
+        modes[mode] = {"coil_samples_per_s": wl["N"] * wl["C"] / (ms * 1e-3), "ms_per_step": ms,
+                       "rel_l2_vs_fp64": rl, "kernel_ms": kt}
+        log(f"{mode}: {ms:.3f} ms/step, {modes[mode]['coil_samples_per_s']:.3e} coil-samples/s, rel-L2 {rl}, "
+            f"kernels {kt}")
+
+    bf = [m for m in modes if m.startswith("bf16") and (modes[m]["rel_l2_vs_fp64"] is None
+                                                       or modes[m]["rel_l2_vs_fp64"] <= 1e-3)]
+    head = max(bf, key=lambda m: modes[m]["coil_samples_per_s"]) if bf else next(iter(modes))
+    hm = modes[head]
+    kname, kms = max(hm["kernel_ms"].items(), key=lambda kv: kv[1])
+    stats["n_parts"] = 2 if head == "bf16x3" else 1
+    ab = alg_bytes(kname, stats)
+    achieved = ab / (kms * 1e-3) / 1e9
+    roof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes": ab, "kernel_ms": kms}
+    traffic_file = ROOT / "profiles" / f"traffic_{args.workload}.json"
+    if traffic_file.exists():
+        tr = json.loads(traffic_file.read_text())
+        if kname in tr:
+            roof["traffic"] = tr[kname]
+    all_roof = {}
+    for m, v in modes.items():
+        stats["n_parts"] = 2 if m == "bf16x3" else 1
+        all_roof[m] = {k: {"GB/s": alg_bytes(k, stats) / (t * 1e-3) / 1e9, "ms": t} for k, t in v["kernel_ms"].items()}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = len(os.sched_getaffinity(0))
+        probe = cpu_baseline(wl, max(1, min(wl["spokes"], 100 if wl["spokes"] > 1000 else wl["spokes"])), threads)
+        rate = probe["coil_samples"] / probe["seconds"]
+        n_sp = int(min(wl["spokes"], max(1, args.cpu_seconds * rate / (wl["n_read"] * wl["C"]))))
+        res = cpu_baseline(wl, n_sp, threads)
+        cpu = {"value": res["coil_samples"] / res["seconds"], "unit": "coil-samples/s", "cores": threads,
+               "kind": res["kind"],
+               "sample": f"first {n_sp} of {wl['spokes']} spokes ({n_sp * wl['n_read']} samples x {wl['C']} coils), "
+                         f"reference interpolate(grid=True) flow, {res['seconds']:.2f} s"}
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": METRIC,
+        "value": hm["coil_samples_per_s"],
+        "unit": "coil-samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": hm["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16" if head.startswith("bf16") else "fp32",
+        "data": "synthetic (seeded trajectory, phantom, coil maps; Gaussian k-space)",
+        "config": {"workload": wl["desc"], "mode": head, "samples": wl["N"], "coils": wl["C"],
+                   "grid": list(wl["shape"]), "table_entries": stats["K"], "replicas_valid": stats["R"]},
+        "modes": modes,
+        "speedup_bf16_vs_fp32": (hm["coil_samples_per_s"] / modes["fp32"]["coil_samples_per_s"]
+                                 if "fp32" in modes else None),
+        "roofline": roof,
+        "roofline_all": all_roof,
+        "cpu_baseline": cpu,
+        "setup_s": setup_s,
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

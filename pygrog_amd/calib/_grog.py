@@ -211,6 +211,35 @@ def compile_units(cell_ptr: torch.Tensor, S: int, G: int, device, chunk: int = U
     return cell_unit, unit_cell, unit_rep
 
 
+class KernelTimer:
+    """Per-kernel HIP-event timing on the launch stream: ``mark(name)``
+    brackets one launch; ``durations()`` (after a synchronize) returns the
+    milliseconds of every bracketed launch, per kernel name."""
+
+    def __init__(self, device):
+        self.device = device
+        self.events: list = []
+
+    def start(self):
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(torch.cuda.current_stream(self.device))
+        return e
+
+    def stop(self, name, e0):
+        e1 = torch.cuda.Event(enable_timing=True)
+        e1.record(torch.cuda.current_stream(self.device))
+        self.events.append((name, e0, e1))
+
+    def durations(self) -> dict:
+        out: dict = {}
+        for name, a, b in self.events:
+            out.setdefault(name, []).append(a.elapsed_time(b))
+        return out
+
+    def reset(self):
+        self.events = []
+
+
 class GridProgram:
     """A ready-to-launch fused gridding job: device frame table + tile map.
 
@@ -234,9 +263,12 @@ class GridProgram:
             n_blocks = min(self.n_tiles, 256 * 12)
         self.n_blocks = int(n_blocks)
 
-    def launch(self, entry: str = "pg_grid_fp32") -> None:
+    def launch(self, entry: str = "pg_grid_fp32", timer: "KernelTimer | None" = None) -> None:
+        e0 = timer.start() if timer is not None else None
         _lib.call(entry, _lib.ptr(self.frames), _lib.ptr(self.tile_ptr), self.F, self.n_tiles, self.C,
                   self.tile_cells, self.n_blocks, _lib.stream(self.device))
+        if timer is not None:
+            timer.stop(entry, e0)
 
 
 # ---------------------------------------------------------------------------
@@ -431,7 +463,7 @@ class GrogInterpolator:
 
     # -- table (reference :382-422) ------------------------------------------
     def calc_interp_table(self, train_data, lamda: float = 0.01, precision: int = 1, *, quant: int | None = None,
-                          axis_order: str = "reference"):
+                          axis_order: str = "reference", keep_fp64: bool = False):
         """Build the operator table (FP64 on the GPU) and quantise every
         replica's shift to a table entry (bit-exact, in the plan kernels).
 
@@ -443,7 +475,7 @@ class GrogInterpolator:
             raise RuntimeError("calc_interp_table: shifts were already released by a previous call")
         dev = comp.device
         res = build_table(train_data, self.plan.radius, precision, lamda, quant=quant, device=dev,
-                          axis_order=axis_order)
+                          axis_order=axis_order, keep_fp64=keep_fp64)
         pfac, stepsize, nsteps = quant_steps(self.plan.radius, precision, quant)
         if res.table.shape[0] != nsteps ** comp.ndim:
             raise RuntimeError("table size does not match the quantisation")
@@ -460,6 +492,7 @@ class GrogInterpolator:
         C = int(res.table.shape[1])
         comp.C = C
         comp.table = res.table
+        comp.table64 = res.table64
         comp.table_host_steps = res.host_steps
         if C <= 64:
             packed = torch.empty(K * ((C + 1) // 2) * C * 4, dtype=torch.float32, device=dev)
@@ -524,12 +557,17 @@ class GrogInterpolator:
                 f.unit_rep = comp.unit_rep.data_ptr() + s * 4
                 f.cell_unit = comp.cell_unit.data_ptr() + s * G * 4
                 if vbuf is not None:
-                    off = (b * S + s) * max(comp.nper, 1) * C * 8
+                    off = (b * S + s) * self._rmax() * C * 8
                     f.vbuf = vbuf.data_ptr() + off
                 f.n_src = N
                 f.n_cell = G
                 frames.append(f)
         return frames
+
+    def _rmax(self) -> int:
+        """Largest valid-replica count over the stack elements (rows of the
+        grouped modes' intermediate buffer per frame)."""
+        return max(1, max(self.plan._compiled.n_valid))
 
     def grid_program(self, y: torch.Tensor, out: torch.Tensor, B: int = 1, vbuf=None, n_blocks=None) -> GridProgram:
         comp = self.plan._compiled
@@ -575,7 +613,8 @@ class GrogInterpolator:
         return comp.table_bf16[n_parts]
 
     # -- gridding on device (no host transfer) --------------------------------
-    def grid_device(self, y: torch.Tensor, mode: str = "fp32", out: torch.Tensor | None = None) -> torch.Tensor:
+    def grid_device(self, y: torch.Tensor, mode: str = "fp32", out: torch.Tensor | None = None,
+                    timer: KernelTimer | None = None, program_cache: dict | None = None) -> torch.Tensor:
         """Grid device-resident ``y`` (B, S, C, N) complex64 -> (B, S, C, G).
 
         mode "fp32": plain fused kernel.  "bf16" / "bf16x3": grouped complex
@@ -592,16 +631,24 @@ class GrogInterpolator:
             raise ValueError(f"y shape {tuple(y.shape)} does not match plan (B, {comp.S}, {comp.C}, {comp.npts})")
         if out is None:
             out = torch.empty(B, S, C, comp.G, dtype=torch.complex64, device=comp.device)
+        cache = program_cache if program_cache is not None else {}
         if mode == "fp32":
-            self.grid_program(y, out, B).launch("pg_grid_fp32")
+            key = ("fp32", y.data_ptr(), out.data_ptr(), B)
+            if key not in cache:
+                cache[key] = self.grid_program(y, out, B)
+            cache[key].launch("pg_grid_fp32", timer)
             return out
         if mode not in ("bf16", "bf16x3"):
             raise ValueError(f"unknown gridding mode {mode!r}")
         n_terms = 1 if mode == "bf16" else 3
         tbl = self._table_bf16(2 if n_terms == 3 else 1)
         groups = self._entry_groups()
-        vbuf = torch.empty(B * S * max(comp.nper, 1) * C, dtype=torch.complex64, device=comp.device)
-        yt = torch.empty(N * C, dtype=torch.complex64, device=comp.device)
+        key = (mode, y.data_ptr(), out.data_ptr(), B)
+        if key not in cache:
+            vbuf = torch.empty(B * S * self._rmax() * C, dtype=torch.complex64, device=comp.device)
+            yt = torch.empty(N * C, dtype=torch.complex64, device=comp.device)
+            cache[key] = (vbuf, yt, self.grid_program(y, out, B, vbuf=vbuf))
+        vbuf, yt, prog = cache[key]
         st = _lib.stream(comp.device)
         K = int(comp.table.shape[0])
         for b in range(B):
@@ -609,11 +656,17 @@ class GrogInterpolator:
                 ent_ptr, ent_rep, mx = groups[s]
                 if comp.n_valid[s] == 0:
                     continue
+                e0 = timer.start() if timer is not None else None
                 _lib.call("pg_transpose_cn_c64", y[b, s].data_ptr(), C, N, _lib.ptr(yt), st)
-                vb = vbuf.data_ptr() + (b * S + s) * max(comp.nper, 1) * C * 8
+                if timer is not None:
+                    timer.stop("pg_transpose_cn_c64", e0)
+                    e0 = timer.start()
+                vb = vbuf.data_ptr() + (b * S + s) * self._rmax() * C * 8
                 _lib.call("pg_apply_grouped_bf16", _lib.ptr(yt), _lib.ptr(tbl), _lib.ptr(ent_ptr), _lib.ptr(ent_rep),
                           comp.rep_src[s].data_ptr(), comp.rep_s[s].data_ptr(), K, C, mx, n_terms, vb, st)
-        self.grid_program(y, out, B, vbuf=vbuf).launch("pg_grid_reduce_v")
+                if timer is not None:
+                    timer.stop("pg_apply_grouped_bf16", e0)
+        prog.launch("pg_grid_reduce_v", timer)
         return out
 
     # -- forward interpolation (reference :437-555) ----------------------------

@@ -174,6 +174,8 @@ class SparseFFT(SolveMixin):
 
     def __init__(self, grid_shape=None, image_shape=None, indices=None, weights=None, smaps=None, device=None, *,
                  plan=None, toeplitz=None):
+        if plan is None and (grid_shape is None or image_shape is None):
+            raise ValueError("Either 'plan' or both 'grid_shape'/'image_shape' required")
         comp_dev = device if (device is not None and torch.device(device).type == "cuda") else None
         self._comp_device = _lib.hip_device(comp_dev)
         dev_fields = None

@@ -1,0 +1,219 @@
+"""CPU tests: the C ABI library loads and exports every symbol the header
+declares; host-side logic (argument checks, shapes, solver maths on a dense
+CPU operator, preconditioner coefficients, error messages)."""
+
+import ctypes
+import pathlib
+import re
+
+import numpy as np
+import pytest
+import torch
+
+from pygrog_amd import _lib
+
+HEADER = pathlib.Path(__file__).resolve().parents[1] / "include" / "pygrog_hip.h"
+
+
+def _header_symbols():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(pg_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_symbols_exported():
+    lib = ctypes.CDLL(str(_lib.lib_path()))
+    syms = _header_symbols()
+    assert len(syms) >= 35
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, f"not exported: {missing}"
+    # every declared entry point has a ctypes signature in the loader
+    assert set(syms) <= set(_lib.exported_symbols())
+
+
+def test_struct_layouts_match_header():
+    # pg_frame: 10 pointers + 2 int64; pg_plan_geom: 2 int32 + 6 x 3 fields + 128 x 3 int32
+    assert ctypes.sizeof(_lib.Frame) == 96
+    assert ctypes.sizeof(_lib.PlanGeom) == 8 + 24 + 24 + 12 * 3 + 128 * 12 + 4  # trailing pad to 8
+
+
+def test_library_version_and_errors():
+    lib = _lib.load()
+    assert lib.pg_abi_version() == 1
+    # a host-side argument error is reported through pg_last_error (no GPU work)
+    rc = lib.pg_grid_fp32(None, None, 0, 0, 1, 128, 0, None)
+    assert rc != 0 and b"frame" in lib.pg_last_error()
+    assert lib.pg_grid_tile_cells(65) == 0 and lib.pg_grid_tile_cells(32) == 128
+    assert lib.pg_table_bf16_elems(10, 32, 2) == 10 * 2 * 4 * 2 * 64 * 8
+
+
+def test_kernel_offsets_and_quant_steps():
+    from pygrog_amd.calib._grappa import quant_steps
+    from pygrog_amd.calib._grog import _kernel_offsets
+
+    assert _kernel_offsets(2, 2, "circle").shape == (3, 2)
+    assert _kernel_offsets(3, 3, "sphere").shape == (19, 3)
+    assert _kernel_offsets(2, 3, "cross").shape == (5, 2)
+    with pytest.raises(ValueError, match="kernel_shape"):
+        _kernel_offsets(2, 2, "hexagon")
+    assert quant_steps(0.5, 1) == (10.0, 0.1, 11)
+    assert quant_steps(0.5, None, 32) == (32.0, 1 / 32, 33)
+
+
+def test_product_offsets_match_oracle():
+    from oracle.plan import kernel_offsets as oko
+    from pygrog_amd.calib._grog import _kernel_offsets
+
+    for nd, w, s in [(2, 2, "circle"), (2, 3, "square"), (3, 2, "sphere"), (3, 3, "cross"), (3, 4, "cube")]:
+        assert np.array_equal(_kernel_offsets(nd, w, s), oko(nd, w, s))
+
+
+def test_maskedfft_requires_plan_or_explicit_args():
+    from pygrog_amd.operator import MaskedFFT
+
+    with pytest.raises(ValueError, match=r"plan.*required"):
+        MaskedFFT()
+
+
+def test_sparsefft_requires_plan_or_shapes():
+    from pygrog_amd.operator import SparseFFT
+
+    with pytest.raises(ValueError, match="grid_shape"):
+        SparseFFT(indices=[0, 1], weights=[1, 1])
+
+
+def test_no_gpu_fails_loudly():
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    from pygrog_amd.calib import GrogInterpolator
+
+    with pytest.raises(RuntimeError, match="HIP"):
+        GrogInterpolator((8, 8), np.zeros((2, 4, 2), np.float32))
+
+
+def test_stack_rejects_mismatch():
+    from pygrog_amd.calib import GrogPlan
+
+    with pytest.raises(ValueError, match="empty"):
+        GrogPlan._stack([])
+    a = GrogPlan(shape=(8, 8), oversamp=(1, 1), grid_shape=(8, 8), kernel_width=2, kernel_shape="circle", radius=0.5,
+                 image_shape=(8, 8), grid_size=64, n_samples=4, natural_shape=(4,))
+    b = GrogPlan(shape=(8, 8), oversamp=(1, 1), grid_shape=(9, 9), kernel_width=2, kernel_shape="circle", radius=0.5,
+                 image_shape=(8, 8), grid_size=81, n_samples=4, natural_shape=(4,))
+    with pytest.raises(ValueError, match="grid_shape"):
+        GrogPlan._stack([a, b])
+
+
+class _DenseOp:
+    """y = A x on the CPU: the solvers only need forward/adjoint/normal."""
+
+    def __init__(self, A):
+        self.A = A
+        self.toeplitz = True
+
+    def forward(self, x):
+        return self.A @ x
+
+    def adjoint(self, y):
+        return self.A.conj().T @ y
+
+    def normal(self, x):
+        return self.adjoint(self.forward(x))
+
+
+def _problem(m=40, n=12, seed=0):
+    g = torch.Generator().manual_seed(seed)
+    A = torch.randn(m, n, dtype=torch.complex128, generator=g)
+    b = torch.randn(m, dtype=torch.complex128, generator=g)
+    return A, b
+
+
+def test_cg_converges_to_normal_equation_solution():
+    from pygrog_amd import cg
+
+    A, b = _problem()
+    damp = 0.1
+    x = cg(_DenseOp(A), b, damp=damp, max_iter=60, tol=1e-14)
+    ref = torch.linalg.solve(A.conj().T @ A + damp * torch.eye(12, dtype=A.dtype), A.conj().T @ b)
+    torch.testing.assert_close(x, ref, rtol=1e-8, atol=1e-10)
+
+
+def test_lsmr_converges_to_least_squares():
+    from pygrog_amd import lsmr
+
+    A, b = _problem(seed=1)
+    x = lsmr(_DenseOp(A), b, max_iter=200, atol=1e-15, btol=1e-15)
+    ref = torch.linalg.lstsq(A, b.unsqueeze(-1)).solution.squeeze(-1)
+    torch.testing.assert_close(x, ref, rtol=1e-7, atol=1e-9)
+    # damped: min ||Ax-b||^2 + damp^2 ||x||^2
+    damp = 0.7
+    x = lsmr(_DenseOp(A), b, damp=damp, max_iter=200, atol=1e-15, btol=1e-15)
+    ref = torch.linalg.solve(A.conj().T @ A + damp**2 * torch.eye(12, dtype=A.dtype), A.conj().T @ b)
+    torch.testing.assert_close(x, ref, rtol=1e-7, atol=1e-9)
+
+
+def test_solve_dispatch_and_errors():
+    from pygrog_amd import solve
+    from pygrog_amd._solve import SolveMixin
+
+    A, b = _problem(seed=2)
+    op = _DenseOp(A)
+    with pytest.raises(ValueError, match="Unknown solver method"):
+        solve(op, b, method="bogus")
+
+    class Op(SolveMixin, _DenseOp):
+        pass
+
+    o = Op(A)
+    with pytest.raises(ValueError, match="Unknown method"):
+        o.solve(b, method="bogus")
+    x1 = o.solve(b, max_iter=5)
+    x2 = o.solve(b, method="cg", max_iter=5)
+    assert torch.equal(x1, x2)
+    xn = o.solve(b.numpy(), method="cg", max_iter=5)
+    assert isinstance(xn, np.ndarray)
+
+
+def test_polynomial_coeffs_match_symbolic_l2_fit():
+    sympy = pytest.importorskip("sympy")
+    from pygrog_amd._solve._polynomial import _l2_optimal_coeffs
+
+    deg, lo, hi = 3, 0.0, 2.5
+    c = sympy.symbols(f"c0:{deg + 1}")
+    x = sympy.symbols("x")
+    p = sum(c[k] * x**k for k in range(deg + 1))
+    J = sympy.integrate((1 - x * p) ** 2, (x, lo, hi))
+    sol = sympy.solve([sympy.diff(J, ck) for ck in c], c)
+    ref = [float(sol[ck]) for ck in c]
+    np.testing.assert_allclose(_l2_optimal_coeffs(deg, lo, hi), ref, rtol=1e-8)
+
+
+def test_polynomial_preconditioner_applies_horner():
+    from pygrog_amd import PolynomialPreconditioner
+
+    A, _ = _problem(seed=3)
+    op = _DenseOp(A)
+    op.image_shape = (12,)
+    op.smaps = torch.ones(1)
+    pc = PolynomialPreconditioner(op, degree=2, spectrum=(0.0, 100.0))
+    x = torch.randn(12, dtype=torch.complex128)
+    T = A.conj().T @ A
+    c = pc.coeffs
+    ref = c[0] * x + c[1] * (T @ x) + c[2] * (T @ (T @ x))
+    torch.testing.assert_close(pc.apply(x), ref, rtol=1e-10, atol=1e-10)
+    with pytest.raises(ValueError):
+        PolynomialPreconditioner(op, degree=-1)
+
+
+def test_resize_and_centered_fft_roundtrip():
+    from pygrog_amd._base import fft, ifft
+    from pygrog_amd._utils import resize
+
+    x = torch.randn(6, 10, dtype=torch.complex64)
+    torch.testing.assert_close(ifft(fft(x)), x, rtol=1e-5, atol=1e-5)
+    big = resize(x, (8, 12))
+    assert big.shape == (8, 12) and torch.equal(resize(big, (6, 10)), x)
+    # centred FFT of a centred delta is flat with ortho norm
+    d = torch.zeros(8, 8, dtype=torch.complex64)
+    d[4, 4] = 1
+    torch.testing.assert_close(fft(d), torch.full((8, 8), 1 / 8, dtype=torch.complex64))
