@@ -114,8 +114,9 @@ def rel_l2(a: torch.Tensor, b: torch.Tensor) -> float:
 def alg_bytes(kernel: str, st: dict) -> float:
     C, N, G, R, U, K = st["C"], st["N"], st["G"], st["R"], st["U"], st["K"]
     if kernel == "pg_grid_fp32":
-        # y read + grid write + replica records (src, entry, sqrt w) + units + cell map + table once
-        return 8 * C * N + 8 * C * G + 12 * R + 8 * U + 4 * G + 8 * C * C * K
+        # source rows of every replica + grid write + replica records (src,
+        # entry, sqrt w) + units + cell map + table once
+        return 8 * C * R + 8 * C * G + 12 * R + 8 * U + 4 * G + 8 * C * C * K
     if kernel == "pg_apply_grouped_bf16":
         parts = st["n_parts"]
         # y_t rows of every replica + table (bf16, real 2C x 2C per entry) + V write + records

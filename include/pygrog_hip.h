@@ -242,18 +242,27 @@ int pg_plan_quantize(const float *distances, const int64_t *inv_perm,
 
 /* One frame = one (source block, plan, table, output grid) job. */
 typedef struct pg_frame {
-  const void *y;            /* (C, n_src) c64, coil-major */
+  const void *y;            /* sources: (n_src, C) sample-major c64 for
+                               pg_grid_fp32, (C, n_src) coil-major for
+                               pg_grid_scatter */
   void *grid;               /* (C, n_cell) c64, fully written */
-  const void *table;        /* packed table, pg_table_pack layout */
+  const void *table;        /* packed table, pg_table_pack_c64 layout */
   const uint32_t *rep_src;  /* (R) source of each replica, cell-sorted */
   const uint32_t *rep_ent;  /* (R) table entry of each replica */
   const float *rep_s;       /* (R) sqrt weight of each replica */
   const uint32_t *unit_cell;/* (U) */
   const uint32_t *unit_rep; /* (U + 1) */
   const uint32_t *cell_unit;/* (n_cell + 1) first unit of each cell */
-  const void *vbuf;         /* (R, C) c64 applied replicas (two-pass modes) */
+  const void *vbuf;         /* (R, C) c64 applied replicas (grouped modes) */
+  const int32_t *unit_hot;  /* (U) hot-cell index of a unit, -1 if none */
+  const uint32_t *hot_units;/* (n_hot_units) units of the hot cells, in order */
+  const uint32_t *hot_ptr;  /* (n_hot + 1) hot cell h owns hot_units[hot_ptr[h]..] */
+  void *p_unit;             /* (n_hot_units, C) c64 scratch: hot unit partials */
+  void *p_cell;             /* (n_hot, C) c64 scratch: hot cell sums */
   int64_t n_src;
   int64_t n_cell;
+  int64_t n_hot;
+  int64_t n_hot_units;
 } pg_frame;
 
 /* Pack a (K, C, C) c64 row-major table into the layout read by the fused
@@ -267,16 +276,25 @@ int pg_table_pack_c64(const void *table, int64_t K, int64_t C, void *packed,
  * dense coil-major grid (zeros in empty cells).  frames: device array of F
  * pg_frame; tile_ptr: device array (F + 1) of cumulative tile counts with
  * tile_cells cells per tile.  Deterministic: the result does not depend on
- * tile size, block count or the other frames in the launch. */
+ * tile size, block count or the other frames in the launch.
+ * Cells with many replicas ("hot", e.g. the radial DC) are reduced first by
+ * two pre-pass kernels -- unit partials in parallel into p_unit, then an
+ * in-order fold per hot cell into p_cell -- and the tile kernel copies the
+ * finished sums, so a hot cell costs one workgroup no more than any other
+ * and the result is the same bits as without the pre-pass.  max_hot_units /
+ * max_hot: largest n_hot_units / n_hot over the frames (0 = no pre-pass). */
 int pg_grid_fp32(const pg_frame *frames, const uint32_t *tile_ptr, int64_t F,
                  int64_t n_tiles, int64_t C, int64_t tile_cells,
-                 int64_t n_blocks, pg_stream_t stream);
+                 int64_t n_blocks, int64_t max_hot_units, int64_t max_hot,
+                 pg_stream_t stream);
 
 /* Second pass of the grouped modes: same reduction as pg_grid_fp32 but v is
  * read from frame.vbuf[r] (cell-sorted rows) instead of being computed. */
 int pg_grid_reduce_v(const pg_frame *frames, const uint32_t *tile_ptr,
                      int64_t F, int64_t n_tiles, int64_t C,
-                     int64_t tile_cells, int64_t n_blocks, pg_stream_t stream);
+                     int64_t tile_cells, int64_t n_blocks,
+                     int64_t max_hot_units, int64_t max_hot,
+                     pg_stream_t stream);
 
 /* Third mode, used by SparseFFT (reference _scatter_ifft_crop,
  * src/pygrog/operator/_sparse_fft.py:396-435, 1154-1165): identity operator,
@@ -284,7 +302,8 @@ int pg_grid_reduce_v(const pg_frame *frames, const uint32_t *tile_ptr,
  * sqrt_weights, reduced per cell exactly like pg_grid_fp32. */
 int pg_grid_scatter(const pg_frame *frames, const uint32_t *tile_ptr,
                     int64_t F, int64_t n_tiles, int64_t C, int64_t tile_cells,
-                    int64_t n_blocks, pg_stream_t stream);
+                    int64_t n_blocks, int64_t max_hot_units, int64_t max_hot,
+                    pg_stream_t stream);
 
 /* Cells per tile the gridding kernels use for a given coil count (0 if
  * C > 64); tile_ptr must be built with this value. */
@@ -313,14 +332,17 @@ int pg_table_pack_bf16(const void *table, int64_t K, int64_t C, int32_t n_parts,
 
 /* Grouped complex GEMM on BF16 MFMA with FP32 accumulation.  Replicas are
  * grouped by table entry (ent_ptr (K+1), ent_rep (R) = cell-sorted replica
- * rank, both from pg_sort_pairs_u32 on rep_ent); for every replica r,
+ * rank, both from pg_sort_pairs_u32 on rep_ent); items (n_items pairs of
+ * uint32: entry, first replica offset inside its group) are the 128-replica
+ * work chunks, one wavefront each; for every replica r,
  * vbuf[r] = (K[e] y_t[rep_src[r]]) * s * s.  n_terms = 1 (hi*hi) or 3
  * (hi*hi + hi*lo + lo*hi split-BF16 products). */
 int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
                           const uint32_t *ent_ptr, const uint32_t *ent_rep,
                           const uint32_t *rep_src, const float *rep_s,
-                          int64_t K, int64_t C, int64_t max_group,
-                          int32_t n_terms, void *vbuf, pg_stream_t stream);
+                          const uint32_t *items, int64_t n_items,
+                          int64_t K, int64_t C, int32_t n_terms, void *vbuf,
+                          pg_stream_t stream);
 /* Element count (bf16) of the pg_table_pack_bf16 layout, -1 if C > 64. */
 int64_t pg_table_bf16_elems(int64_t K, int64_t C, int32_t n_parts);
 

@@ -59,8 +59,15 @@ class Frame(ctypes.Structure):
         ("unit_rep", c_void_p),
         ("cell_unit", c_void_p),
         ("vbuf", c_void_p),
+        ("unit_hot", c_void_p),
+        ("hot_units", c_void_p),
+        ("hot_ptr", c_void_p),
+        ("p_unit", c_void_p),
+        ("p_cell", c_void_p),
         ("n_src", c_int64),
         ("n_cell", c_int64),
+        ("n_hot", c_int64),
+        ("n_hot_units", c_int64),
     ]
 
 
@@ -94,16 +101,16 @@ _SIGS = {
     "pg_plan_units": [c_void_p, c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p, c_void_p],
     "pg_plan_quantize": [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int32, c_float, c_float, c_float, c_int64, c_int64, c_void_p, c_void_p, c_int64, c_void_p],
     "pg_table_pack_c64": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
-    "pg_grid_fp32": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
-    "pg_grid_reduce_v": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
-    "pg_grid_scatter": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
+    "pg_grid_fp32": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
+    "pg_grid_reduce_v": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
+    "pg_grid_scatter": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
     "pg_grid_tile_cells": [c_int64],
     "pg_gather_perm_c64": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p],
     "pg_unit_count": [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_transpose_cn_c64": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
     "pg_table_pack_bf16": [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_table_bf16_elems": [c_int64, c_int64, c_int32],
-    "pg_apply_grouped_bf16": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
+    "pg_apply_grouped_bf16": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_sparse_apply_c64": [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
     "pg_csr_from_sorted": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
     "pg_csr_from_sorted_u32": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],

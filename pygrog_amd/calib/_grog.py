@@ -211,6 +211,46 @@ def compile_units(cell_ptr: torch.Tensor, S: int, G: int, device, chunk: int = U
     return cell_unit, unit_cell, unit_rep
 
 
+# Cells with at least this many reduction units are "hot": reduced by the
+# pre-pass kernels instead of inside their tile.
+HOT_MIN_UNITS = 2
+
+
+def hot_lists(cell_unit: torch.Tensor, S: int, G: int, device, min_units: int | None = None):
+    """Hot-cell work lists per stack element.
+
+    Returns (unit_hot (U) int32: element-local hot-cell index of each unit or
+    -1, and per element a dict with hot_units (HU) int32 global unit ids in
+    cell then unit order, hot_ptr (H + 1) int32, n_hot, n_hot_units).
+    """
+    if min_units is None:
+        min_units = HOT_MIN_UNITS
+    cu = cell_unit.to(torch.int64)
+    U = int(cu[-1].item())
+    cnt = cu[1:] - cu[:-1]
+    unit_hot = torch.full((max(U, 1),), -1, dtype=torch.int32, device=device)
+    per = []
+    hot_all = torch.nonzero(cnt >= min_units).squeeze(1)
+    for s in range(S):
+        hs = hot_all[(hot_all >= s * G) & (hot_all < (s + 1) * G)]
+        lens = cnt[hs]
+        H = int(hs.numel())
+        ptr = torch.zeros(H + 1, dtype=torch.int64, device=device)
+        if H:
+            ptr[1:] = torch.cumsum(lens, 0)
+        HU = int(ptr[-1].item()) if H else 0
+        if HU:
+            owner = torch.repeat_interleave(torch.arange(H, device=device), lens)
+            units = cu[hs][owner] + (torch.arange(HU, device=device) - ptr[:-1][owner])
+            unit_hot[units] = owner.to(torch.int32)
+            units = units.to(torch.int32)
+        else:
+            units = torch.zeros(1, dtype=torch.int32, device=device)
+        per.append({"hot_units": units.contiguous(), "hot_ptr": ptr.to(torch.int32).contiguous(), "n_hot": H,
+                    "n_hot_units": HU})
+    return unit_hot, per
+
+
 class KernelTimer:
     """Per-kernel HIP-event timing on the launch stream: ``mark(name)``
     brackets one launch; ``durations()`` (after a synchronize) returns the
@@ -253,6 +293,19 @@ class GridProgram:
         self.C = int(C)
         self.F = len(frames)
         self.tile_cells = int(tile_cells)
+        # per-frame scratch of the hot-cell pre-pass
+        self._scratch = []
+        self.max_hot_units = 0
+        self.max_hot = 0
+        for f in frames:
+            if f.n_hot_units > 0:
+                pu = torch.empty(f.n_hot_units * self.C, dtype=torch.complex64, device=device)
+                pc = torch.empty(f.n_hot * self.C, dtype=torch.complex64, device=device)
+                self._scratch += [pu, pc]
+                f.p_unit = pu.data_ptr()
+                f.p_cell = pc.data_ptr()
+                self.max_hot_units = max(self.max_hot_units, int(f.n_hot_units))
+                self.max_hot = max(self.max_hot, int(f.n_hot))
         arr = (_lib.Frame * self.F)(*frames)
         raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
         self.frames = raw.to(device)
@@ -266,7 +319,7 @@ class GridProgram:
     def launch(self, entry: str = "pg_grid_fp32", timer: "KernelTimer | None" = None) -> None:
         e0 = timer.start() if timer is not None else None
         _lib.call(entry, _lib.ptr(self.frames), _lib.ptr(self.tile_ptr), self.F, self.n_tiles, self.C,
-                  self.tile_cells, self.n_blocks, _lib.stream(self.device))
+                  self.tile_cells, self.n_blocks, self.max_hot_units, self.max_hot, _lib.stream(self.device))
         if timer is not None:
             timer.stop(entry, e0)
 
@@ -372,6 +425,7 @@ def _create_plan(shape, coords, oversamp, kernel_width, kernel_shape, time_map, 
               _lib.ptr(density), _lib.ptr(unit_count), max(nper, 1), st)
     del vals_s, gauss_sorted, gauss
     cell_unit, unit_cell, unit_rep = compile_units(cell_ptr, S, G, dev, UNIT_CHUNK, unit_count)
+    unit_hot, hot = hot_lists(cell_unit, S, G, dev)
     n_inv = n_invalid.cpu().tolist()
 
     if time_map is not None:
@@ -398,7 +452,7 @@ def _create_plan(shape, coords, oversamp, kernel_width, kernel_shape, time_map, 
     plan._compiled = SimpleNamespace(
         device=dev, S=S, npts=npts, kw=kw, nper=nper, G=G, ndim=ndim, spatial=spatial,
         cell_ptr=cell_ptr, rep_src=rep_src, rep_s=rep_s, rep_ent=None, cell_unit=cell_unit, unit_cell=unit_cell,
-        unit_rep=unit_rep, density=density, n_invalid=n_invalid, n_inv=n_inv,
+        unit_rep=unit_rep, unit_hot=unit_hot, hot=hot, density=density, n_invalid=n_invalid, n_inv=n_inv,
         n_valid=[nper - v for v in n_inv], target_flat=target_idx, distances=distances, inv_perm_flat=inv_perm,
         interp_idx=None, entry_groups=None,
     )
@@ -540,6 +594,8 @@ class GrogInterpolator:
         return data.to(comp.device).to(torch.complex64).reshape(B, S, C, comp.npts).contiguous()
 
     def _frames(self, y: torch.Tensor, out: torch.Tensor, B: int, vbuf: torch.Tensor | None = None):
+        """Frames of a (B, S) launch; ``y`` holds the per-frame sources
+        (sample-major (N, C) rows for the fp32 mode) back to back."""
         comp = self.plan._compiled
         S, C, N, G = comp.S, comp.C, comp.npts, comp.G
         frames = []
@@ -559,6 +615,12 @@ class GrogInterpolator:
                 if vbuf is not None:
                     off = (b * S + s) * self._rmax() * C * 8
                     f.vbuf = vbuf.data_ptr() + off
+                hot = comp.hot[s]
+                f.unit_hot = comp.unit_hot.data_ptr()
+                f.hot_units = hot["hot_units"].data_ptr()
+                f.hot_ptr = hot["hot_ptr"].data_ptr()
+                f.n_hot = hot["n_hot"]
+                f.n_hot_units = hot["n_hot_units"]
                 f.n_src = N
                 f.n_cell = G
                 frames.append(f)
@@ -575,9 +637,12 @@ class GrogInterpolator:
         return GridProgram(self._frames(y, out, B, vbuf), comp.C, comp.G, comp.device, tc, n_blocks)
 
     # -- grouped (BF16 MFMA) structures ---------------------------------------
+    GROUP_CHUNK = 128  # replicas per MFMA work item (8 tiles of 16)
+
     def _entry_groups(self):
         """Per stack element: replicas of each table entry (stable radix sort of
-        the entries), as (ent_ptr (K+1), ent_rep (R), max group size)."""
+        the entries) as (ent_ptr (K+1), ent_rep (R), items (n, 2) work list of
+        (entry, first replica offset) chunks of GROUP_CHUNK)."""
         comp = self.plan._compiled
         if comp.entry_groups is not None:
             return comp.entry_groups
@@ -596,8 +661,17 @@ class GrogInterpolator:
                                         _lib.ptr(vals_o), R, max(1, (K - 1).bit_length()), st)
             ent_ptr = torch.empty(K + 1, dtype=torch.int32, device=dev)
             _lib.call("pg_csr_from_sorted_u32", _lib.ptr(keys_o), R, K, _lib.ptr(ent_ptr), st)
-            mx = int((ent_ptr[1:] - ent_ptr[:-1]).max().item()) if K > 0 else 0
-            groups.append((ent_ptr, vals_o, mx))
+            n_e = (ent_ptr[1:] - ent_ptr[:-1]).to(torch.int64)
+            chunks = (n_e + self.GROUP_CHUNK - 1) // self.GROUP_CHUNK
+            total = int(chunks.sum().item())
+            if total:
+                ent = torch.repeat_interleave(torch.arange(K, device=dev), chunks)
+                first = torch.cumsum(chunks, 0) - chunks
+                off = (torch.arange(total, device=dev) - first[ent]) * self.GROUP_CHUNK
+                items = torch.stack([ent, off], 1).to(torch.int32).contiguous()
+            else:
+                items = torch.zeros(1, 2, dtype=torch.int32, device=dev)
+            groups.append((ent_ptr, vals_o, items, total))
         comp.entry_groups = groups
         return groups
 
@@ -613,59 +687,75 @@ class GrogInterpolator:
         return comp.table_bf16[n_parts]
 
     # -- gridding on device (no host transfer) --------------------------------
+    def _transpose_all(self, y: torch.Tensor, yt: torch.Tensor, timer) -> None:
+        """(B, S, C, N) coil-major -> per frame (N, C) sample-major rows."""
+        comp = self.plan._compiled
+        B, S, C, N = y.shape
+        st = _lib.stream(comp.device)
+        e0 = timer.start() if timer is not None else None
+        for b in range(B):
+            for s in range(S):
+                _lib.call("pg_transpose_cn_c64", y[b, s].data_ptr(), C, N,
+                          yt.data_ptr() + (b * S + s) * C * N * 8, st)
+        if timer is not None:
+            timer.stop("pg_transpose_cn_c64", e0)
+
     def grid_device(self, y: torch.Tensor, mode: str = "fp32", out: torch.Tensor | None = None,
                     timer: KernelTimer | None = None, program_cache: dict | None = None) -> torch.Tensor:
         """Grid device-resident ``y`` (B, S, C, N) complex64 -> (B, S, C, G).
 
-        mode "fp32": plain fused kernel.  "bf16" / "bf16x3": grouped complex
-        GEMM on BF16 MFMA (1 or 3 split products, FP32 accumulate), then the
-        same deterministic reduction.
+        Every mode first turns the coil-major sources into sample-major rows
+        (one transpose pass), so a replica reads one contiguous row.
+        mode "fp32": plain fused kernel (per-replica operator apply).
+        "bf16" / "bf16x3": grouped complex GEMM on BF16 MFMA (1 or 3 split
+        products, FP32 accumulate), then the same deterministic reduction.
         """
         if not self._interpolator_set:
             raise RuntimeError("GRAPPA kernels not set. Call calc_interp_table() first.")
         comp = self.plan._compiled
         if comp.C > 64:
             raise RuntimeError(f"fused gridding supports up to 64 coils, got {comp.C}")
+        if mode not in ("fp32", "bf16", "bf16x3"):
+            raise ValueError(f"unknown gridding mode {mode!r}")
         B, S, C, N = y.shape
         if (S, C, N) != (comp.S, comp.C, comp.npts):
             raise ValueError(f"y shape {tuple(y.shape)} does not match plan (B, {comp.S}, {comp.C}, {comp.npts})")
+        if not y.is_contiguous() or y.dtype != torch.complex64:
+            raise ValueError("y must be a contiguous complex64 tensor")
         if out is None:
             out = torch.empty(B, S, C, comp.G, dtype=torch.complex64, device=comp.device)
         cache = program_cache if program_cache is not None else {}
+        key = (mode, y.data_ptr(), out.data_ptr(), B)
+        if key not in cache:
+            yt = torch.empty(B * S * N * C, dtype=torch.complex64, device=comp.device)
+            if mode == "fp32":
+                cache[key] = (None, yt, self.grid_program(yt, out, B))
+            else:
+                vbuf = torch.empty(B * S * self._rmax() * C, dtype=torch.complex64, device=comp.device)
+                cache[key] = (vbuf, yt, self.grid_program(yt, out, B, vbuf=vbuf))
+        vbuf, yt, prog = cache[key]
+        self._transpose_all(y, yt, timer)
         if mode == "fp32":
-            key = ("fp32", y.data_ptr(), out.data_ptr(), B)
-            if key not in cache:
-                cache[key] = self.grid_program(y, out, B)
-            cache[key].launch("pg_grid_fp32", timer)
+            prog.launch("pg_grid_fp32", timer)
             return out
-        if mode not in ("bf16", "bf16x3"):
-            raise ValueError(f"unknown gridding mode {mode!r}")
         n_terms = 1 if mode == "bf16" else 3
         tbl = self._table_bf16(2 if n_terms == 3 else 1)
         groups = self._entry_groups()
-        key = (mode, y.data_ptr(), out.data_ptr(), B)
-        if key not in cache:
-            vbuf = torch.empty(B * S * self._rmax() * C, dtype=torch.complex64, device=comp.device)
-            yt = torch.empty(N * C, dtype=torch.complex64, device=comp.device)
-            cache[key] = (vbuf, yt, self.grid_program(y, out, B, vbuf=vbuf))
-        vbuf, yt, prog = cache[key]
         st = _lib.stream(comp.device)
         K = int(comp.table.shape[0])
+        e0 = timer.start() if timer is not None else None
         for b in range(B):
             for s in range(S):
-                ent_ptr, ent_rep, mx = groups[s]
-                if comp.n_valid[s] == 0:
+                ent_ptr, ent_rep, items, n_items = groups[s]
+                if comp.n_valid[s] == 0 or n_items == 0:
                     continue
-                e0 = timer.start() if timer is not None else None
-                _lib.call("pg_transpose_cn_c64", y[b, s].data_ptr(), C, N, _lib.ptr(yt), st)
-                if timer is not None:
-                    timer.stop("pg_transpose_cn_c64", e0)
-                    e0 = timer.start()
+                yf = yt.data_ptr() + (b * S + s) * C * N * 8
                 vb = vbuf.data_ptr() + (b * S + s) * self._rmax() * C * 8
-                _lib.call("pg_apply_grouped_bf16", _lib.ptr(yt), _lib.ptr(tbl), _lib.ptr(ent_ptr), _lib.ptr(ent_rep),
-                          comp.rep_src[s].data_ptr(), comp.rep_s[s].data_ptr(), K, C, mx, n_terms, vb, st)
-                if timer is not None:
-                    timer.stop("pg_apply_grouped_bf16", e0)
+                _lib.call("pg_apply_grouped_bf16", yf, _lib.ptr(tbl), _lib.ptr(ent_ptr), _lib.ptr(ent_rep),
+                          comp.rep_src[s].data_ptr(), comp.rep_s[s].data_ptr(), _lib.ptr(items), n_items, K, C,
+                          n_terms, vb, st)
+        if timer is not None:
+            timer.stop("pg_apply_grouped_bf16", e0)
         prog.launch("pg_grid_reduce_v", timer)
         return out
 

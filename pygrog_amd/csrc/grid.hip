@@ -20,20 +20,23 @@
 // one unit at a time, the unit partials are staged in LDS, folded per cell
 // into an LDS tile accumulator and the tile is written coil-row by coil-row
 // (coalesced, zeros included, no read-modify-write, no atomics).  The tile
-// loop is persistent and XCD-aware: the 8 XCDs each walk a contiguous range
-// of tiles, so neighbouring cells (which share source samples) stay in one
-// XCD's L2.
+// loop is persistent (grid-stride over tiles).  Cells with many replicas (the
+// radial DC and its neighbours) are reduced by a two-kernel pre-pass -- hot
+// units in parallel, then an in-order fold per hot cell -- whose sums the
+// tile kernel copies, so no workgroup serialises on a hot cell.
 #include "common.h"
 
 namespace pg {
 
 constexpr int kGridThreads = 256;
-constexpr int kLdsAcc = 4096;   // GS * TC complex accumulators (32 KiB)
-constexpr int kLdsPart = 2048;  // GS * NB complex unit partials (16 KiB)
+constexpr int kLdsAcc = 2048;   // GS * TC complex accumulators (16 KiB)
+constexpr int kLdsPart = 1024;  // GS * NB complex unit partials (8 KiB)
+constexpr int kLdsRows = 2048;  // GS * RB staged V rows, MODE 1 (16 KiB)
 
 template <int GS> struct TileCfg {
-  static constexpr int TC = kLdsAcc / GS;
-  static constexpr int NB = kLdsPart / GS;
+  static constexpr int TC = kLdsAcc / GS;  // cells per tile
+  static constexpr int NB = kLdsPart / GS; // units per batch
+  static constexpr int RB = kLdsRows / GS; // staged rows per batch (MODE 1)
 };
 
 __device__ __forceinline__ int find_frame(const uint32_t *__restrict__ tp,
@@ -47,51 +50,128 @@ __device__ __forceinline__ int find_frame(const uint32_t *__restrict__ tp,
   return lo;
 }
 
-// MODE 0: v = (K[e] y[:, src]) * s * s        (plain FP32 GROG apply)
+// MODE 0: v = (K[e] y_t[src, :]) * s * s     (plain FP32 GROG apply; y is
+//         sample-major (n_src, C): one contiguous row per replica)
 // MODE 1: v = vbuf[r]                          (grouped modes, second pass)
-// MODE 2: v = y[:, src] * s                    (SparseFFT scatter, identity op)
+// MODE 2: v = y[:, src] * s                    (SparseFFT scatter, identity op,
+//         y coil-major (C, n_src))
+struct Rec {
+  uint32_t src, ent;
+  float s;
+};
+
+template <int MODE>
+__device__ __forceinline__ Rec load_rec(const pg_frame &fr, uint32_t r) {
+  Rec q{0u, 0u, 0.0f};
+  if (MODE != 1) {
+    q.src = fr.rep_src[r];
+    q.s = fr.rep_s[r];
+    if (MODE == 0) q.ent = fr.rep_ent[r];
+  }
+  return q;
+}
+
 template <int GS, int MODE>
 __device__ __forceinline__ float2 replica_value(const pg_frame &fr, uint32_t r,
-                                                int lane, int C, int P) {
+                                                const Rec &q, int lane, int C,
+                                                int P, const float2 *rows,
+                                                uint32_t rbase) {
   float vx = 0.0f, vy = 0.0f;
   if (MODE == 1) {
-    if (lane < C)
+    if (lane < C) {
+      if (rows != nullptr) return rows[(r - rbase) * GS + lane];
       return reinterpret_cast<const float2 *>(fr.vbuf)[(int64_t)r * C + lane];
+    }
     return make_float2(0.0f, 0.0f);
   }
-  const uint32_t src = fr.rep_src[r];
-  const float s = fr.rep_s[r];
-  const float2 *y = reinterpret_cast<const float2 *>(fr.y);
   if (MODE == 2) {
     if (lane < C) {
-      const float2 a = y[(int64_t)lane * fr.n_src + src];
-      vx = a.x * s;
-      vy = a.y * s;
+      const float2 a =
+          reinterpret_cast<const float2 *>(fr.y)[(int64_t)lane * fr.n_src + q.src];
+      vx = a.x * q.s;
+      vy = a.y * q.s;
     }
     return make_float2(vx, vy);
   }
-  const uint32_t e = fr.rep_ent[r];
-  const float4 *tb =
-      reinterpret_cast<const float4 *>(fr.table) + (int64_t)e * P * C + lane;
+  const float4 *tb = reinterpret_cast<const float4 *>(fr.table) +
+                     (int64_t)q.ent * P * C + lane;
+  const float2 *yr = reinterpret_cast<const float2 *>(fr.y) + (int64_t)q.src * C;
   if (lane < C) {
+    if ((C & 1) == 0) {
+      // row of C complex = P float4 pairs (16-byte aligned for even C)
+      const float4 *y4 = reinterpret_cast<const float4 *>(yr);
 #pragma unroll
-    for (int p = 0; p < (GS + 1) / 2; ++p) {
-      if (p < P) {
-        const float4 k = tb[p * C];
-        const float2 y0 = y[(int64_t)(2 * p) * fr.n_src + src];
-        vx += k.x * y0.x - k.y * y0.y;
-        vy += k.x * y0.y + k.y * y0.x;
-        if (2 * p + 1 < C) {
-          const float2 y1 = y[(int64_t)(2 * p + 1) * fr.n_src + src];
-          vx += k.z * y1.x - k.w * y1.y;
-          vy += k.z * y1.y + k.w * y1.x;
+      for (int p = 0; p < (GS + 1) / 2; ++p) {
+        if (p < P) {
+          const float4 k = tb[p * C];
+          const float4 yy = y4[p];
+          vx += k.x * yy.x - k.y * yy.y;
+          vy += k.x * yy.y + k.y * yy.x;
+          vx += k.z * yy.z - k.w * yy.w;
+          vy += k.z * yy.w + k.w * yy.z;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < (GS + 1) / 2; ++p) {
+        if (p < P) {
+          const float4 k = tb[p * C];
+          const float2 y0 = yr[2 * p];
+          vx += k.x * y0.x - k.y * y0.y;
+          vy += k.x * y0.y + k.y * y0.x;
+          if (2 * p + 1 < C) {
+            const float2 y1 = yr[2 * p + 1];
+            vx += k.z * y1.x - k.w * y1.y;
+            vy += k.z * y1.y + k.w * y1.x;
+          }
         }
       }
     }
   }
-  vx = vx * s;
-  vy = vy * s;
-  return make_float2(vx * s, vy * s);
+  vx = vx * q.s;
+  vy = vy * q.s;
+  return make_float2(vx * q.s, vy * q.s);
+}
+
+// Canonical fold of one unit: ((0 + v_r0) + v_r0+1) + ... ; the next
+// replica's record is fetched while the current one is applied.
+template <int GS, int MODE>
+__device__ __forceinline__ float2 unit_partial(const pg_frame &fr, uint32_t u,
+                                               int lane, int C, int P,
+                                               const float2 *rows = nullptr,
+                                               uint32_t rbase = 0) {
+  const uint32_t r0 = fr.unit_rep[u], r1 = fr.unit_rep[u + 1];
+  float px = 0.0f, py = 0.0f;
+  if (r0 >= r1) return make_float2(px, py);
+  if (MODE == 1 && rows == nullptr) {
+    // contiguous rows: issue 8 loads, then add them in order
+    const float2 *vb = reinterpret_cast<const float2 *>(fr.vbuf) + lane;
+    if (lane >= C) return make_float2(px, py);
+    for (uint32_t rb = r0; rb < r1; rb += 8) {
+      float2 v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        v[k] = (rb + k < r1) ? vb[(int64_t)(rb + k) * C] : make_float2(0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (rb + k < r1) {
+          px = px + v[k].x;
+          py = py + v[k].y;
+        }
+      }
+    }
+    return make_float2(px, py);
+  }
+  Rec q = load_rec<MODE>(fr, r0);
+  for (uint32_t r = r0; r < r1; ++r) {
+    Rec nq = q;
+    if (r + 1 < r1) nq = load_rec<MODE>(fr, r + 1);
+    const float2 v = replica_value<GS, MODE>(fr, r, q, lane, C, P, rows, rbase);
+    px = px + v.x;
+    py = py + v.y;
+    q = nq;
+  }
+  return make_float2(px, py);
 }
 
 template <int GS, int MODE>
@@ -100,21 +180,20 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(
     int F, int C, uint32_t n_tiles) {
   constexpr int TC = TileCfg<GS>::TC;
   constexpr int NB = TileCfg<GS>::NB;
+  constexpr int RB = TileCfg<GS>::RB;
   constexpr int NG = kGridThreads / GS;
   __shared__ float2 acc[GS * TC];
   __shared__ float2 part[NB * GS];
+  // MODE 1 stages the batch's V rows (contiguous: units are cell-sorted)
+  // with one coalesced cooperative load, for memory-level parallelism.
+  __shared__ float2 rows_lds[MODE == 1 ? RB * GS : 1];
   const int lane = threadIdx.x % GS;
   const int grp = threadIdx.x / GS;
   const int P = (C + 1) / 2;
 
-  // XCD-aware persistent order: iteration `it` of block b runs on XCD
-  // it % 8 (gridDim.x is a multiple of 8), which owns tiles
-  // [xcd * per_xcd, (xcd + 1) * per_xcd).
-  const uint32_t per_xcd = (n_tiles + 7) / 8;
-  const uint32_t n_it = per_xcd * 8;
-  for (uint32_t it = blockIdx.x; it < n_it; it += gridDim.x) {
-    const uint32_t L = (it % 8) * per_xcd + it / 8;
-    if (L >= n_tiles) continue;
+  // Round-robin tile order: neighbouring tiles (the dense k-space centre
+  // spans many consecutive tiles) land on different workgroups and XCDs.
+  for (uint32_t L = blockIdx.x; L < n_tiles; L += gridDim.x) {
     const int f = find_frame(tile_ptr, F, L);
     const pg_frame fr = frames[f];
     const int64_t cell0 = (int64_t)(L - tile_ptr[f]) * TC;
@@ -122,6 +201,7 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(
     const int ncell = rem < TC ? (int)rem : TC;
     const uint32_t u0 = fr.cell_unit[cell0];
     const uint32_t u1 = fr.cell_unit[cell0 + ncell];
+    const bool has_hot = fr.n_hot > 0;
 
     for (int i = threadIdx.x; i < GS * TC; i += kGridThreads)
       acc[i] = make_float2(0.0f, 0.0f);
@@ -129,20 +209,33 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(
 
     for (uint32_t ub = u0; ub < u1; ub += NB) {
       const int nbat = (u1 - ub) < (uint32_t)NB ? (int)(u1 - ub) : NB;
+      const float2 *rows = nullptr;
+      uint32_t rbase = 0;
+      if (MODE == 1) {
+        rbase = fr.unit_rep[ub];
+        const uint32_t nrows = fr.unit_rep[ub + nbat] - rbase;
+        if (nrows <= (uint32_t)RB) {
+          const float2 *vb = reinterpret_cast<const float2 *>(fr.vbuf) +
+                             (int64_t)rbase * C;
+          for (int i = threadIdx.x; i < (int)nrows * C; i += kGridThreads) {
+            const int row = i / C, co = i - row * C;
+            rows_lds[row * GS + co] = vb[i];
+          }
+          rows = rows_lds;
+          __syncthreads();
+        }
+      }
       for (int j = grp; j < nbat; j += NG) {
         const uint32_t u = ub + j;
-        const uint32_t r0 = fr.unit_rep[u], r1 = fr.unit_rep[u + 1];
-        float px = 0.0f, py = 0.0f;
-        for (uint32_t r = r0; r < r1; ++r) {
-          const float2 v = replica_value<GS, MODE>(fr, r, lane, C, P);
-          px = px + v.x;
-          py = py + v.y;
-        }
-        part[j * GS + lane] = make_float2(px, py);
+        float2 pv = make_float2(0.0f, 0.0f);
+        if (!has_hot || fr.unit_hot[u] < 0)
+          pv = unit_partial<GS, MODE>(fr, u, lane, C, P, rows, rbase);
+        part[j * GS + lane] = pv;
       }
       __syncthreads();
       // Fold unit partials into the cell accumulators, units of a cell in
-      // order (one thread per (run head, coil)).
+      // order (one thread per (run head, coil)); a hot cell takes its sum
+      // from the pre-pass once, at the batch holding its first unit.
       for (int i = threadIdx.x; i < nbat * GS; i += kGridThreads) {
         const int j = i / GS;
         const int co = i % GS;
@@ -150,6 +243,18 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(
         const uint32_t cell = fr.unit_cell[ub + j];
         if (j > 0 && fr.unit_cell[ub + j - 1] == cell) continue;
         const int t = (int)((int64_t)cell - cell0);
+        const int h = has_hot ? fr.unit_hot[ub + j] : -1;
+        if (h >= 0) {
+          if (ub + j == fr.cell_unit[cell]) {
+            const float2 pc =
+                reinterpret_cast<const float2 *>(fr.p_cell)[(int64_t)h * C + co];
+            float2 a = acc[co * TC + t];
+            a.x = a.x + pc.x;
+            a.y = a.y + pc.y;
+            acc[co * TC + t] = a;
+          }
+          continue;
+        }
         float2 a = acc[co * TC + t];
         for (int jj = j; jj < nbat && fr.unit_cell[ub + jj] == cell; ++jj) {
           const float2 p = part[jj * GS + co];
@@ -168,6 +273,51 @@ __global__ __launch_bounds__(kGridThreads) void k_grid(
     }
     __syncthreads();
   }
+}
+
+// Hot pre-pass 1: partial of every unit of every hot cell, in parallel
+// (grid.x over hot units, grid.y over frames).
+template <int GS, int MODE>
+__global__ __launch_bounds__(kGridThreads) void k_hot_units(
+    const pg_frame *__restrict__ frames, int C) {
+  constexpr int NG = kGridThreads / GS;
+  const pg_frame fr = frames[blockIdx.y];
+  const int lane = threadIdx.x % GS;
+  const int64_t i = (int64_t)blockIdx.x * NG + threadIdx.x / GS;
+  if (i >= fr.n_hot_units) return;
+  const float2 pv = unit_partial<GS, MODE>(fr, fr.hot_units[i], lane, C, (C + 1) / 2);
+  if (lane < C) reinterpret_cast<float2 *>(fr.p_unit)[i * C + lane] = pv;
+}
+
+// Hot pre-pass 2: in-order fold of each hot cell's unit partials.  The
+// partials are fetched 16 at a time (independent loads in flight) and added
+// in order, so the dependent chain is the adds, not the loads.
+template <int GS>
+__global__ __launch_bounds__(kGridThreads) void k_hot_cells(
+    const pg_frame *__restrict__ frames, int C) {
+  constexpr int NG = kGridThreads / GS;
+  constexpr int KB = 16;
+  const pg_frame fr = frames[blockIdx.y];
+  const int lane = threadIdx.x % GS;
+  const int64_t h = (int64_t)blockIdx.x * NG + threadIdx.x / GS;
+  if (h >= fr.n_hot || lane >= C) return;
+  const float2 *pu = reinterpret_cast<const float2 *>(fr.p_unit);
+  float ax = 0.0f, ay = 0.0f;
+  const uint32_t i0 = fr.hot_ptr[h], i1 = fr.hot_ptr[h + 1];
+  for (uint32_t ib = i0; ib < i1; ib += KB) {
+    float2 v[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k)
+      v[k] = (ib + k < i1) ? pu[(int64_t)(ib + k) * C + lane] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      if (ib + k < i1) {
+        ax = ax + v[k].x;
+        ay = ay + v[k].y;
+      }
+    }
+  }
+  reinterpret_cast<float2 *>(fr.p_cell)[h * C + lane] = make_float2(ax, ay);
 }
 
 // packed[e][p][co] = (K[e][co][2p], K[e][co][2p+1])
@@ -269,26 +419,41 @@ __global__ void k_unit_count(const uint32_t *__restrict__ cp, int64_t S,
 template <int MODE>
 static int launch_grid(const pg_frame *frames, const uint32_t *tile_ptr,
                        int64_t F, int64_t n_tiles, int64_t C,
-                       int64_t tile_cells, int64_t n_blocks, pg_stream_t s) {
+                       int64_t tile_cells, int64_t n_blocks,
+                       int64_t max_hot_units, int64_t max_hot, pg_stream_t s) {
   PG_CHECK(C >= 1 && C <= 64, "grid: coil count must be in [1, 64], got %lld",
            (long long)C);
   PG_CHECK(F >= 1 && frames && tile_ptr, "grid: empty frame list");
+  PG_CHECK(F < 65536, "grid: too many frames in one launch");
   PG_CHECK(n_tiles >= 0 && n_tiles < (int64_t)UINT32_MAX, "grid: bad tiles");
+  PG_CHECK(max_hot_units >= 0 && max_hot >= 0, "grid: bad hot counts");
   if (n_tiles == 0) return 0;
   int64_t nb = n_blocks > 0 ? n_blocks : 2048;
   if (nb > n_tiles) nb = n_tiles;
   nb = ((nb + 7) / 8) * 8;
   const hipStream_t st = as_stream(s);
   const dim3 grid((unsigned)nb), block(kGridThreads);
-#define PG_GRID_CASE(GSV)                                                  \
-  if (C <= GSV) {                                                          \
-    PG_CHECK(tile_cells == TileCfg<GSV>::TC,                               \
-             "grid: tile_cells must be %d for C=%lld", TileCfg<GSV>::TC,   \
-             (long long)C);                                                \
-    hipLaunchKernelGGL((k_grid<GSV, MODE>), grid, block, 0, st, frames,    \
-                       tile_ptr, (int)F, (int)C, (uint32_t)n_tiles);       \
-    PG_LAUNCHED();                                                         \
-    return 0;                                                              \
+#define PG_GRID_CASE(GSV)                                                    \
+  if (C <= GSV) {                                                            \
+    PG_CHECK(tile_cells == TileCfg<GSV>::TC,                                 \
+             "grid: tile_cells must be %d for C=%lld", TileCfg<GSV>::TC,     \
+             (long long)C);                                                  \
+    constexpr int NG = kGridThreads / GSV;                                   \
+    if (max_hot_units > 0) {                                                 \
+      hipLaunchKernelGGL((k_hot_units<GSV, MODE>),                           \
+                         dim3((unsigned)ceil_div(max_hot_units, NG),         \
+                              (unsigned)F),                                  \
+                         block, 0, st, frames, (int)C);                      \
+      PG_LAUNCHED();                                                         \
+      hipLaunchKernelGGL((k_hot_cells<GSV>),                                 \
+                         dim3((unsigned)ceil_div(max_hot, NG), (unsigned)F), \
+                         block, 0, st, frames, (int)C);                      \
+      PG_LAUNCHED();                                                         \
+    }                                                                        \
+    hipLaunchKernelGGL((k_grid<GSV, MODE>), grid, block, 0, st, frames,      \
+                       tile_ptr, (int)F, (int)C, (uint32_t)n_tiles);         \
+    PG_LAUNCHED();                                                           \
+    return 0;                                                                \
   }
   PG_GRID_CASE(4)
   PG_GRID_CASE(8)
@@ -329,24 +494,27 @@ int pg_table_pack_c64(const void *table, int64_t K, int64_t C, void *packed,
 
 int pg_grid_fp32(const pg_frame *frames, const uint32_t *tile_ptr, int64_t F,
                  int64_t n_tiles, int64_t C, int64_t tile_cells,
-                 int64_t n_blocks, pg_stream_t stream) {
+                 int64_t n_blocks, int64_t max_hot_units, int64_t max_hot,
+                 pg_stream_t stream) {
   return launch_grid<0>(frames, tile_ptr, F, n_tiles, C, tile_cells, n_blocks,
-                        stream);
+                        max_hot_units, max_hot, stream);
 }
 
 int pg_grid_reduce_v(const pg_frame *frames, const uint32_t *tile_ptr,
                      int64_t F, int64_t n_tiles, int64_t C,
                      int64_t tile_cells, int64_t n_blocks,
+                     int64_t max_hot_units, int64_t max_hot,
                      pg_stream_t stream) {
   return launch_grid<1>(frames, tile_ptr, F, n_tiles, C, tile_cells, n_blocks,
-                        stream);
+                        max_hot_units, max_hot, stream);
 }
 
 int pg_grid_scatter(const pg_frame *frames, const uint32_t *tile_ptr,
                     int64_t F, int64_t n_tiles, int64_t C, int64_t tile_cells,
-                    int64_t n_blocks, pg_stream_t stream) {
+                    int64_t n_blocks, int64_t max_hot_units, int64_t max_hot,
+                    pg_stream_t stream) {
   return launch_grid<2>(frames, tile_ptr, F, n_tiles, C, tile_cells, n_blocks,
-                        stream);
+                        max_hot_units, max_hot, stream);
 }
 
 int pg_transpose_cn_c64(const void *y, int64_t C, int64_t N, void *y_t,

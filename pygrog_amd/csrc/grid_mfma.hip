@@ -78,14 +78,17 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
     const float *__restrict__ y_t, const unsigned short *__restrict__ tbl,
     const uint32_t *__restrict__ ent_ptr, const uint32_t *__restrict__ ent_rep,
     const uint32_t *__restrict__ rep_src, const float *__restrict__ rep_s,
-    int64_t nK, int C, float *__restrict__ vbuf) {
+    const uint32_t *__restrict__ items, int64_t n_items, int64_t nK, int C,
+    float *__restrict__ vbuf) {
   constexpr int NP = (NTERMS > 1) ? 2 : 1;
   const int wave = threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
-  const int64_t e = (int64_t)blockIdx.x * 4 + wave;
+  const int64_t item = (int64_t)blockIdx.x * 4 + wave;
+  if (item >= n_items) return;
+  const int64_t e = items[2 * item];
   if (e >= nK) return;
-  const uint32_t b0 = ent_ptr[e], b1 = ent_ptr[e + 1];
-  const uint32_t start = b0 + blockIdx.y * (uint32_t)(16 * kTilesPerWave);
+  const uint32_t b1 = ent_ptr[e + 1];
+  const uint32_t start = ent_ptr[e] + items[2 * item + 1];
   if (start >= b1) return;
   uint32_t stop = start + 16 * kTilesPerWave;
   if (stop > b1) stop = b1;
@@ -189,23 +192,23 @@ template <int MT, int KS>
 static int launch_grouped(const void *y_t, const void *tbl,
                           const uint32_t *ent_ptr, const uint32_t *ent_rep,
                           const uint32_t *rep_src, const float *rep_s,
-                          int64_t K, int64_t C, int64_t max_group,
-                          int32_t n_terms, void *vbuf, hipStream_t st) {
-  const int64_t per_wave = 16 * kTilesPerWave;
-  const int64_t gy = max_group <= 0 ? 1 : ceil_div(max_group, per_wave);
-  PG_CHECK(gy < 65536, "grouped apply: entry group too large");
-  const dim3 grid((unsigned)ceil_div(K, 4), (unsigned)gy);
+                          const uint32_t *items, int64_t n_items, int64_t K,
+                          int64_t C, int32_t n_terms, void *vbuf,
+                          hipStream_t st) {
+  if (n_items == 0) return 0;
+  PG_CHECK(items != nullptr, "grouped apply: null work list");
+  const dim3 grid((unsigned)ceil_div(n_items, 4));
   if (n_terms == 3)
     hipLaunchKernelGGL((k_apply_grouped<MT, KS, 3>), grid, dim3(256), 0, st,
                        static_cast<const float *>(y_t),
                        static_cast<const unsigned short *>(tbl), ent_ptr,
-                       ent_rep, rep_src, rep_s, K, (int)C,
+                       ent_rep, rep_src, rep_s, items, n_items, K, (int)C,
                        static_cast<float *>(vbuf));
   else
     hipLaunchKernelGGL((k_apply_grouped<MT, KS, 1>), grid, dim3(256), 0, st,
                        static_cast<const float *>(y_t),
                        static_cast<const unsigned short *>(tbl), ent_ptr,
-                       ent_rep, rep_src, rep_s, K, (int)C,
+                       ent_rep, rep_src, rep_s, items, n_items, K, (int)C,
                        static_cast<float *>(vbuf));
   PG_LAUNCHED();
   return 0;
@@ -243,8 +246,9 @@ int64_t pg_table_bf16_elems(int64_t K, int64_t C, int32_t n_parts) {
 int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
                           const uint32_t *ent_ptr, const uint32_t *ent_rep,
                           const uint32_t *rep_src, const float *rep_s,
-                          int64_t K, int64_t C, int64_t max_group,
-                          int32_t n_terms, void *vbuf, pg_stream_t stream) {
+                          const uint32_t *items, int64_t n_items,
+                          int64_t K, int64_t C, int32_t n_terms, void *vbuf,
+                          pg_stream_t stream) {
   PG_CHECK(C >= 1 && C <= 64, "grouped apply: coil count must be in [1, 64]");
   PG_CHECK(n_terms == 1 || n_terms == 3, "grouped apply: n_terms 1 or 3");
   PG_CHECK(K >= 1, "grouped apply: empty table");
@@ -254,8 +258,8 @@ int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
 #define PG_GROUPED_CASE(MTV, KSV)                                            \
   if (MT == MTV && KS == KSV)                                                \
     return launch_grouped<MTV, KSV>(y_t, table_bf16, ent_ptr, ent_rep,       \
-                                    rep_src, rep_s, K, C, max_group, n_terms, \
-                                    vbuf, st);
+                                    rep_src, rep_s, items, n_items, K, C,     \
+                                    n_terms, vbuf, st);
   PG_GROUPED_CASE(1, 1)
   PG_GROUPED_CASE(2, 1)
   PG_GROUPED_CASE(3, 2)

@@ -80,7 +80,7 @@ class _ScatterPlan:
     CSR over cells, chunked reduction units, permutations)."""
 
     def __init__(self, indices, sqrt_w, sort_perm, inv_perm, grid_size: int, S: int, n: int, device):
-        from ..calib._grog import UNIT_CHUNK, compile_units
+        from ..calib._grog import UNIT_CHUNK, compile_units, hot_lists
 
         self.device = device
         self.S, self.n, self.G = S, n, grid_size
@@ -103,8 +103,8 @@ class _ScatterPlan:
         for s in range(S):
             _lib.call("pg_csr_from_sorted", idx[s].data_ptr(), n, grid_size, self.cell_ptr[s].data_ptr(), st)
         self.cell_unit, self.unit_cell, self.unit_rep = compile_units(self.cell_ptr, S, grid_size, device, UNIT_CHUNK)
+        self.unit_hot, self.hot = hot_lists(self.cell_unit, S, grid_size, device)
         self.rep_src = self.sort_perm.to(torch.int32).contiguous()
-        self.tile_cells_cache = {}
 
     def frames(self, y_ptr: int, grid_ptr: int, B: int, C: int, y_stride_s: int | None = None):
         """Frames for data (B, S, C, n) -> grid (B, S, C, G)."""
@@ -120,6 +120,12 @@ class _ScatterPlan:
                 f.unit_cell = self.unit_cell.data_ptr()
                 f.unit_rep = self.unit_rep.data_ptr() + s * 4
                 f.cell_unit = self.cell_unit.data_ptr() + s * G * 4
+                hot = self.hot[s]
+                f.unit_hot = self.unit_hot.data_ptr()
+                f.hot_units = hot["hot_units"].data_ptr()
+                f.hot_ptr = hot["hot_ptr"].data_ptr()
+                f.n_hot = hot["n_hot"]
+                f.n_hot_units = hot["n_hot_units"]
                 f.n_src = n
                 f.n_cell = G
                 fr.append(f)
@@ -133,7 +139,6 @@ class _ScatterPlan:
         out = torch.empty(B, S, C, self.G, dtype=torch.complex64, device=self.device)
         if C == 0 or B == 0:
             return out
-        tc = _lib.load().pg_grid_tile_cells(min(C, 64))
         for c0 in range(0, C, 64):
             c1 = min(C, c0 + 64)
             if c0 == 0 and c1 == C:
@@ -147,7 +152,6 @@ class _ScatterPlan:
             prog.launch("pg_grid_scatter")
             if dst is not out:
                 out[:, :, c0:c1] = dst
-        _ = tc
         return out
 
     def gather(self, kgrid: torch.Tensor) -> torch.Tensor:

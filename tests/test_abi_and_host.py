@@ -32,8 +32,8 @@ def test_header_symbols_exported():
 
 
 def test_struct_layouts_match_header():
-    # pg_frame: 10 pointers + 2 int64; pg_plan_geom: 2 int32 + 6 x 3 fields + 128 x 3 int32
-    assert ctypes.sizeof(_lib.Frame) == 96
+    # pg_frame: 15 pointers + 4 int64; pg_plan_geom: 2 int32 + 6 x 3 fields + 128 x 3 int32
+    assert ctypes.sizeof(_lib.Frame) == 15 * 8 + 4 * 8
     assert ctypes.sizeof(_lib.PlanGeom) == 8 + 24 + 24 + 12 * 3 + 128 * 12 + 4  # trailing pad to 8
 
 
@@ -41,9 +41,12 @@ def test_library_version_and_errors():
     lib = _lib.load()
     assert lib.pg_abi_version() == 1
     # a host-side argument error is reported through pg_last_error (no GPU work)
-    rc = lib.pg_grid_fp32(None, None, 0, 0, 1, 128, 0, None)
+    rc = lib.pg_grid_fp32(None, None, 0, 0, 1, 128, 0, 0, 0, None)
     assert rc != 0 and b"frame" in lib.pg_last_error()
-    assert lib.pg_grid_tile_cells(65) == 0 and lib.pg_grid_tile_cells(32) == 128
+    assert lib.pg_grid_tile_cells(65) == 0
+    for c, gs in ((3, 4), (8, 8), (16, 16), (32, 32), (64, 64)):
+        tc = lib.pg_grid_tile_cells(c)
+        assert tc > 0 and tc * gs == 2048  # 16 KiB of complex accumulators
     assert lib.pg_table_bf16_elems(10, 32, 2) == 10 * 2 * 4 * 2 * 64 * 8
 
 

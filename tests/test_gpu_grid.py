@@ -166,3 +166,38 @@ def test_stacked_plan_matches_per_element(device):
         np.testing.assert_array_equal(g.plan.target_idx[s].cpu().numpy(), gs.plan.target_idx.cpu().numpy())
         k1, _ = gs.interpolate(y[s], grid=True)
         assert np.array_equal(np.asarray(ks[s]), np.asarray(k1))
+
+
+def test_hot_prepass_and_schedule_do_not_change_bits(device, monkeypatch):
+    """The hot-cell pre-pass and the block count only redistribute work:
+    the reduction order is fixed by the plan, so the grids are identical."""
+    import pygrog_amd.calib._grog as grog_mod
+
+    coords = _radial(64, 256, 9)  # DC and the first rings are hot cells
+    calib = _calib(8, (24, 24))
+    rng = np.random.default_rng(4)
+    y = torch.from_numpy((rng.standard_normal((8, 64, 256)) + 1j * rng.standard_normal((8, 64, 256)))
+                         .astype(np.complex64)).to(device)
+
+    def build():
+        g = GrogInterpolator((128, 128), coords, kernel_width=3, device=device)
+        g.calc_interp_table(calib, lamda=1e-3, quant=32)
+        return g
+
+    g_hot = build()
+    assert g_hot.plan._compiled.hot[0]["n_hot"] > 0
+    monkeypatch.setattr(grog_mod, "HOT_MIN_UNITS", 10**9)
+    g_cold = build()
+    assert g_cold.plan._compiled.hot[0]["n_hot"] == 0
+    yy = y.reshape(1, 1, 8, -1).contiguous()
+    for mode in ("fp32", "bf16x3", "bf16"):
+        a = g_hot.grid_device(yy, mode)
+        b = g_cold.grid_device(yy, mode)
+        assert torch.equal(a, b), mode
+    # a different persistent grid size (tile -> workgroup mapping)
+    comp = g_hot.plan._compiled
+    out = torch.empty(1, 1, 8, comp.G, dtype=torch.complex64, device=device)
+    yt = torch.empty(comp.npts * 8, dtype=torch.complex64, device=device)
+    g_hot._transpose_all(yy, yt, None)
+    g_hot.grid_program(yt, out, 1, n_blocks=8).launch("pg_grid_fp32")
+    assert torch.equal(out, g_hot.grid_device(yy, "fp32"))
