@@ -129,6 +129,78 @@ def alg_bytes(kernel: str, st: dict) -> float:
 
 
 # ---------------------------------------------------------------------------
+# HBM traffic from rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; KiB/dispatch)
+# ---------------------------------------------------------------------------
+# kernels launched by one call of each entry point; k_hot_cells is shared by
+# the fp32 and reduce entries (same plan, same hot cells)
+ENTRY_KERNELS = {
+    "pg_grid_fp32": [r"k_hot_units<\d+, 0>", r"k_hot_cells<", r"k_grid<\d+, 0>"],
+    "pg_grid_reduce_v": [r"k_hot_units<\d+, 1>", r"k_hot_cells<", r"k_grid<\d+, 1>"],
+    "pg_apply_grouped_bf16:1": [r"k_apply_grouped<\d+, \d+, 1>"],
+    "pg_apply_grouped_bf16:3": [r"k_apply_grouped<\d+, \d+, 3>"],
+    "pg_transpose_cn_c64": [r"k_transpose_cn"],
+}
+COPY_KERNEL = r"neg_kernel"
+
+
+def _pmc_dispatches(path: str, counter: str) -> list:
+    """[(dispatch id, kernel name, value)] of ``counter``, in dispatch order."""
+    import csv
+    import re
+
+    per = {}
+    names = {}
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            if r["Counter_Name"] != counter:
+                continue
+            d = int(r["Dispatch_Id"])
+            per[d] = per.get(d, 0.0) + float(r["Counter_Value"])
+            names[d] = re.sub(r"\(.*$", "", r["Kernel_Name"]).replace("void ", "")
+    return [(d, names[d], per[d]) for d in sorted(per)]
+
+
+def _pmc_means(rows: list) -> dict:
+    out = {}
+    for _, name, v in rows:
+        out.setdefault(name, []).append(v)
+    return {k: float(np.mean(v)) for k, v in out.items()}
+
+
+def summarize_pmc(fetch_csv: str, write_csv: str, copy_bytes: int, workload: str) -> dict:
+    import re
+
+    fetch_rows = _pmc_dispatches(fetch_csv, "FETCH_SIZE")
+    write_rows = _pmc_dispatches(write_csv, "WRITE_SIZE")
+    fetch, write = _pmc_means(fetch_rows), _pmc_means(write_rows)
+
+    def copy_kib(rows):
+        # the calibration copy is the first negation the benchmark dispatches
+        return next(v for _, n, v in rows if re.search(COPY_KERNEL, n))
+
+    def pick(table, pat):
+        hits = [v for k, v in table.items() if re.search(pat, k)]
+        if len(hits) != 1:
+            raise SystemExit(f"pattern {pat!r} matches {len(hits)} kernels")
+        return hits[0]
+
+    ff = copy_bytes / (copy_kib(fetch_rows) * 1024.0)
+    wf = copy_bytes / (copy_kib(write_rows) * 1024.0)
+    entries = {}
+    for entry, pats in ENTRY_KERNELS.items():
+        try:
+            fb = sum(pick(fetch, p) for p in pats) * 1024.0 * ff
+            wb = sum(pick(write, p) for p in pats) * 1024.0 * wf
+        except SystemExit:
+            continue
+        entries[entry] = {"fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb}
+    return {"workload": workload, "copy_bytes": copy_bytes, "fetch_factor": ff, "write_factor": wf,
+            "entries": entries,
+            "kernels_kib": {k: {"FETCH_SIZE": fetch.get(k), "WRITE_SIZE": write.get(k)} for k in sorted(fetch)
+                            if k.startswith("pg::")}}
+
+
+# ---------------------------------------------------------------------------
 # reference CPU baseline (bounded sample)
 # ---------------------------------------------------------------------------
 def cpu_baseline(wl, n_spokes: int, threads: int) -> dict:
@@ -211,7 +283,21 @@ def main(argv=None) -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work (s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 rel-L2 check")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="run one plain device copy (float32 negation) of BYTES first: the PMC scale factor")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="turn two rocprofv3 --pmc counter_collection.csv files into profiles/traffic_<workload>.json "
+                         "(no GPU needed)")
     args = ap.parse_args(argv)
+
+    if args.summarize_pmc:
+        if args.copy_calibration <= 0:
+            raise SystemExit("--summarize-pmc needs --copy-calibration BYTES of the profiled run")
+        res = summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.copy_calibration, args.workload)
+        path = ROOT / "profiles" / f"traffic_{args.workload}.json"
+        path.write_text(json.dumps(res, indent=1) + "\n")
+        print(json.dumps(res["entries"]))
+        return 0
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus N>1 is not supported (see DESIGN.md, "
@@ -223,6 +309,12 @@ def main(argv=None) -> int:
     from pygrog_amd.calib._grog import KernelTimer
 
     dev = torch.device("cuda:0")
+    if args.copy_calibration > 0:
+        src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     t_setup = time.perf_counter()
     wl = make_workload(args.workload, dev)
     g = wl["g"]
@@ -274,10 +366,12 @@ def main(argv=None) -> int:
     roof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes": ab, "kernel_ms": kms}
     traffic_file = ROOT / "profiles" / f"traffic_{args.workload}.json"
+    tkey = kname + (f":{3 if head == 'bf16x3' else 1}" if kname == "pg_apply_grouped_bf16" else "")
     if traffic_file.exists():
-        tr = json.loads(traffic_file.read_text())
-        if kname in tr:
-            roof["traffic"] = tr[kname]
+        tr = json.loads(traffic_file.read_text()).get("entries", {})
+        if tkey in tr:
+            roof["traffic"] = tr[tkey]["traffic_bytes"]
+            roof["traffic_source"] = f"profiles/{traffic_file.name} (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
     all_roof = {}
     for m, v in modes.items():
         stats["n_parts"] = 2 if m == "bf16x3" else 1
