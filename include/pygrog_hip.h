@@ -259,6 +259,10 @@ typedef struct pg_frame {
   const uint32_t *hot_ptr;  /* (n_hot + 1) hot cell h owns hot_units[hot_ptr[h]..] */
   void *p_unit;             /* (n_hot_units, C) c64 scratch: hot unit partials */
   void *p_cell;             /* (n_hot, C) c64 scratch: hot cell sums */
+  const uint32_t *cell_row; /* (n_cell + 1) first replica of each cell (CSR) */
+  const uint32_t *cell_vrow;/* (n_cell + 1) replicas of ordinary cells (not
+                               hot, one unit) before each cell: the tile
+                               kernels' row numbering */
   int64_t n_src;
   int64_t n_cell;
   int64_t n_hot;
@@ -305,9 +309,14 @@ int pg_grid_scatter(const pg_frame *frames, const uint32_t *tile_ptr,
                     int64_t n_blocks, int64_t max_hot_units, int64_t max_hot,
                     pg_stream_t stream);
 
-/* Cells per tile the gridding kernels use for a given coil count (0 if
- * C > 64); tile_ptr must be built with this value. */
+/* Default cells per tile of the gridding kernels for a coil count (0 if
+ * C > 64). */
 int pg_grid_tile_cells(int64_t C);
+
+/* The tile sizes compiled for a coil count, largest first, written to
+ * opts[0 .. min(n, max_opts)); returns n (0 if C is out of [1, 64]).  A
+ * launch's tile_cells must be one of them and its tile_ptr built with it. */
+int pg_grid_tile_options(int64_t C, int32_t *opts, int32_t max_opts);
 
 /* out[b, m] = weights[inv_perm[m]] * grid[b, indices[inv_perm[m]]]:
  * SparseFFT gather fused with the unsort (reference _fft_pad_gather,
@@ -323,6 +332,16 @@ int pg_unit_count(const uint32_t *cell_ptr, int64_t S, int64_t G,
 /* y (C, N) c64 -> y_t (N, C) c64 sample-major. */
 int pg_transpose_cn_c64(const void *y, int64_t C, int64_t N, void *y_t,
                         pg_stream_t stream);
+
+/* Frames of uneven sample counts, packed back to back: frame f is a (C, N_f)
+ * coil-major block at sample offset frame_off[f] (element offset
+ * C * frame_off[f]); its (N_f, C) sample-major rows are written at the same
+ * offset of y_t.  frame_off (F + 1 entries, device) is non-decreasing and
+ * max_n >= every N_f.  No reference counterpart: it feeds the batched
+ * multi-frame gridding launch. */
+int pg_transpose_frames_c64(const void *y, const int64_t *frame_off, int64_t F,
+                            int64_t max_n, int64_t C, void *y_t,
+                            pg_stream_t stream);
 
 /* Pack a (K, C, C) c64 table for the BF16 MFMA grouped GEMM (real 2C x 2C
  * form, hi and optionally lo bf16 parts).  n_parts = 1 or 2; packed holds

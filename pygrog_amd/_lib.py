@@ -64,6 +64,8 @@ class Frame(ctypes.Structure):
         ("hot_ptr", c_void_p),
         ("p_unit", c_void_p),
         ("p_cell", c_void_p),
+        ("cell_row", c_void_p),
+        ("cell_vrow", c_void_p),
         ("n_src", c_int64),
         ("n_cell", c_int64),
         ("n_hot", c_int64),
@@ -105,9 +107,11 @@ _SIGS = {
     "pg_grid_reduce_v": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
     "pg_grid_scatter": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
     "pg_grid_tile_cells": [c_int64],
+    "pg_grid_tile_options": [c_int64, c_void_p, ctypes.c_int32],
     "pg_gather_perm_c64": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p],
     "pg_unit_count": [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_transpose_cn_c64": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
+    "pg_transpose_frames_c64": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p],
     "pg_table_pack_bf16": [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_table_bf16_elems": [c_int64, c_int64, c_int32],
     "pg_apply_grouped_bf16": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
@@ -167,6 +171,13 @@ def ptr(t: torch.Tensor | None) -> int | None:
     if t is None:
         return None
     return t.data_ptr()
+
+
+def tile_options(C: int) -> list:
+    """Tile sizes (cells) compiled for ``C`` coils, largest first."""
+    buf = (ctypes.c_int32 * 8)()
+    n = load().pg_grid_tile_options(int(C), buf, 8)
+    return [int(buf[i]) for i in range(min(n, 8))]
 
 
 def stream(device: torch.device | None = None) -> int:

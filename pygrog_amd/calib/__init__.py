@@ -3,5 +3,7 @@ src/pygrog/calib/__init__.py)."""
 
 from ._grappa import KernelTable, KernelTableResult, build_table, train_grappa
 from ._grog import GrogInterpolator, GrogPlan
+from ._batch import GrogFrameBatch
 
-__all__ = ["GrogInterpolator", "GrogPlan", "KernelTable", "KernelTableResult", "build_table", "train_grappa"]
+__all__ = ["GrogFrameBatch", "GrogInterpolator", "GrogPlan", "KernelTable", "KernelTableResult", "build_table",
+           "train_grappa"]

@@ -251,6 +251,25 @@ def hot_lists(cell_unit: torch.Tensor, S: int, G: int, device, min_units: int | 
     return unit_hot, per
 
 
+def ordinary_rows(cell_ptr: torch.Tensor, cell_unit: torch.Tensor, S: int, G: int,
+                  min_units: int | None = None) -> torch.Tensor:
+    """(S, G + 1) int32: replicas of ordinary cells (one unit, not hot)
+    before each cell, per stack element -- the tile kernels' row numbering,
+    which leaves out the cells the hot pre-pass or the multi-unit path
+    reduce.  ``min_units`` must be the one ``hot_lists`` used."""
+    if min_units is None:
+        min_units = HOT_MIN_UNITS
+    cp = cell_ptr.reshape(S, G + 1).to(torch.int64)
+    cnt = cp[:, 1:] - cp[:, :-1]
+    cu = cell_unit.to(torch.int64)
+    nu = (cu[1:] - cu[:-1]).reshape(S, G)
+    special = (nu >= max(int(min_units), 1)) | (nu > 1)
+    vc = torch.where(special, torch.zeros_like(cnt), cnt)
+    out = torch.zeros(S, G + 1, dtype=torch.int64, device=cell_ptr.device)
+    out[:, 1:] = torch.cumsum(vc, 1)
+    return out.to(torch.int32).contiguous()
+
+
 class KernelTimer:
     """Per-kernel HIP-event timing on the launch stream: ``mark(name)``
     brackets one launch; ``durations()`` (after a synchronize) returns the
@@ -278,6 +297,34 @@ class KernelTimer:
 
     def reset(self):
         self.events = []
+
+
+def group_by_entry(rep_ent: torch.Tensor, R: int, K: int, device, chunk: int):
+    """Replicas of each table entry: a stable radix sort of ``rep_ent[:R]``
+    -> (ent_ptr (K+1) int32, ent_rep (R) int32 replica ids, items (n, 2)
+    int32 work list of (entry, first replica offset) chunks of ``chunk``,
+    n)."""
+    st = _lib.stream(device)
+    keys = rep_ent[:max(R, 1)].to(torch.int32).contiguous()
+    vals = torch.arange(max(R, 1), dtype=torch.int32, device=device)
+    keys_o = torch.empty_like(keys)
+    vals_o = torch.empty_like(vals)
+    if R > 0:
+        _lib.run_with_workspace("pg_sort_pairs_u32", device, _lib.ptr(keys), _lib.ptr(keys_o), _lib.ptr(vals),
+                                _lib.ptr(vals_o), R, max(1, (K - 1).bit_length()), st)
+    ent_ptr = torch.empty(K + 1, dtype=torch.int32, device=device)
+    _lib.call("pg_csr_from_sorted_u32", _lib.ptr(keys_o), R, K, _lib.ptr(ent_ptr), st)
+    n_e = (ent_ptr[1:] - ent_ptr[:-1]).to(torch.int64)
+    chunks = (n_e + chunk - 1) // chunk
+    total = int(chunks.sum().item())
+    if total:
+        ent = torch.repeat_interleave(torch.arange(K, device=device), chunks)
+        first = torch.cumsum(chunks, 0) - chunks
+        off = (torch.arange(total, device=device) - first[ent]) * chunk
+        items = torch.stack([ent, off], 1).to(torch.int32).contiguous()
+    else:
+        items = torch.zeros(1, 2, dtype=torch.int32, device=device)
+    return ent_ptr, vals_o, items, total
 
 
 class GridProgram:
@@ -309,12 +356,25 @@ class GridProgram:
         arr = (_lib.Frame * self.F)(*frames)
         raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
         self.frames = raw.to(device)
-        tiles_per = (G + self.tile_cells - 1) // self.tile_cells
+        self.G = int(G)
+        self.set_shape(tile_cells, n_blocks)
+
+    def set_shape(self, tile_cells: int, n_blocks: int | None = None) -> None:
+        """Launch shape: cells per tile (one of ``_lib.tile_options(C)``) and
+        the persistent grid size (None: one workgroup per tile up to 12 per
+        CU).  The shape never changes the result bits."""
+        if int(tile_cells) not in _lib.tile_options(self.C):
+            raise ValueError(f"tile_cells {tile_cells} not compiled for C={self.C}: {_lib.tile_options(self.C)}")
+        self.tile_cells = int(tile_cells)
+        tiles_per = (self.G + self.tile_cells - 1) // self.tile_cells
         self.n_tiles = tiles_per * self.F
-        self.tile_ptr = (torch.arange(self.F + 1, dtype=torch.int64) * tiles_per).to(torch.int32).to(device)
+        if getattr(self, "tile_ptr", None) is not None:
+            # a graph captured earlier may still name the old map: keep it alive
+            self._old_maps = getattr(self, "_old_maps", []) + [self.tile_ptr]
+        self.tile_ptr = (torch.arange(self.F + 1, dtype=torch.int64) * tiles_per).to(torch.int32).to(self.device)
         if n_blocks is None:
             n_blocks = min(self.n_tiles, 256 * 12)
-        self.n_blocks = int(n_blocks)
+        self.n_blocks = max(1, int(n_blocks))
 
     def launch(self, entry: str = "pg_grid_fp32", timer: "KernelTimer | None" = None) -> None:
         e0 = timer.start() if timer is not None else None
@@ -425,7 +485,9 @@ def _create_plan(shape, coords, oversamp, kernel_width, kernel_shape, time_map, 
               _lib.ptr(density), _lib.ptr(unit_count), max(nper, 1), st)
     del vals_s, gauss_sorted, gauss
     cell_unit, unit_cell, unit_rep = compile_units(cell_ptr, S, G, dev, UNIT_CHUNK, unit_count)
-    unit_hot, hot = hot_lists(cell_unit, S, G, dev)
+    min_units = HOT_MIN_UNITS
+    unit_hot, hot = hot_lists(cell_unit, S, G, dev, min_units)
+    cell_vrow = ordinary_rows(cell_ptr, cell_unit, S, G, min_units)
     n_inv = n_invalid.cpu().tolist()
 
     if time_map is not None:
@@ -452,7 +514,8 @@ def _create_plan(shape, coords, oversamp, kernel_width, kernel_shape, time_map, 
     plan._compiled = SimpleNamespace(
         device=dev, S=S, npts=npts, kw=kw, nper=nper, G=G, ndim=ndim, spatial=spatial,
         cell_ptr=cell_ptr, rep_src=rep_src, rep_s=rep_s, rep_ent=None, cell_unit=cell_unit, unit_cell=unit_cell,
-        unit_rep=unit_rep, unit_hot=unit_hot, hot=hot, density=density, n_invalid=n_invalid, n_inv=n_inv,
+        unit_rep=unit_rep, unit_hot=unit_hot, hot=hot, cell_vrow=cell_vrow, density=density, n_invalid=n_invalid,
+        n_inv=n_inv,
         n_valid=[nper - v for v in n_inv], target_flat=target_idx, distances=distances, inv_perm_flat=inv_perm,
         interp_idx=None, entry_groups=None,
     )
@@ -527,9 +590,17 @@ class GrogInterpolator:
         comp = self.plan._compiled
         if comp.distances is None:
             raise RuntimeError("calc_interp_table: shifts were already released by a previous call")
-        dev = comp.device
-        res = build_table(train_data, self.plan.radius, precision, lamda, quant=quant, device=dev,
+        res = build_table(train_data, self.plan.radius, precision, lamda, quant=quant, device=comp.device,
                           axis_order=axis_order, keep_fp64=keep_fp64)
+        self._attach_table(res, precision, quant)
+
+    def _attach_table(self, res, precision: int = 1, quant: int | None = None, packed: torch.Tensor | None = None):
+        """Quantise every replica's shift against the fitted table ``res``
+        and keep the table (``packed``: an already packed copy to share)."""
+        comp = self.plan._compiled
+        if comp.distances is None:
+            raise RuntimeError("calc_interp_table: shifts were already released by a previous call")
+        dev = comp.device
         pfac, stepsize, nsteps = quant_steps(self.plan.radius, precision, quant)
         if res.table.shape[0] != nsteps ** comp.ndim:
             raise RuntimeError("table size does not match the quantisation")
@@ -549,8 +620,9 @@ class GrogInterpolator:
         comp.table64 = res.table64
         comp.table_host_steps = res.host_steps
         if C <= 64:
-            packed = torch.empty(K * ((C + 1) // 2) * C * 4, dtype=torch.float32, device=dev)
-            _lib.call("pg_table_pack_c64", _lib.ptr(res.table), K, C, _lib.ptr(packed), _lib.stream(dev))
+            if packed is None:
+                packed = torch.empty(K * ((C + 1) // 2) * C * 4, dtype=torch.float32, device=dev)
+                _lib.call("pg_table_pack_c64", _lib.ptr(res.table), K, C, _lib.ptr(packed), _lib.stream(dev))
             comp.table_packed = packed
         else:
             comp.table_packed = None
@@ -621,6 +693,8 @@ class GrogInterpolator:
                 f.hot_ptr = hot["hot_ptr"].data_ptr()
                 f.n_hot = hot["n_hot"]
                 f.n_hot_units = hot["n_hot_units"]
+                f.cell_row = comp.cell_ptr[s].data_ptr()
+                f.cell_vrow = comp.cell_vrow[s].data_ptr()
                 f.n_src = N
                 f.n_cell = G
                 frames.append(f)
@@ -647,31 +721,11 @@ class GrogInterpolator:
         if comp.entry_groups is not None:
             return comp.entry_groups
         dev = comp.device
-        st = _lib.stream(dev)
         K = int(comp.table.shape[0])
         groups = []
         for s in range(comp.S):
             R = comp.n_valid[s]
-            keys = comp.rep_ent[s, :max(R, 1)].contiguous()
-            vals = torch.arange(max(R, 1), dtype=torch.int32, device=dev)
-            keys_o = torch.empty_like(keys)
-            vals_o = torch.empty_like(vals)
-            if R > 0:
-                _lib.run_with_workspace("pg_sort_pairs_u32", dev, _lib.ptr(keys), _lib.ptr(keys_o), _lib.ptr(vals),
-                                        _lib.ptr(vals_o), R, max(1, (K - 1).bit_length()), st)
-            ent_ptr = torch.empty(K + 1, dtype=torch.int32, device=dev)
-            _lib.call("pg_csr_from_sorted_u32", _lib.ptr(keys_o), R, K, _lib.ptr(ent_ptr), st)
-            n_e = (ent_ptr[1:] - ent_ptr[:-1]).to(torch.int64)
-            chunks = (n_e + self.GROUP_CHUNK - 1) // self.GROUP_CHUNK
-            total = int(chunks.sum().item())
-            if total:
-                ent = torch.repeat_interleave(torch.arange(K, device=dev), chunks)
-                first = torch.cumsum(chunks, 0) - chunks
-                off = (torch.arange(total, device=dev) - first[ent]) * self.GROUP_CHUNK
-                items = torch.stack([ent, off], 1).to(torch.int32).contiguous()
-            else:
-                items = torch.zeros(1, 2, dtype=torch.int32, device=dev)
-            groups.append((ent_ptr, vals_o, items, total))
+            groups.append(group_by_entry(comp.rep_ent[s, :max(R, 1)], R, K, dev, self.GROUP_CHUNK))
         comp.entry_groups = groups
         return groups
 

@@ -15,39 +15,27 @@
 //   grid[c]   = ((0 + p_{u0}) + p_{u0+1}) + ...        over the cell's units
 // so results are bitwise reproducible run to run and alone vs batched.
 //
-// Work decomposition: a workgroup (256 threads) owns a tile of TC cells.  Its
-// lanes form groups of GS >= C lanes (lane = output coil); each group folds
-// one unit at a time, the unit partials are staged in LDS, folded per cell
-// into an LDS tile accumulator and the tile is written coil-row by coil-row
-// (coalesced, zeros included, no read-modify-write, no atomics).  The tile
-// loop is persistent (grid-stride over tiles).  Cells with many replicas (the
-// radial DC and its neighbours) are reduced by a two-kernel pre-pass -- hot
-// units in parallel, then an in-order fold per hot cell -- whose sums the
-// tile kernel copies, so no workgroup serialises on a hot cell.
+// Work decomposition: a workgroup (256 threads) owns a tile of TC consecutive
+// cells (TC picked per launch among the compiled shapes); replica values
+// are staged in LDS rows and folded per cell in registers; the tile is
+// written coil-row by coil-row (coalesced, zeros included, no
+// read-modify-write, no atomics).  The tile loop is persistent (grid-stride
+// over tiles).  Cells with many replicas (the radial DC and its neighbours)
+// are reduced by a two-kernel pre-pass -- hot units in parallel, then an
+// in-order fold per hot cell -- whose sums the tile kernel copies, so no
+// workgroup serialises on a hot cell.
 #include "common.h"
+
+extern "C" int pg_grid_tile_options(int64_t C, int32_t *opts, int32_t max_opts);
 
 namespace pg {
 
 constexpr int kGridThreads = 256;
-constexpr int kLdsAcc = 2048;   // GS * TC complex accumulators (16 KiB)
-constexpr int kLdsPart = 1024;  // GS * NB complex unit partials (8 KiB)
-constexpr int kLdsRows = 2048;  // GS * RB staged V rows, MODE 1 (16 KiB)
 
-template <int GS> struct TileCfg {
-  static constexpr int TC = kLdsAcc / GS;  // cells per tile
-  static constexpr int NB = kLdsPart / GS; // units per batch
-  static constexpr int RB = kLdsRows / GS; // staged rows per batch (MODE 1)
-};
-
-__device__ __forceinline__ int find_frame(const uint32_t *__restrict__ tp,
-                                          int F, uint32_t L) {
-  int lo = 0, hi = F - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (tp[mid] <= L) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
+// Tile sizes compiled per coil group GS (threads per cell = 256 / TC, each
+// thread owning GS * TC / 256 <= 16 coils); the first is the default.
+__host__ __device__ constexpr int default_tile(int GS) {
+  return GS <= 8 ? 256 : GS == 16 ? 128 : GS == 32 ? 64 : 32;
 }
 
 // MODE 0: v = (K[e] y_t[src, :]) * s * s     (plain FP32 GROG apply; y is
@@ -71,82 +59,88 @@ __device__ __forceinline__ Rec load_rec(const pg_frame &fr, uint32_t r) {
   return q;
 }
 
-template <int GS, int MODE>
-__device__ __forceinline__ float2 replica_value(const pg_frame &fr, uint32_t r,
-                                                const Rec &q, int lane, int C,
-                                                int P, const float2 *rows,
-                                                uint32_t rbase) {
+// K[e][co][:] . y_row, then (v * s) * s.  tb points at the packed table
+// pairs of (entry, coil) (stride C float4), yr at the replica's sample row
+// (global memory or an LDS copy; 16-byte aligned when C is even).  The
+// products are spelled out as fmaf chains, so every caller gets the same bits.
+template <int GS>
+__device__ __forceinline__ float2 apply_row(const float4 *__restrict__ tb,
+                                           const float2 *yr, int C, int P,
+                                           float s) {
   float vx = 0.0f, vy = 0.0f;
-  if (MODE == 1) {
-    if (lane < C) {
-      if (rows != nullptr) return rows[(r - rbase) * GS + lane];
-      return reinterpret_cast<const float2 *>(fr.vbuf)[(int64_t)r * C + lane];
-    }
-    return make_float2(0.0f, 0.0f);
-  }
-  if (MODE == 2) {
-    if (lane < C) {
-      const float2 a =
-          reinterpret_cast<const float2 *>(fr.y)[(int64_t)lane * fr.n_src + q.src];
-      vx = a.x * q.s;
-      vy = a.y * q.s;
-    }
-    return make_float2(vx, vy);
-  }
-  const float4 *tb = reinterpret_cast<const float4 *>(fr.table) +
-                     (int64_t)q.ent * P * C + lane;
-  const float2 *yr = reinterpret_cast<const float2 *>(fr.y) + (int64_t)q.src * C;
-  if (lane < C) {
-    if ((C & 1) == 0) {
-      // row of C complex = P float4 pairs (16-byte aligned for even C)
-      const float4 *y4 = reinterpret_cast<const float4 *>(yr);
+  if ((C & 1) == 0) {
+    const float4 *y4 = reinterpret_cast<const float4 *>(yr);
 #pragma unroll
-      for (int p = 0; p < (GS + 1) / 2; ++p) {
-        if (p < P) {
-          const float4 k = tb[p * C];
-          const float4 yy = y4[p];
-          vx += k.x * yy.x - k.y * yy.y;
-          vy += k.x * yy.y + k.y * yy.x;
-          vx += k.z * yy.z - k.w * yy.w;
-          vy += k.z * yy.w + k.w * yy.z;
-        }
+    for (int p = 0; p < (GS + 1) / 2; ++p) {
+      if (p < P) {
+        const float4 k = tb[p * C];
+        const float4 yy = y4[p];
+        vx = fmaf(k.x, yy.x, vx);
+        vx = fmaf(-k.y, yy.y, vx);
+        vy = fmaf(k.x, yy.y, vy);
+        vy = fmaf(k.y, yy.x, vy);
+        vx = fmaf(k.z, yy.z, vx);
+        vx = fmaf(-k.w, yy.w, vx);
+        vy = fmaf(k.z, yy.w, vy);
+        vy = fmaf(k.w, yy.z, vy);
       }
-    } else {
+    }
+  } else {
 #pragma unroll
-      for (int p = 0; p < (GS + 1) / 2; ++p) {
-        if (p < P) {
-          const float4 k = tb[p * C];
-          const float2 y0 = yr[2 * p];
-          vx += k.x * y0.x - k.y * y0.y;
-          vy += k.x * y0.y + k.y * y0.x;
-          if (2 * p + 1 < C) {
-            const float2 y1 = yr[2 * p + 1];
-            vx += k.z * y1.x - k.w * y1.y;
-            vy += k.z * y1.y + k.w * y1.x;
-          }
+    for (int p = 0; p < (GS + 1) / 2; ++p) {
+      if (p < P) {
+        const float4 k = tb[p * C];
+        const float2 y0 = yr[2 * p];
+        vx = fmaf(k.x, y0.x, vx);
+        vx = fmaf(-k.y, y0.y, vx);
+        vy = fmaf(k.x, y0.y, vy);
+        vy = fmaf(k.y, y0.x, vy);
+        if (2 * p + 1 < C) {
+          const float2 y1 = yr[2 * p + 1];
+          vx = fmaf(k.z, y1.x, vx);
+          vx = fmaf(-k.w, y1.y, vx);
+          vy = fmaf(k.z, y1.y, vy);
+          vy = fmaf(k.w, y1.x, vy);
         }
       }
     }
   }
-  vx = vx * q.s;
-  vy = vy * q.s;
-  return make_float2(vx * q.s, vy * q.s);
+  vx = vx * s;
+  vy = vy * s;
+  return make_float2(vx * s, vy * s);
 }
 
-// Canonical fold of one unit: ((0 + v_r0) + v_r0+1) + ... ; the next
-// replica's record is fetched while the current one is applied.
+// One output coil of one replica's value (before the fold).  The products
+// are spelled out as fmaf chains, so every kernel that evaluates a replica
+// (tile kernel, hot pre-pass, multi-unit path) produces the same bits.
+template <int GS, int MODE>
+__device__ __forceinline__ float2 coil_value(const pg_frame &fr, uint32_t r,
+                                             const Rec &q, int co, int C,
+                                             int P) {
+  if (MODE == 1)
+    return reinterpret_cast<const float2 *>(fr.vbuf)[(int64_t)r * C + co];
+  if (MODE == 2) {
+    const float2 a =
+        reinterpret_cast<const float2 *>(fr.y)[(int64_t)co * fr.n_src + q.src];
+    return make_float2(a.x * q.s, a.y * q.s);
+  }
+  const float4 *tb = reinterpret_cast<const float4 *>(fr.table) +
+                     (int64_t)q.ent * P * C + co;
+  const float2 *yr = reinterpret_cast<const float2 *>(fr.y) + (int64_t)q.src * C;
+  return apply_row<GS>(tb, yr, C, P, q.s);
+}
+
+// Canonical fold of one unit for output coil co: ((0 + v_r0) + v_r0+1) + ...;
+// the next replica's record is fetched while the current one is applied.
 template <int GS, int MODE>
 __device__ __forceinline__ float2 unit_partial(const pg_frame &fr, uint32_t u,
-                                               int lane, int C, int P,
-                                               const float2 *rows = nullptr,
-                                               uint32_t rbase = 0) {
+                                               int co, int C, int P) {
   const uint32_t r0 = fr.unit_rep[u], r1 = fr.unit_rep[u + 1];
   float px = 0.0f, py = 0.0f;
-  if (r0 >= r1) return make_float2(px, py);
-  if (MODE == 1 && rows == nullptr) {
+  if (r0 >= r1 || co >= C) return make_float2(px, py);
+  if (MODE == 1) {
     // contiguous rows: issue 8 loads, then add them in order
-    const float2 *vb = reinterpret_cast<const float2 *>(fr.vbuf) + lane;
-    if (lane >= C) return make_float2(px, py);
+    const float2 *vb = reinterpret_cast<const float2 *>(fr.vbuf) + co;
     for (uint32_t rb = r0; rb < r1; rb += 8) {
       float2 v[8];
 #pragma unroll
@@ -166,7 +160,7 @@ __device__ __forceinline__ float2 unit_partial(const pg_frame &fr, uint32_t u,
   for (uint32_t r = r0; r < r1; ++r) {
     Rec nq = q;
     if (r + 1 < r1) nq = load_rec<MODE>(fr, r + 1);
-    const float2 v = replica_value<GS, MODE>(fr, r, q, lane, C, P, rows, rbase);
+    const float2 v = coil_value<GS, MODE>(fr, r, q, co, C, P);
     px = px + v.x;
     py = py + v.y;
     q = nq;
@@ -174,102 +168,245 @@ __device__ __forceinline__ float2 unit_partial(const pg_frame &fr, uint32_t u,
   return make_float2(px, py);
 }
 
+// Tile kernel.  A workgroup owns TC consecutive cells of one frame.  The
+// rows (replicas) of its ordinary cells -- not hot, one unit -- are numbered
+// 0 .. NR-1 in cell order ("virtual rows": the rows of hot cells are left
+// out, so a tile holding the k-space centre does no more work than any
+// other) and processed in batches of RB rows:
+//   phase A  cell offsets and tags into LDS (one coalesced load of
+//            cell_unit, then unit_rep / unit_hot), then a block scan of the
+//            ordinary cells' row counts;
+//   B1       each batch row's replica index, and for MODE 0/2 its record
+//            (source, entry, weight), into LDS: one round of loads;
+//   B2       MODE 1: the V rows, MODE 2: the scaled source values, MODE 0:
+//            the source sample rows, into LDS rows: one round of loads;
+//   B3       MODE 0: groups of GS lanes (lane = output coil) apply the table
+//            row of each replica to its staged sample row, in place;
+//   C        each thread owns one cell and CPT coils and adds its cell's
+//            rows of the batch in order, in registers.
+// The tile is then written coil-major, consecutive threads on consecutive
+// cells (zeros included, no read-modify-write, no atomics).  Hot cells take
+// their sum from the pre-pass; cells with several units that are not hot
+// (only when the pre-pass threshold is raised) fold their units in canonical
+// order straight from memory.  The tile loop is persistent.
+constexpr int kRowBytes = 24576;
+
+// Cell with several units that is not reduced by the hot pre-pass: fold its
+// unit partials for one coil in canonical order, straight from memory.
 template <int GS, int MODE>
-__global__ __launch_bounds__(kGridThreads) void k_grid(
+__device__ __forceinline__ float2 multi_unit_coil(const pg_frame *__restrict__ frames,
+                                               int f, uint32_t ua, uint32_t ub,
+                                               int co, int C) {
+  const pg_frame fr = frames[f];
+  const int P = (C + 1) / 2;
+  float cx = 0.0f, cy = 0.0f;
+  for (uint32_t u = ua; u < ub; ++u) {
+    const float2 p = unit_partial<GS, MODE>(fr, u, co, C, P);
+    cx = cx + p.x;
+    cy = cy + p.y;
+  }
+  return make_float2(cx, cy);
+}
+
+// Largest t in [0, ncell) with vr[t] <= k (vr non-decreasing, k < vr[ncell]).
+__device__ __forceinline__ int find_row_cell(const uint32_t *vr, int ncell,
+                                             uint32_t k) {
+  int lo = 0, hi = ncell - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (vr[mid] <= k) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int MODE, int GS> struct TileRows {
+  static constexpr int RS = MODE == 0 ? GS + 2 : GS + 1;  // row stride (float2)
+  static constexpr int REC = MODE == 1 ? 4 : 12;          // record bytes / row
+  static constexpr int RB = kRowBytes / (RS * 8 + REC);   // rows per batch
+};
+
+template <int GS, int MODE, int TC>
+__global__ __launch_bounds__(kGridThreads) void k_tile(
     const pg_frame *__restrict__ frames, const uint32_t *__restrict__ tile_ptr,
     int F, int C, uint32_t n_tiles) {
-  constexpr int TC = TileCfg<GS>::TC;
-  constexpr int NB = TileCfg<GS>::NB;
-  constexpr int RB = TileCfg<GS>::RB;
-  constexpr int NG = kGridThreads / GS;
-  __shared__ float2 acc[GS * TC];
-  __shared__ float2 part[NB * GS];
-  // MODE 1 stages the batch's V rows (contiguous: units are cell-sorted)
-  // with one coalesced cooperative load, for memory-level parallelism.
-  __shared__ float2 rows_lds[MODE == 1 ? RB * GS : 1];
-  const int lane = threadIdx.x % GS;
-  const int grp = threadIdx.x / GS;
-  const int P = (C + 1) / 2;
+  constexpr int RS = TileRows<MODE, GS>::RS;
+  constexpr int RB = TileRows<MODE, GS>::RB;
+  constexpr int TPC = kGridThreads / TC;  // threads per cell
+  constexpr int CPT = GS / TPC;           // coils per thread
+  constexpr int NG = kGridThreads / GS;   // apply groups
+  static_assert(TPC * TC == kGridThreads && CPT >= 1 && CPT <= 16, "tile shape");
+  __shared__ __align__(16) float2 vrow[RB * RS];
+  __shared__ uint32_t rmap[RB];                    // MODE 1: replica of row
+  __shared__ uint32_t rsrc[MODE == 1 ? 1 : RB];    // MODE 0/2: record
+  __shared__ uint32_t rent[MODE == 0 ? RB : 1];
+  __shared__ float rsw[MODE == 1 ? 1 : RB];
+  __shared__ uint32_t crow[TC + 1];
+  __shared__ uint32_t vr[TC + 1];
+  __shared__ int tag[TC];
+  __shared__ uint32_t cu_lo[TC], cu_hi[TC];
 
-  // Round-robin tile order: neighbouring tiles (the dense k-space centre
-  // spans many consecutive tiles) land on different workgroups and XCDs.
+  const int tid = threadIdx.x;
+  const int lane = tid % GS;
+  const int grp = tid / GS;
+  const int P = (C + 1) / 2;
+  const int t_own = tid % TC;
+  const int co0 = (tid / TC) * CPT;
+  const uint32_t tp1 = tile_ptr[1];
+
   for (uint32_t L = blockIdx.x; L < n_tiles; L += gridDim.x) {
-    const int f = find_frame(tile_ptr, F, L);
-    const pg_frame fr = frames[f];
-    const int64_t cell0 = (int64_t)(L - tile_ptr[f]) * TC;
+    // frame of tile L: the first guess is exact when every frame has tp1
+    // tiles, so the frame record and the check load together
+    int f = tp1 > 0 ? (int)(L / tp1) : 0;
+    if (f > F - 1) f = F - 1;
+    pg_frame fr = frames[f];
+    uint32_t t0 = tile_ptr[f];
+    if (t0 > L || tile_ptr[f + 1] <= L) {
+      while (f > 0 && tile_ptr[f] > L) --f;
+      while (f + 1 < F && tile_ptr[f + 1] <= L) ++f;
+      fr = frames[f];
+      t0 = tile_ptr[f];
+    }
+    const int64_t cell0 = (int64_t)(L - t0) * TC;
     const int64_t rem = fr.n_cell - cell0;
     const int ncell = rem < TC ? (int)rem : TC;
-    const uint32_t u0 = fr.cell_unit[cell0];
-    const uint32_t u1 = fr.cell_unit[cell0 + ncell];
-    const bool has_hot = fr.n_hot > 0;
 
-    for (int i = threadIdx.x; i < GS * TC; i += kGridThreads)
-      acc[i] = make_float2(0.0f, 0.0f);
+    // phase A: one round of coalesced loads (CSR row starts, ordinary-row
+    // prefix); cells with rows but no ordinary rows are hot or multi-unit
+    for (int t = tid; t <= ncell; t += kGridThreads) {
+      crow[t] = fr.cell_row[cell0 + t];
+      vr[t] = fr.cell_vrow[cell0 + t];
+    }
     __syncthreads();
+    for (int t = tid; t < ncell; t += kGridThreads) {
+      int tg = -1;
+      if (vr[t + 1] == vr[t] && crow[t + 1] > crow[t]) {
+        const uint32_t a = fr.cell_unit[cell0 + t];
+        const uint32_t b = fr.cell_unit[cell0 + t + 1];
+        const int h = fr.n_hot > 0 ? fr.unit_hot[a] : -1;
+        tg = h >= 0 ? h : -2;
+        cu_lo[t] = a;
+        cu_hi[t] = b;
+      }
+      tag[t] = tg;
+    }
+    __syncthreads();
+    const uint32_t V0 = vr[0], NR = vr[ncell] - V0;
 
-    for (uint32_t ub = u0; ub < u1; ub += NB) {
-      const int nbat = (u1 - ub) < (uint32_t)NB ? (int)(u1 - ub) : NB;
-      const float2 *rows = nullptr;
-      uint32_t rbase = 0;
-      if (MODE == 1) {
-        rbase = fr.unit_rep[ub];
-        const uint32_t nrows = fr.unit_rep[ub + nbat] - rbase;
-        if (nrows <= (uint32_t)RB) {
-          const float2 *vb = reinterpret_cast<const float2 *>(fr.vbuf) +
-                             (int64_t)rbase * C;
-          for (int i = threadIdx.x; i < (int)nrows * C; i += kGridThreads) {
-            const int row = i / C, co = i - row * C;
-            rows_lds[row * GS + co] = vb[i];
+    const bool own = t_own < ncell;
+    const uint32_t my_lo = own ? vr[t_own] - V0 : 0u;
+    const uint32_t my_hi = own ? vr[t_own + 1] - V0 : 0u;
+    float2 acc[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) acc[k] = make_float2(0.0f, 0.0f);
+
+    for (uint32_t k0 = 0; k0 < NR; k0 += RB) {
+      const int nk = (NR - k0) < (uint32_t)RB ? (int)(NR - k0) : RB;
+      // B1: row -> replica (and its record)
+      for (int j = tid; j < nk; j += kGridThreads) {
+        const uint32_t k = V0 + k0 + j;
+        const int t = find_row_cell(vr, ncell, k);
+        const uint32_t r = crow[t] + (k - vr[t]);
+        if (MODE == 1) {
+          rmap[j] = r;
+        } else {
+          rsrc[j] = fr.rep_src[r];
+          rsw[j] = fr.rep_s[r];
+          if (MODE == 0) rent[j] = fr.rep_ent[r];
+        }
+      }
+      __syncthreads();
+      // B2: stage the rows
+      if (MODE != 2 && (C & 1) == 0) {
+        // rows of C complex as C/2 float4 (16-byte aligned for even C)
+        const int H = C >> 1;
+        for (int i = tid; i < nk * H; i += kGridThreads) {
+          const int j = i / H;
+          const int c2 = i - j * H;
+          const int64_t row = MODE == 1 ? (int64_t)rmap[j] : (int64_t)rsrc[j];
+          const float4 v = reinterpret_cast<const float4 *>(MODE == 1 ? fr.vbuf : fr.y)[row * H + c2];
+          float2 *d = &vrow[j * RS + 2 * c2];
+          d[0] = make_float2(v.x, v.y);
+          d[1] = make_float2(v.z, v.w);
+        }
+      } else {
+        for (int i = tid; i < nk * C; i += kGridThreads) {
+          const int j = i / C;
+          const int c = i - j * C;
+          float2 v;
+          if (MODE == 1) {
+            v = reinterpret_cast<const float2 *>(fr.vbuf)[(int64_t)rmap[j] * C + c];
+          } else if (MODE == 2) {
+            const float2 a = reinterpret_cast<const float2 *>(fr.y)[(int64_t)c * fr.n_src + rsrc[j]];
+            v = make_float2(a.x * rsw[j], a.y * rsw[j]);
+          } else {
+            v = reinterpret_cast<const float2 *>(fr.y)[(int64_t)rsrc[j] * C + c];
           }
-          rows = rows_lds;
+          vrow[j * RS + c] = v;
+        }
+      }
+      __syncthreads();
+      // B3 (MODE 0): apply the table in place, two rows per group per round
+      if (MODE == 0) {
+        const int rounds = (nk + 2 * NG - 1) / (2 * NG);
+        for (int rd = 0; rd < rounds; ++rd) {
+          const int j1 = rd * 2 * NG + grp, j2 = j1 + NG;
+          float2 v1 = make_float2(0.0f, 0.0f), v2 = v1;
+          if (lane < C) {
+            const float4 *tb = reinterpret_cast<const float4 *>(fr.table) + lane;
+            if (j1 < nk) v1 = apply_row<GS>(tb + (int64_t)rent[j1] * P * C, &vrow[j1 * RS], C, P, rsw[j1]);
+            if (j2 < nk) v2 = apply_row<GS>(tb + (int64_t)rent[j2] * P * C, &vrow[j2 * RS], C, P, rsw[j2]);
+          }
+          __syncthreads();
+          if (lane < C) {
+            if (j1 < nk) vrow[j1 * RS + lane] = v1;
+            if (j2 < nk) vrow[j2 * RS + lane] = v2;
+          }
           __syncthreads();
         }
       }
-      for (int j = grp; j < nbat; j += NG) {
-        const uint32_t u = ub + j;
-        float2 pv = make_float2(0.0f, 0.0f);
-        if (!has_hot || fr.unit_hot[u] < 0)
-          pv = unit_partial<GS, MODE>(fr, u, lane, C, P, rows, rbase);
-        part[j * GS + lane] = pv;
-      }
-      __syncthreads();
-      // Fold unit partials into the cell accumulators, units of a cell in
-      // order (one thread per (run head, coil)); a hot cell takes its sum
-      // from the pre-pass once, at the batch holding its first unit.
-      for (int i = threadIdx.x; i < nbat * GS; i += kGridThreads) {
-        const int j = i / GS;
-        const int co = i % GS;
-        if (co >= C) continue;
-        const uint32_t cell = fr.unit_cell[ub + j];
-        if (j > 0 && fr.unit_cell[ub + j - 1] == cell) continue;
-        const int t = (int)((int64_t)cell - cell0);
-        const int h = has_hot ? fr.unit_hot[ub + j] : -1;
-        if (h >= 0) {
-          if (ub + j == fr.cell_unit[cell]) {
-            const float2 pc =
-                reinterpret_cast<const float2 *>(fr.p_cell)[(int64_t)h * C + co];
-            float2 a = acc[co * TC + t];
-            a.x = a.x + pc.x;
-            a.y = a.y + pc.y;
-            acc[co * TC + t] = a;
+      // C: fold this batch's rows of the owned cell
+      {
+        const uint32_t lo = my_lo > k0 ? my_lo : k0;
+        const uint32_t hi = my_hi < k0 + nk ? my_hi : k0 + nk;
+        for (uint32_t k = lo; k < hi; ++k) {
+          const float2 *vp = &vrow[(k - k0) * RS + co0];
+#pragma unroll
+          for (int c = 0; c < CPT; ++c) {
+            const float2 v = vp[c];
+            acc[c].x = acc[c].x + v.x;
+            acc[c].y = acc[c].y + v.y;
           }
-          continue;
         }
-        float2 a = acc[co * TC + t];
-        for (int jj = j; jj < nbat && fr.unit_cell[ub + jj] == cell; ++jj) {
-          const float2 p = part[jj * GS + co];
-          a.x = a.x + p.x;
-          a.y = a.y + p.y;
-        }
-        acc[co * TC + t] = a;
       }
       __syncthreads();
     }
-    float2 *grid = reinterpret_cast<float2 *>(fr.grid);
-    for (int i = threadIdx.x; i < C * TC; i += kGridThreads) {
-      const int co = i / TC;
-      const int t = i - co * TC;
-      if (t < ncell) grid[(int64_t)co * fr.n_cell + cell0 + t] = acc[co * TC + t];
+
+    if (own) {
+      const int tg = tag[t_own];
+      float2 val[CPT];
+      if (tg == -1) {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+          val[k] = make_float2(0.0f + acc[k].x, 0.0f + acc[k].y);
+      } else if (tg >= 0) {
+        const float2 *pc = reinterpret_cast<const float2 *>(fr.p_cell) + (int64_t)tg * C;
+#pragma unroll
+        for (int k = 0; k < CPT; ++k) {
+          const float2 v = (co0 + k < C) ? pc[co0 + k] : make_float2(0.0f, 0.0f);
+          val[k] = make_float2(0.0f + v.x, 0.0f + v.y);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < CPT; ++k)
+          val[k] = (co0 + k < C)
+                       ? multi_unit_coil<GS, MODE>(frames, f, cu_lo[t_own], cu_hi[t_own], co0 + k, C)
+                       : make_float2(0.0f, 0.0f);
+      }
+      float2 *g = reinterpret_cast<float2 *>(fr.grid) + cell0 + t_own;
+#pragma unroll
+      for (int k = 0; k < CPT; ++k)
+        if (co0 + k < C) g[(int64_t)(co0 + k) * fr.n_cell] = val[k];
     }
     __syncthreads();
   }
@@ -355,6 +492,31 @@ __global__ void k_transpose_cn(const float2 *__restrict__ y, int64_t C,
   }
 }
 
+// Frames of uneven length back to back: frame f holds (C, N_f) coil-major
+// at sample offset off[f] (element offset C * off[f]); its (N_f, C)
+// sample-major rows go to the same offset of yt.  blockIdx.z = frame.
+__global__ void k_transpose_frames(const float2 *__restrict__ y,
+                                   const int64_t *__restrict__ off, int64_t C,
+                                   float2 *__restrict__ yt) {
+  __shared__ float2 tile[32][33];
+  const int64_t base = off[blockIdx.z];
+  const int64_t N = off[blockIdx.z + 1] - base;
+  const int64_t n0 = (int64_t)blockIdx.x * 32;
+  if (n0 >= N) return;
+  const int64_t c0 = (int64_t)blockIdx.y * 32;
+  const float2 *yf = y + C * base;
+  float2 *tf = yt + C * base;
+  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+    const int64_t c = c0 + k, n = n0 + threadIdx.x;
+    if (c < C && n < N) tile[k][threadIdx.x] = yf[c * N + n];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
+    const int64_t n = n0 + k, c = c0 + threadIdx.x;
+    if (c < C && n < N) tf[n * C + c] = tile[threadIdx.x][k];
+  }
+}
+
 // out[b, co, m] = K[idx[m]][co] . y[b, :, m / kw]
 __global__ void k_sparse_apply(const float2 *__restrict__ y,
                                const int64_t *__restrict__ idx,
@@ -427,17 +589,29 @@ static int launch_grid(const pg_frame *frames, const uint32_t *tile_ptr,
   PG_CHECK(F < 65536, "grid: too many frames in one launch");
   PG_CHECK(n_tiles >= 0 && n_tiles < (int64_t)UINT32_MAX, "grid: bad tiles");
   PG_CHECK(max_hot_units >= 0 && max_hot >= 0, "grid: bad hot counts");
+  {
+    int32_t opts[3];
+    const int n = pg_grid_tile_options(C, opts, 3);
+    bool ok = false;
+    for (int i = 0; i < n; ++i) ok |= (opts[i] == tile_cells);
+    PG_CHECK(ok, "grid: tile_cells %lld is not compiled for C=%lld",
+             (long long)tile_cells, (long long)C);
+  }
   if (n_tiles == 0) return 0;
   int64_t nb = n_blocks > 0 ? n_blocks : 2048;
   if (nb > n_tiles) nb = n_tiles;
   nb = ((nb + 7) / 8) * 8;
   const hipStream_t st = as_stream(s);
   const dim3 grid((unsigned)nb), block(kGridThreads);
-#define PG_GRID_CASE(GSV)                                                    \
-  if (C <= GSV) {                                                            \
-    PG_CHECK(tile_cells == TileCfg<GSV>::TC,                                 \
-             "grid: tile_cells must be %d for C=%lld", TileCfg<GSV>::TC,     \
-             (long long)C);                                                  \
+#define PG_TILE_LAUNCH(GSV, TCV)                                             \
+  if (tile_cells == TCV) {                                                   \
+    hipLaunchKernelGGL((k_tile<GSV, MODE, TCV>), grid, block, 0, st, frames, \
+                       tile_ptr, (int)F, (int)C, (uint32_t)n_tiles);         \
+    PG_LAUNCHED();                                                           \
+    return 0;                                                                \
+  }
+#define PG_HOT_LAUNCH(GSV)                                                   \
+  {                                                                          \
     constexpr int NG = kGridThreads / GSV;                                   \
     if (max_hot_units > 0) {                                                 \
       hipLaunchKernelGGL((k_hot_units<GSV, MODE>),                           \
@@ -450,17 +624,27 @@ static int launch_grid(const pg_frame *frames, const uint32_t *tile_ptr,
                          block, 0, st, frames, (int)C);                      \
       PG_LAUNCHED();                                                         \
     }                                                                        \
-    hipLaunchKernelGGL((k_grid<GSV, MODE>), grid, block, 0, st, frames,      \
-                       tile_ptr, (int)F, (int)C, (uint32_t)n_tiles);         \
-    PG_LAUNCHED();                                                           \
-    return 0;                                                                \
   }
-  PG_GRID_CASE(4)
-  PG_GRID_CASE(8)
-  PG_GRID_CASE(16)
-  PG_GRID_CASE(32)
-  PG_GRID_CASE(64)
-#undef PG_GRID_CASE
+  if (C <= 4) {
+    PG_HOT_LAUNCH(4)
+    PG_TILE_LAUNCH(4, 256) PG_TILE_LAUNCH(4, 128) PG_TILE_LAUNCH(4, 64)
+  } else if (C <= 8) {
+    PG_HOT_LAUNCH(8)
+    PG_TILE_LAUNCH(8, 256) PG_TILE_LAUNCH(8, 128) PG_TILE_LAUNCH(8, 64)
+  } else if (C <= 16) {
+    PG_HOT_LAUNCH(16)
+    PG_TILE_LAUNCH(16, 256) PG_TILE_LAUNCH(16, 128) PG_TILE_LAUNCH(16, 64)
+  } else if (C <= 32) {
+    PG_HOT_LAUNCH(32)
+    PG_TILE_LAUNCH(32, 128) PG_TILE_LAUNCH(32, 64)
+  } else {
+    PG_HOT_LAUNCH(64)
+    PG_TILE_LAUNCH(64, 64) PG_TILE_LAUNCH(64, 32)
+  }
+  PG_CHECK(false, "grid: tile_cells %lld is not compiled for C=%lld",
+           (long long)tile_cells, (long long)C);
+#undef PG_TILE_LAUNCH
+#undef PG_HOT_LAUNCH
   return 1;
 }
 
@@ -471,12 +655,23 @@ using namespace pg;
 extern "C" {
 
 int pg_grid_tile_cells(int64_t C) {
-  if (C <= 4) return TileCfg<4>::TC;
-  if (C <= 8) return TileCfg<8>::TC;
-  if (C <= 16) return TileCfg<16>::TC;
-  if (C <= 32) return TileCfg<32>::TC;
-  if (C <= 64) return TileCfg<64>::TC;
+  if (C <= 4) return default_tile(4);
+  if (C <= 8) return default_tile(8);
+  if (C <= 16) return default_tile(16);
+  if (C <= 32) return default_tile(32);
+  if (C <= 64) return default_tile(64);
   return 0;
+}
+
+int pg_grid_tile_options(int64_t C, int32_t *opts, int32_t max_opts) {
+  int32_t o[3] = {0, 0, 0};
+  int n = 0;
+  if (C < 1 || C > 64) return 0;
+  if (C <= 16) { o[0] = 256; o[1] = 128; o[2] = 64; n = 3; }
+  else if (C <= 32) { o[0] = 128; o[1] = 64; n = 2; }
+  else { o[0] = 64; o[1] = 32; n = 2; }
+  for (int i = 0; i < n && i < max_opts; ++i) opts[i] = o[i];
+  return n;
 }
 
 int pg_table_pack_c64(const void *table, int64_t K, int64_t C, void *packed,
@@ -525,6 +720,23 @@ int pg_transpose_cn_c64(const void *y, int64_t C, int64_t N, void *y_t,
   hipLaunchKernelGGL(k_transpose_cn, grid, dim3(32, 8), 0, as_stream(stream),
                      static_cast<const float2 *>(y), C, N,
                      static_cast<float2 *>(y_t));
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_transpose_frames_c64(const void *y, const int64_t *frame_off,
+                            int64_t F, int64_t max_n, int64_t C, void *y_t,
+                            pg_stream_t stream) {
+  PG_CHECK(F >= 0 && F < 65536 && C >= 0 && C < 65535 * 32 && max_n >= 0,
+           "transpose_frames: bad sizes (F=%lld, C=%lld)", (long long)F,
+           (long long)C);
+  PG_CHECK(F == 0 || frame_off, "transpose_frames: null offsets");
+  if (F == 0 || C == 0 || max_n == 0) return 0;
+  const dim3 grid((unsigned)ceil_div(max_n, 32), (unsigned)ceil_div(C, 32),
+                  (unsigned)F);
+  hipLaunchKernelGGL(k_transpose_frames, grid, dim3(32, 8), 0,
+                     as_stream(stream), static_cast<const float2 *>(y),
+                     frame_off, C, static_cast<float2 *>(y_t));
   PG_LAUNCHED();
   return 0;
 }

@@ -80,7 +80,7 @@ class _ScatterPlan:
     CSR over cells, chunked reduction units, permutations)."""
 
     def __init__(self, indices, sqrt_w, sort_perm, inv_perm, grid_size: int, S: int, n: int, device):
-        from ..calib._grog import UNIT_CHUNK, compile_units, hot_lists
+        from ..calib._grog import HOT_MIN_UNITS, UNIT_CHUNK, compile_units, hot_lists, ordinary_rows
 
         self.device = device
         self.S, self.n, self.G = S, n, grid_size
@@ -103,7 +103,9 @@ class _ScatterPlan:
         for s in range(S):
             _lib.call("pg_csr_from_sorted", idx[s].data_ptr(), n, grid_size, self.cell_ptr[s].data_ptr(), st)
         self.cell_unit, self.unit_cell, self.unit_rep = compile_units(self.cell_ptr, S, grid_size, device, UNIT_CHUNK)
-        self.unit_hot, self.hot = hot_lists(self.cell_unit, S, grid_size, device)
+        min_units = HOT_MIN_UNITS
+        self.unit_hot, self.hot = hot_lists(self.cell_unit, S, grid_size, device, min_units)
+        self.cell_vrow = ordinary_rows(self.cell_ptr, self.cell_unit, S, grid_size, min_units)
         self.rep_src = self.sort_perm.to(torch.int32).contiguous()
 
     def frames(self, y_ptr: int, grid_ptr: int, B: int, C: int, y_stride_s: int | None = None):
@@ -126,6 +128,8 @@ class _ScatterPlan:
                 f.hot_ptr = hot["hot_ptr"].data_ptr()
                 f.n_hot = hot["n_hot"]
                 f.n_hot_units = hot["n_hot_units"]
+                f.cell_row = self.cell_ptr[s].data_ptr()
+                f.cell_vrow = self.cell_vrow[s].data_ptr()
                 f.n_src = n
                 f.n_cell = G
                 fr.append(f)

@@ -32,8 +32,15 @@ def test_header_symbols_exported():
 
 
 def test_struct_layouts_match_header():
-    # pg_frame: 15 pointers + 4 int64; pg_plan_geom: 2 int32 + 6 x 3 fields + 128 x 3 int32
-    assert ctypes.sizeof(_lib.Frame) == 15 * 8 + 4 * 8
+    # pg_frame: the ctypes mirror names the header's fields in the header's order
+    hdr = HEADER.read_text()
+    body = hdr[hdr.index("typedef struct pg_frame {"):hdr.index("} pg_frame;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = re.findall(r"(\w+);", body)
+    assert names == [f[0] for f in _lib.Frame._fields_]
+    n_ptr = sum(1 for f in _lib.Frame._fields_ if f[1] is ctypes.c_void_p)
+    assert ctypes.sizeof(_lib.Frame) == n_ptr * 8 + (len(names) - n_ptr) * 8
+    # pg_plan_geom: 2 int32 + 6 x 3 fields + 128 x 3 int32
     assert ctypes.sizeof(_lib.PlanGeom) == 8 + 24 + 24 + 12 * 3 + 128 * 12 + 4  # trailing pad to 8
 
 
@@ -44,9 +51,16 @@ def test_library_version_and_errors():
     rc = lib.pg_grid_fp32(None, None, 0, 0, 1, 128, 0, 0, 0, None)
     assert rc != 0 and b"frame" in lib.pg_last_error()
     assert lib.pg_grid_tile_cells(65) == 0
+    assert _lib.tile_options(65) == [] and _lib.tile_options(0) == []
     for c, gs in ((3, 4), (8, 8), (16, 16), (32, 32), (64, 64)):
         tc = lib.pg_grid_tile_cells(c)
-        assert tc > 0 and tc * gs == 2048  # 16 KiB of complex accumulators
+        opts = _lib.tile_options(c)
+        assert tc in opts and opts == sorted(opts, reverse=True)
+        # each thread of a 256-thread tile owns one cell and at most 16 coils
+        assert all(256 % t == 0 and 1 <= gs * t // 256 <= 16 for t in opts)
+    # a tile size that is not compiled is rejected before any launch
+    rc = lib.pg_grid_fp32(1, 1, 1, 1, 16, 100, 0, 0, 0, None)
+    assert rc != 0 and b"tile_cells" in lib.pg_last_error()
     assert lib.pg_table_bf16_elems(10, 32, 2) == 10 * 2 * 4 * 2 * 64 * 8
 
 
