@@ -28,6 +28,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ...]
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import pathlib
@@ -112,18 +113,23 @@ def rel_l2(a: torch.Tensor, b: torch.Tensor) -> float:
 # algorithmic bytes per kernel launch (the roofline numerators)
 # ---------------------------------------------------------------------------
 def alg_bytes(kernel: str, st: dict) -> float:
-    C, N, G, R, U, K = st["C"], st["N"], st["G"], st["R"], st["U"], st["K"]
+    """Bytes a launch must move at least (inputs read once, outputs written
+    once), with C coils, N samples, G cells, R valid replicas, K entries."""
+    C, N, G, R, K = st["C"], st["N"], st["G"], st["R"], st["K"]
     if kernel == "pg_grid_fp32":
-        # source rows of every replica + grid write + replica records (src,
-        # entry, sqrt w) + units + cell map + table once
-        return 8 * C * R + 8 * C * G + 12 * R + 8 * U + 4 * G + 8 * C * C * K
+        # sample row of every replica + dense grid + replica records (source,
+        # entry, sqrt weight) + per-cell row offsets (2 x u32) + table once
+        return 8 * C * R + 8 * C * G + 12 * R + 8 * G + 8 * C * C * K
     if kernel == "pg_apply_grouped_bf16":
         parts = st["n_parts"]
-        # y_t rows of every replica + table (bf16, real 2C x 2C per entry) + V write + records
-        return 8 * C * R + 2 * parts * 4 * C * C * K + 8 * C * R + 12 * R + 4 * K
+        # sample rows + bf16 table (real 2C x 2C per entry and part) + V rows
+        # written + records (source, sqrt weight, replica id)
+        return 8 * C * R + 8 * parts * C * C * K + 8 * C * R + 12 * R
     if kernel == "pg_grid_reduce_v":
-        return 8 * C * R + 8 * C * G + 8 * U + 4 * G
-    if kernel == "pg_transpose_cn_c64":
+        return 8 * C * R + 8 * C * G + 8 * G
+    if kernel == "pg_grid_scatter":
+        return 8 * C * R + 8 * C * G + 8 * R + 8 * G
+    if kernel in ("pg_transpose_cn_c64", "pg_transpose_frames_c64"):
         return 16 * C * N
     raise KeyError(kernel)
 
@@ -352,8 +358,7 @@ def run_frames2d(args, dev) -> dict:
         stats["n_parts"] = 2 if m == "bf16x3" else 1
         all_roof[m] = {}
         for k, t in v["kernel_ms"].items():
-            kk = "pg_transpose_cn_c64" if k == "pg_transpose_frames_c64" else k
-            all_roof[m][k] = {"GB/s": alg_bytes(kk, stats) / (t * 1e-3) / 1e9, "ms": t}
+            all_roof[m][k] = {"GB/s": alg_bytes(k, stats) / (t * 1e-3) / 1e9, "ms": t}
     head = "fp32" if "fp32" in modes else next(iter(modes))
     kname = "pg_grid_fp32" if head == "fp32" else max(modes[head]["kernel_ms"], key=modes[head]["kernel_ms"].get)
     kms = modes[head]["kernel_ms"][kname]
@@ -384,12 +389,175 @@ def run_frames2d(args, dev) -> dict:
 
 
 # ---------------------------------------------------------------------------
+# serving: Poisson stream of mixed 2D / 3D jobs, each with its own table
+# ---------------------------------------------------------------------------
+SERVE_DESC = ("Poisson stream mixing 2D 256^2 8-coil golden-angle radial jobs (40-400 spokes x 512; 3x3 kernel, q=1/32) "
+              "and 3D 128^3 16-coil golden-means kooshball jobs (2k-10k spokes x 256; 2-wide sphere, q=1/10), "
+              "each with its own ACR-fit table (lamda=1e-3); HBM-budget admission, 4 preparation streams, "
+              "continuous batching of the gridding launches")
+
+
+def make_serve_jobs(dev, n_jobs: int, seed: int = 5):
+    from pygrog_amd import synth
+    from pygrog_amd.serve import GridJob
+
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for i in range(n_jobs):
+        if rng.random() < 0.25:
+            shape, C = (128, 128, 128), 16
+            coords = synth.kooshball_3d(int(rng.integers(2000, 10001)), 256, first=int(rng.integers(0, 100000)))
+            calib = synth.calibration(shape, C, (16, 16, 16), seed=seed + i, device=dev)
+            kw = dict(kernel_width=2, kernel_shape="sphere", oversamp=(1, 1, 1), precision=1)
+        else:
+            shape, C = (256, 256), 8
+            coords = synth.radial_2d(int(rng.integers(40, 401)), 512, offset=float(rng.uniform(0, np.pi)))
+            calib = synth.calibration(shape, C, (24, 24), seed=seed + i, device=dev)
+            kw = dict(kernel_width=3, quant=32)
+        y = synth.complex_gaussian((C, coords.shape[0] * coords.shape[1]), seed=seed + 1000 + i, device=dev)
+        jobs.append(GridJob(shape=shape, coords=coords, calib=calib, kspace=y, job_id=i, lamda=1e-3, **kw))
+    return jobs
+
+
+def run_serving(dev, n_jobs: int, modes=("fp32", "bf16x3"), loads=(0.5, 0.8), seed: int = 5) -> dict:
+    """Saturation throughput first (all jobs at t=0), then Poisson streams at
+    fractions of it; p50/p99 latency per (mode, rate)."""
+    from pygrog_amd.serve import GrogServer, poisson_arrivals
+
+    jobs = make_serve_jobs(dev, n_jobs, seed)
+    torch.cuda.synchronize()
+    # warm the kernels and allocator once, untimed
+    GrogServer(dev, mode="fp32").run([dataclasses.replace(j, arrival=0.0) for j in jobs[:2]])
+    out = {"workload": SERVE_DESC, "jobs": n_jobs,
+           "coil_samples_per_job_mean": float(np.mean([j.coils * j.samples for j in jobs])), "modes": {}}
+    for mode in modes:
+        sat = GrogServer(dev, mode=mode).run([dataclasses.replace(j, arrival=0.0) for j in jobs])
+        mu = sat["throughput_jobs_s"]
+        res = {"saturation": sat, "rates": {}}
+        for frac in loads:
+            lam = frac * mu
+            arr = poisson_arrivals(n_jobs, lam, seed + int(frac * 100))
+            st = GrogServer(dev, mode=mode).run([dataclasses.replace(j, arrival=float(a)) for j, a in zip(jobs, arr)])
+            st["rate_jobs_s"] = lam
+            res["rates"][f"{frac:.2f}"] = st
+            log(f"serve {mode} @ {lam:.2f} jobs/s ({frac:.0%} of saturation): p50 {st['p50_ms']:.1f} ms, "
+                f"p99 {st['p99_ms']:.1f} ms, {st['throughput_jobs_s']:.2f} jobs/s, mean batch {st['mean_batch']:.2f}")
+        out["modes"][mode] = res
+        log(f"serve {mode} saturation: {mu:.2f} jobs/s, {sat['coil_samples_per_s']:.3e} coil-samples/s, "
+            f"p99 {sat['p99_ms']:.1f} ms")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# iterative recon: SENSE CG through SparseFFT with the Toeplitz PSF operator
+# ---------------------------------------------------------------------------
+CG_DESC = ("SENSE CG, 30 iterations, SparseFFT with Toeplitz PSF normal operator, 320^2, 20 coils, 24-interleaf "
+           "variable-density spiral x 4096 (98,304 samples), FP64 NUDFT of a 10-ellipse phantom x Gaussian maps "
+           "+ complex Gaussian noise at 30 dB, 3x3 GROG table at q=1/32 (lamda=1e-3, 24^2 ACR at 30 dB); FP32 matvecs "
+           "with FP64 CG scalars vs the same CG in FP64")
+
+
+class _Normal64:
+    """The SENSE Toeplitz normal equations of a SparseFFT in complex128
+    (torch on the device): the FP64 reference of the CG benchmark."""
+
+    def __init__(self, op, dev):
+        sp = op._sp()
+        self.shape = tuple(op.image_shape)
+        self.grid = tuple(op.grid_shape)
+        self.idx = sp.indices[0]
+        self.sqw = sp.sqrt_w[0].to(torch.float64)
+        self.perm = sp.sort_perm[0]
+        self.smaps = op.smaps.to(dev, torch.complex128)
+        G = int(np.prod(self.grid))
+        self.psf = torch.zeros(G, dtype=torch.float64, device=dev).index_add_(0, self.idx, self.sqw * self.sqw)
+        self.psf = self.psf.reshape(self.grid)
+
+    def _fft(self, x, inv=False):
+        ax = (-2, -1)
+        f = torch.fft.ifftn if inv else torch.fft.fftn
+        return torch.fft.fftshift(f(torch.fft.ifftshift(x, dim=ax), dim=ax, norm="ortho"), dim=ax)
+
+    def adjoint(self, b):
+        b = b.reshape(b.shape[0], -1).to(torch.complex128)
+        G = int(np.prod(self.grid))
+        g = torch.zeros(b.shape[0], G, dtype=torch.complex128, device=b.device)
+        g.index_add_(1, self.idx, b[:, self.perm] * self.sqw)
+        img = self._fft(g.reshape(-1, *self.grid), inv=True)
+        return (img * self.smaps.conj()).sum(0)
+
+    def normal(self, x):
+        k = self._fft(self.smaps * x) * self.psf
+        return (self._fft(k, inv=True) * self.smaps.conj()).sum(0)
+
+
+def run_cg(args, dev) -> dict:
+    from pygrog_amd import cg, synth
+    from pygrog_amd.calib import GrogInterpolator
+    from pygrog_amd.operator import SparseFFT
+
+    shape, C, n_il, n_s = (320, 320), 20, 24, 4096
+    coords = synth.spiral_vd_2d(n_il, n_s, 320)
+    ph = synth.ellipse_phantom(shape, 10, seed=41, device=dev)
+    maps = synth.gaussian_coil_maps(shape, C, 0.4, 0.5, seed=42, device=dev)
+    y64 = synth.nudft(maps * ph, coords.reshape(-1, 2))
+    noise = synth.complex_gaussian(tuple(y64.shape), seed=43, device=dev, dtype=torch.complex128)
+    noise *= torch.linalg.vector_norm(y64) / torch.linalg.vector_norm(noise) / 10 ** (30 / 20)
+    y = (y64 + noise).to(torch.complex64).reshape(C, n_il, n_s)
+    # calibration region with the acquisition's noise level (a noiseless ACR
+    # makes the unit operators near-singular and their negative fractional
+    # powers reach norms of 1e14)
+    acr = synth.acr(maps * ph, (24, 24))
+    acr_noise = synth.complex_gaussian(tuple(acr.shape), seed=45, device=dev, dtype=acr.dtype)
+    acr = acr + acr_noise * (torch.linalg.vector_norm(acr) / torch.linalg.vector_norm(acr_noise) / 10 ** (30 / 20))
+    g = GrogInterpolator(shape, coords, kernel_width=3, device=dev)
+    g.calc_interp_table(acr.to(torch.complex64), lamda=1e-3, quant=32)
+    sparse = g.interpolate(y)
+    op = SparseFFT(plan=g.plan, smaps=maps.to(torch.complex64), device=dev, toeplitz=True)
+    b = (sparse * g.plan.pre_weights.to(dev)).reshape(C, *op.natural_shape)
+    iters = 30
+    op.solve(b, method="cg", max_iter=2, tol=0.0)  # warm-up: PSF, plans, FFT plans
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(max(1, min(args.steps, 5))):
+        t0 = time.perf_counter()
+        x = op.solve(b, method="cg", max_iter=iters, tol=0.0)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    op64 = _Normal64(op, dev)
+
+    def rel(a, b_):
+        return float(torch.linalg.vector_norm(a.to(torch.complex128) - b_) / torch.linalg.vector_norm(b_))
+
+    # the FP32 operators against their FP64 restatement
+    xr = synth.complex_gaussian(shape, seed=44, device=dev)
+    op_rel = {"adjoint": rel(op.adjoint(b), op64.adjoint(b)),
+              "normal": rel(op.normal(xr), op64.normal(xr.to(torch.complex128)))}
+    x64 = cg(op64, b.to(torch.complex128), max_iter=iters, tol=0.0)
+    res_rel = rel(x, x64)
+    # the same solve with a small Tikhonov term (better conditioned)
+    damp = 1e-2
+    xd = op.solve(b, method="cg", max_iter=iters, tol=0.0, damp=damp)
+    xd64 = cg(op64, b.to(torch.complex128), max_iter=iters, tol=0.0, damp=damp)
+    t = float(np.median(times))
+    res = {"workload": CG_DESC, "iterations": iters, "solve_ms": t * 1e3, "ms_per_iteration": t * 1e3 / iters,
+           "rel_l2_vs_fp64_cg": res_rel, "rel_l2_vs_fp64_cg_damp_1e-2": rel(xd, xd64),
+           "operator_rel_l2_vs_fp64": op_rel, "runs": len(times)}
+    log(f"cg: {t * 1e3:.1f} ms for {iters} iterations ({t * 1e3 / iters:.2f} ms/iter), rel-L2 vs FP64 CG "
+        f"{res_rel:.2e} (damp 1e-2: {res['rel_l2_vs_fp64_cg_damp_1e-2']:.2e}), operators vs FP64 {op_rel}")
+    return res
+
+
+# ---------------------------------------------------------------------------
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="kooshball3d", choices=["kooshball3d", "radial2d", "frames2d"])
+    ap.add_argument("--workload", default="kooshball3d", choices=["kooshball3d", "radial2d", "frames2d", "serve", "cg"])
+    ap.add_argument("--serve-jobs", type=int, default=48, help="jobs per serving run (0: skip serving)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload only: skip the 200-frame batch and SENSE-CG legs")
     ap.add_argument("--modes", default="fp32,bf16,bf16x3")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work (s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -423,6 +591,13 @@ def main(argv=None) -> int:
     if args.workload == "frames2d":
         line = run_frames2d(args, dev)
         print(json.dumps(line))
+        return 0
+    if args.workload == "cg":
+        print(json.dumps({"metric": METRIC, "cg": run_cg(args, dev)}))
+        return 0
+    if args.workload == "serve":
+        res = run_serving(dev, max(args.serve_jobs, 4))
+        print(json.dumps({"metric": METRIC, "serving": res}))
         return 0
     if args.copy_calibration > 0:
         src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device=dev)
@@ -495,15 +670,38 @@ def main(argv=None) -> int:
     cpu = None
     if not args.no_cpu_baseline:
         threads = len(os.sched_getaffinity(0))
-        probe = cpu_baseline(wl, max(1, min(wl["spokes"], 100 if wl["spokes"] > 1000 else wl["spokes"])), threads)
-        rate = probe["coil_samples"] / probe["seconds"]
-        n_sp = int(min(wl["spokes"], max(1, args.cpu_seconds * rate / (wl["n_read"] * wl["C"]))))
+        n_sp = max(1, min(wl["spokes"], 100 if wl["spokes"] > 1000 else wl["spokes"]))
         res = cpu_baseline(wl, n_sp, threads)
+        # rescale (at most twice) until the sample is about --cpu-seconds of work
+        for _ in range(2):
+            if res["seconds"] >= 0.6 * args.cpu_seconds or n_sp >= wl["spokes"]:
+                break
+            rate = res["coil_samples"] / res["seconds"]
+            n_sp = int(min(wl["spokes"], max(n_sp + 1, args.cpu_seconds * rate / (wl["n_read"] * wl["C"]))))
+            res = cpu_baseline(wl, n_sp, threads)
         cpu = {"value": res["coil_samples"] / res["seconds"], "unit": "coil-samples/s", "cores": threads,
                "kind": res["kind"],
                "sample": f"first {n_sp} of {wl['spokes']} spokes ({n_sp * wl['n_read']} samples x {wl['C']} coils), "
                          f"reference interpolate(grid=True) flow, {res['seconds']:.2f} s"}
         log(f"cpu baseline: {cpu}")
+
+    hbm_peak_gb = torch.cuda.max_memory_allocated(dev) / 1e9
+    wl_info = {k: wl[k] for k in ("desc", "N", "C", "shape")}
+    serving = None
+    if args.serve_jobs > 0:
+        del wl, g, comp, out, ref64, cache, y
+        import gc
+
+        gc.collect()
+        torch.cuda.empty_cache()
+        serving = run_serving(dev, args.serve_jobs)
+    extras = {}
+    if not args.no_extras and args.workload == "kooshball3d":
+        torch.cuda.empty_cache()
+        fr = run_frames2d(args, dev)
+        extras["frames2d"] = {k: fr[k] for k in ("value", "ms_per_step", "config", "modes", "roofline")}
+        torch.cuda.empty_cache()
+        extras["sense_cg"] = run_cg(args, dev)
 
     line = {
         "metric": METRIC,
@@ -518,8 +716,8 @@ def main(argv=None) -> int:
         "vs_baseline": None,
         "dtype": "bf16" if head.startswith("bf16") else "fp32",
         "data": "synthetic (seeded trajectory, phantom, coil maps; Gaussian k-space)",
-        "config": {"workload": wl["desc"], "mode": head, "samples": wl["N"], "coils": wl["C"],
-                   "grid": list(wl["shape"]), "table_entries": stats["K"], "replicas_valid": stats["R"]},
+        "config": {"workload": wl_info["desc"], "mode": head, "samples": wl_info["N"], "coils": wl_info["C"],
+                   "grid": list(wl_info["shape"]), "table_entries": stats["K"], "replicas_valid": stats["R"]},
         "modes": modes,
         "speedup_bf16_vs_fp32": (hm["coil_samples_per_s"] / modes["fp32"]["coil_samples_per_s"]
                                  if "fp32" in modes else None),
@@ -527,7 +725,16 @@ def main(argv=None) -> int:
         "roofline_all": all_roof,
         "cpu_baseline": cpu,
         "setup_s": setup_s,
+        "hbm_peak_gb": hbm_peak_gb,
     }
+    if extras:
+        line["extras"] = extras
+    if serving is not None:
+        sus = {m: v["rates"][max(v["rates"])] for m, v in serving["modes"].items()}
+        line["serving"] = serving
+        line["serve_p99_ms"] = {m: v["p99_ms"] for m, v in sus.items()}
+        line["serve_p50_ms"] = {m: v["p50_ms"] for m, v in sus.items()}
+        line["serve_rate_jobs_s"] = {m: v["rate_jobs_s"] for m, v in sus.items()}
     print(json.dumps(line))
     return 0
 

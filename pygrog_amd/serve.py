@@ -1,0 +1,279 @@
+"""Serving loop for GROG gridding jobs on one GPU: HBM-budget admission,
+per-job preparation overlapped on several streams, and continuous batching
+of the gridding launches.
+
+A request is one reconstruction's gridding: a trajectory, its own
+calibration region (ACR) and its k-space, as a user of the reference would
+pass to ``GrogInterpolator(shape, coords, ...)``,
+``calc_interp_table(acr, ...)`` and ``interpolate(data, grid=True)``
+(src/pygrog/calib/_grog.py:302-331, 382-422, 437-555).  The loop:
+
+  * admits arrived requests in arrival order while the HBM budget (an
+    up-front estimate of every buffer the job will hold) and a preparation
+    slot are free;
+  * prepares each admitted job -- GPU plan compiler and FP64 table fit -- on
+    its slot's own stream, from a worker thread, so preparations of
+    different jobs overlap each other and the gridding on the device;
+  * every time the gridding stream has room, gathers all prepared jobs
+    (continuous batching: jobs that became ready while earlier batches were
+    on the GPU join the next one) and grids them together: one transpose per
+    job, then one fused launch over all frames of the same coil count and
+    grid size (each frame carries its own table), or for the BF16 modes one
+    grouped MFMA apply per job and one shared reduction launch;
+  * retires a batch when its event completes, records each job's latency
+    (arrival to gridded k-space resident on the device) and releases its
+    budget.
+
+A job's result is bitwise the one its own interpolator produces alone with
+the same mode: every cell folds its replicas in a fixed order whatever
+launch the job shares.
+"""
+
+from __future__ import annotations
+
+__all__ = ["GridJob", "GrogServer", "estimate_job_bytes", "poisson_arrivals"]
+
+import concurrent.futures as cf
+import dataclasses
+import threading
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .calib._grog import GridProgram, GrogInterpolator, _default_oversamp, _kernel_offsets
+
+
+@dataclasses.dataclass
+class GridJob:
+    """One gridding request.  ``kspace`` (C, N) complex64 and ``calib``
+    (C, *acr) are device tensors (the request payload, staged on the GPU);
+    ``coords`` is the trajectory (..., ndim) on the host."""
+
+    shape: tuple
+    coords: np.ndarray
+    calib: torch.Tensor
+    kspace: torch.Tensor
+    kernel_width: int = 2
+    kernel_shape: str = "circle"
+    oversamp: object = None
+    lamda: float = 1e-3
+    precision: int = 1
+    quant: int | None = None
+    arrival: float = 0.0
+    job_id: int = 0
+    # filled by the server
+    t_admit: float | None = None
+    t_ready: float | None = None
+    t_done: float | None = None
+    est_bytes: int = 0
+    result: torch.Tensor | None = None
+    interp: GrogInterpolator | None = None
+    _ready_event: torch.cuda.Event | None = None
+
+    @property
+    def coils(self) -> int:
+        return int(self.kspace.shape[0])
+
+    @property
+    def samples(self) -> int:
+        return int(self.kspace.shape[-1])
+
+    @property
+    def latency(self) -> float | None:
+        return None if self.t_done is None else self.t_done - self.arrival
+
+
+def estimate_job_bytes(job: GridJob, mode: str) -> int:
+    """Upper estimate of the device memory a job holds from admission to
+    completion: plan (per candidate replica and per cell), table and its
+    packed copies, the sample-major copy of the k-space, the grid and, for
+    the grouped modes, the applied replicas."""
+    ndim = len(job.shape)
+    os_ = _default_oversamp(ndim, job.oversamp)
+    grid = [int(np.ceil(s * o)) for s, o in zip(job.shape, os_)]
+    G = int(np.prod(grid))
+    kw = int(len(_kernel_offsets(ndim, job.kernel_width, job.kernel_shape)))
+    N, C = job.samples, job.coils
+    if job.quant is not None:
+        nsteps = job.quant + 1
+    else:
+        nsteps = 10 ** job.precision + 1
+    K = nsteps ** ndim
+    per_rep = 8 + 4 * ndim + 4 + 8 + 4 + 8 + 8 + 4 * 3 + 8 + 16   # plan fields + records + sort buffers
+    plan = N * kw * per_rep + G * (4 * 4 + 4 + 4)
+    table = K * C * C * (8 + 8 + 8 + 2 * 8 * 2)  # c64, fp64 pair, fp32 packed, bf16 hi/lo real form
+    data = 2 * 8 * C * N + 8 * C * G
+    vbuf = 8 * C * N * (ndim if mode != "fp32" else 0)
+    return int(1.25 * (plan + table + data + vbuf)) + (16 << 20)
+
+
+def poisson_arrivals(n: int, rate: float, seed: int) -> np.ndarray:
+    """Arrival times (s) of a Poisson process of ``rate`` jobs/s."""
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))
+
+
+class GrogServer:
+    """Serve ``GridJob`` requests on one HIP device (see the module notes).
+
+    ``hbm_budget`` bytes bound the admitted jobs' estimated footprint,
+    ``slots`` the jobs in preparation at once (one stream and one worker
+    thread each), ``max_inflight`` the gridding batches on the device."""
+
+    def __init__(self, device=None, *, mode: str = "fp32", hbm_budget: int | None = None, slots: int = 4,
+                 max_inflight: int = 2, keep_results: bool = False):
+        if mode not in ("fp32", "bf16", "bf16x3"):
+            raise ValueError(f"unknown gridding mode {mode!r}")
+        self.device = _lib.hip_device(device)
+        self.mode = mode
+        if hbm_budget is None:
+            free, total = torch.cuda.mem_get_info(self.device)
+            hbm_budget = int(0.5 * free)
+        self.hbm_budget = int(hbm_budget)
+        self.slots = int(slots)
+        self.max_inflight = int(max_inflight)
+        self.keep_results = keep_results
+        self._streams = [torch.cuda.Stream(device=self.device) for _ in range(self.slots)]
+        self._grid_stream = torch.cuda.Stream(device=self.device)
+        self._free_streams = list(range(self.slots))
+        self._lock = threading.Lock()
+
+    # -- preparation (worker thread, own stream) --------------------------------
+    def _prepare(self, job: GridJob, slot: int) -> GridJob:
+        st = self._streams[slot]
+        with torch.cuda.device(self.device), torch.cuda.stream(st):
+            g = GrogInterpolator(job.shape, job.coords, job.oversamp, job.kernel_width, job.kernel_shape,
+                                 device=self.device)
+            g.calc_interp_table(job.calib, lamda=job.lamda, precision=job.precision, quant=job.quant)
+            if self.mode != "fp32":
+                g._entry_groups()
+                g._table_bf16(2 if self.mode == "bf16x3" else 1)
+            ev = torch.cuda.Event()
+            ev.record(st)
+        job.interp = g
+        job._ready_event = ev
+        return job
+
+    # -- one gridding batch on the grid stream ---------------------------------
+    def _launch_batch(self, jobs: list) -> tuple:
+        st = self._grid_stream
+        keep = []
+        groups = {}
+        with torch.cuda.device(self.device), torch.cuda.stream(st):
+            for job in jobs:
+                st.wait_event(job._ready_event)
+                g = job.interp
+                comp = g.plan._compiled
+                y = job.kspace
+                if y.device != self.device or y.dtype != torch.complex64 or not y.is_contiguous():
+                    y = y.to(self.device, torch.complex64).contiguous()
+                yt = torch.empty(comp.npts * comp.C, dtype=torch.complex64, device=self.device)
+                out = torch.empty(comp.C, comp.G, dtype=torch.complex64, device=self.device)
+                _lib.call("pg_transpose_cn_c64", _lib.ptr(y), comp.C, comp.npts, _lib.ptr(yt), st.cuda_stream)
+                vbuf = None
+                if self.mode != "fp32":
+                    vbuf = torch.empty(g._rmax() * comp.C, dtype=torch.complex64, device=self.device)
+                    n_terms = 1 if self.mode == "bf16" else 3
+                    tbl = g._table_bf16(2 if n_terms == 3 else 1)
+                    ent_ptr, ent_rep, items, n_items = g._entry_groups()[0]
+                    if n_items > 0:
+                        _lib.call("pg_apply_grouped_bf16", _lib.ptr(yt), _lib.ptr(tbl), _lib.ptr(ent_ptr),
+                                  _lib.ptr(ent_rep), comp.rep_src[0].data_ptr(), comp.rep_s[0].data_ptr(),
+                                  _lib.ptr(items), n_items, int(comp.table.shape[0]), comp.C, n_terms,
+                                  _lib.ptr(vbuf), st.cuda_stream)
+                frames = g._frames(yt, out, 1, vbuf)
+                groups.setdefault((comp.C, comp.G), []).extend(frames)
+                keep += [y, yt, out, vbuf]
+                job.result = out
+            for (C, G), frames in groups.items():
+                prog = GridProgram(frames, C, G, self.device, _lib.load().pg_grid_tile_cells(C))
+                prog.launch("pg_grid_fp32" if self.mode == "fp32" else "pg_grid_reduce_v")
+                keep.append(prog)
+            ev = torch.cuda.Event()
+            ev.record(st)
+        return ev, keep
+
+    # -- the loop ------------------------------------------------------------------
+    def run(self, jobs: list, *, timeout: float = 600.0) -> dict:
+        """Serve ``jobs`` (arrival times relative to the start of the run)
+        and return latency / throughput statistics."""
+        pending = sorted(jobs, key=lambda j: j.arrival)
+        for j in pending:
+            j.est_bytes = estimate_job_bytes(j, self.mode)
+            if j.est_bytes > self.hbm_budget:
+                raise ValueError(f"job {j.job_id} needs ~{j.est_bytes / 1e9:.2f} GB, above the budget "
+                                 f"{self.hbm_budget / 1e9:.2f} GB")
+        used = 0
+        peak_used = 0
+        preparing = {}
+        ready = []
+        inflight = []
+        done = []
+        batches = []
+        pool = cf.ThreadPoolExecutor(max_workers=self.slots)
+        free_slots = list(range(self.slots))
+        t0 = time.perf_counter()
+        try:
+            while len(done) < len(jobs):
+                now = time.perf_counter() - t0
+                if now > timeout:
+                    raise RuntimeError(f"serving run exceeded {timeout:.0f} s")
+                progressed = False
+                # admit in arrival order while budget and a slot are free
+                while pending and pending[0].arrival <= now and free_slots and used + pending[0].est_bytes <= self.hbm_budget:
+                    job = pending.pop(0)
+                    slot = free_slots.pop(0)
+                    used += job.est_bytes
+                    peak_used = max(peak_used, used)
+                    job.t_admit = now
+                    preparing[pool.submit(self._prepare, job, slot)] = (job, slot)
+                    progressed = True
+                # collect finished preparations
+                for fut in [f for f in preparing if f.done()]:
+                    job, slot = preparing.pop(fut)
+                    fut.result()  # re-raise a preparation error
+                    job.t_ready = time.perf_counter() - t0
+                    free_slots.append(slot)
+                    ready.append(job)
+                    progressed = True
+                # continuous batching: everything ready joins the next launch
+                if ready and len(inflight) < self.max_inflight:
+                    batch, ready = ready, []
+                    ev, keep = self._launch_batch(batch)
+                    inflight.append((ev, batch, keep))
+                    batches.append(len(batch))
+                    progressed = True
+                # retire completed batches
+                still = []
+                for ev, batch, keep in inflight:
+                    if ev.query():
+                        t = time.perf_counter() - t0
+                        for job in batch:
+                            job.t_done = t
+                            used -= job.est_bytes
+                            if not self.keep_results:
+                                job.result = None
+                            job.interp = None
+                            done.append(job)
+                        progressed = True
+                    else:
+                        still.append((ev, batch, keep))
+                inflight = still
+                if not progressed:
+                    time.sleep(0.0002)
+        finally:
+            pool.shutdown(wait=True)
+        makespan = max(j.t_done for j in done) if done else 0.0
+        lat = np.array([j.latency for j in done])
+        cs = sum(j.coils * j.samples for j in done)
+        return {
+            "mode": self.mode, "jobs": len(done), "batches": len(batches),
+            "mean_batch": float(np.mean(batches)) if batches else 0.0,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3), "max_ms": float(lat.max() * 1e3),
+            "throughput_jobs_s": len(done) / makespan if makespan > 0 else 0.0,
+            "coil_samples_per_s": cs / makespan if makespan > 0 else 0.0,
+            "makespan_s": makespan, "hbm_budget_gb": self.hbm_budget / 1e9, "peak_admitted_gb": peak_used / 1e9,
+        }
