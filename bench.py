@@ -625,6 +625,7 @@ def main(argv=None) -> int:
     modes = {}
     cache = {}
     for mode in [m for m in args.modes.split(",") if m]:
+        tune = g.autotune(y, mode, out, program_cache=cache, reps=3)
         for _ in range(args.warmup):
             g.grid_device(y, mode, out, program_cache=cache)
         torch.cuda.synchronize()
@@ -641,7 +642,8 @@ def main(argv=None) -> int:
         ms = e0.elapsed_time(e1) / args.steps
         kt = {k: float(np.mean(v)) for k, v in timer.durations().items()}
         modes[mode] = {"coil_samples_per_s": wl["N"] * wl["C"] / (ms * 1e-3), "ms_per_step": ms,
-                       "rel_l2_vs_fp64": rl, "kernel_ms": kt}
+                       "rel_l2_vs_fp64": rl, "kernel_ms": kt,
+                       "autotune": {"tile_cells": tune["tile_cells"], "n_blocks": tune["n_blocks"]}}
         log(f"{mode}: {ms:.3f} ms/step, {modes[mode]['coil_samples_per_s']:.3e} coil-samples/s, rel-L2 {rl}, "
             f"kernels {kt}")
 

@@ -258,7 +258,6 @@ typedef struct pg_frame {
   const uint32_t *hot_units;/* (n_hot_units) units of the hot cells, in order */
   const uint32_t *hot_ptr;  /* (n_hot + 1) hot cell h owns hot_units[hot_ptr[h]..] */
   void *p_unit;             /* (n_hot_units, C) c64 scratch: hot unit partials */
-  void *p_cell;             /* (n_hot, C) c64 scratch: hot cell sums */
   const uint32_t *cell_row; /* (n_cell + 1) first replica of each cell (CSR) */
   const uint32_t *cell_vrow;/* (n_cell + 1) replicas of ordinary cells (not
                                hot, one unit) before each cell: the tile
@@ -281,12 +280,14 @@ int pg_table_pack_c64(const void *table, int64_t K, int64_t C, void *packed,
  * pg_frame; tile_ptr: device array (F + 1) of cumulative tile counts with
  * tile_cells cells per tile.  Deterministic: the result does not depend on
  * tile size, block count or the other frames in the launch.
- * Cells with many replicas ("hot", e.g. the radial DC) are reduced first by
- * two pre-pass kernels -- unit partials in parallel into p_unit, then an
- * in-order fold per hot cell into p_cell -- and the tile kernel copies the
- * finished sums, so a hot cell costs one workgroup no more than any other
- * and the result is the same bits as without the pre-pass.  max_hot_units /
- * max_hot: largest n_hot_units / n_hot over the frames (0 = no pre-pass). */
+ * Cells with many replicas ("hot", e.g. the radial DC) are reduced by two
+ * pre-pass kernels -- unit partials in parallel into p_unit, then an
+ * in-order fold per hot cell written straight into the grid -- on a helper
+ * stream forked from and joined back into `stream` (graph-capturable), so
+ * they overlap the tile kernel, which leaves hot cells alone; a hot cell
+ * costs one workgroup no more than any other and the result is the same
+ * bits as without the pre-pass.  max_hot_units / max_hot: largest
+ * n_hot_units / n_hot over the frames (0 = no pre-pass). */
 int pg_grid_fp32(const pg_frame *frames, const uint32_t *tile_ptr, int64_t F,
                  int64_t n_tiles, int64_t C, int64_t tile_cells,
                  int64_t n_blocks, int64_t max_hot_units, int64_t max_hot,

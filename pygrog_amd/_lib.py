@@ -63,7 +63,6 @@ class Frame(ctypes.Structure):
         ("hot_units", c_void_p),
         ("hot_ptr", c_void_p),
         ("p_unit", c_void_p),
-        ("p_cell", c_void_p),
         ("cell_row", c_void_p),
         ("cell_vrow", c_void_p),
         ("n_src", c_int64),

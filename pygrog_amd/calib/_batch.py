@@ -34,7 +34,7 @@ import torch
 
 from .. import _lib
 from ._grappa import build_table
-from ._grog import GridProgram, GrogInterpolator, KernelTimer, group_by_entry
+from ._grog import GridProgram, GrogInterpolator, KernelTimer, group_by_entry, tune_program
 
 _TUNE_CACHE: dict = {}
 
@@ -230,27 +230,9 @@ class GrogFrameBatch:
         key = (mode, self.C, self.G, self.F, int(self.replicas().sum()))
         yt, vbuf, prog = self._program(mode, y, out)
         if key not in _TUNE_CACHE:
-            n_cu = torch.cuda.get_device_properties(self.device).multi_processor_count
-            entry = "pg_grid_fp32" if mode == "fp32" else "pg_grid_reduce_v"
             self._run(mode, y, out)
-            times = {}
-            for tc in _lib.tile_options(self.C):
-                prog.set_shape(tc, None)
-                cands = candidates or [n_cu * k for k in (2, 4, 8, 16)] + [prog.n_tiles]
-                for nb in sorted({max(8, min(int(c), prog.n_tiles)) for c in cands}):
-                    prog.set_shape(tc, nb)
-                    prog.launch(entry)
-                    e0 = torch.cuda.Event(enable_timing=True)
-                    e1 = torch.cuda.Event(enable_timing=True)
-                    e0.record()
-                    for _ in range(reps):
-                        prog.launch(entry)
-                    e1.record()
-                    torch.cuda.synchronize(self.device)
-                    times[(tc, nb)] = e0.elapsed_time(e1) / reps
-            best = min(times, key=times.get)
-            _TUNE_CACHE[key] = {"tile_cells": best[0], "n_blocks": best[1], "ms": times[best],
-                                "tried": {f"{k[0]}x{k[1]}": v for k, v in times.items()}}
+            entry = "pg_grid_fp32" if mode == "fp32" else "pg_grid_reduce_v"
+            _TUNE_CACHE[key] = tune_program(prog, entry, reps, candidates)
         self.tuned[mode] = dict(_TUNE_CACHE[key])
         for k, (yt_, vbuf_, pr) in self._programs.items():
             if k[0] == mode:

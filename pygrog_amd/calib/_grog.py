@@ -347,10 +347,8 @@ class GridProgram:
         for f in frames:
             if f.n_hot_units > 0:
                 pu = torch.empty(f.n_hot_units * self.C, dtype=torch.complex64, device=device)
-                pc = torch.empty(f.n_hot * self.C, dtype=torch.complex64, device=device)
-                self._scratch += [pu, pc]
+                self._scratch.append(pu)
                 f.p_unit = pu.data_ptr()
-                f.p_cell = pc.data_ptr()
                 self.max_hot_units = max(self.max_hot_units, int(f.n_hot_units))
                 self.max_hot = max(self.max_hot, int(f.n_hot))
         arr = (_lib.Frame * self.F)(*frames)
@@ -382,6 +380,32 @@ class GridProgram:
                   self.tile_cells, self.n_blocks, self.max_hot_units, self.max_hot, _lib.stream(self.device))
         if timer is not None:
             timer.stop(entry, e0)
+
+
+def tune_program(prog: GridProgram, entry: str, reps: int = 5, candidates=None) -> dict:
+    """Time ``prog`` over its compiled tile sizes and candidate persistent
+    grid sizes (the inputs the launch reads must already hold real data) and
+    leave it at the fastest shape.  The shape never changes the bits."""
+    n_cu = torch.cuda.get_device_properties(prog.device).multi_processor_count
+    times = {}
+    for tc in _lib.tile_options(prog.C):
+        prog.set_shape(tc, None)
+        cands = candidates or [n_cu * k for k in (4, 8, 16, 32, 64)] + [prog.n_tiles]
+        for nb in sorted({max(8, min(int(c), prog.n_tiles)) for c in cands}):
+            prog.set_shape(tc, nb)
+            prog.launch(entry)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                prog.launch(entry)
+            e1.record()
+            torch.cuda.synchronize(prog.device)
+            times[(tc, nb)] = e0.elapsed_time(e1) / reps
+    best = min(times, key=times.get)
+    prog.set_shape(*best)
+    return {"tile_cells": best[0], "n_blocks": best[1], "ms": times[best],
+            "tried": {f"{k[0]}x{k[1]}": v for k, v in times.items()}}
 
 
 # ---------------------------------------------------------------------------
@@ -705,9 +729,13 @@ class GrogInterpolator:
         grouped modes' intermediate buffer per frame)."""
         return max(1, max(self.plan._compiled.n_valid))
 
-    def grid_program(self, y: torch.Tensor, out: torch.Tensor, B: int = 1, vbuf=None, n_blocks=None) -> GridProgram:
+    def grid_program(self, y: torch.Tensor, out: torch.Tensor, B: int = 1, vbuf=None, n_blocks=None,
+                     mode: str | None = None) -> GridProgram:
         comp = self.plan._compiled
         tc = _lib.load().pg_grid_tile_cells(comp.C)
+        tuned = getattr(self, "_tuned", {}).get(mode)
+        if tuned is not None and n_blocks is None:
+            tc, n_blocks = tuned
         return GridProgram(self._frames(y, out, B, vbuf), comp.C, comp.G, comp.device, tc, n_blocks)
 
     # -- grouped (BF16 MFMA) structures ---------------------------------------
@@ -783,10 +811,10 @@ class GrogInterpolator:
         if key not in cache:
             yt = torch.empty(B * S * N * C, dtype=torch.complex64, device=comp.device)
             if mode == "fp32":
-                cache[key] = (None, yt, self.grid_program(yt, out, B))
+                cache[key] = (None, yt, self.grid_program(yt, out, B, mode=mode))
             else:
                 vbuf = torch.empty(B * S * self._rmax() * C, dtype=torch.complex64, device=comp.device)
-                cache[key] = (vbuf, yt, self.grid_program(yt, out, B, vbuf=vbuf))
+                cache[key] = (vbuf, yt, self.grid_program(yt, out, B, vbuf=vbuf, mode=mode))
         vbuf, yt, prog = cache[key]
         self._transpose_all(y, yt, timer)
         if mode == "fp32":
@@ -812,6 +840,21 @@ class GrogInterpolator:
             timer.stop("pg_apply_grouped_bf16", e0)
         prog.launch("pg_grid_reduce_v", timer)
         return out
+
+    def autotune(self, y: torch.Tensor, mode: str = "fp32", out: torch.Tensor | None = None,
+                 program_cache: dict | None = None, reps: int = 5) -> dict:
+        """Pick the gridding kernel's tile size and grid size for ``y`` on
+        this device (timed on the real data; the bits do not change) and
+        keep them for later ``grid_device`` calls with the same buffers and
+        for new programs of this interpolator."""
+        cache = program_cache if program_cache is not None else {}
+        out = self.grid_device(y, mode, out, program_cache=cache)
+        B = y.shape[0]
+        _, _, prog = cache[(mode, y.data_ptr(), out.data_ptr(), B)]
+        res = tune_program(prog, "pg_grid_fp32" if mode == "fp32" else "pg_grid_reduce_v", reps)
+        self._tuned = getattr(self, "_tuned", {})
+        self._tuned[mode] = (res["tile_cells"], res["n_blocks"])
+        return res
 
     # -- forward interpolation (reference :437-555) ----------------------------
     def interpolate(self, data, shot_index=None, ret_image: bool = False, grid: bool = False, *, mode: str = "fp32"):

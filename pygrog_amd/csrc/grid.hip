@@ -26,11 +26,14 @@
 // workgroup serialises on a hot cell.
 #include "common.h"
 
+#include <mutex>
+
 extern "C" int pg_grid_tile_options(int64_t C, int32_t *opts, int32_t max_opts);
 
 namespace pg {
 
 constexpr int kGridThreads = 256;
+constexpr int kMaxDevices = 64;
 
 // Tile sizes compiled per coil group GS (threads per cell = 256 / TC, each
 // thread owning GS * TC / 256 <= 16 coils); the first is the default.
@@ -189,7 +192,7 @@ __device__ __forceinline__ float2 unit_partial(const pg_frame &fr, uint32_t u,
 // their sum from the pre-pass; cells with several units that are not hot
 // (only when the pre-pass threshold is raised) fold their units in canonical
 // order straight from memory.  The tile loop is persistent.
-constexpr int kRowBytes = 24576;
+constexpr int kRowBytes = 16384;
 
 // Cell with several units that is not reduced by the hot pre-pass: fold its
 // unit partials for one coil in canonical order, straight from memory.
@@ -221,9 +224,10 @@ __device__ __forceinline__ int find_row_cell(const uint32_t *vr, int ncell,
 }
 
 template <int MODE, int GS> struct TileRows {
-  static constexpr int RS = MODE == 0 ? GS + 2 : GS + 1;  // row stride (float2)
+  static constexpr int RS = MODE == 0 ? GS + 2 : GS + 1;  // staged row stride (float2)
+  static constexpr int OS = GS + 1;                       // MODE 0 output row stride
   static constexpr int REC = MODE == 1 ? 4 : 12;          // record bytes / row
-  static constexpr int RB = kRowBytes / (RS * 8 + REC);   // rows per batch
+  static constexpr int RB = kRowBytes / (RS * 8 + (MODE == 0 ? OS * 8 : 0) + REC);  // rows per batch
 };
 
 template <int GS, int MODE, int TC>
@@ -231,12 +235,15 @@ __global__ __launch_bounds__(kGridThreads) void k_tile(
     const pg_frame *__restrict__ frames, const uint32_t *__restrict__ tile_ptr,
     int F, int C, uint32_t n_tiles) {
   constexpr int RS = TileRows<MODE, GS>::RS;
+  constexpr int OS = TileRows<MODE, GS>::OS;
   constexpr int RB = TileRows<MODE, GS>::RB;
+  constexpr int FS = MODE == 0 ? OS : RS;  // stride of the rows phase C folds
   constexpr int TPC = kGridThreads / TC;  // threads per cell
   constexpr int CPT = GS / TPC;           // coils per thread
   constexpr int NG = kGridThreads / GS;   // apply groups
   static_assert(TPC * TC == kGridThreads && CPT >= 1 && CPT <= 16, "tile shape");
   __shared__ __align__(16) float2 vrow[RB * RS];
+  __shared__ float2 vout[MODE == 0 ? RB * OS : 1];  // MODE 0: applied rows
   __shared__ uint32_t rmap[RB];                    // MODE 1: replica of row
   __shared__ uint32_t rsrc[MODE == 1 ? 1 : RB];    // MODE 0/2: record
   __shared__ uint32_t rent[MODE == 0 ? RB : 1];
@@ -254,30 +261,60 @@ __global__ __launch_bounds__(kGridThreads) void k_tile(
   const int co0 = (tid / TC) * CPT;
   const uint32_t tp1 = tile_ptr[1];
 
-  for (uint32_t L = blockIdx.x; L < n_tiles; L += gridDim.x) {
-    // frame of tile L: the first guess is exact when every frame has tp1
-    // tiles, so the frame record and the check load together
+  // frame of tile L: the first guess is exact when every frame has tp1 tiles
+  auto frame_of = [&](uint32_t L) -> int {
     int f = tp1 > 0 ? (int)(L / tp1) : 0;
     if (f > F - 1) f = F - 1;
-    pg_frame fr = frames[f];
-    uint32_t t0 = tile_ptr[f];
-    if (t0 > L || tile_ptr[f + 1] <= L) {
-      while (f > 0 && tile_ptr[f] > L) --f;
-      while (f + 1 < F && tile_ptr[f + 1] <= L) ++f;
-      fr = frames[f];
-      t0 = tile_ptr[f];
+    while (f > 0 && tile_ptr[f] > L) --f;
+    while (f + 1 < F && tile_ptr[f + 1] <= L) ++f;
+    return f;
+  };
+  // the phase-A values this thread stages: entries tid and (thread 0) TC
+  auto fetch_head = [&](uint32_t L, int f, uint32_t &c0v, uint32_t &v0v, uint32_t &c1v, uint32_t &v1v) {
+    const pg_frame *q = frames + f;
+    const int64_t cell0 = (int64_t)(L - tile_ptr[f]) * TC;
+    const int64_t rem = q->n_cell - cell0;
+    const int nc = rem < TC ? (int)rem : TC;
+    if (tid <= nc) {
+      c0v = q->cell_row[cell0 + tid];
+      v0v = q->cell_vrow[cell0 + tid];
     }
-    const int64_t cell0 = (int64_t)(L - t0) * TC;
+    if (tid == 0 && nc == kGridThreads) {
+      c1v = q->cell_row[cell0 + kGridThreads];
+      v1v = q->cell_vrow[cell0 + kGridThreads];
+    }
+  };
+
+  uint32_t L = blockIdx.x;
+  if (L >= n_tiles) return;
+  int f = frame_of(L);
+  uint32_t pc0 = 0, pv0 = 0, pc1 = 0, pv1 = 0;
+  fetch_head(L, f, pc0, pv0, pc1, pv1);
+  for (;;) {
+    const pg_frame fr = frames[f];
+    const int64_t cell0 = (int64_t)(L - tile_ptr[f]) * TC;
     const int64_t rem = fr.n_cell - cell0;
     const int ncell = rem < TC ? (int)rem : TC;
 
-    // phase A: one round of coalesced loads (CSR row starts, ordinary-row
-    // prefix); cells with rows but no ordinary rows are hot or multi-unit
-    for (int t = tid; t <= ncell; t += kGridThreads) {
-      crow[t] = fr.cell_row[cell0 + t];
-      vr[t] = fr.cell_vrow[cell0 + t];
+    // phase A: this tile's CSR row starts and ordinary-row prefix were
+    // fetched during the previous tile; fetch the next tile's now
+    if (tid <= ncell) {
+      crow[tid] = pc0;
+      vr[tid] = pv0;
+    }
+    if (tid == 0 && ncell == kGridThreads) {
+      crow[kGridThreads] = pc1;
+      vr[kGridThreads] = pv1;
+    }
+    const uint32_t Ln = L + gridDim.x;
+    const bool more = Ln < n_tiles;
+    int fn = f;
+    if (more) {
+      fn = frame_of(Ln);
+      fetch_head(Ln, fn, pc0, pv0, pc1, pv1);
     }
     __syncthreads();
+    // cells with rows but no ordinary rows are hot or multi-unit
     for (int t = tid; t < ncell; t += kGridThreads) {
       int tg = -1;
       if (vr[t + 1] == vr[t] && crow[t + 1] > crow[t]) {
@@ -346,31 +383,28 @@ __global__ __launch_bounds__(kGridThreads) void k_tile(
         }
       }
       __syncthreads();
-      // B3 (MODE 0): apply the table in place, two rows per group per round
+      // B3 (MODE 0): apply each replica's table row to its staged sample
+      // row, two rows per group in flight, into the output rows
       if (MODE == 0) {
-        const int rounds = (nk + 2 * NG - 1) / (2 * NG);
-        for (int rd = 0; rd < rounds; ++rd) {
-          const int j1 = rd * 2 * NG + grp, j2 = j1 + NG;
-          float2 v1 = make_float2(0.0f, 0.0f), v2 = v1;
-          if (lane < C) {
-            const float4 *tb = reinterpret_cast<const float4 *>(fr.table) + lane;
-            if (j1 < nk) v1 = apply_row<GS>(tb + (int64_t)rent[j1] * P * C, &vrow[j1 * RS], C, P, rsw[j1]);
+        if (lane < C) {
+          const float4 *tb = reinterpret_cast<const float4 *>(fr.table) + lane;
+          for (int j1 = grp; j1 < nk; j1 += 2 * NG) {
+            const int j2 = j1 + NG;
+            const float2 v1 = apply_row<GS>(tb + (int64_t)rent[j1] * P * C, &vrow[j1 * RS], C, P, rsw[j1]);
+            float2 v2 = make_float2(0.0f, 0.0f);
             if (j2 < nk) v2 = apply_row<GS>(tb + (int64_t)rent[j2] * P * C, &vrow[j2 * RS], C, P, rsw[j2]);
+            vout[j1 * OS + lane] = v1;
+            if (j2 < nk) vout[j2 * OS + lane] = v2;
           }
-          __syncthreads();
-          if (lane < C) {
-            if (j1 < nk) vrow[j1 * RS + lane] = v1;
-            if (j2 < nk) vrow[j2 * RS + lane] = v2;
-          }
-          __syncthreads();
         }
+        __syncthreads();
       }
       // C: fold this batch's rows of the owned cell
       {
         const uint32_t lo = my_lo > k0 ? my_lo : k0;
         const uint32_t hi = my_hi < k0 + nk ? my_hi : k0 + nk;
         for (uint32_t k = lo; k < hi; ++k) {
-          const float2 *vp = &vrow[(k - k0) * RS + co0];
+          const float2 *vp = MODE == 0 ? &vout[(k - k0) * FS + co0] : &vrow[(k - k0) * FS + co0];
 #pragma unroll
           for (int c = 0; c < CPT; ++c) {
             const float2 v = vp[c];
@@ -382,33 +416,26 @@ __global__ __launch_bounds__(kGridThreads) void k_tile(
       __syncthreads();
     }
 
-    if (own) {
-      const int tg = tag[t_own];
-      float2 val[CPT];
-      if (tg == -1) {
-#pragma unroll
-        for (int k = 0; k < CPT; ++k)
-          val[k] = make_float2(0.0f + acc[k].x, 0.0f + acc[k].y);
-      } else if (tg >= 0) {
-        const float2 *pc = reinterpret_cast<const float2 *>(fr.p_cell) + (int64_t)tg * C;
-#pragma unroll
-        for (int k = 0; k < CPT; ++k) {
-          const float2 v = (co0 + k < C) ? pc[co0 + k] : make_float2(0.0f, 0.0f);
-          val[k] = make_float2(0.0f + v.x, 0.0f + v.y);
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < CPT; ++k)
-          val[k] = (co0 + k < C)
-                       ? multi_unit_coil<GS, MODE>(frames, f, cu_lo[t_own], cu_hi[t_own], co0 + k, C)
-                       : make_float2(0.0f, 0.0f);
-      }
+    if (own && tag[t_own] != -1 && tag[t_own] < 0) {
+      // multi-unit cell that the pre-pass does not reduce
       float2 *g = reinterpret_cast<float2 *>(fr.grid) + cell0 + t_own;
 #pragma unroll
       for (int k = 0; k < CPT; ++k)
-        if (co0 + k < C) g[(int64_t)(co0 + k) * fr.n_cell] = val[k];
+        if (co0 + k < C)
+          g[(int64_t)(co0 + k) * fr.n_cell] =
+              multi_unit_coil<GS, MODE>(frames, f, cu_lo[t_own], cu_hi[t_own], co0 + k, C);
+    } else if (own && tag[t_own] == -1) {
+      float2 *g = reinterpret_cast<float2 *>(fr.grid) + cell0 + t_own;
+#pragma unroll
+      for (int k = 0; k < CPT; ++k)
+        if (co0 + k < C)
+          g[(int64_t)(co0 + k) * fr.n_cell] = make_float2(0.0f + acc[k].x, 0.0f + acc[k].y);
     }
+    // hot cells (tag >= 0) are written by the pre-pass
     __syncthreads();
+    if (!more) break;
+    L = Ln;
+    f = fn;
   }
 }
 
@@ -426,7 +453,8 @@ __global__ __launch_bounds__(kGridThreads) void k_hot_units(
   if (lane < C) reinterpret_cast<float2 *>(fr.p_unit)[i * C + lane] = pv;
 }
 
-// Hot pre-pass 2: in-order fold of each hot cell's unit partials.  The
+// Hot pre-pass 2: in-order fold of each hot cell's unit partials, written
+// straight into the grid (the tile kernel leaves hot cells alone).  The
 // partials are fetched 16 at a time (independent loads in flight) and added
 // in order, so the dependent chain is the adds, not the loads.
 template <int GS>
@@ -454,7 +482,10 @@ __global__ __launch_bounds__(kGridThreads) void k_hot_cells(
       }
     }
   }
-  reinterpret_cast<float2 *>(fr.p_cell)[h * C + lane] = make_float2(ax, ay);
+  // grid[c] = ((0 + p_u0) + p_u1) + ... : 0 + (the fold) as for every cell
+  const uint32_t cell = fr.unit_cell[fr.hot_units[i0]];
+  reinterpret_cast<float2 *>(fr.grid)[(int64_t)lane * fr.n_cell + cell] =
+      make_float2(0.0f + ax, 0.0f + ay);
 }
 
 // packed[e][p][co] = (K[e][co][2p], K[e][co][2p+1])
@@ -578,6 +609,98 @@ __global__ void k_unit_count(const uint32_t *__restrict__ cp, int64_t S,
   }
 }
 
+template <int GS, int MODE>
+static int launch_tile(int64_t tile_cells, dim3 grid, hipStream_t st,
+                       const pg_frame *frames, const uint32_t *tile_ptr,
+                       int F, int C, uint32_t n_tiles) {
+  const dim3 block(kGridThreads);
+#define PG_TILE_LAUNCH(TCV)                                                   \
+  if (tile_cells == TCV) {                                                    \
+    hipLaunchKernelGGL((k_tile<GS, MODE, TCV>), grid, block, 0, st, frames,   \
+                       tile_ptr, F, C, n_tiles);                              \
+    PG_LAUNCHED();                                                            \
+    return 0;                                                                 \
+  }
+  if constexpr (GS <= 16) {
+    PG_TILE_LAUNCH(256) PG_TILE_LAUNCH(128) PG_TILE_LAUNCH(64)
+  } else if constexpr (GS == 32) {
+    PG_TILE_LAUNCH(128) PG_TILE_LAUNCH(64)
+  } else {
+    PG_TILE_LAUNCH(64) PG_TILE_LAUNCH(32)
+  }
+#undef PG_TILE_LAUNCH
+  PG_CHECK(false, "grid: tile_cells %lld is not compiled for C=%d",
+           (long long)tile_cells, C);
+}
+
+template <int GS, int MODE>
+static int launch_hot(hipStream_t st, const pg_frame *frames, int F, int C,
+                      int64_t max_hot_units, int64_t max_hot) {
+  constexpr int NG = kGridThreads / GS;
+  const dim3 block(kGridThreads);
+  hipLaunchKernelGGL((k_hot_units<GS, MODE>),
+                     dim3((unsigned)ceil_div(max_hot_units, NG), (unsigned)F),
+                     block, 0, st, frames, C);
+  PG_LAUNCHED();
+  hipLaunchKernelGGL((k_hot_cells<GS>),
+                     dim3((unsigned)ceil_div(max_hot, NG), (unsigned)F), block,
+                     0, st, frames, C);
+  PG_LAUNCHED();
+  return 0;
+}
+
+// Helper stream per device for the hot pre-pass, forked from and joined back
+// into the caller's stream with two events (created once, outside any graph
+// capture; default priority).  The lock keeps one call's record/wait pairs
+// together.
+struct HotFork {
+  std::mutex m;
+  hipStream_t helper[kMaxDevices] = {};
+  hipEvent_t fork[kMaxDevices] = {};
+  hipEvent_t join[kMaxDevices] = {};
+};
+static HotFork &hot_fork() {
+  static HotFork h;
+  return h;
+}
+
+template <int GS, int MODE>
+static int launch_gs(int64_t tile_cells, dim3 grid, hipStream_t st,
+                     const pg_frame *frames, const uint32_t *tile_ptr, int F,
+                     int C, uint32_t n_tiles, int64_t max_hot_units,
+                     int64_t max_hot) {
+  if (max_hot_units <= 0)
+    return launch_tile<GS, MODE>(tile_cells, grid, st, frames, tile_ptr, F, C, n_tiles);
+  int dev = 0;
+  PG_HIP(hipGetDevice(&dev));
+  HotFork &hf = hot_fork();
+  std::lock_guard<std::mutex> lk(hf.m);
+  bool fork = dev >= 0 && dev < kMaxDevices;
+  if (fork && hf.helper[dev] == nullptr) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    PG_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) {
+      fork = false;  // no resource creation inside a capture: run in line
+    } else {
+      PG_HIP(hipStreamCreateWithFlags(&hf.helper[dev], hipStreamNonBlocking));
+      PG_HIP(hipEventCreateWithFlags(&hf.fork[dev], hipEventDisableTiming));
+      PG_HIP(hipEventCreateWithFlags(&hf.join[dev], hipEventDisableTiming));
+    }
+  }
+  if (!fork) {
+    if (launch_hot<GS, MODE>(st, frames, F, C, max_hot_units, max_hot)) return 1;
+    return launch_tile<GS, MODE>(tile_cells, grid, st, frames, tile_ptr, F, C, n_tiles);
+  }
+  hipStream_t hs = hf.helper[dev];
+  PG_HIP(hipEventRecord(hf.fork[dev], st));
+  PG_HIP(hipStreamWaitEvent(hs, hf.fork[dev], 0));
+  if (launch_hot<GS, MODE>(hs, frames, F, C, max_hot_units, max_hot)) return 1;
+  PG_HIP(hipEventRecord(hf.join[dev], hs));
+  const int rc = launch_tile<GS, MODE>(tile_cells, grid, st, frames, tile_ptr, F, C, n_tiles);
+  PG_HIP(hipStreamWaitEvent(st, hf.join[dev], 0));
+  return rc;
+}
+
 template <int MODE>
 static int launch_grid(const pg_frame *frames, const uint32_t *tile_ptr,
                        int64_t F, int64_t n_tiles, int64_t C,
@@ -600,52 +723,15 @@ static int launch_grid(const pg_frame *frames, const uint32_t *tile_ptr,
   if (n_tiles == 0) return 0;
   int64_t nb = n_blocks > 0 ? n_blocks : 2048;
   if (nb > n_tiles) nb = n_tiles;
-  nb = ((nb + 7) / 8) * 8;
   const hipStream_t st = as_stream(s);
-  const dim3 grid((unsigned)nb), block(kGridThreads);
-#define PG_TILE_LAUNCH(GSV, TCV)                                             \
-  if (tile_cells == TCV) {                                                   \
-    hipLaunchKernelGGL((k_tile<GSV, MODE, TCV>), grid, block, 0, st, frames, \
-                       tile_ptr, (int)F, (int)C, (uint32_t)n_tiles);         \
-    PG_LAUNCHED();                                                           \
-    return 0;                                                                \
-  }
-#define PG_HOT_LAUNCH(GSV)                                                   \
-  {                                                                          \
-    constexpr int NG = kGridThreads / GSV;                                   \
-    if (max_hot_units > 0) {                                                 \
-      hipLaunchKernelGGL((k_hot_units<GSV, MODE>),                           \
-                         dim3((unsigned)ceil_div(max_hot_units, NG),         \
-                              (unsigned)F),                                  \
-                         block, 0, st, frames, (int)C);                      \
-      PG_LAUNCHED();                                                         \
-      hipLaunchKernelGGL((k_hot_cells<GSV>),                                 \
-                         dim3((unsigned)ceil_div(max_hot, NG), (unsigned)F), \
-                         block, 0, st, frames, (int)C);                      \
-      PG_LAUNCHED();                                                         \
-    }                                                                        \
-  }
-  if (C <= 4) {
-    PG_HOT_LAUNCH(4)
-    PG_TILE_LAUNCH(4, 256) PG_TILE_LAUNCH(4, 128) PG_TILE_LAUNCH(4, 64)
-  } else if (C <= 8) {
-    PG_HOT_LAUNCH(8)
-    PG_TILE_LAUNCH(8, 256) PG_TILE_LAUNCH(8, 128) PG_TILE_LAUNCH(8, 64)
-  } else if (C <= 16) {
-    PG_HOT_LAUNCH(16)
-    PG_TILE_LAUNCH(16, 256) PG_TILE_LAUNCH(16, 128) PG_TILE_LAUNCH(16, 64)
-  } else if (C <= 32) {
-    PG_HOT_LAUNCH(32)
-    PG_TILE_LAUNCH(32, 128) PG_TILE_LAUNCH(32, 64)
-  } else {
-    PG_HOT_LAUNCH(64)
-    PG_TILE_LAUNCH(64, 64) PG_TILE_LAUNCH(64, 32)
-  }
-  PG_CHECK(false, "grid: tile_cells %lld is not compiled for C=%lld",
-           (long long)tile_cells, (long long)C);
-#undef PG_TILE_LAUNCH
-#undef PG_HOT_LAUNCH
-  return 1;
+  const dim3 grid((unsigned)nb);
+  const int c = (int)C, f = (int)F;
+  const uint32_t nt = (uint32_t)n_tiles;
+  if (C <= 4) return launch_gs<4, MODE>(tile_cells, grid, st, frames, tile_ptr, f, c, nt, max_hot_units, max_hot);
+  if (C <= 8) return launch_gs<8, MODE>(tile_cells, grid, st, frames, tile_ptr, f, c, nt, max_hot_units, max_hot);
+  if (C <= 16) return launch_gs<16, MODE>(tile_cells, grid, st, frames, tile_ptr, f, c, nt, max_hot_units, max_hot);
+  if (C <= 32) return launch_gs<32, MODE>(tile_cells, grid, st, frames, tile_ptr, f, c, nt, max_hot_units, max_hot);
+  return launch_gs<64, MODE>(tile_cells, grid, st, frames, tile_ptr, f, c, nt, max_hot_units, max_hot);
 }
 
 }  // namespace pg
