@@ -122,9 +122,9 @@ def alg_bytes(kernel: str, st: dict) -> float:
         return 8 * C * R + 8 * C * G + 12 * R + 8 * G + 8 * C * C * K
     if kernel == "pg_apply_grouped_bf16":
         parts = st["n_parts"]
-        # sample rows + bf16 table (real 2C x 2C per entry and part) + V rows
+        # sample rows + bf16 table (complex C x C per entry and part) + V rows
         # written + records (source, sqrt weight, replica id)
-        return 8 * C * R + 8 * parts * C * C * K + 8 * C * R + 12 * R
+        return 8 * C * R + 4 * parts * C * C * K + 8 * C * R + 12 * R
     if kernel == "pg_grid_reduce_v":
         return 8 * C * R + 8 * C * G + 8 * G
     if kernel == "pg_grid_scatter":

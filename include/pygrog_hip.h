@@ -344,9 +344,11 @@ int pg_transpose_frames_c64(const void *y, const int64_t *frame_off, int64_t F,
                             int64_t max_n, int64_t C, void *y_t,
                             pg_stream_t stream);
 
-/* Pack a (K, C, C) c64 table for the BF16 MFMA grouped GEMM (real 2C x 2C
- * form, hi and optionally lo bf16 parts).  n_parts = 1 or 2; packed holds
- * K * n_parts * (2C)^2 bf16. */
+/* Pack a (K, C, C) c64 table for the BF16 MFMA grouped GEMM: per entry and
+ * part, (re, im) bf16 pairs of K[co][ci] padded to 8*ceil(2C/16) rows by
+ * 16*ceil(2C/32) columns; part 0 = hi = bf16(x), part 1 = lo =
+ * bf16(x - hi).  The kernel expands each pair into the real 2C x 2C form in
+ * registers.  n_parts = 1 or 2; packed holds pg_table_bf16_elems bf16. */
 int pg_table_pack_bf16(const void *table, int64_t K, int64_t C, int32_t n_parts,
                        void *packed, pg_stream_t stream);
 

@@ -35,38 +35,33 @@ __device__ __forceinline__ float bf2f(unsigned short h) {
   return __uint_as_float(((uint32_t)h) << 16);
 }
 
-// packed[e][part][mt][ks][lane][i], lane = (row % 16) + 16 * ((col % 32) / 8),
-// i = col % 8, row = 16 mt + row % 16, col = 32 ks + col % 32.
+// Compact complex table: packed[e][part][co][ci] = (re, im) as two bf16,
+// co < CO = 8 MT, ci < CI = 16 KS (zero padded).  part 0 = hi = bf16(x),
+// part 1 = lo = bf16(x - hi).  A lane's MFMA A fragment (row 16 mt + j,
+// k = 32 ks + 8q .. +7) is the 4 complex values K[co][16 ks + 4q .. +3] of
+// co = 8 mt + j/2, expanded in registers: even rows (Re of the output)
+// take (re, -im) pairs, odd rows (Im) take (im, re) pairs.  Negating a bf16
+// is exact and both rounding splits are odd functions, so these are the
+// same bf16 values as a table stored in real 2C x 2C form, at half the bytes.
 __global__ void k_table_pack_bf16(const float2 *__restrict__ K, int64_t nK,
-                                  int C, int MT, int KS, int n_parts,
+                                  int C, int CO, int CI, int n_parts,
                                   unsigned short *__restrict__ packed) {
-  const int64_t per_part = (int64_t)MT * KS * 64 * 8;
-  const int64_t total = nK * per_part;
+  const int64_t per_part = (int64_t)CO * CI * 2;
+  const int64_t total = nK * (int64_t)CO * CI;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t e = t / per_part;
-    int64_t rem = t - e * per_part;
-    const int i = (int)(rem % 8);
-    rem /= 8;
-    const int lane = (int)(rem % 64);
-    rem /= 64;
-    const int ks = (int)(rem % KS);
-    const int mt = (int)(rem / KS);
-    const int row = 16 * mt + (lane % 16);
-    const int col = 32 * ks + 8 * (lane / 16) + i;
-    const int co = row / 2, ci = col / 2;
-    float a = 0.0f;
-    if (co < C && ci < C) {
-      const float2 k = K[(e * C + co) * C + ci];
-      const bool re_row = (row % 2) == 0, re_col = (col % 2) == 0;
-      a = re_row ? (re_col ? k.x : -k.y) : (re_col ? k.y : k.x);
-    }
-    const unsigned short hi = f2bf(a);
-    const int64_t base = e * n_parts * per_part;
-    packed[base + (t - e * per_part)] = hi;
+    const int64_t e = t / ((int64_t)CO * CI);
+    const int rem = (int)(t - e * CO * CI);
+    const int co = rem / CI, ci = rem - (rem / CI) * CI;
+    float2 k = make_float2(0.0f, 0.0f);
+    if (co < C && ci < C) k = K[(e * C + co) * C + ci];
+    const unsigned short hr = f2bf(k.x), hi_ = f2bf(k.y);
+    unsigned short *dst = packed + e * n_parts * per_part + (int64_t)rem * 2;
+    dst[0] = hr;
+    dst[1] = hi_;
     if (n_parts > 1) {
-      const unsigned short lo = f2bf(a - bf2f(hi));
-      packed[base + per_part + (t - e * per_part)] = lo;
+      dst[per_part] = f2bf(k.x - bf2f(hr));
+      dst[per_part + 1] = f2bf(k.y - bf2f(hi_));
     }
   }
 }
@@ -93,60 +88,93 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
   uint32_t stop = start + 16 * kTilesPerWave;
   if (stop > b1) stop = b1;
   const int twoC = 2 * C;
-  const int64_t per_part = (int64_t)MT * KS * 64 * 8;
+  constexpr int CO = 8 * MT, CI = 16 * KS;
+  constexpr int64_t per_part = (int64_t)CO * CI * 2;
 
   bf16x8_t ahi[MT][KS];
   bf16x8_t alo[MT][KS];
   const unsigned short *tb = tbl + e * NP * per_part;
+  {
+    const int jj = lane % 16, qq = lane / 16;
+    const bool im_row = (jj & 1) != 0;
 #pragma unroll
-  for (int mt = 0; mt < MT; ++mt)
+    for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const int64_t off = ((int64_t)(mt * KS + ks) * 64 + lane) * 8;
-      u16x8_t h = *reinterpret_cast<const u16x8_t *>(tb + off);
-      ahi[mt][ks] = __builtin_bit_cast(bf16x8_t, h);
-      if (NP > 1) {
-        u16x8_t l = *reinterpret_cast<const u16x8_t *>(tb + per_part + off);
-        alo[mt][ks] = __builtin_bit_cast(bf16x8_t, l);
+      for (int ks = 0; ks < KS; ++ks) {
+        const int co = 8 * mt + (jj >> 1);
+        const int64_t off = ((int64_t)co * CI + 16 * ks + 4 * qq) * 2;
+#pragma unroll
+        for (int part = 0; part < NP; ++part) {
+          const u16x8_t c = *reinterpret_cast<const u16x8_t *>(tb + part * per_part + off);
+          u16x8_t a;
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const unsigned short re = c[2 * i], im = c[2 * i + 1];
+            a[2 * i] = im_row ? im : re;
+            a[2 * i + 1] = im_row ? re : (unsigned short)(im ^ 0x8000u);
+          }
+          if (part == 0) ahi[mt][ks] = __builtin_bit_cast(bf16x8_t, a);
+          else alo[mt][ks] = __builtin_bit_cast(bf16x8_t, a);
+        }
       }
-    }
+  }
 
   const int j = lane % 16, q = lane / 16;
-  for (uint32_t n0 = start; n0 < stop; n0 += 16) {
-    const uint32_t idx = n0 + j;
-    const bool valid = idx < stop;
-    uint32_t r = 0, src = 0;
-    float s = 0.0f;
-    if (valid) {
-      r = ent_rep[idx];
-      src = rep_src[r];
-      s = rep_s[r];
+  // every tile's replica, source and weight for this lane's column j: two
+  // rounds of independent loads for the whole work item
+  uint32_t rr[kTilesPerWave], src[kTilesPerWave];
+  float sw[kTilesPerWave];
+#pragma unroll
+  for (int t = 0; t < kTilesPerWave; ++t) {
+    const uint32_t idx = start + 16 * t + j;
+    rr[t] = idx < stop ? ent_rep[idx] : 0u;
+  }
+#pragma unroll
+  for (int t = 0; t < kTilesPerWave; ++t) {
+    const bool v = start + 16 * t + j < stop;
+    src[t] = v ? rep_src[rr[t]] : 0u;
+    sw[t] = v ? rep_s[rr[t]] : 0.0f;
+  }
+  const bool vec = (twoC & 3) == 0;
+  // sample-row fragments of one tile: KS x 8 floats per lane
+  auto load_rows = [&](int t, float (&f)[KS][8]) {
+    const bool v = start + 16 * t + j < stop;
+    const float *yr = y_t + (int64_t)src[t] * twoC;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int col0 = 32 * ks + 8 * q;
+      if (v && vec && col0 + 8 <= twoC) {
+        const float4 a0 = *reinterpret_cast<const float4 *>(yr + col0);
+        const float4 a1 = *reinterpret_cast<const float4 *>(yr + col0 + 4);
+        f[ks][0] = a0.x; f[ks][1] = a0.y; f[ks][2] = a0.z; f[ks][3] = a0.w;
+        f[ks][4] = a1.x; f[ks][5] = a1.y; f[ks][6] = a1.z; f[ks][7] = a1.w;
+      } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          f[ks][i] = (v && col0 + i < twoC) ? yr[col0 + i] : 0.0f;
+      }
     }
+  };
+  const int n_tiles = (int)((stop - start + 15) / 16);
+  float fa[KS][8], fb[KS][8];
+  load_rows(0, fa);
+#pragma unroll
+  for (int t = 0; t < kTilesPerWave; ++t) {
+    if (t >= n_tiles) break;
+    float (&cur)[KS][8] = (t & 1) ? fb : fa;
+    float (&nxt)[KS][8] = (t & 1) ? fa : fb;
+    if (t + 1 < n_tiles) load_rows(t + 1, nxt);
     floatx4_t acc[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) acc[mt] = floatx4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      const int col0 = 32 * ks + 8 * q;
-      float f[8];
-      const float *yr = y_t + (int64_t)src * twoC + col0;
-      // rows are 16-byte aligned only when 2C is a multiple of 4
-      if (valid && col0 + 8 <= twoC && (twoC & 3) == 0) {
-        const float4 a0 = *reinterpret_cast<const float4 *>(yr);
-        const float4 a1 = *reinterpret_cast<const float4 *>(yr + 4);
-        f[0] = a0.x; f[1] = a0.y; f[2] = a0.z; f[3] = a0.w;
-        f[4] = a1.x; f[5] = a1.y; f[6] = a1.z; f[7] = a1.w;
-      } else {
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-          f[i] = (valid && col0 + i < twoC) ? yr[i] : 0.0f;
-      }
       u16x8_t bh, bl;
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        const unsigned short h = f2bf(f[i]);
+        const unsigned short h = f2bf(cur[ks][i]);
         bh[i] = h;
-        bl[i] = f2bf(f[i] - bf2f(h));
+        bl[i] = f2bf(cur[ks][i] - bf2f(h));
       }
       const bf16x8_t bhi = __builtin_bit_cast(bf16x8_t, bh);
       const bf16x8_t blo = __builtin_bit_cast(bf16x8_t, bl);
@@ -162,15 +190,16 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
         }
       }
     }
-    if (valid) {
-      float *vr = vbuf + (int64_t)r * twoC;
+    if (start + 16 * t + j < stop) {
+      const float s = sw[t];
+      float *vr = vbuf + (int64_t)rr[t] * twoC;
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
         const int row = 16 * mt + 4 * q;
         float o[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) o[i] = (acc[mt][i] * s) * s;
-        if (row + 4 <= twoC && (twoC & 3) == 0) {
+        if (row + 4 <= twoC && vec) {
           *reinterpret_cast<float4 *>(vr + row) = make_float4(o[0], o[1], o[2], o[3]);
         } else {
 #pragma unroll
@@ -226,12 +255,12 @@ int pg_table_pack_bf16(const void *table, int64_t K, int64_t C,
   PG_CHECK(n_parts == 1 || n_parts == 2, "table_pack_bf16: n_parts 1 or 2");
   int MT, KS;
   mfma_shape(C, MT, KS);
-  const int64_t total = K * (int64_t)MT * KS * 64 * 8;
+  const int64_t total = K * (int64_t)(8 * MT) * (16 * KS);
   const int threads = 256;
   hipLaunchKernelGGL(k_table_pack_bf16, dim3(grid_for(total, threads)),
                      dim3(threads), 0, as_stream(stream),
-                     static_cast<const float2 *>(table), K, (int)C, MT, KS,
-                     (int)n_parts, static_cast<unsigned short *>(packed));
+                     static_cast<const float2 *>(table), K, (int)C, 8 * MT,
+                     16 * KS, (int)n_parts, static_cast<unsigned short *>(packed));
   PG_LAUNCHED();
   return 0;
 }
@@ -240,7 +269,7 @@ int64_t pg_table_bf16_elems(int64_t K, int64_t C, int32_t n_parts) {
   if (C < 1 || C > 64) return -1;
   int MT, KS;
   mfma_shape(C, MT, KS);
-  return K * n_parts * (int64_t)MT * KS * 64 * 8;
+  return K * n_parts * (int64_t)(8 * MT) * (16 * KS) * 2;
 }
 
 int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,

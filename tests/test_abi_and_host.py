@@ -61,7 +61,9 @@ def test_library_version_and_errors():
     # a tile size that is not compiled is rejected before any launch
     rc = lib.pg_grid_fp32(1, 1, 1, 1, 16, 100, 0, 0, 0, None)
     assert rc != 0 and b"tile_cells" in lib.pg_last_error()
-    assert lib.pg_table_bf16_elems(10, 32, 2) == 10 * 2 * 4 * 2 * 64 * 8
+    # compact complex bf16 table: (re, im) per (co, ci), padded to 8 MT x 16 KS
+    assert lib.pg_table_bf16_elems(10, 32, 2) == 10 * 2 * 32 * 32 * 2
+    assert lib.pg_table_bf16_elems(3, 5, 1) == 3 * 1 * 8 * 16 * 2
 
 
 def test_kernel_offsets_and_quant_steps():
