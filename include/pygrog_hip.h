@@ -353,15 +353,17 @@ int pg_table_pack_bf16(const void *table, int64_t K, int64_t C, int32_t n_parts,
                        void *packed, pg_stream_t stream);
 
 /* Grouped complex GEMM on BF16 MFMA with FP32 accumulation.  Replicas are
- * grouped by table entry (ent_ptr (K+1), ent_rep (R) = cell-sorted replica
- * rank, both from pg_sort_pairs_u32 on rep_ent); items (n_items pairs of
- * uint32: entry, first replica offset inside its group) are the 128-replica
- * work chunks, one wavefront each; for every replica r,
- * vbuf[r] = (K[e] y_t[rep_src[r]]) * s * s.  n_terms = 1 (hi*hi) or 3
+ * grouped by table entry: ent_ptr (K+1) and ent_rep (R) = the cell-sorted
+ * replica index at each position of the entry order (both from
+ * pg_sort_pairs_u32 on the replicas' entries); ent_src / ent_s (R) = that
+ * replica's source sample and sqrt weight, in the same entry order.  items
+ * (n_items pairs of uint32: entry, first offset inside its group) are the
+ * 128-replica work chunks, one wavefront each; for the replica r at each
+ * position, vbuf[r] = (K[e] y_t[src]) * s * s.  n_terms = 1 (hi*hi) or 3
  * (hi*hi + hi*lo + lo*hi split-BF16 products). */
 int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
                           const uint32_t *ent_ptr, const uint32_t *ent_rep,
-                          const uint32_t *rep_src, const float *rep_s,
+                          const uint32_t *ent_src, const float *ent_s,
                           const uint32_t *items, int64_t n_items,
                           int64_t K, int64_t C, int32_t n_terms, void *vbuf,
                           pg_stream_t stream);

@@ -122,10 +122,8 @@ class GrogFrameBatch:
         rep_src = torch.cat(src).to(torch.int32).contiguous()
         rep_s = torch.cat(sw).contiguous()
         rep_ent = torch.cat(ent).contiguous()
-        ent_ptr, ent_rep, items, n_items = group_by_entry(rep_ent, Rt, self.K, self.device,
-                                                          GrogInterpolator.GROUP_CHUNK)
-        self._merged = {"rep_off": rep_off, "R": Rt, "rep_src": rep_src, "rep_s": rep_s, "ent_ptr": ent_ptr,
-                        "ent_rep": ent_rep, "items": items, "n_items": n_items}
+        grp = group_by_entry(rep_ent, Rt, self.K, self.device, GrogInterpolator.GROUP_CHUNK, rep_src, rep_s)
+        self._merged = {"rep_off": rep_off, "R": Rt, "groups": grp, "n_items": grp.n_items}
         return self._merged
 
     def _program(self, mode: str, y: torch.Tensor, out: torch.Tensor):
@@ -180,9 +178,10 @@ class GrogFrameBatch:
         tbl = self.frames[0]._table_bf16(2 if n_terms == 3 else 1)
         e0 = timer.start() if timer is not None else None
         if m["n_items"] > 0:
-            _lib.call("pg_apply_grouped_bf16", _lib.ptr(yt), _lib.ptr(tbl), _lib.ptr(m["ent_ptr"]),
-                      _lib.ptr(m["ent_rep"]), _lib.ptr(m["rep_src"]), _lib.ptr(m["rep_s"]), _lib.ptr(m["items"]),
-                      m["n_items"], self.K, self.C, n_terms, _lib.ptr(vbuf), st)
+            grp = m["groups"]
+            _lib.call("pg_apply_grouped_bf16", _lib.ptr(yt), _lib.ptr(tbl), _lib.ptr(grp.ent_ptr),
+                      _lib.ptr(grp.ent_rep), _lib.ptr(grp.ent_src), _lib.ptr(grp.ent_s), _lib.ptr(grp.items),
+                      grp.n_items, self.K, self.C, n_terms, _lib.ptr(vbuf), st)
         if timer is not None:
             timer.stop("pg_apply_grouped_bf16", e0)
         prog.launch("pg_grid_reduce_v", timer)

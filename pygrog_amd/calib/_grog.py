@@ -31,6 +31,7 @@ __all__ = ["GrogInterpolator", "GrogPlan"]
 import copy
 import ctypes
 import gc
+from typing import NamedTuple
 from types import SimpleNamespace
 
 import numpy as np
@@ -299,11 +300,23 @@ class KernelTimer:
         self.events = []
 
 
-def group_by_entry(rep_ent: torch.Tensor, R: int, K: int, device, chunk: int):
-    """Replicas of each table entry: a stable radix sort of ``rep_ent[:R]``
-    -> (ent_ptr (K+1) int32, ent_rep (R) int32 replica ids, items (n, 2)
-    int32 work list of (entry, first replica offset) chunks of ``chunk``,
-    n)."""
+class EntryGroups(NamedTuple):
+    """Replicas grouped by table entry, for the grouped MFMA apply."""
+
+    ent_ptr: torch.Tensor   # (K + 1) int32: group of entry e = positions [ent_ptr[e], ent_ptr[e+1])
+    ent_rep: torch.Tensor   # (R) int32: replica (cell-sorted index) at each position
+    items: torch.Tensor     # (n_items, 2) int32: (entry, first offset) work chunks
+    n_items: int
+    ent_src: torch.Tensor   # (R) int32: source sample of each position's replica
+    ent_s: torch.Tensor     # (R) float32: sqrt weight of each position's replica
+
+
+def group_by_entry(rep_ent: torch.Tensor, R: int, K: int, device, chunk: int, rep_src: torch.Tensor,
+                   rep_s: torch.Tensor) -> EntryGroups:
+    """Replicas of each table entry (a stable radix sort of ``rep_ent[:R]``),
+    the work list of chunks of ``chunk`` replicas, and each replica's source
+    and weight gathered into that order (so the apply reads its records
+    contiguously instead of gathering them)."""
     st = _lib.stream(device)
     keys = rep_ent[:max(R, 1)].to(torch.int32).contiguous()
     vals = torch.arange(max(R, 1), dtype=torch.int32, device=device)
@@ -324,7 +337,14 @@ def group_by_entry(rep_ent: torch.Tensor, R: int, K: int, device, chunk: int):
         items = torch.stack([ent, off], 1).to(torch.int32).contiguous()
     else:
         items = torch.zeros(1, 2, dtype=torch.int32, device=device)
-    return ent_ptr, vals_o, items, total
+    if R > 0:
+        order = vals_o[:R].to(torch.int64)
+        ent_src = rep_src[:R][order].to(torch.int32).contiguous()
+        ent_s = rep_s[:R][order].contiguous()
+    else:
+        ent_src = torch.zeros(1, dtype=torch.int32, device=device)
+        ent_s = torch.zeros(1, dtype=torch.float32, device=device)
+    return EntryGroups(ent_ptr, vals_o, items, total, ent_src, ent_s)
 
 
 class GridProgram:
@@ -753,7 +773,8 @@ class GrogInterpolator:
         groups = []
         for s in range(comp.S):
             R = comp.n_valid[s]
-            groups.append(group_by_entry(comp.rep_ent[s, :max(R, 1)], R, K, dev, self.GROUP_CHUNK))
+            groups.append(group_by_entry(comp.rep_ent[s, :max(R, 1)], R, K, dev, self.GROUP_CHUNK,
+                                         comp.rep_src[s], comp.rep_s[s]))
         comp.entry_groups = groups
         return groups
 
@@ -828,13 +849,13 @@ class GrogInterpolator:
         e0 = timer.start() if timer is not None else None
         for b in range(B):
             for s in range(S):
-                ent_ptr, ent_rep, items, n_items = groups[s]
-                if comp.n_valid[s] == 0 or n_items == 0:
+                grp = groups[s]
+                if comp.n_valid[s] == 0 or grp.n_items == 0:
                     continue
                 yf = yt.data_ptr() + (b * S + s) * C * N * 8
                 vb = vbuf.data_ptr() + (b * S + s) * self._rmax() * C * 8
-                _lib.call("pg_apply_grouped_bf16", yf, _lib.ptr(tbl), _lib.ptr(ent_ptr), _lib.ptr(ent_rep),
-                          comp.rep_src[s].data_ptr(), comp.rep_s[s].data_ptr(), _lib.ptr(items), n_items, K, C,
+                _lib.call("pg_apply_grouped_bf16", yf, _lib.ptr(tbl), _lib.ptr(grp.ent_ptr), _lib.ptr(grp.ent_rep),
+                          _lib.ptr(grp.ent_src), _lib.ptr(grp.ent_s), _lib.ptr(grp.items), grp.n_items, K, C,
                           n_terms, vb, st)
         if timer is not None:
             timer.stop("pg_apply_grouped_bf16", e0)

@@ -72,7 +72,7 @@ template <int MT, int KS, int NTERMS>
 __global__ __launch_bounds__(256) void k_apply_grouped(
     const float *__restrict__ y_t, const unsigned short *__restrict__ tbl,
     const uint32_t *__restrict__ ent_ptr, const uint32_t *__restrict__ ent_rep,
-    const uint32_t *__restrict__ rep_src, const float *__restrict__ rep_s,
+    const uint32_t *__restrict__ ent_src, const float *__restrict__ ent_s,
     const uint32_t *__restrict__ items, int64_t n_items, int64_t nK, int C,
     float *__restrict__ vbuf) {
   constexpr int NP = (NTERMS > 1) ? 2 : 1;
@@ -120,20 +120,18 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
   }
 
   const int j = lane % 16, q = lane / 16;
-  // every tile's replica, source and weight for this lane's column j: two
-  // rounds of independent loads for the whole work item
+  // every tile's replica, source and weight for this lane's column j: one
+  // round of independent, contiguous loads for the whole work item (the
+  // records are stored in entry order next to ent_rep)
   uint32_t rr[kTilesPerWave], src[kTilesPerWave];
   float sw[kTilesPerWave];
 #pragma unroll
   for (int t = 0; t < kTilesPerWave; ++t) {
     const uint32_t idx = start + 16 * t + j;
-    rr[t] = idx < stop ? ent_rep[idx] : 0u;
-  }
-#pragma unroll
-  for (int t = 0; t < kTilesPerWave; ++t) {
-    const bool v = start + 16 * t + j < stop;
-    src[t] = v ? rep_src[rr[t]] : 0u;
-    sw[t] = v ? rep_s[rr[t]] : 0.0f;
+    const bool v = idx < stop;
+    rr[t] = v ? ent_rep[idx] : 0u;
+    src[t] = v ? ent_src[idx] : 0u;
+    sw[t] = v ? ent_s[idx] : 0.0f;
   }
   const bool vec = (twoC & 3) == 0;
   // sample-row fragments of one tile: KS x 8 floats per lane
@@ -220,7 +218,7 @@ static void mfma_shape(int64_t C, int &MT, int &KS) {
 template <int MT, int KS>
 static int launch_grouped(const void *y_t, const void *tbl,
                           const uint32_t *ent_ptr, const uint32_t *ent_rep,
-                          const uint32_t *rep_src, const float *rep_s,
+                          const uint32_t *ent_src, const float *ent_s,
                           const uint32_t *items, int64_t n_items, int64_t K,
                           int64_t C, int32_t n_terms, void *vbuf,
                           hipStream_t st) {
@@ -231,13 +229,13 @@ static int launch_grouped(const void *y_t, const void *tbl,
     hipLaunchKernelGGL((k_apply_grouped<MT, KS, 3>), grid, dim3(256), 0, st,
                        static_cast<const float *>(y_t),
                        static_cast<const unsigned short *>(tbl), ent_ptr,
-                       ent_rep, rep_src, rep_s, items, n_items, K, (int)C,
+                       ent_rep, ent_src, ent_s, items, n_items, K, (int)C,
                        static_cast<float *>(vbuf));
   else
     hipLaunchKernelGGL((k_apply_grouped<MT, KS, 1>), grid, dim3(256), 0, st,
                        static_cast<const float *>(y_t),
                        static_cast<const unsigned short *>(tbl), ent_ptr,
-                       ent_rep, rep_src, rep_s, items, n_items, K, (int)C,
+                       ent_rep, ent_src, ent_s, items, n_items, K, (int)C,
                        static_cast<float *>(vbuf));
   PG_LAUNCHED();
   return 0;
@@ -274,7 +272,7 @@ int64_t pg_table_bf16_elems(int64_t K, int64_t C, int32_t n_parts) {
 
 int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
                           const uint32_t *ent_ptr, const uint32_t *ent_rep,
-                          const uint32_t *rep_src, const float *rep_s,
+                          const uint32_t *ent_src, const float *ent_s,
                           const uint32_t *items, int64_t n_items,
                           int64_t K, int64_t C, int32_t n_terms, void *vbuf,
                           pg_stream_t stream) {
@@ -287,7 +285,7 @@ int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
 #define PG_GROUPED_CASE(MTV, KSV)                                            \
   if (MT == MTV && KS == KSV)                                                \
     return launch_grouped<MTV, KSV>(y_t, table_bf16, ent_ptr, ent_rep,       \
-                                    rep_src, rep_s, items, n_items, K, C,     \
+                                    ent_src, ent_s, items, n_items, K, C,     \
                                     n_terms, vbuf, st);
   PG_GROUPED_CASE(1, 1)
   PG_GROUPED_CASE(2, 1)

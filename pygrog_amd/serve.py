@@ -177,11 +177,11 @@ class GrogServer:
                     vbuf = torch.empty(g._rmax() * comp.C, dtype=torch.complex64, device=self.device)
                     n_terms = 1 if self.mode == "bf16" else 3
                     tbl = g._table_bf16(2 if n_terms == 3 else 1)
-                    ent_ptr, ent_rep, items, n_items = g._entry_groups()[0]
-                    if n_items > 0:
-                        _lib.call("pg_apply_grouped_bf16", _lib.ptr(yt), _lib.ptr(tbl), _lib.ptr(ent_ptr),
-                                  _lib.ptr(ent_rep), comp.rep_src[0].data_ptr(), comp.rep_s[0].data_ptr(),
-                                  _lib.ptr(items), n_items, int(comp.table.shape[0]), comp.C, n_terms,
+                    grp = g._entry_groups()[0]
+                    if grp.n_items > 0:
+                        _lib.call("pg_apply_grouped_bf16", _lib.ptr(yt), _lib.ptr(tbl), _lib.ptr(grp.ent_ptr),
+                                  _lib.ptr(grp.ent_rep), _lib.ptr(grp.ent_src), _lib.ptr(grp.ent_s),
+                                  _lib.ptr(grp.items), grp.n_items, int(comp.table.shape[0]), comp.C, n_terms,
                                   _lib.ptr(vbuf), st.cuda_stream)
                 frames = g._frames(yt, out, 1, vbuf)
                 groups.setdefault((comp.C, comp.G), []).extend(frames)
