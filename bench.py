@@ -422,9 +422,11 @@ def make_serve_jobs(dev, n_jobs: int, seed: int = 5):
     return jobs
 
 
-def run_serving(dev, n_jobs: int, modes=("fp32", "bf16x3"), loads=(0.5, 0.8), seed: int = 5) -> dict:
+def run_serving(dev, n_jobs: int, modes=("fp32", "bf16x3"), loads=(0.3, 0.5, 0.7), seed: int = 5) -> dict:
     """Saturation throughput first (all jobs at t=0), then Poisson streams at
-    fractions of it; p50/p99 latency per (mode, rate)."""
+    fractions of it; p50/p99 latency per (mode, rate).  The sustained rate is
+    the highest offered rate the server kept up with (completed >= 95 % of
+    the offered rate)."""
     from pygrog_amd.serve import GrogServer, poisson_arrivals
 
     jobs = make_serve_jobs(dev, n_jobs, seed)
@@ -445,6 +447,8 @@ def run_serving(dev, n_jobs: int, modes=("fp32", "bf16x3"), loads=(0.5, 0.8), se
             res["rates"][f"{frac:.2f}"] = st
             log(f"serve {mode} @ {lam:.2f} jobs/s ({frac:.0%} of saturation): p50 {st['p50_ms']:.1f} ms, "
                 f"p99 {st['p99_ms']:.1f} ms, {st['throughput_jobs_s']:.2f} jobs/s, mean batch {st['mean_batch']:.2f}")
+        kept = [k for k, st in res["rates"].items() if st["throughput_jobs_s"] >= 0.95 * st["rate_jobs_s"]]
+        res["sustained"] = max(kept, key=float) if kept else min(res["rates"], key=float)
         out["modes"][mode] = res
         log(f"serve {mode} saturation: {mu:.2f} jobs/s, {sat['coil_samples_per_s']:.3e} coil-samples/s, "
             f"p99 {sat['p99_ms']:.1f} ms")
@@ -735,7 +739,7 @@ def main(argv=None) -> int:
     if extras:
         line["extras"] = extras
     if serving is not None:
-        sus = {m: v["rates"][max(v["rates"])] for m, v in serving["modes"].items()}
+        sus = {m: v["rates"][v["sustained"]] for m, v in serving["modes"].items()}
         line["serving"] = serving
         line["serve_p99_ms"] = {m: v["p99_ms"] for m, v in sus.items()}
         line["serve_p50_ms"] = {m: v["p50_ms"] for m, v in sus.items()}

@@ -180,21 +180,45 @@ __global__ void k_plan_weights(const uint32_t *__restrict__ vals,
     const int64_t r0 = cp[c], r1 = cp[c + 1];
     const int64_t base = s * nper + n_invalid[s];
     // gauss_sum.scatter_add_ in float64, valid replicas in natural order
-    // (the stable sort keeps that order inside a cell).
+    // (the stable sort keeps that order inside a cell).  Loads go 16 at a
+    // time (independent, in flight together) and are added in order, so a
+    // cell with thousands of replicas (the k-space centre) is not a chain
+    // of dependent load latencies.
+    constexpr int KB = 16;
+    const float *gs = gauss_sorted + base;
     double gsum = 0.0;
-    for (int64_t r = r0; r < r1; ++r) gsum += (double)gauss_sorted[base + r];
+    for (int64_t rb = r0; rb < r1; rb += KB) {
+      float g[KB];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) g[k] = (rb + k < r1) ? gs[rb + k] : 0.0f;
+#pragma unroll
+      for (int k = 0; k < KB; ++k)
+        if (rb + k < r1) gsum += (double)g[k];
+    }
     // gauss_sum[...].clamp(min=1e-12).float()
     const float denom = (float)(gsum > 1e-12 ? gsum : 1e-12);
     float dens = 0.0f;
-    for (int64_t r = r0; r < r1; ++r) {
-      const float w = gauss_sorted[base + r] / denom;
-      const float sw = sqrtf(w > 0.0f ? w : 0.0f);
-      const uint32_t v = vals[base + r];
-      weights[v] = w;
-      sqrt_weights[base + r] = sw;
-      rep_s[s * rep_stride + r] = sw;
-      rep_src[s * rep_stride + r] = (uint32_t)(((int64_t)v - s * nper) / kw);
-      dens = dens + sw * sw;
+    for (int64_t rb = r0; rb < r1; rb += KB) {
+      float g[KB];
+      uint32_t v[KB];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        g[k] = (rb + k < r1) ? gs[rb + k] : 0.0f;
+        v[k] = (rb + k < r1) ? vals[base + rb + k] : 0u;
+      }
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        if (rb + k < r1) {
+          const int64_t r = rb + k;
+          const float w = g[k] / denom;
+          const float sw = sqrtf(w > 0.0f ? w : 0.0f);
+          weights[v[k]] = w;
+          sqrt_weights[base + r] = sw;
+          rep_s[s * rep_stride + r] = sw;
+          rep_src[s * rep_stride + r] = (uint32_t)(((int64_t)v[k] - s * nper) / kw);
+          dens = dens + sw * sw;
+        }
+      }
     }
     density[t] = dens;
     unit_count[t] = (uint32_t)((r1 - r0 + chunk - 1) / chunk);
