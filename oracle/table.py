@@ -42,6 +42,28 @@ def unit_operators(calib: np.ndarray, lamda: float):
     return {"x": Gx, "y": Gy, "z": Gz}
 
 
+def radial_unit_operators(calib: np.ndarray, coords: np.ndarray, lamda: float = 0.0):
+    """Radial self-calibration (reference _grappa.py:290-323): per spoke a
+    least-squares operator from each readout sample to the next
+    (T = S G, row vectors), log of each, a least-squares fit
+    log G_spoke = dx log Gx + dy log Gy over the spokes with the logs
+    flattened column-major and unflattened row-major, then expm.  calib
+    (coils, spokes, readout), coords (spokes, readout, 2)."""
+    c = np.asarray(calib, dtype=np.complex128)
+    c = c / np.linalg.norm(c)
+    c = np.moveaxis(c, 0, -1)  # (nr, ns, nc)
+    nr, ns, nc = c.shape
+    lth = np.empty((nr, nc * nc), np.complex128)
+    for r in range(nr):
+        G = ridge(c[r, :-1], c[r, 1:], lamda)
+        lth[r] = logm_eig(G).T.reshape(-1)
+    k = np.asarray(coords, np.float64)
+    dx = np.diff(k[..., 0], axis=1).mean(axis=1)
+    dy = np.diff(k[..., 1], axis=1).mean(axis=1)
+    lG = ridge(np.stack([dx, dy], -1).astype(np.complex128), lth, lamda)
+    return {"x": sla.expm(lG[0].reshape(nc, nc)), "y": sla.expm(lG[1].reshape(nc, nc)), "z": None}
+
+
 def logm_eig(G: np.ndarray) -> np.ndarray:
     w, V = np.linalg.eig(np.asarray(G, dtype=np.complex128))
     return V @ np.diag(np.log(w)) @ np.linalg.inv(V)

@@ -152,15 +152,52 @@ def _grappa_op_3d(calib, lamda, host_steps):
     return Gz, Gy, Gx
 
 
+def _radial_grappa_op(calib, lamda, coords, host_steps):
+    """Unit operators from radial data (reference _grappa.py:290-323), FP64.
+
+    One operator per spoke maps each readout sample to the next (least
+    squares over the spoke), its matrix log is fitted as dx * log Gx +
+    dy * log Gy over all spokes (dx, dy = the spoke's mean readout step),
+    and the exponentials are returned as (Gy, Gx).  The logs are flattened
+    column-major and unflattened row-major, as in the reference, which turns
+    the row-vector fit into the column-vector operators the table uses."""
+    calib = calib.movedim(0, -1)  # (nr, ns, nc)
+    nr, _ns, nc = calib.shape
+    c = coords.to(calib.device, torch.float64)
+    xcoord = c[..., 0].T
+    ycoord = c[..., 1].T
+    S = calib[:, :-1, ...]
+    T = calib[:, 1:, ...]
+    Gtheta = lstsq(S, T, lamda, host_steps)  # (nr, nc, nc)
+    lGtheta = _matrix_logm(Gtheta, host_steps).permute(0, 2, 1).reshape(nr, nc * nc).contiguous()
+    dx = torch.diff(xcoord, dim=0).mean(dim=0)
+    dy = torch.diff(ycoord, dim=0).mean(dim=0)
+    dxy = torch.stack([dx, dy], dim=-1).to(lGtheta.dtype)
+    lG = lstsq(dxy, lGtheta, lamda, host_steps)  # (2, nc^2)
+    lGx = lG[0].reshape(nc, nc)
+    lGy = lG[1].reshape(nc, nc)
+    return torch.linalg.matrix_exp(lGy), torch.linalg.matrix_exp(lGx)
+
+
 def train_grappa(train_data, lamda: float | None = None, coords=None, *, device=None, host_steps: list | None = None) -> SimpleNamespace:
-    """Unit-shift GROG operators (reference _grappa.py:211-287), FP64."""
-    if coords is not None:
-        raise NotImplementedError("radial self-calibration (coords=...) is not part of this build")
+    """Unit-shift GROG operators (reference _grappa.py:211-287), FP64.
+
+    Cartesian calibration ``(coils, *2 or 3 spatial dims)``, or with
+    ``coords`` (spokes, readout, 2) radial self-calibration from
+    ``(coils, spokes, readout)`` data (lamda defaults to 0 then, as in the
+    reference)."""
     if lamda is None:
-        lamda = 0.01
+        lamda = 0.01 if coords is None else 0.0
     dev = torch.device(device) if device is not None else torch.device("cpu")
     calib = _to_dev(train_data, dev, torch.complex128)
     calib = calib / torch.linalg.norm(calib)
+    if coords is not None:
+        c = torch.as_tensor(np.asarray(coords) if not isinstance(coords, torch.Tensor) else coords)
+        if c.shape[-1] != 2 or calib.ndim != 3 or tuple(c.shape[:-1]) != tuple(calib.shape[1:]):
+            raise ValueError(f"radial self-calibration takes (coils, spokes, readout) data and (spokes, readout, 2) "
+                             f"coords, got {tuple(calib.shape)} and {tuple(c.shape)}")
+        gy, gx = _radial_grappa_op(calib, lamda, c, host_steps)
+        return SimpleNamespace(x=gx, y=gy, z=None)
     ndim = calib.ndim - 1
     if ndim == 2:
         gy, gx = _grappa_op_2d(calib, lamda, host_steps)

@@ -278,3 +278,33 @@ def test_grid_fp64_matches_brute_force():
     gr, dens = ogrid.grid_reference(y.astype(np.complex64), p, e, t.astype(np.complex64))
     assert ogrid.rel_l2(gr, brute) < 1e-6
     np.testing.assert_allclose(dens[dens > 0], 1.0, rtol=1e-6)
+
+
+def test_radial_unit_operators_recover_known_generators():
+    """Spokes generated by y_{i+1} = y_i expm(dx_s A + dy_s B) (row vectors)
+    with commuting A, B: the radial fit returns expm(A^T), expm(B^T), the
+    column-vector operators (a transposed or swapped result fails)."""
+    import scipy.linalg as sla
+
+    from oracle import table as otable
+
+    rng = np.random.default_rng(2)
+    nc, nr, ns = 4, 12, 24
+    Q = np.linalg.qr(rng.standard_normal((nc, nc)) + 1j * rng.standard_normal((nc, nc)))[0]
+    Qi = np.linalg.inv(Q)
+    A = Q @ np.diag(0.3 * rng.standard_normal(nc) + 0.2j * rng.standard_normal(nc)) @ Qi
+    B = Q @ np.diag(0.3 * rng.standard_normal(nc) + 0.2j * rng.standard_normal(nc)) @ Qi
+    th = np.pi * np.arange(nr) / nr
+    coords = np.zeros((nr, ns, 2))
+    calib = np.zeros((nc, nr, ns), np.complex128)
+    for r in range(nr):
+        d = np.array([np.cos(th[r]), np.sin(th[r])]) * 0.7
+        coords[r] = (np.arange(ns) - ns // 2)[:, None] * d[None]
+        G = sla.expm(d[0] * A + d[1] * B)
+        y = rng.standard_normal(nc) + 1j * rng.standard_normal(nc)
+        for i in range(ns):
+            calib[:, r, i] = y
+            y = y @ G
+    ops = otable.radial_unit_operators(calib, coords, 0.0)
+    np.testing.assert_allclose(ops["x"], sla.expm(A.T), atol=1e-8)
+    np.testing.assert_allclose(ops["y"], sla.expm(B.T), atol=1e-8)
