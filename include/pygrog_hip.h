@@ -370,6 +370,40 @@ int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
 /* Element count (bf16) of the pg_table_pack_bf16 layout, -1 if C > 64. */
 int64_t pg_table_bf16_elems(int64_t K, int64_t C, int32_t n_parts);
 
+/* SENSE expand + zero pad + centring sign, one pass (reference
+ * SparseFFT._fft_pad_gather / _adjoint_single image side,
+ * src/pygrog/operator/_sparse_fft.py:703-789, 1167-1177, and
+ * _base/_fftc.py:114-139): out (B, C, *grid) c64,
+ * out[b,c,g] = s(g) * smaps[c, g-off] * x[b, g-off] inside the centred image
+ * window, 0 outside; smaps NULL: x is (B, C, *image) and out = s(g) * x.
+ * s(g) = (-1)^(sum of g's indices) if modulate else 1; off = (grid -
+ * image) // 2.  With modulate, fftn(out) times (-1)^(k - grid/2) per axis
+ * is the centred FFT of the padded image (even grid sizes).  image_shape,
+ * grid_shape: host arrays of ndim (1..3) entries. */
+int pg_sense_expand_c64(const void *x, const void *smaps, int64_t B, int64_t C,
+                        int32_t ndim, const int64_t *image_shape,
+                        const int64_t *grid_shape, int32_t modulate, void *out,
+                        pg_stream_t stream);
+
+/* Crop + centring sign + conj(smaps) coil sum, one pass (reference
+ * SparseFFT._scatter_ifft_crop / _forward_single combine,
+ * src/pygrog/operator/_sparse_fft.py:574-621, 1154-1165): full (B, C, *grid)
+ * c64 -> out (B, *image) = sign * sum_c conj(smaps[c]) * s(p+off) *
+ * full[b, c, p+off] (coils summed in order); smaps NULL: out (B, C, *image)
+ * = sign * s(p+off) * full[b, c, p+off]. */
+int pg_sense_combine_c64(const void *full, const void *smaps, int64_t B,
+                         int64_t C, int32_t ndim, const int64_t *image_shape,
+                         const int64_t *grid_shape, int32_t modulate,
+                         float sign, void *out, pg_stream_t stream);
+
+/* out[0] + i out[1] = sum_i conj(a_i) * b_i for complex64 a, b (n values),
+ * accumulated in double (the FP64 CG / LSMR scalars), deterministic.
+ * workspace: device buffer of pg_vdot_workspace_bytes() bytes; out: device
+ * array of 2 doubles. */
+int pg_vdot_c64_f64(const void *a, const void *b, int64_t n, void *workspace,
+                    double *out, pg_stream_t stream);
+int64_t pg_vdot_workspace_bytes(void);
+
 /* interpolate(ret_image=False) fused: out[b, co, n*kw + k] =
  *   K[interp_idx[n, k]] @ y[b, :, n]  for every candidate replica
  * (including invalid ones, as the reference does).  y (B, C, N) c64,

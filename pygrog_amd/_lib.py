@@ -111,6 +111,12 @@ _SIGS = {
     "pg_unit_count": [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_transpose_cn_c64": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
     "pg_transpose_frames_c64": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p],
+    "pg_vdot_c64_f64": [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p],
+    "pg_vdot_workspace_bytes": [],
+    "pg_sense_expand_c64": [c_void_p, c_void_p, c_int64, c_int64, ctypes.c_int32, c_void_p, c_void_p,
+                            ctypes.c_int32, c_void_p, c_void_p],
+    "pg_sense_combine_c64": [c_void_p, c_void_p, c_int64, c_int64, ctypes.c_int32, c_void_p, c_void_p,
+                             ctypes.c_int32, ctypes.c_float, c_void_p, c_void_p],
     "pg_table_pack_bf16": [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_table_bf16_elems": [c_int64, c_int64, c_int32],
     "pg_apply_grouped_bf16": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
@@ -121,6 +127,7 @@ _SIGS = {
 _RESTYPES = {
     "pg_last_error": ctypes.c_char_p,
     "pg_table_bf16_elems": c_int64,
+    "pg_vdot_workspace_bytes": c_int64,
 }
 
 _lib = None
@@ -177,6 +184,11 @@ def tile_options(C: int) -> list:
     buf = (ctypes.c_int32 * 8)()
     n = load().pg_grid_tile_options(int(C), buf, 8)
     return [int(buf[i]) for i in range(min(n, 8))]
+
+
+def dims(shape) -> "ctypes.Array":
+    """Host int64 array of a shape, for the entries that take one."""
+    return (ctypes.c_int64 * 3)(*[int(v) for v in shape], *([1] * (3 - len(shape))))
 
 
 def stream(device: torch.device | None = None) -> int:

@@ -32,17 +32,36 @@ def _normalize_kspace_shape(op, b: torch.Tensor) -> torch.Tensor:
     return b.reshape(*b.shape[:-1], *nat)
 
 
+def _vdot_c64_f64(a: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+    """<a, b> of two complex64 device vectors summed in float64 by the HIP
+    reduction (no complex128 copies); a (2,) float64 device tensor."""
+    from .. import _lib
+
+    a = a.reshape(-1).resolve_conj().contiguous()
+    b = b.reshape(-1).resolve_conj().contiguous()
+    ws = torch.empty(int(_lib.load().pg_vdot_workspace_bytes()), dtype=torch.uint8, device=a.device)
+    out = torch.empty(2, dtype=torch.float64, device=a.device)
+    _lib.call("pg_vdot_c64_f64", _lib.ptr(a), _lib.ptr(b), a.numel(), _lib.ptr(ws), _lib.ptr(out),
+              _lib.stream(a.device))
+    return out
+
+
 def _dot(a: torch.Tensor, b: torch.Tensor, dt) -> torch.Tensor:
     """Re <a, b> accumulated in ``dt`` (a real dtype)."""
     ct = torch.complex128 if dt == torch.float64 else torch.complex64
     a = a.reshape(-1)
     b = b.reshape(-1)
+    if (dt == torch.float64 and a.dtype == torch.complex64 and b.dtype == torch.complex64 and a.is_cuda
+            and b.device == a.device):
+        return _vdot_c64_f64(a, b)[0]
     if a.is_complex() or b.is_complex():
         return torch.vdot(a.to(ct), b.to(ct)).real
     return torch.dot(a.to(dt), b.to(dt))
 
 
 def _norm(a: torch.Tensor, dt) -> torch.Tensor:
+    if dt == torch.float64 and a.dtype == torch.complex64 and a.is_cuda:
+        return torch.sqrt(_vdot_c64_f64(a, a)[0])
     ct = torch.complex128 if dt == torch.float64 else torch.complex64
     return torch.linalg.vector_norm(a.to(ct) if a.is_complex() else a.to(dt))
 
@@ -65,18 +84,22 @@ def cg(op, b, *, damp: float = 0.0, x0=None, max_iter: int = 20, tol: float = 1e
     rz_old = _dot(r, z, dt)
     r0 = _norm(r, dt) + 1e-30
     for k in range(max_iter):
-        Ap = AHA(p) + damp * p
+        Ap = AHA(p)
+        if damp != 0.0:
+            Ap = Ap + damp * p
         alpha = rz_old / (_dot(p, Ap, dt) + 1e-30)
         a = alpha.to(x.real.dtype if x.is_complex() else x.dtype)
         x = x + a * p
         r = r - a * Ap
-        rel = (_norm(r, dt) / r0).item()
+        # ||r||^2 as <r, r>: the residual test, and <r, z> itself when z = r
+        rr = _dot(r, r, dt)
+        rel = (torch.sqrt(rr) / r0).item()
         if callback is not None:
             callback(k, x, rel)
         if rel < tol:
             break
         z = preconditioner.apply(r) if preconditioner is not None else r
-        rz_new = _dot(r, z, dt)
+        rz_new = _dot(r, z, dt) if preconditioner is not None else rr
         beta = rz_new / (rz_old + 1e-30)
         p = z + beta.to(a.dtype) * p
         rz_old = rz_new

@@ -19,6 +19,7 @@ import torch
 from .. import _lib
 from .._base._fftc import fft, ifft
 from .._utils import resize
+from ..operator._centred import combine, even_grid, expand
 
 
 class GrogToeplitzOp:
@@ -66,14 +67,25 @@ class GrogToeplitzOp:
         src, dtype = x.device, x.dtype
         ctype = dtype if dtype.is_complex else torch.complex64
         xd = x.to(self.device).to(ctype)
+        C = int(sf.smaps.shape[0]) if sf.smaps is not None else int(x.shape[-len(self.image_shape) - 1])
+        if ctype == torch.complex64 and even_grid(self.grid_shape):
+            # (-1)^n sign + pad in the expand pass; the k-space signs of the two
+            # centred transforms cancel around the real PSF; crop + (-1)^n +
+            # conj(S) coil sum in the combine pass
+            padded = expand(xd.reshape(B * S, *(() if sf.smaps is not None else (C,)), *self.image_shape),
+                            sf.smaps, B * S, C, self.image_shape, self.grid_shape)
+            k = torch.fft.fftn(padded, dim=self.fft_axes, norm="ortho").reshape(B, S, C, *self.grid_shape)
+            k *= self._psf_c.to(ctype).reshape(1, S, 1, *self.grid_shape)
+            full = torch.fft.ifftn(k.reshape(B * S, C, *self.grid_shape), dim=self.fft_axes, norm="ortho")
+            out = combine(full, sf.smaps, B * S, C, self.image_shape, self.grid_shape, 1.0)
+            single = tuple(self.image_shape) if sf.smaps is not None else (C, *self.image_shape)
+            return out.reshape(*B_shape, *s_shape, *single).to(src)
+        psf = self._psf_c.to(ctype).reshape(1, S, 1, *self.grid_shape)
         if sf.smaps is not None:
             sm = sf.smaps.to(self.device, ctype)
-            C = int(sm.shape[0])
             coil = xd.reshape(B, S, 1, *self.image_shape) * sm
         else:
-            C = int(x.shape[-len(self.image_shape) - 1])
             coil = xd.reshape(B, S, C, *self.image_shape)
-        psf = self._psf_c.to(ctype).reshape(1, S, 1, *self.grid_shape)
         if self.grid_shape == self.image_shape:
             full = ifft(fft(coil, axes=self.fft_axes) * psf, axes=self.fft_axes)
         else:

@@ -32,6 +32,7 @@ SOURCES = {
     "plan.hip": ["-ffp-contract=off"],
     "grid.hip": [],
     "grid_mfma.hip": [],
+    "sense.hip": [],
 }
 COMMON_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -1,0 +1,260 @@
+// SENSE expand / combine around torch.fft with the centring folded in.
+//
+// For an even length N (M = N/2) the centred transforms of the reference
+// (src/pygrog/_base/_fftc.py:114-139: ifftshift -> fftn/ifftn -> fftshift)
+// are plain transforms between two sign modulations:
+//   fftc(x)[k]  = (-1)^(k-M) * fftn((-1)^n x)[k]
+//   ifftc(x)[n] = (-1)^(n-M) * ifftn((-1)^k x)[n]
+// (shift theorem, per axis; multiplying by +-1 is exact).  These kernels do
+// the image-side half of that in the same pass as the coil expansion + zero
+// pad (forward direction) or the crop + conj(S) coil sum (adjoint
+// direction); the k-space-side signs are folded into the scatter / gather
+// weights by the callers.  Rolls, pad buffers and the separate multiply and
+// reduction passes of the unfused path disappear.
+//
+//   expand:  out[b, c, g] = s(g) * S[c, g - off] * x[b, g - off]   (g inside
+//            the image window, else 0); without maps x is (B, C, *image)
+//   combine: out[b, p]    = sign * sum_c conj(S[c, p]) * s(p + off) * full[b, c, p + off]
+//            (coils in order); without maps out[b, c, p] = sign * s(p + off) * full[b, c, p + off]
+// with s(g) = (-1)^(g0 + g1 + g2) when modulate, else 1, and off = (grid -
+// image) // 2 per axis (the reference's centred crop / pad).
+#include "common.h"
+
+namespace pg {
+
+struct Dims {
+  int nd;
+  int64_t img[3], grd[3], off[3];
+  int64_t P, G;  // image and grid sizes
+};
+
+__device__ __forceinline__ float parity_sign(int64_t a, int64_t b, int64_t c) {
+  return ((a + b + c) & 1) ? -1.0f : 1.0f;
+}
+
+__global__ void k_sense_expand(const float2 *__restrict__ x,
+                               const float2 *__restrict__ smaps, int64_t B,
+                               int64_t C, Dims d, int modulate,
+                               float2 *__restrict__ out) {
+  const int64_t total = B * C * d.G;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t bc = t / d.G;
+    int64_t g = t - bc * d.G;
+    const int64_t b = bc / C, c = bc - b * C;
+    int64_t gi[3] = {0, 0, 0};
+    for (int a = d.nd - 1; a >= 0; --a) {
+      gi[a] = g % d.grd[a];
+      g /= d.grd[a];
+    }
+    bool inside = true;
+    int64_t p = 0;
+    for (int a = 0; a < d.nd; ++a) {
+      const int64_t q = gi[a] - d.off[a];
+      inside = inside && q >= 0 && q < d.img[a];
+      p = p * d.img[a] + q;
+    }
+    float2 v = make_float2(0.0f, 0.0f);
+    if (inside) {
+      if (smaps) {
+        const float2 s = smaps[c * d.P + p];
+        const float2 xv = x[b * d.P + p];
+        v = make_float2(s.x * xv.x - s.y * xv.y, s.x * xv.y + s.y * xv.x);
+      } else {
+        v = x[(b * C + c) * d.P + p];
+      }
+      if (modulate) {
+        const float sg = parity_sign(gi[0], gi[1], gi[2]);
+        v.x *= sg;
+        v.y *= sg;
+      }
+    }
+    out[t] = v;
+  }
+}
+
+__global__ void k_sense_combine(const float2 *__restrict__ full,
+                                const float2 *__restrict__ smaps, int64_t B,
+                                int64_t C, Dims d, int modulate, float sign,
+                                float2 *__restrict__ out) {
+  const int64_t per_b = smaps ? d.P : C * d.P;
+  const int64_t total = B * per_b;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = t / per_b;
+    const int64_t rem = t - b * per_b;
+    const int64_t c1 = smaps ? 0 : rem / d.P;
+    int64_t p = rem - c1 * d.P;
+    const int64_t p_flat = p;
+    int64_t gi[3] = {0, 0, 0};
+    int64_t g = 0;
+    {
+      int64_t pi[3] = {0, 0, 0};
+      for (int a = d.nd - 1; a >= 0; --a) {
+        pi[a] = p % d.img[a];
+        p /= d.img[a];
+      }
+      for (int a = 0; a < d.nd; ++a) {
+        gi[a] = pi[a] + d.off[a];
+        g = g * d.grd[a] + gi[a];
+      }
+    }
+    const float sg = sign * (modulate ? parity_sign(gi[0], gi[1], gi[2]) : 1.0f);
+    float2 acc;
+    if (smaps) {
+      float ax = 0.0f, ay = 0.0f;
+      for (int64_t c = 0; c < C; ++c) {
+        const float2 s = smaps[c * d.P + p_flat];
+        const float2 f = full[(b * C + c) * d.G + g];
+        // conj(s) * f
+        ax = ax + (s.x * f.x + s.y * f.y);
+        ay = ay + (s.x * f.y - s.y * f.x);
+      }
+      acc = make_float2(ax * sg, ay * sg);
+    } else {
+      const float2 f = full[(b * C + c1) * d.G + g];
+      acc = make_float2(f.x * sg, f.y * sg);
+    }
+    out[t] = acc;
+  }
+}
+
+static int make_dims(int32_t ndim, const int64_t *image_shape,
+                     const int64_t *grid_shape, Dims &d) {
+  PG_CHECK(ndim >= 1 && ndim <= 3, "sense: ndim must be 1..3, got %d", ndim);
+  PG_CHECK(image_shape && grid_shape, "sense: null shapes");
+  d.nd = ndim;
+  d.P = 1;
+  d.G = 1;
+  for (int a = 0; a < 3; ++a) {
+    d.img[a] = a < ndim ? image_shape[a] : 1;
+    d.grd[a] = a < ndim ? grid_shape[a] : 1;
+    PG_CHECK(d.img[a] >= 1 && d.grd[a] >= d.img[a],
+             "sense: image axis %d (%lld) must be in [1, grid %lld]", a,
+             (long long)d.img[a], (long long)d.grd[a]);
+    d.off[a] = (d.grd[a] - d.img[a]) / 2;
+    d.P *= d.img[a];
+    d.G *= d.grd[a];
+  }
+  return 0;
+}
+
+}  // namespace pg
+
+using namespace pg;
+
+extern "C" {
+
+int pg_sense_expand_c64(const void *x, const void *smaps, int64_t B, int64_t C,
+                        int32_t ndim, const int64_t *image_shape,
+                        const int64_t *grid_shape, int32_t modulate, void *out,
+                        pg_stream_t stream) {
+  Dims d;
+  if (make_dims(ndim, image_shape, grid_shape, d)) return 1;
+  PG_CHECK(B >= 0 && C >= 1 && x && out, "sense_expand: bad arguments");
+  const int64_t total = B * C * d.G;
+  if (total == 0) return 0;
+  const int threads = 256;
+  hipLaunchKernelGGL(k_sense_expand, dim3(grid_for(total, threads)),
+                     dim3(threads), 0, as_stream(stream),
+                     static_cast<const float2 *>(x),
+                     static_cast<const float2 *>(smaps), B, C, d, (int)modulate,
+                     static_cast<float2 *>(out));
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_sense_combine_c64(const void *full, const void *smaps, int64_t B,
+                         int64_t C, int32_t ndim, const int64_t *image_shape,
+                         const int64_t *grid_shape, int32_t modulate,
+                         float sign, void *out, pg_stream_t stream) {
+  Dims d;
+  if (make_dims(ndim, image_shape, grid_shape, d)) return 1;
+  PG_CHECK(B >= 0 && C >= 1 && full && out, "sense_combine: bad arguments");
+  const int64_t total = B * (smaps ? d.P : C * d.P);
+  if (total == 0) return 0;
+  const int threads = 256;
+  hipLaunchKernelGGL(k_sense_combine, dim3(grid_for(total, threads)),
+                     dim3(threads), 0, as_stream(stream),
+                     static_cast<const float2 *>(full),
+                     static_cast<const float2 *>(smaps), B, C, d, (int)modulate,
+                     sign, static_cast<float2 *>(out));
+  PG_LAUNCHED();
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// FP64-accumulated inner product of complex64 vectors (the CG / LSMR scalars):
+// out[0] + i out[1] = sum_i conj(a_i) b_i, summed in double, deterministic
+// (fixed per-block ranges, fixed tree, then the block partials in order).
+namespace pg {
+
+constexpr int kDotThreads = 256;
+constexpr int kDotBlocks = 1024;
+
+__global__ __launch_bounds__(kDotThreads) void k_vdot_partial(
+    const float2 *__restrict__ a, const float2 *__restrict__ b, int64_t n,
+    double2 *__restrict__ part) {
+  __shared__ double sr[kDotThreads], si[kDotThreads];
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * per;
+  const int64_t i1 = i0 + per < n ? i0 + per : n;
+  double re = 0.0, im = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kDotThreads) {
+    const float2 x = a[i], y = b[i];
+    re += (double)x.x * (double)y.x + (double)x.y * (double)y.y;
+    im += (double)x.x * (double)y.y - (double)x.y * (double)y.x;
+  }
+  sr[threadIdx.x] = re;
+  si[threadIdx.x] = im;
+  __syncthreads();
+  for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      si[threadIdx.x] += si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(sr[0], si[0]);
+}
+
+__global__ void k_vdot_final(const double2 *__restrict__ part, int nb,
+                             double *__restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double re = 0.0, im = 0.0;
+  for (int i = 0; i < nb; ++i) {
+    re += part[i].x;
+    im += part[i].y;
+  }
+  out[0] = re;
+  out[1] = im;
+}
+
+}  // namespace pg
+
+extern "C" {
+
+int64_t pg_vdot_workspace_bytes(void) { return (int64_t)kDotBlocks * 16; }
+
+int pg_vdot_c64_f64(const void *a, const void *b, int64_t n, void *workspace,
+                    double *out, pg_stream_t stream) {
+  PG_CHECK(n >= 0 && out && workspace, "vdot: bad arguments");
+  PG_CHECK(n == 0 || (a && b), "vdot: null vectors");
+  const hipStream_t st = as_stream(stream);
+  int nb = (int)((n + kDotThreads - 1) / kDotThreads);
+  if (nb > kDotBlocks) nb = kDotBlocks;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(k_vdot_partial, dim3(nb), dim3(kDotThreads), 0, st,
+                     static_cast<const float2 *>(a),
+                     static_cast<const float2 *>(b), n,
+                     static_cast<double2 *>(workspace));
+  PG_LAUNCHED();
+  hipLaunchKernelGGL(k_vdot_final, dim3(1), dim3(64), 0, st,
+                     static_cast<const double2 *>(workspace), nb, out);
+  PG_LAUNCHED();
+  return 0;
+}
+
+}  // extern "C"

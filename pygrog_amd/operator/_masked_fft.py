@@ -20,6 +20,7 @@ from .. import _lib
 from .._base._fftc import fft, ifft
 from .._solve._mixin import SolveMixin
 from .._utils import resize
+from ._centred import combine, even_grid, expand, grid_signs, half_sign
 from ._sparse_fft import numpy_io
 
 
@@ -106,6 +107,19 @@ class MaskedFFT(SolveMixin):
             self._mask_dev = self.mask.to(self._comp_device).reshape(S, *self.grid_shape)
         return self._mask_dev
 
+    def _signed_mask(self, S: int, half: bool) -> torch.Tensor:
+        """mask * (-1)^(k0 + k1 + ...) (* prod (-1)^(N/2) when ``half``):
+        the k-space side of the centred transforms folded into the mask."""
+        key = ("signed", half)
+        cache = self.__dict__.setdefault("_signed_cache", {})
+        if key not in cache:
+            m = self._mask_s(S).to(torch.float32)
+            sg = grid_signs(self.grid_shape, m.device)
+            if half:
+                sg = sg * half_sign(self.grid_shape)
+            cache[key] = (m * sg).contiguous()
+        return cache[key]
+
     def _stack_mask(self, s_flat_idx: int):
         if not self.stack_shape:
             return self.mask, self.density
@@ -135,6 +149,13 @@ class MaskedFFT(SolveMixin):
         d = self._comp_device
         ctype = dtype if dtype.is_complex else torch.complex64
         xd = x.to(d).to(ctype).reshape(B, S, C, *self.grid_shape)
+        if ctype == torch.complex64 and even_grid(self.grid_shape):
+            # signed mask, plain IFFT, one pass for sign + crop + conj(S) sum
+            masked = xd * self._signed_mask(S, False).reshape(1, S, 1, *self.grid_shape)
+            full = torch.fft.ifftn(masked.reshape(B * S, C, *self.grid_shape), dim=self.fft_axes, norm="ortho")
+            img = combine(full, self.smaps, B * S, C, self.image_shape, self.grid_shape, half_sign(self.grid_shape))
+            single = tuple(self.image_shape) if self.smaps is not None else (C, *self.image_shape)
+            return img.reshape(*B_shape, *self.stack_shape, *single).to(src_device)
         m = self._mask_s(S).to(torch.float64 if ctype == torch.complex128 else torch.float32)
         masked = xd * m.reshape(1, S, 1, *self.grid_shape)
         img = resize(ifft(masked, axes=self.fft_axes), (B, S, C, *self.image_shape))
@@ -158,12 +179,18 @@ class MaskedFFT(SolveMixin):
         d = self._comp_device
         ctype = dtype if dtype.is_complex else torch.complex64
         xd = x.to(d).to(ctype)
+        C = int(self.smaps.shape[0]) if self.smaps is not None else int(x.shape[-len(self.image_shape) - 1])
+        if ctype == torch.complex64 and even_grid(self.grid_shape):
+            # one pass for S_c x + zero pad + sign, plain FFT, signed mask
+            padded = expand(xd.reshape(B * S, *(() if self.smaps is not None else (C,)), *self.image_shape),
+                            self.smaps, B * S, C, self.image_shape, self.grid_shape)
+            k = torch.fft.fftn(padded, dim=self.fft_axes, norm="ortho").reshape(B, S, C, *self.grid_shape)
+            k = k * self._signed_mask(S, True).reshape(1, S, 1, *self.grid_shape)
+            return k.reshape(*B_shape, *self.stack_shape, C, *self.grid_shape).to(src_device)
         if self.smaps is not None:
             sm = self.smaps.to(d, ctype)
-            C = int(sm.shape[0])
             coil = xd.reshape(B, S, 1, *self.image_shape) * sm
         else:
-            C = int(x.shape[-len(self.image_shape) - 1])
             coil = xd.reshape(B, S, C, *self.image_shape)
         padded = torch.zeros(B, S, C, *self.grid_shape, dtype=ctype, device=d)
         padded[(slice(None), slice(None), slice(None), *self._pad_slices)] = coil

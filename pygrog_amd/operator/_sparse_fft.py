@@ -29,6 +29,7 @@ from .. import _lib
 from .._base._fftc import fft, ifft
 from .._solve._mixin import SolveMixin
 from .._utils import resize
+from ._centred import combine, even_grid, expand, half_sign
 
 
 def _get_torch_ext():
@@ -107,8 +108,23 @@ class _ScatterPlan:
         self.unit_hot, self.hot = hot_lists(self.cell_unit, S, grid_size, device, min_units)
         self.cell_vrow = ordinary_rows(self.cell_ptr, self.cell_unit, S, grid_size, min_units)
         self.rep_src = self.sort_perm.to(torch.int32).contiguous()
+        self._signed = {}
 
-    def frames(self, y_ptr: int, grid_ptr: int, B: int, C: int, y_stride_s: int | None = None):
+    def signed_weights(self, grid_shape, half: bool) -> torch.Tensor:
+        """sqrt weights times (-1)^(k0 + k1 + ...) of each sample's cell
+        (times prod (-1)^(N/2) when ``half``): the k-space side of the
+        centred transforms, folded into the scatter / gather weights."""
+        if half not in self._signed:
+            from ._centred import cell_signs, half_sign
+
+            sg = cell_signs(self.indices, grid_shape)
+            if half:
+                sg = sg * half_sign(grid_shape)
+            self._signed[half] = (self.sqrt_w * sg).contiguous()
+        return self._signed[half]
+
+    def frames(self, y_ptr: int, grid_ptr: int, B: int, C: int, y_stride_s: int | None = None,
+               rep_s: torch.Tensor | None = None):
         """Frames for data (B, S, C, n) -> grid (B, S, C, G)."""
         fr = []
         S, n, G = self.S, self.n, self.G
@@ -118,7 +134,7 @@ class _ScatterPlan:
                 f.y = y_ptr + ((b * S + s) * C * n) * 8
                 f.grid = grid_ptr + ((b * S + s) * C * G) * 8
                 f.rep_src = self.rep_src.data_ptr() + s * n * 4
-                f.rep_s = self.sqrt_w.data_ptr() + s * n * 4
+                f.rep_s = (self.sqrt_w if rep_s is None else rep_s).data_ptr() + s * n * 4
                 f.unit_cell = self.unit_cell.data_ptr()
                 f.unit_rep = self.unit_rep.data_ptr() + s * 4
                 f.cell_unit = self.cell_unit.data_ptr() + s * G * 4
@@ -135,8 +151,9 @@ class _ScatterPlan:
                 fr.append(f)
         return fr
 
-    def scatter(self, data: torch.Tensor) -> torch.Tensor:
-        """data (B, S, C, n) complex64 on device -> grid (B, S, C, G)."""
+    def scatter(self, data: torch.Tensor, rep_s: torch.Tensor | None = None) -> torch.Tensor:
+        """data (B, S, C, n) complex64 on device -> grid (B, S, C, G);
+        ``rep_s`` replaces the sqrt weights (signed weights)."""
         from ..calib._grog import GridProgram
 
         B, S, C, n = data.shape
@@ -151,23 +168,25 @@ class _ScatterPlan:
                 src = data[:, :, c0:c1].contiguous()
                 dst = torch.empty(B, S, c1 - c0, self.G, dtype=torch.complex64, device=self.device)
             tcc = _lib.load().pg_grid_tile_cells(c1 - c0)
-            prog = GridProgram(self.frames(src.data_ptr(), dst.data_ptr(), B, c1 - c0), c1 - c0, self.G, self.device,
-                               tcc)
+            prog = GridProgram(self.frames(src.data_ptr(), dst.data_ptr(), B, c1 - c0, rep_s=rep_s), c1 - c0,
+                               self.G, self.device, tcc)
             prog.launch("pg_grid_scatter")
             if dst is not out:
                 out[:, :, c0:c1] = dst
         return out
 
-    def gather(self, kgrid: torch.Tensor) -> torch.Tensor:
-        """kgrid (B, S, C, G) -> (B, S, C, n) natural order, sqrt_w applied."""
+    def gather(self, kgrid: torch.Tensor, weights: torch.Tensor | None = None) -> torch.Tensor:
+        """kgrid (B, S, C, G) -> (B, S, C, n) natural order, sqrt_w (or
+        ``weights``, signed) applied."""
         B, S, C, G = kgrid.shape
         out = torch.empty(B, S, C, self.n, dtype=torch.complex64, device=self.device)
         st = _lib.stream(self.device)
         kg = kgrid.contiguous()
+        w = self.sqrt_w if weights is None else weights
         for b in range(B):
             for s in range(S):
                 _lib.call("pg_gather_perm_c64", kg[b, s].data_ptr(), G, self.indices[s].data_ptr(),
-                          self.sqrt_w[s].data_ptr(), self.inv_perm[s].data_ptr(), self.n, C, out[b, s].data_ptr(), st)
+                          w[s].data_ptr(), self.inv_perm[s].data_ptr(), self.n, C, out[b, s].data_ptr(), st)
         return out
 
 
@@ -304,15 +323,23 @@ class SparseFFT(SolveMixin):
         C = int(x.shape[-2])
         sp = self._sp()
         xd = x.to(sp.device).to(torch.complex64).reshape(B, S, C, self.n_samples).contiguous()
-        g = sp.scatter(xd)
-        img = ifft(g.reshape(B, S, C, *self.grid_shape), axes=self.fft_axes)
-        img = resize(img, (B, S, C, *self.image_shape))
-        if self.smaps is not None:
-            cs = self._conj_smaps.to(sp.device, torch.complex64)
-            img = (img * cs).sum(2)
-            single = tuple(self.image_shape)
+        if even_grid(self.grid_shape):
+            # scatter with (-1)^k-signed weights, plain IFFT, then one pass for
+            # the (-1)^(n - M) sign, the crop and the conj(S) coil sum
+            g = sp.scatter(xd, rep_s=sp.signed_weights(self.grid_shape, False))
+            full = torch.fft.ifftn(g.reshape(B * S, C, *self.grid_shape), dim=self.fft_axes, norm="ortho")
+            img = combine(full, self.smaps, B * S, C, self.image_shape, self.grid_shape, half_sign(self.grid_shape))
+            single = tuple(self.image_shape) if self.smaps is not None else (C, *self.image_shape)
         else:
-            single = (C, *self.image_shape)
+            g = sp.scatter(xd)
+            img = ifft(g.reshape(B, S, C, *self.grid_shape), axes=self.fft_axes)
+            img = resize(img, (B, S, C, *self.image_shape))
+            if self.smaps is not None:
+                cs = self._conj_smaps.to(sp.device, torch.complex64)
+                img = (img * cs).sum(2)
+                single = tuple(self.image_shape)
+            else:
+                single = (C, *self.image_shape)
         out = img.reshape(*B_shape, *self.stack_shape, *single)
         if out.dtype != dtype and dtype.is_complex:
             out = out.to(dtype)
@@ -342,16 +369,26 @@ class SparseFFT(SolveMixin):
             xd = xd.to(torch.complex64)
         xd = xd.to(torch.complex64)
         if self.smaps is not None:
-            sm = self.smaps.to(sp.device, torch.complex64)
-            C = int(sm.shape[0])
-            coil = xd.reshape(B, S, 1, *self.image_shape) * sm
+            C = int(self.smaps.shape[0])
         else:
             C = int(x.shape[-len(self.image_shape) - 1])
-            coil = xd.reshape(B, S, C, *self.image_shape)
-        padded = torch.zeros(B, S, C, *self.grid_shape, dtype=torch.complex64, device=sp.device)
-        padded[(slice(None), slice(None), slice(None), *self._pad_slices)] = coil
-        k = fft(padded, axes=self.fft_axes).reshape(B, S, C, self.grid_size)
-        out = sp.gather(k)
+        if even_grid(self.grid_shape):
+            # one pass for S_c x, the zero pad and the (-1)^n sign; plain FFT;
+            # the (-1)^(k - M) sign rides on the gather weights
+            padded = expand(xd.reshape(B * S, *(() if self.smaps is not None else (C,)), *self.image_shape),
+                            self.smaps, B * S, C, self.image_shape, self.grid_shape)
+            k = torch.fft.fftn(padded, dim=self.fft_axes, norm="ortho").reshape(B, S, C, self.grid_size)
+            out = sp.gather(k, sp.signed_weights(self.grid_shape, True))
+        else:
+            if self.smaps is not None:
+                sm = self.smaps.to(sp.device, torch.complex64)
+                coil = xd.reshape(B, S, 1, *self.image_shape) * sm
+            else:
+                coil = xd.reshape(B, S, C, *self.image_shape)
+            padded = torch.zeros(B, S, C, *self.grid_shape, dtype=torch.complex64, device=sp.device)
+            padded[(slice(None), slice(None), slice(None), *self._pad_slices)] = coil
+            k = fft(padded, axes=self.fft_axes).reshape(B, S, C, self.grid_size)
+            out = sp.gather(k)
         out = out.reshape(*B_shape, *self.stack_shape, C, self.n_samples)
         if dtype.is_complex and out.dtype != dtype:
             out = out.to(dtype)

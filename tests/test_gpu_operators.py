@@ -392,6 +392,29 @@ def test_polynomial_preconditioner_on_device_operator(grog_setup):
     assert pc2.spectrum[1] > 0.0
 
 
+@pytest.mark.parametrize("n", [0, 1, 255, 257, 300_001, 4_000_003])
+def test_fp64_vdot_kernel(device, n):
+    """The CG scalars' reduction: <a, b> of complex64 vectors summed in
+    float64 against numpy's complex128 dot of the same values (exact
+    products, so only the summation order differs), and deterministic."""
+    from pygrog_amd._solve._solvers import _vdot_c64_f64
+
+    rng = np.random.default_rng(n)
+    a = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    b = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    ref = np.vdot(a.astype(np.complex128), b.astype(np.complex128))
+    ad, bd = torch.from_numpy(a).to(device), torch.from_numpy(b).to(device)
+    got = _vdot_c64_f64(ad, bd).cpu().numpy()
+    scale = max(1.0, float(np.abs(a).astype(np.float64) @ np.abs(b).astype(np.float64)))
+    assert abs(got[0] - ref.real) <= 1e-13 * scale and abs(got[1] - ref.imag) <= 1e-13 * scale
+    assert np.array_equal(_vdot_c64_f64(ad, bd).cpu().numpy(), got)
+    if n > 1:
+        # strided and lazily conjugated views are materialised first
+        conj = _vdot_c64_f64(ad[::2].conj(), bd[::2]).cpu().numpy()
+        r2 = np.vdot(np.conj(a[::2]).astype(np.complex128), b[::2].astype(np.complex128))
+        assert abs(conj[0] - r2.real) <= 1e-13 * scale and abs(conj[1] - r2.imag) <= 1e-13 * scale
+
+
 def test_cg_fp64_scalars_vs_oracle_cg(device):
     """SENSE CG through the Toeplitz normal operator, FP32 matvecs with
     FP64 scalars, against the oracle's float64 CG on the same system."""
