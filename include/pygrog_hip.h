@@ -270,9 +270,12 @@ typedef struct pg_frame {
 
 /* Pack a (K, C, C) c64 row-major table into the layout read by the fused
  * kernels: packed[e][p][co] = (K[co][2p], K[co][2p+1]) as float4,
- * p < ceil(C/2) (zero padded).  packed holds K * ceil(C/2) * C float4. */
+ * p < G/2 where G is the coil group of C (the smallest of 4, 8, 16, 32, 64
+ * that holds C), zero padded past C.  packed holds
+ * pg_table_packed_elems(K, C) = K * (G/2) * C float4. */
 int pg_table_pack_c64(const void *table, int64_t K, int64_t C, void *packed,
                       pg_stream_t stream);
+int64_t pg_table_packed_elems(int64_t K, int64_t C);
 
 /* Plain FP32 fused gridding: per replica v = (K[e] y[:, src]) * s * s,
  * reduced per cell in plan order (chunked sequential folds), written as a

@@ -102,6 +102,7 @@ _SIGS = {
     "pg_plan_units": [c_void_p, c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p, c_void_p],
     "pg_plan_quantize": [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int32, c_float, c_float, c_float, c_int64, c_int64, c_void_p, c_void_p, c_int64, c_void_p],
     "pg_table_pack_c64": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
+    "pg_table_packed_elems": [c_int64, c_int64],
     "pg_grid_fp32": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
     "pg_grid_reduce_v": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
     "pg_grid_scatter": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
@@ -127,6 +128,7 @@ _SIGS = {
 _RESTYPES = {
     "pg_last_error": ctypes.c_char_p,
     "pg_table_bf16_elems": c_int64,
+    "pg_table_packed_elems": c_int64,
     "pg_vdot_workspace_bytes": c_int64,
 }
 

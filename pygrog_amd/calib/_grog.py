@@ -665,7 +665,8 @@ class GrogInterpolator:
         comp.table_host_steps = res.host_steps
         if C <= 64:
             if packed is None:
-                packed = torch.empty(K * ((C + 1) // 2) * C * 4, dtype=torch.float32, device=dev)
+                packed = torch.empty(int(_lib.load().pg_table_packed_elems(K, C)) * 4, dtype=torch.float32,
+                                     device=dev)
                 _lib.call("pg_table_pack_c64", _lib.ptr(res.table), K, C, _lib.ptr(packed), _lib.stream(dev))
             comp.table_packed = packed
         else:

@@ -31,6 +31,50 @@ inline unsigned grid_for(int64_t n, int threads, int64_t cap = 65536) {
   return static_cast<unsigned>(b);
 }
 
+// Loads and stores through a global-address-space pointer.  Pointers held in
+// device structs (the frame tables) are generic, so plain dereferences
+// compile to flat accesses; an integer round trip into address space 1 makes
+// them global_load / global_store with a scalar base where possible.
+#define PG_GAS __attribute__((address_space(1)))
+typedef float pg_f2n __attribute__((ext_vector_type(2)));
+typedef float pg_f4n __attribute__((ext_vector_type(4)));
+template <class T> __device__ __forceinline__ T gld(const T *p) {
+  return *(const PG_GAS T *)(uintptr_t)p;
+}
+template <> __device__ __forceinline__ float2 gld<float2>(const float2 *p) {
+  const pg_f2n v = *(const PG_GAS pg_f2n *)(uintptr_t)p;
+  return make_float2(v.x, v.y);
+}
+template <> __device__ __forceinline__ float4 gld<float4>(const float4 *p) {
+  const pg_f4n v = *(const PG_GAS pg_f4n *)(uintptr_t)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+template <class T> __device__ __forceinline__ void gst(T *p, T v) {
+  *(PG_GAS T *)(uintptr_t)p = v;
+}
+template <> __device__ __forceinline__ void gst<float2>(float2 *p, float2 v) {
+  pg_f2n n;
+  n.x = v.x;
+  n.y = v.y;
+  *(PG_GAS pg_f2n *)(uintptr_t)p = n;
+}
+template <> __device__ __forceinline__ void gst<float4>(float4 *p, float4 v) {
+  pg_f4n n;
+  n.x = v.x;
+  n.y = v.y;
+  n.z = v.z;
+  n.w = v.w;
+  *(PG_GAS pg_f4n *)(uintptr_t)p = n;
+}
+
+// Orders one wavefront's LDS writes before its later LDS reads (other lanes
+// of the same wavefront); no other wavefront is involved.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class T> struct V2;
 template <> struct V2<float> { using type = float2; };
 template <> struct V2<double> { using type = double2; };

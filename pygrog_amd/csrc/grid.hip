@@ -15,15 +15,19 @@
 //   grid[c]   = ((0 + p_{u0}) + p_{u0+1}) + ...        over the cell's units
 // so results are bitwise reproducible run to run and alone vs batched.
 //
-// Work decomposition: a workgroup (256 threads) owns a tile of TC consecutive
-// cells (TC picked per launch among the compiled shapes); replica values
-// are staged in LDS rows and folded per cell in registers; the tile is
-// written coil-row by coil-row (coalesced, zeros included, no
-// read-modify-write, no atomics).  The tile loop is persistent (grid-stride
-// over tiles).  Cells with many replicas (the radial DC and its neighbours)
-// are reduced by a two-kernel pre-pass -- hot units in parallel, then an
-// in-order fold per hot cell -- whose sums the tile kernel copies, so no
-// workgroup serialises on a hot cell.
+// Work decomposition (the launch shape picks one; none changes a bit):
+//   tile_cells 256: wave tiles -- each wavefront owns 64 consecutive cells,
+//     lane = cell for the bookkeeping, the fold and the coalesced write
+//     (k_wtile: rows staged in LDS, no workgroup barrier; k_wrows, the V
+//     reduction above 16 coils: rows read straight into registers);
+//   tile_cells 128 / 64 / 32: workgroup tiles (k_tile) -- 256 threads own TC
+//     consecutive cells, rows staged in LDS between workgroup barriers.
+// Either way the grid is written coil row by coil row (coalesced, zeros
+// included, no read-modify-write, no atomics) and the tile loop is
+// persistent (grid-stride over tiles).  Cells with many replicas (the radial
+// DC and its neighbours) are reduced by a two-kernel pre-pass -- hot units in
+// parallel, then an in-order fold per hot cell -- written straight into the
+// grid, so no tile serialises on a hot cell.
 #include "common.h"
 
 #include <mutex>
@@ -35,10 +39,19 @@ namespace pg {
 constexpr int kGridThreads = 256;
 constexpr int kMaxDevices = 64;
 
-// Tile sizes compiled per coil group GS (threads per cell = 256 / TC, each
-// thread owning GS * TC / 256 <= 16 coils); the first is the default.
+// Default tile (cells per workgroup) per coil group GS: 256 selects the
+// wave-tile kernels (one wavefront per 64 cells); 64 and 32 the workgroup
+// tiles, whose threads own one cell and GS * TC / 256 coils.
 __host__ __device__ constexpr int default_tile(int GS) {
-  return GS <= 8 ? 256 : GS == 16 ? 128 : GS == 32 ? 64 : 32;
+  return GS <= 16 ? 256 : GS == 32 ? 64 : 32;
+}
+
+// Coil group of a coil count: the lanes that evaluate one replica (one per
+// output coil).  The packed table holds GS / 2 coil pairs per (entry, coil),
+// zero padded past C, so a staged (zero padded) sample row can be applied
+// without a branch per pair.
+__host__ __device__ constexpr int coil_group(int64_t C) {
+  return C <= 4 ? 4 : C <= 8 ? 8 : C <= 16 ? 16 : C <= 32 ? 32 : 64;
 }
 
 // MODE 0: v = (K[e] y_t[src, :]) * s * s     (plain FP32 GROG apply; y is
@@ -64,48 +77,98 @@ __device__ __forceinline__ Rec load_rec(const pg_frame &fr, uint32_t r) {
 
 // K[e][co][:] . y_row, then (v * s) * s.  tb points at the packed table
 // pairs of (entry, coil) (stride C float4), yr at the replica's sample row
-// (global memory or an LDS copy; 16-byte aligned when C is even).  The
-// products are spelled out as fmaf chains, so every caller gets the same bits.
+// in global memory (16-byte aligned when C is even); P = ceil(C / 2) pairs
+// are used.  The products are spelled out as fmaf chains in ascending input
+// coil, the same chain as apply_row_lds: a padding pair adds fmaf(0, 0, v),
+// which leaves v unchanged but for the sign of a zero, and every fold starts
+// from +0, so both give the same grid bits.  The table values of up to 16
+// pairs are loaded together, every load from an in-bounds address.
 template <int GS>
 __device__ __forceinline__ float2 apply_row(const float4 *__restrict__ tb,
                                            const float2 *yr, int C, int P,
                                            float s) {
+  constexpr int PM = (GS + 1) / 2;        // pairs at most
+  constexpr int PB = PM < 16 ? PM : 16;   // pairs loaded together
   float vx = 0.0f, vy = 0.0f;
-  if ((C & 1) == 0) {
-    const float4 *y4 = reinterpret_cast<const float4 *>(yr);
 #pragma unroll
-    for (int p = 0; p < (GS + 1) / 2; ++p) {
-      if (p < P) {
-        const float4 k = tb[p * C];
-        const float4 yy = y4[p];
-        vx = fmaf(k.x, yy.x, vx);
-        vx = fmaf(-k.y, yy.y, vx);
-        vy = fmaf(k.x, yy.y, vy);
-        vy = fmaf(k.y, yy.x, vy);
-        vx = fmaf(k.z, yy.z, vx);
-        vx = fmaf(-k.w, yy.w, vx);
-        vy = fmaf(k.z, yy.w, vy);
-        vy = fmaf(k.w, yy.z, vy);
-      }
+  for (int p0 = 0; p0 < PM; p0 += PB) {
+    if (p0 >= P) break;
+    float4 k[PB];
+#pragma unroll
+    for (int i = 0; i < PB; ++i) {
+      const int p = p0 + i;
+      k[i] = gld(tb + (p < P ? p : P - 1) * C);
     }
-  } else {
+    if ((C & 1) == 0) {
+      const float4 *y4 = reinterpret_cast<const float4 *>(yr);
 #pragma unroll
-    for (int p = 0; p < (GS + 1) / 2; ++p) {
-      if (p < P) {
-        const float4 k = tb[p * C];
-        const float2 y0 = yr[2 * p];
-        vx = fmaf(k.x, y0.x, vx);
-        vx = fmaf(-k.y, y0.y, vx);
-        vy = fmaf(k.x, y0.y, vy);
-        vy = fmaf(k.y, y0.x, vy);
-        if (2 * p + 1 < C) {
-          const float2 y1 = yr[2 * p + 1];
-          vx = fmaf(k.z, y1.x, vx);
-          vx = fmaf(-k.w, y1.y, vx);
-          vy = fmaf(k.z, y1.y, vy);
-          vy = fmaf(k.w, y1.x, vy);
+      for (int i = 0; i < PB; ++i) {
+        const int p = p0 + i;
+        if (p < P) {
+          const float4 yy = y4[p];
+          vx = fmaf(k[i].x, yy.x, vx);
+          vx = fmaf(-k[i].y, yy.y, vx);
+          vy = fmaf(k[i].x, yy.y, vy);
+          vy = fmaf(k[i].y, yy.x, vy);
+          vx = fmaf(k[i].z, yy.z, vx);
+          vx = fmaf(-k[i].w, yy.w, vx);
+          vy = fmaf(k[i].z, yy.w, vy);
+          vy = fmaf(k[i].w, yy.z, vy);
         }
       }
+    } else {
+#pragma unroll
+      for (int i = 0; i < PB; ++i) {
+        const int p = p0 + i;
+        if (p < P) {
+          const float2 y0 = yr[2 * p];
+          vx = fmaf(k[i].x, y0.x, vx);
+          vx = fmaf(-k[i].y, y0.y, vx);
+          vy = fmaf(k[i].x, y0.y, vy);
+          vy = fmaf(k[i].y, y0.x, vy);
+          if (2 * p + 1 < C) {
+            const float2 y1 = yr[2 * p + 1];
+            vx = fmaf(k[i].z, y1.x, vx);
+            vx = fmaf(-k[i].w, y1.y, vx);
+            vy = fmaf(k[i].z, y1.y, vy);
+            vy = fmaf(k[i].w, y1.x, vy);
+          }
+        }
+      }
+    }
+  }
+  vx = vx * s;
+  vy = vy * s;
+  return make_float2(vx * s, vy * s);
+}
+
+// The same product for a staged sample row: y4 is the row in LDS as GS / 2
+// float4 pairs, zero padded past C, and the table is padded alike, so every
+// pair is evaluated without a branch and the loads of a block of up to 16
+// pairs (and of two rows, when the caller evaluates two) issue together.
+template <int GS>
+__device__ __forceinline__ float2 apply_row_lds(const float4 *__restrict__ tb,
+                                               const float4 *y4, int C,
+                                               float s) {
+  constexpr int PM = GS / 2;
+  constexpr int PB = PM < 16 ? PM : 16;
+  float vx = 0.0f, vy = 0.0f;
+#pragma unroll
+  for (int p0 = 0; p0 < PM; p0 += PB) {
+    float4 k[PB];
+#pragma unroll
+    for (int i = 0; i < PB; ++i) k[i] = gld(tb + (p0 + i) * C);
+#pragma unroll
+    for (int i = 0; i < PB; ++i) {
+      const float4 yy = y4[p0 + i];
+      vx = fmaf(k[i].x, yy.x, vx);
+      vx = fmaf(-k[i].y, yy.y, vx);
+      vy = fmaf(k[i].x, yy.y, vy);
+      vy = fmaf(k[i].y, yy.x, vy);
+      vx = fmaf(k[i].z, yy.z, vx);
+      vx = fmaf(-k[i].w, yy.w, vx);
+      vy = fmaf(k[i].z, yy.w, vy);
+      vy = fmaf(k[i].w, yy.z, vy);
     }
   }
   vx = vx * s;
@@ -128,7 +191,7 @@ __device__ __forceinline__ float2 coil_value(const pg_frame &fr, uint32_t r,
     return make_float2(a.x * q.s, a.y * q.s);
   }
   const float4 *tb = reinterpret_cast<const float4 *>(fr.table) +
-                     (int64_t)q.ent * P * C + co;
+                     (int64_t)q.ent * (GS / 2) * C + co;
   const float2 *yr = reinterpret_cast<const float2 *>(fr.y) + (int64_t)q.src * C;
   return apply_row<GS>(tb, yr, C, P, q.s);
 }
@@ -241,7 +304,8 @@ __global__ __launch_bounds__(kGridThreads) void k_tile(
   constexpr int TPC = kGridThreads / TC;  // threads per cell
   constexpr int CPT = GS / TPC;           // coils per thread
   constexpr int NG = kGridThreads / GS;   // apply groups
-  static_assert(TPC * TC == kGridThreads && CPT >= 1 && CPT <= 16, "tile shape");
+  static_assert(TPC * TC == kGridThreads && CPT >= 1 && CPT <= 16 && TC < kGridThreads,
+                "tile shape");
   __shared__ __align__(16) float2 vrow[RB * RS];
   __shared__ float2 vout[MODE == 0 ? RB * OS : 1];  // MODE 0: applied rows
   __shared__ uint32_t rmap[RB];                    // MODE 1: replica of row
@@ -269,8 +333,8 @@ __global__ __launch_bounds__(kGridThreads) void k_tile(
     while (f + 1 < F && tile_ptr[f + 1] <= L) ++f;
     return f;
   };
-  // the phase-A values this thread stages: entries tid and (thread 0) TC
-  auto fetch_head = [&](uint32_t L, int f, uint32_t &c0v, uint32_t &v0v, uint32_t &c1v, uint32_t &v1v) {
+  // the phase-A values this thread stages: entry tid (tid <= TC < 256)
+  auto fetch_head = [&](uint32_t L, int f, uint32_t &c0v, uint32_t &v0v) {
     const pg_frame *q = frames + f;
     const int64_t cell0 = (int64_t)(L - tile_ptr[f]) * TC;
     const int64_t rem = q->n_cell - cell0;
@@ -279,17 +343,13 @@ __global__ __launch_bounds__(kGridThreads) void k_tile(
       c0v = q->cell_row[cell0 + tid];
       v0v = q->cell_vrow[cell0 + tid];
     }
-    if (tid == 0 && nc == kGridThreads) {
-      c1v = q->cell_row[cell0 + kGridThreads];
-      v1v = q->cell_vrow[cell0 + kGridThreads];
-    }
   };
 
   uint32_t L = blockIdx.x;
   if (L >= n_tiles) return;
   int f = frame_of(L);
-  uint32_t pc0 = 0, pv0 = 0, pc1 = 0, pv1 = 0;
-  fetch_head(L, f, pc0, pv0, pc1, pv1);
+  uint32_t pc0 = 0, pv0 = 0;
+  fetch_head(L, f, pc0, pv0);
   for (;;) {
     const pg_frame fr = frames[f];
     const int64_t cell0 = (int64_t)(L - tile_ptr[f]) * TC;
@@ -302,16 +362,12 @@ __global__ __launch_bounds__(kGridThreads) void k_tile(
       crow[tid] = pc0;
       vr[tid] = pv0;
     }
-    if (tid == 0 && ncell == kGridThreads) {
-      crow[kGridThreads] = pc1;
-      vr[kGridThreads] = pv1;
-    }
     const uint32_t Ln = L + gridDim.x;
     const bool more = Ln < n_tiles;
     int fn = f;
     if (more) {
       fn = frame_of(Ln);
-      fetch_head(Ln, fn, pc0, pv0, pc1, pv1);
+      fetch_head(Ln, fn, pc0, pv0);
     }
     __syncthreads();
     // cells with rows but no ordinary rows are hot or multi-unit
@@ -390,9 +446,9 @@ __global__ __launch_bounds__(kGridThreads) void k_tile(
           const float4 *tb = reinterpret_cast<const float4 *>(fr.table) + lane;
           for (int j1 = grp; j1 < nk; j1 += 2 * NG) {
             const int j2 = j1 + NG;
-            const float2 v1 = apply_row<GS>(tb + (int64_t)rent[j1] * P * C, &vrow[j1 * RS], C, P, rsw[j1]);
+            const float2 v1 = apply_row<GS>(tb + (int64_t)rent[j1] * (GS / 2) * C, &vrow[j1 * RS], C, P, rsw[j1]);
             float2 v2 = make_float2(0.0f, 0.0f);
-            if (j2 < nk) v2 = apply_row<GS>(tb + (int64_t)rent[j2] * P * C, &vrow[j2 * RS], C, P, rsw[j2]);
+            if (j2 < nk) v2 = apply_row<GS>(tb + (int64_t)rent[j2] * (GS / 2) * C, &vrow[j2 * RS], C, P, rsw[j2]);
             vout[j1 * OS + lane] = v1;
             if (j2 < nk) vout[j2 * OS + lane] = v2;
           }
@@ -436,6 +492,433 @@ __global__ __launch_bounds__(kGridThreads) void k_tile(
     if (!more) break;
     L = Ln;
     f = fn;
+  }
+}
+
+// Wave-tile kernel (tile_cells = 256).  Same canonical arithmetic and the
+// same cells as k_tile, but the unit of work is one wavefront owning 64
+// consecutive cells of a frame: lane t is cell t for the row bookkeeping, the
+// fold and the write, so no phase waits on another wavefront (wavefront-local
+// LDS ordering only, no workgroup barrier).  A 256-thread workgroup runs the
+// four wave tiles of a 256-cell tile; the tile loop is grid-stride.  The
+// ordinary rows of the wave tile (virtual-row order, as in k_tile) go through
+// batches of at most RB rows:
+//   R  lane t writes the records of its cell's rows into LDS (replica for
+//      MODE 1; source and weight for MODE 2; plus entry for MODE 0);
+//   S  the wavefront stages the rows with 16-byte loads (MODE 0: sample rows
+//      of all C coils; MODE 1: V rows; MODE 2: scaled coil-major values);
+//   A  MODE 0: groups of CW lanes (lane = output coil) read their row's
+//      staged sample values (LDS broadcast) and the table row from memory and
+//      write the applied values over the staged row (every lane of the group
+//      has read the row before any lane writes it: one instruction stream);
+//   F  lane t adds its cell's rows in order, in registers.
+// After the last batch each lane writes its cell's coils: consecutive lanes,
+// consecutive cells (512-byte stores per coil).  More than 32 coils are done
+// in passes of 32 output coils.
+constexpr int kWaveCells = 64;
+constexpr int kWaveTile = (kGridThreads / kWave) * kWaveCells;
+
+template <int GS> struct WaveRows {
+  static constexpr int CW = GS < 32 ? GS : 32;  // output coils per pass
+  static constexpr int RS = GS + 2;              // staged row stride (float2): 16-byte rows
+  // rows per batch: at least the rows of one ordinary cell (one unit of at
+  // most 32 replicas), so a batch always holds whole cells
+  static constexpr int RB = GS <= 8 ? 64 : GS == 16 ? 48 : 32;
+};
+
+// Table pairs of one (entry, coil), N at a time, and their products with a
+// staged row (the chain of apply_row / apply_row_lds).
+template <int N>
+__device__ __forceinline__ void load_pairs(const float4 *__restrict__ tb, int C,
+                                           float4 (&k)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) k[i] = gld(tb + i * C);
+}
+template <int N>
+__device__ __forceinline__ void dot_pairs(const float4 (&k)[N], const float4 *y4,
+                                          float &vx, float &vy) {
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const float4 yy = y4[i];
+    vx = fmaf(k[i].x, yy.x, vx);
+    vx = fmaf(-k[i].y, yy.y, vx);
+    vy = fmaf(k[i].x, yy.y, vy);
+    vy = fmaf(k[i].y, yy.x, vy);
+    vx = fmaf(k[i].z, yy.z, vx);
+    vx = fmaf(-k[i].w, yy.w, vx);
+    vy = fmaf(k[i].z, yy.w, vy);
+    vy = fmaf(k[i].w, yy.z, vy);
+  }
+}
+
+template <int GS, int MODE>
+__global__ __launch_bounds__(kGridThreads) void k_wtile(
+    const pg_frame *__restrict__ frames, const uint32_t *__restrict__ tile_ptr,
+    int F, int C, uint32_t n_tiles) {
+  constexpr int CW = WaveRows<GS>::CW;
+  constexpr int RS = WaveRows<GS>::RS;
+  constexpr int RB = WaveRows<GS>::RB;
+  constexpr int NW = kGridThreads / kWave;
+  constexpr int PM = GS / 2;  // table pairs per (entry, coil)
+  static_assert(GS % 2 == 0 && (RS % 2) == 0 && kWave % CW == 0, "wave tile shape");
+  __shared__ __align__(16) float2 s_rows[NW][RB * RS];
+  __shared__ uint32_t s_ra[NW][RB];                 // MODE 1: replica; else source
+  __shared__ uint32_t s_rb[NW][MODE == 0 ? RB : 1];  // MODE 0: entry
+  __shared__ float s_rw[NW][MODE == 1 ? 1 : RB];     // MODE 0/2: weight
+
+  const int w = threadIdx.x / kWave;
+  const int l = threadIdx.x % kWave;
+  float2 *rows = s_rows[w];
+  uint32_t *ra = s_ra[w];
+  uint32_t *rb = s_rb[w];
+  float *rw = s_rw[w];
+  const uint32_t tp1 = tile_ptr[1];
+
+  for (uint32_t L = blockIdx.x; L < n_tiles; L += gridDim.x) {
+    int f = tp1 > 0 ? (int)(L / tp1) : 0;
+    if (f > F - 1) f = F - 1;
+    while (f > 0 && tile_ptr[f] > L) --f;
+    while (f + 1 < F && tile_ptr[f + 1] <= L) ++f;
+    const pg_frame fr = frames[f];
+    const int64_t c0 = (int64_t)(L - tile_ptr[f]) * kWaveTile + (int64_t)w * kWaveCells;
+    if (c0 >= fr.n_cell) continue;  // uniform over the wavefront
+    const int64_t remc = fr.n_cell - c0;
+    const int ncw = remc < kWaveCells ? (int)remc : kWaveCells;
+    const bool own = l < ncw;
+    // record slots hold valid indices (source / replica / entry 0) until the
+    // batches overwrite them: the staging loads of lanes past a batch's rows
+    // read them (the frame has at least one source and replica once it has a
+    // row)
+    for (int j = l; j < RB; j += kWave) {
+      ra[j] = 0u;
+      if (MODE == 0) rb[j] = 0u;
+      if (MODE != 1) rw[j] = 0.0f;
+    }
+
+    // the cell's row ranges; cells with rows but no ordinary rows are hot
+    // (written by the pre-pass) or multi-unit (folded from memory below)
+    const int64_t cl = c0 + (own ? l : 0);
+    const uint32_t vr0 = gld(fr.cell_vrow + cl), vr1 = gld(fr.cell_vrow + cl + 1);
+    const uint32_t cr0 = gld(fr.cell_row + cl), cr1 = gld(fr.cell_row + cl + 1);
+    // (a cell with more ordinary rows than a batch holds -- only with a
+    // unit chunk above RB -- is also folded from memory; its rows still pass
+    // through the batches, so every staged row slot holds a real record)
+    const bool special = own && vr1 == vr0 && cr1 > cr0;
+    const bool big = own && vr1 - vr0 > (uint32_t)RB;
+    int tag = -1;
+    uint32_t ua = 0, ub = 0;
+    if (special || big) {
+      ua = gld(fr.cell_unit + cl);
+      ub = gld(fr.cell_unit + cl + 1);
+      const int h = (special && fr.n_hot > 0) ? gld(fr.unit_hot + ua) : -1;
+      tag = h >= 0 ? h : -2;
+    }
+    const uint32_t base = __builtin_amdgcn_readfirstlane(vr0);
+    const uint32_t lo = own ? vr0 - base : 0u;
+    const uint32_t hi = own ? vr1 - base : 0u;
+    const uint32_t hf = big ? lo : hi;  // rows the batch must hold for this cell
+
+    for (int cc = 0; cc < C; cc += CW) {
+      // batches of whole cells [t0, t1) whose rows fit in RB
+      for (int t0 = 0; t0 < ncw;) {
+        // (wave-uniform values: read from one lane into scalar registers)
+        const uint32_t k0 = __builtin_amdgcn_readfirstlane(__shfl(lo, t0));
+        const uint64_t over = __ballot(own && l >= t0 && hf - k0 > (uint32_t)RB);
+        const int t1 = over ? __ffsll((unsigned long long)over) - 1 : ncw;  // > t0
+        const int nk = (int)(__builtin_amdgcn_readfirstlane(__shfl(hf, t1 - 1)) - k0);
+        const bool in_b = own && l >= t0 && l < t1;
+        // this lane's rows inside the batch rows [k0, k0 + nk)
+        const uint32_t a = lo > k0 ? lo : k0;
+        const uint32_t bb = hi < k0 + nk ? hi : k0 + nk;
+        const uint32_t b = bb > a ? bb : a;
+        // R: records of this lane's rows in the batch, four rows' loads at a
+        // time (rows past the lane's last re-read its first)
+        for (uint32_t k1 = a; k1 < b; k1 += 4) {
+          uint32_t r[4], q0[4], q1[4];
+          float q2[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) r[i] = cr0 + ((k1 + i < b ? k1 + i : k1) - lo);
+          if (MODE != 1) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              q0[i] = gld(fr.rep_src + r[i]);
+              q2[i] = gld(fr.rep_s + r[i]);
+              if (MODE == 0) q1[i] = gld(fr.rep_ent + r[i]);
+            }
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            if (k1 + i < b) {
+              const int j = (int)(k1 + i - k0);
+              if (MODE == 1) {
+                ra[j] = r[i];
+              } else {
+                ra[j] = q0[i];
+                rw[j] = q2[i];
+                if (MODE == 0) rb[j] = q1[i];
+              }
+            }
+          }
+        }
+        wave_lds_sync();
+        // S: stage the rows: lane = (row, coil) with SW coils per row (all GS
+        // coils for MODE 0, zero padded past C; the pass's CW coils else),
+        // up to 16 loads per lane issued before their LDS writes.  Record
+        // slots past the batch hold valid indices (reset per tile), so the
+        // loads of lanes past the batch are in bounds; they store nothing.
+        {
+          constexpr int SW = MODE == 0 ? GS : CW;
+          constexpr int RPI = kWave / SW;       // rows per load instruction
+          constexpr int NB = 16 * RPI;          // rows per load round
+          const int c = l % SW;
+          const int j0 = l / SW;
+          const int cg = MODE == 0 ? c : cc + c;  // coil of this lane's values
+          const bool cv = cg < C;
+          const int cs = cv ? cg : 0;
+          for (int jb = 0; jb < nk; jb += NB) {
+            float2 v[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+              if (jb + i * RPI >= nk) break;
+              const uint32_t q = ra[jb + j0 + i * RPI];
+              if (MODE == 0)
+                v[i] = gld(reinterpret_cast<const float2 *>(fr.y) + (int64_t)q * C + cs);
+              else if (MODE == 1)
+                v[i] = gld(reinterpret_cast<const float2 *>(fr.vbuf) + (int64_t)q * C + cs);
+              else
+                v[i] = gld(reinterpret_cast<const float2 *>(fr.y) + (int64_t)cs * fr.n_src + q);
+            }
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+              if (jb + i * RPI >= nk) break;
+              const int j = jb + j0 + i * RPI;
+              if (j < nk) {
+                float2 x = cv ? v[i] : make_float2(0.0f, 0.0f);
+                if (MODE == 2) {
+                  const float sj = rw[j];
+                  x = make_float2(x.x * sj, x.y * sj);
+                }
+                if (MODE == 0 || cv) rows[j * RS + c] = x;
+              }
+            }
+          }
+        }
+        wave_lds_sync();
+        // A (MODE 0): apply each row's table row.  Up to 8 coils: two rows
+        // per group, their 2 x GS/2 table loads issued together; 16 and 32
+        // coils: one row; 64 coils: one row in two halves.
+        if (MODE == 0) {
+          constexpr int NG = kWave / CW;
+          constexpr int RIF = GS <= 8 ? 2 : 1;
+          const int co = l % CW;
+          const int g = l / CW;
+          const bool cv = cc + co < C;
+          const float4 *tb = reinterpret_cast<const float4 *>(fr.table) + cc + (cv ? co : 0);
+          for (int j1 = g; j1 < nk; j1 += RIF * NG) {
+            const float4 *y1 = reinterpret_cast<const float4 *>(&rows[j1 * RS]);
+            float ax = 0.0f, ay = 0.0f;
+            if (RIF == 2) {
+              // the second row's slot holds a valid entry even past the batch
+              const int j2 = j1 + NG;
+              const float4 *y2 = reinterpret_cast<const float4 *>(&rows[j2 * RS]);
+              float4 k1[PM], k2[PM];
+              load_pairs<PM>(tb + (int64_t)rb[j1] * PM * C, C, k1);
+              load_pairs<PM>(tb + (int64_t)rb[j2] * PM * C, C, k2);
+              float bx = 0.0f, by = 0.0f;
+              dot_pairs<PM>(k1, y1, ax, ay);
+              dot_pairs<PM>(k2, y2, bx, by);
+              const float s2 = rw[j2];
+              bx = bx * s2;
+              by = by * s2;
+              if (cv && j2 < nk) rows[j2 * RS + co] = make_float2(bx * s2, by * s2);
+            } else {
+              constexpr int PB = PM < 16 ? PM : 16;
+              const float4 *t1 = tb + (int64_t)rb[j1] * PM * C;
+#pragma unroll
+              for (int p0 = 0; p0 < PM; p0 += PB) {
+                float4 k1[PB];
+                load_pairs<PB>(t1 + p0 * C, C, k1);
+                dot_pairs<PB>(k1, y1 + p0, ax, ay);
+              }
+            }
+            const float s1 = rw[j1];
+            ax = ax * s1;
+            ay = ay * s1;
+            if (cv) rows[j1 * RS + co] = make_float2(ax * s1, ay * s1);
+          }
+          wave_lds_sync();
+        }
+        // F: this lane's rows, in order, then its cell's coils of this pass
+        float2 *g = reinterpret_cast<float2 *>(fr.grid) + c0 + l;
+        if (in_b && tag == -1) {
+          // eight coils at a time (few live registers; the rows stay in LDS)
+          constexpr int FC = CW < 8 ? CW : 8;
+#pragma unroll 1
+          for (int c8 = 0; c8 < CW; c8 += FC) {
+            float2 acc[FC];
+#pragma unroll
+            for (int c = 0; c < FC; ++c) acc[c] = make_float2(0.0f, 0.0f);
+#pragma unroll 1
+            for (uint32_t k = a; k < b; ++k) {
+              const float2 *vp = &rows[(k - k0) * RS + c8];
+#pragma unroll
+              for (int c = 0; c < FC; ++c) {
+                const float2 v = vp[c];
+                acc[c].x = acc[c].x + v.x;
+                acc[c].y = acc[c].y + v.y;
+              }
+            }
+#pragma unroll
+            for (int c = 0; c < FC; ++c)
+              if (cc + c8 + c < C)
+                gst(g + (int64_t)(cc + c8 + c) * fr.n_cell, make_float2(0.0f + acc[c].x, 0.0f + acc[c].y));
+          }
+        } else if (in_b && tag == -2) {
+#pragma unroll 1
+          for (int c = 0; c < CW; ++c)
+            if (cc + c < C)
+              gst(g + (int64_t)(cc + c) * fr.n_cell, multi_unit_coil<GS, MODE>(frames, f, ua, ub, cc + c, C));
+        }
+        wave_lds_sync();
+        t0 = t1;
+      }
+    }
+  }
+}
+
+// Row-direct reduction (MODE 1, tile_cells = 256): lane t of a wavefront owns
+// cell t of 64 consecutive cells and reads its cell's V rows straight from
+// memory (the rows of consecutive cells are consecutive, so the wavefront's
+// loads cover one contiguous block) and adds them in order in registers; the
+// grid is then written coil row by coil row, consecutive lanes on
+// consecutive cells.  No LDS, no barrier.  The first two rows of every cell
+// (all coils up to 32) are loaded in one round (addresses clamped in bounds,
+// unused values discarded), so a wavefront waits on memory about once per
+// tile; further rows follow one at a time.  The next tile's cell ranges are
+// fetched while the current tile is processed.  Same canonical fold as
+// k_tile: ((0 + v_r0) + v_r1) + ..., written as 0 + sum.
+template <int W>
+__device__ __forceinline__ void load_row(const float2 *vr, int C, int cc, bool even,
+                                         float2 (&v)[W]) {
+  if (even) {
+#pragma unroll
+    for (int i = 0; i < W / 2; ++i) {
+      const int c = cc + 2 * i < C ? 2 * i : 0;
+      const float4 q = gld(reinterpret_cast<const float4 *>(vr + c));
+      v[2 * i] = make_float2(q.x, q.y);
+      v[2 * i + 1] = make_float2(q.z, q.w);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < W; ++i) v[i] = gld(vr + (cc + i < C ? i : 0));
+  }
+}
+
+#ifndef PG_WROWS_CW
+#define PG_WROWS_CW 16
+#endif
+#ifndef PG_WROWS_TWO
+#define PG_WROWS_TWO 0
+#endif
+template <int GS>
+__global__ __launch_bounds__(kGridThreads) void k_wrows(
+    const pg_frame *__restrict__ frames, const uint32_t *__restrict__ tile_ptr,
+    int F, int C, uint32_t n_tiles) {
+  constexpr int CW = GS < PG_WROWS_CW ? GS : PG_WROWS_CW;  // coils per pass
+  constexpr bool TWO = PG_WROWS_TWO != 0;  // second row loaded with the first
+  const int w = threadIdx.x / kWave;
+  const int l = threadIdx.x % kWave;
+  const uint32_t tp1 = tile_ptr[1];
+  const bool even = (C & 1) == 0;
+  auto frame_of = [&](uint32_t L) -> int {
+    int f = tp1 > 0 ? (int)(L / tp1) : 0;
+    if (f > F - 1) f = F - 1;
+    while (f > 0 && tile_ptr[f] > L) --f;
+    while (f + 1 < F && tile_ptr[f + 1] <= L) ++f;
+    return f;
+  };
+  // this lane's cell of tile L and its row ranges
+  struct Head {
+    int f;
+    int64_t cl;
+    uint32_t vr0, vr1, cr0, cr1;
+  };
+  auto head = [&](uint32_t L) -> Head {
+    Head h;
+    h.f = frame_of(L);
+    const pg_frame *q = frames + h.f;
+    const int64_t nc = q->n_cell;
+    h.cl = (int64_t)(L - tile_ptr[h.f]) * kWaveTile + (int64_t)w * kWaveCells + l;
+    const int64_t c = h.cl < nc ? h.cl : nc - 1;  // in bounds; unused past the end
+    h.vr0 = gld(q->cell_vrow + c);
+    h.vr1 = gld(q->cell_vrow + c + 1);
+    h.cr0 = gld(q->cell_row + c);
+    h.cr1 = gld(q->cell_row + c + 1);
+    return h;
+  };
+  uint32_t L = blockIdx.x;
+  if (L >= n_tiles) return;
+  Head h = head(L);
+  for (;;) {
+    const uint32_t Ln = L + gridDim.x;
+    const bool more = Ln < n_tiles;
+    Head hn = h;
+    if (more) hn = head(Ln);
+    const pg_frame fr = frames[h.f];
+    if (h.cl < fr.n_cell) {
+      float2 *g = reinterpret_cast<float2 *>(fr.grid) + h.cl;
+      if (h.vr1 == h.vr0 && h.cr1 > h.cr0) {
+        // hot cell (the pre-pass writes it) or several units: fold from memory
+        const uint32_t ua = gld(fr.cell_unit + h.cl), ub = gld(fr.cell_unit + h.cl + 1);
+        if (!(fr.n_hot > 0 && gld(fr.unit_hot + ua) >= 0)) {
+#pragma unroll 1
+          for (int c = 0; c < C; ++c)
+            gst(g + (int64_t)c * fr.n_cell, multi_unit_coil<GS, 1>(frames, h.f, ua, ub, c, C));
+        }
+      } else {
+        const uint32_t n = h.cr1 - h.cr0;
+        // rows 0 and 1 of the cell: in bounds even for fewer rows (row 0 of
+        // the frame's buffer always exists)
+        const uint32_t r0 = n >= 1 ? h.cr0 : 0u;
+        const uint32_t r1 = n >= 2 ? h.cr0 + 1 : r0;
+        const uint32_t rn = TWO ? h.cr0 + 2 : h.cr0 + 1;  // first row of the loop
+        const float2 *vb = reinterpret_cast<const float2 *>(fr.vbuf);
+#pragma unroll 1
+        for (int cc = 0; cc < C; cc += CW) {
+          float2 a[CW], b[CW];
+          load_row<CW>(vb + (int64_t)r0 * C + cc, C, cc, even, a);
+          if (TWO) load_row<CW>(vb + (int64_t)r1 * C + cc, C, cc, even, b);
+          float2 acc[CW];
+#pragma unroll
+          for (int c = 0; c < CW; ++c) {
+            acc[c] = make_float2(0.0f, 0.0f);
+            if (n >= 1) {
+              acc[c].x = acc[c].x + a[c].x;
+              acc[c].y = acc[c].y + a[c].y;
+            }
+            if (TWO && n >= 2) {
+              acc[c].x = acc[c].x + b[c].x;
+              acc[c].y = acc[c].y + b[c].y;
+            }
+          }
+#pragma unroll 1
+          for (uint32_t r = rn; r < h.cr1; ++r) {
+            load_row<CW>(vb + (int64_t)r * C + cc, C, cc, even, a);
+#pragma unroll
+            for (int c = 0; c < CW; ++c) {
+              acc[c].x = acc[c].x + a[c].x;
+              acc[c].y = acc[c].y + a[c].y;
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < CW; ++c)
+            if (cc + c < C) gst(g + (int64_t)(cc + c) * fr.n_cell, make_float2(0.0f + acc[c].x, 0.0f + acc[c].y));
+        }
+      }
+    }
+    if (!more) break;
+    L = Ln;
+    h = hn;
   }
 }
 
@@ -488,10 +971,11 @@ __global__ __launch_bounds__(kGridThreads) void k_hot_cells(
       make_float2(0.0f + ax, 0.0f + ay);
 }
 
-// packed[e][p][co] = (K[e][co][2p], K[e][co][2p+1])
+// packed[e][p][co] = (K[e][co][2p], K[e][co][2p+1]), p < coil_group(C) / 2,
+// zero past C
 __global__ void k_table_pack(const float2 *__restrict__ K, int64_t nK, int C,
                              float4 *__restrict__ packed) {
-  const int P = (C + 1) / 2;
+  const int P = coil_group(C) / 2;
   const int64_t total = nK * P * C;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -500,52 +984,79 @@ __global__ void k_table_pack(const float2 *__restrict__ K, int64_t nK, int C,
     const int p = rem / C;
     const int co = rem - p * C;
     const float2 *row = K + (e * C + co) * C;
-    const float2 a = row[2 * p];
+    const float2 a = (2 * p < C) ? row[2 * p] : make_float2(0.f, 0.f);
     const float2 b = (2 * p + 1 < C) ? row[2 * p + 1] : make_float2(0.f, 0.f);
     packed[t] = make_float4(a.x, a.y, b.x, b.y);
   }
 }
 
-// (C, N) -> (N, C) through a 32x33 LDS tile.
-__global__ void k_transpose_cn(const float2 *__restrict__ y, int64_t C,
-                               int64_t N, float2 *__restrict__ yt) {
-  __shared__ float2 tile[32][33];
-  const int64_t n0 = (int64_t)blockIdx.x * 32;
-  const int64_t c0 = (int64_t)blockIdx.y * 32;
-  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
-    const int64_t c = c0 + k, n = n0 + threadIdx.x;
-    if (c < C && n < N) tile[k][threadIdx.x] = y[c * N + n];
+// (C, N) coil-major -> (N, C) sample-major through an LDS tile of 64 samples
+// x 32 coils per 256-thread block: each thread reads four 16-byte pairs of
+// samples of one coil and writes four 16-byte pairs of coils of one sample,
+// so both sides move whole 512-byte / 256-byte runs.  Odd C or N (pairs not
+// 16-byte aligned) and ragged tiles take an element-wise path.
+constexpr int kTrS = 64, kTrC = 32;
+
+__device__ __forceinline__ void transpose_tile(const float2 *__restrict__ y,
+                                               int64_t C, int64_t N,
+                                               float2 *__restrict__ yt,
+                                               int64_t n0, int64_t c0,
+                                               float2 (*tile)[kTrS + 1]) {
+  const int t = threadIdx.x;
+  const bool full = n0 + kTrS <= N && ((C | N) & 1) == 0;
+  if (full) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int cl = t / 32 + 8 * i, sp = t % 32;
+      if (c0 + cl < C) {
+        const float4 v = gld(reinterpret_cast<const float4 *>(y + (c0 + cl) * N + n0) + sp);
+        tile[cl][2 * sp] = make_float2(v.x, v.y);
+        tile[cl][2 * sp + 1] = make_float2(v.z, v.w);
+      }
+    }
+  } else {
+    for (int i = t; i < kTrS * kTrC; i += 256) {
+      const int cl = i / kTrS, sl = i % kTrS;
+      if (c0 + cl < C && n0 + sl < N) tile[cl][sl] = y[(c0 + cl) * N + n0 + sl];
+    }
   }
   __syncthreads();
-  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
-    const int64_t n = n0 + k, c = c0 + threadIdx.x;
-    if (c < C && n < N) yt[n * C + c] = tile[threadIdx.x][k];
+  const int64_t cw = C - c0 < kTrC ? C - c0 : kTrC;  // coils of this tile
+  if (full) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int sl = t / 16 + 16 * i, cp = t % 16;
+      if (2 * cp < cw) {
+        const float2 a = tile[2 * cp][sl], b = tile[2 * cp + 1][sl];
+        gst(reinterpret_cast<float4 *>(yt + (n0 + sl) * C + c0) + cp, make_float4(a.x, a.y, b.x, b.y));
+      }
+    }
+  } else {
+    for (int i = t; i < kTrS * kTrC; i += 256) {
+      const int sl = i / kTrC, cl = i % kTrC;
+      if (cl < cw && n0 + sl < N) yt[(n0 + sl) * C + c0 + cl] = tile[cl][sl];
+    }
   }
+}
+
+__global__ __launch_bounds__(256) void k_transpose_cn(const float2 *__restrict__ y, int64_t C,
+                                                      int64_t N, float2 *__restrict__ yt) {
+  __shared__ float2 tile[kTrC][kTrS + 1];
+  transpose_tile(y, C, N, yt, (int64_t)blockIdx.x * kTrS, (int64_t)blockIdx.y * kTrC, tile);
 }
 
 // Frames of uneven length back to back: frame f holds (C, N_f) coil-major
 // at sample offset off[f] (element offset C * off[f]); its (N_f, C)
 // sample-major rows go to the same offset of yt.  blockIdx.z = frame.
-__global__ void k_transpose_frames(const float2 *__restrict__ y,
-                                   const int64_t *__restrict__ off, int64_t C,
-                                   float2 *__restrict__ yt) {
-  __shared__ float2 tile[32][33];
+__global__ __launch_bounds__(256) void k_transpose_frames(const float2 *__restrict__ y,
+                                                          const int64_t *__restrict__ off,
+                                                          int64_t C, float2 *__restrict__ yt) {
+  __shared__ float2 tile[kTrC][kTrS + 1];
   const int64_t base = off[blockIdx.z];
   const int64_t N = off[blockIdx.z + 1] - base;
-  const int64_t n0 = (int64_t)blockIdx.x * 32;
+  const int64_t n0 = (int64_t)blockIdx.x * kTrS;
   if (n0 >= N) return;
-  const int64_t c0 = (int64_t)blockIdx.y * 32;
-  const float2 *yf = y + C * base;
-  float2 *tf = yt + C * base;
-  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
-    const int64_t c = c0 + k, n = n0 + threadIdx.x;
-    if (c < C && n < N) tile[k][threadIdx.x] = yf[c * N + n];
-  }
-  __syncthreads();
-  for (int k = threadIdx.y; k < 32; k += blockDim.y) {
-    const int64_t n = n0 + k, c = c0 + threadIdx.x;
-    if (c < C && n < N) tf[n * C + c] = tile[threadIdx.x][k];
-  }
+  transpose_tile(y + C * base, C, N, yt + C * base, n0, (int64_t)blockIdx.y * kTrC, tile);
 }
 
 // out[b, co, m] = K[idx[m]][co] . y[b, :, m / kw]
@@ -621,9 +1132,19 @@ static int launch_tile(int64_t tile_cells, dim3 grid, hipStream_t st,
     PG_LAUNCHED();                                                            \
     return 0;                                                                 \
   }
-  if constexpr (GS <= 16) {
-    PG_TILE_LAUNCH(256) PG_TILE_LAUNCH(128) PG_TILE_LAUNCH(64)
-  } else if constexpr (GS == 32) {
+  if (tile_cells == kWaveTile) {
+    // the V reduction: row-direct above 16 coils (rows of 256 bytes and
+    // more), LDS-staged wave tiles below (measured on MI355X)
+    if (MODE == 1 && GS >= 32)
+      hipLaunchKernelGGL((k_wrows<GS>), grid, block, 0, st, frames, tile_ptr,
+                         F, C, n_tiles);
+    else
+      hipLaunchKernelGGL((k_wtile<GS, MODE>), grid, block, 0, st, frames,
+                         tile_ptr, F, C, n_tiles);
+    PG_LAUNCHED();
+    return 0;
+  }
+  if constexpr (GS <= 32) {
     PG_TILE_LAUNCH(128) PG_TILE_LAUNCH(64)
   } else {
     PG_TILE_LAUNCH(64) PG_TILE_LAUNCH(32)
@@ -753,17 +1274,21 @@ int pg_grid_tile_options(int64_t C, int32_t *opts, int32_t max_opts) {
   int32_t o[3] = {0, 0, 0};
   int n = 0;
   if (C < 1 || C > 64) return 0;
-  if (C <= 16) { o[0] = 256; o[1] = 128; o[2] = 64; n = 3; }
-  else if (C <= 32) { o[0] = 128; o[1] = 64; n = 2; }
-  else { o[0] = 64; o[1] = 32; n = 2; }
+  if (C <= 32) { o[0] = kWaveTile; o[1] = 128; o[2] = 64; n = 3; }
+  else { o[0] = kWaveTile; o[1] = 64; o[2] = 32; n = 3; }
   for (int i = 0; i < n && i < max_opts; ++i) opts[i] = o[i];
   return n;
+}
+
+int64_t pg_table_packed_elems(int64_t K, int64_t C) {
+  if (K < 1 || C < 1 || C > 64) return 0;
+  return K * (coil_group(C) / 2) * C;
 }
 
 int pg_table_pack_c64(const void *table, int64_t K, int64_t C, void *packed,
                       pg_stream_t stream) {
   PG_CHECK(K >= 1 && C >= 1 && C <= 64, "table_pack: bad sizes");
-  const int64_t total = K * ((C + 1) / 2) * C;
+  const int64_t total = pg_table_packed_elems(K, C);
   const int threads = 256;
   hipLaunchKernelGGL(k_table_pack, dim3(grid_for(total, threads)),
                      dim3(threads), 0, as_stream(stream),
@@ -802,8 +1327,8 @@ int pg_transpose_cn_c64(const void *y, int64_t C, int64_t N, void *y_t,
                         pg_stream_t stream) {
   PG_CHECK(C >= 0 && N >= 0 && C < 65535 * 32, "transpose: bad sizes");
   if (C == 0 || N == 0) return 0;
-  const dim3 grid((unsigned)ceil_div(N, 32), (unsigned)ceil_div(C, 32));
-  hipLaunchKernelGGL(k_transpose_cn, grid, dim3(32, 8), 0, as_stream(stream),
+  const dim3 grid((unsigned)ceil_div(N, kTrS), (unsigned)ceil_div(C, kTrC));
+  hipLaunchKernelGGL(k_transpose_cn, grid, dim3(256), 0, as_stream(stream),
                      static_cast<const float2 *>(y), C, N,
                      static_cast<float2 *>(y_t));
   PG_LAUNCHED();
@@ -818,9 +1343,9 @@ int pg_transpose_frames_c64(const void *y, const int64_t *frame_off,
            (long long)C);
   PG_CHECK(F == 0 || frame_off, "transpose_frames: null offsets");
   if (F == 0 || C == 0 || max_n == 0) return 0;
-  const dim3 grid((unsigned)ceil_div(max_n, 32), (unsigned)ceil_div(C, 32),
+  const dim3 grid((unsigned)ceil_div(max_n, kTrS), (unsigned)ceil_div(C, kTrC),
                   (unsigned)F);
-  hipLaunchKernelGGL(k_transpose_frames, grid, dim3(32, 8), 0,
+  hipLaunchKernelGGL(k_transpose_frames, grid, dim3(256), 0,
                      as_stream(stream), static_cast<const float2 *>(y),
                      frame_off, C, static_cast<float2 *>(y_t));
   PG_LAUNCHED();

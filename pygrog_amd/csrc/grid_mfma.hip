@@ -90,6 +90,12 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
   const int twoC = 2 * C;
   constexpr int CO = 8 * MT, CI = 16 * KS;
   constexpr int64_t per_part = (int64_t)CO * CI * 2;
+  // per-wavefront staging of one 16-replica output tile (rows of 16 MT
+  // floats, padded): the tile is written to LDS in the MFMA layout and read
+  // back so that 16 consecutive lanes store one replica's whole V row
+  constexpr int RLP = 16 * MT + 4;
+  __shared__ __align__(16) float s_out[4][16 * RLP];
+  float *vt = s_out[threadIdx.x / 64];
 
   bf16x8_t ahi[MT][KS];
   bf16x8_t alo[MT][KS];
@@ -154,14 +160,19 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
     }
   };
   const int n_tiles = (int)((stop - start + 15) / 16);
-  float fa[KS][8], fb[KS][8];
-  load_rows(0, fa);
+  // sample-row fragments DEPTH tiles ahead (DEPTH + 1 buffers in rotation;
+  // two ahead up to 32 coils, where the registers allow it)
+  constexpr int DEPTH = KS <= 2 ? 2 : 1;
+  constexpr int NBUF = DEPTH + 1;
+  float fr_[NBUF][KS][8];
+#pragma unroll
+  for (int d = 0; d < DEPTH; ++d)
+    if (d < n_tiles) load_rows(d, fr_[d]);
 #pragma unroll
   for (int t = 0; t < kTilesPerWave; ++t) {
     if (t >= n_tiles) break;
-    float (&cur)[KS][8] = (t & 1) ? fb : fa;
-    float (&nxt)[KS][8] = (t & 1) ? fa : fb;
-    if (t + 1 < n_tiles) load_rows(t + 1, nxt);
+    float (&cur)[KS][8] = fr_[t % NBUF];
+    if (t + DEPTH < n_tiles) load_rows(t + DEPTH, fr_[(t + DEPTH) % NBUF]);
     floatx4_t acc[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) acc[mt] = floatx4_t{0.f, 0.f, 0.f, 0.f};
@@ -188,22 +199,40 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
         }
       }
     }
-    if (start + 16 * t + j < stop) {
+    if (vec) {
+      // even C: through LDS, then whole rows (16 lanes x 16 bytes per
+      // replica row piece, consecutive lanes on consecutive bytes)
+      const float s = sw[t];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        float4 o;
+        o.x = (acc[mt][0] * s) * s;
+        o.y = (acc[mt][1] * s) * s;
+        o.z = (acc[mt][2] * s) * s;
+        o.w = (acc[mt][3] * s) * s;
+        *reinterpret_cast<float4 *>(&vt[j * RLP + 16 * mt + 4 * q]) = o;
+      }
+      wave_lds_sync();
+      constexpr int CH = 4 * MT;  // float4 pieces per (padded) row
+#pragma unroll
+      for (int k = 0; k < CH / 4; ++k) {
+        const int idx = 64 * k + lane;
+        const int jr = idx / CH, ch = idx % CH;
+        const uint32_t rj = __shfl(rr[t], jr);
+        if (4 * ch < twoC && start + 16 * t + jr < stop)
+          gst(reinterpret_cast<float4 *>(vbuf + (int64_t)rj * twoC + 4 * ch),
+              *reinterpret_cast<const float4 *>(&vt[jr * RLP + 4 * ch]));
+      }
+      wave_lds_sync();
+    } else if (start + 16 * t + j < stop) {
       const float s = sw[t];
       float *vr = vbuf + (int64_t)rr[t] * twoC;
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
         const int row = 16 * mt + 4 * q;
-        float o[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) o[i] = (acc[mt][i] * s) * s;
-        if (row + 4 <= twoC && vec) {
-          *reinterpret_cast<float4 *>(vr + row) = make_float4(o[0], o[1], o[2], o[3]);
-        } else {
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-            if (row + i < twoC) vr[row + i] = o[i];
-        }
+        for (int i = 0; i < 4; ++i)
+          if (row + i < twoC) vr[row + i] = (acc[mt][i] * s) * s;
       }
     }
   }

@@ -56,14 +56,22 @@ def test_library_version_and_errors():
         tc = lib.pg_grid_tile_cells(c)
         opts = _lib.tile_options(c)
         assert tc in opts and opts == sorted(opts, reverse=True)
-        # each thread of a 256-thread tile owns one cell and at most 16 coils
-        assert all(256 % t == 0 and 1 <= gs * t // 256 <= 16 for t in opts)
+        # 256 is the wave-tile kernel (one wavefront per 64 cells, every coil
+        # count); in the other shapes each thread of the 256-thread tile owns
+        # one cell and at most 16 coils
+        assert opts[0] == 256
+        assert all(256 % t == 0 and 1 <= gs * t // 256 <= 16 for t in opts[1:])
     # a tile size that is not compiled is rejected before any launch
     rc = lib.pg_grid_fp32(1, 1, 1, 1, 16, 100, 0, 0, 0, None)
     assert rc != 0 and b"tile_cells" in lib.pg_last_error()
     # compact complex bf16 table: (re, im) per (co, ci), padded to 8 MT x 16 KS
     assert lib.pg_table_bf16_elems(10, 32, 2) == 10 * 2 * 32 * 32 * 2
     assert lib.pg_table_bf16_elems(3, 5, 1) == 3 * 1 * 8 * 16 * 2
+    # FP32 packed table: coil pairs padded to half the coil group (4/8/16/32/64)
+    assert lib.pg_table_packed_elems(10, 16) == 10 * 8 * 16
+    assert lib.pg_table_packed_elems(3, 5) == 3 * 4 * 5
+    assert lib.pg_table_packed_elems(2, 33) == 2 * 32 * 33
+    assert lib.pg_table_packed_elems(1, 65) == 0 and lib.pg_table_packed_elems(0, 8) == 0
 
 
 def test_kernel_offsets_and_quant_steps():

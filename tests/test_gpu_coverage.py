@@ -118,3 +118,71 @@ def test_grouped_modes_linear_on_kooshball(device):
         rhs = 2 * g.grid_device(a, mode).clone() + g.grid_device(b, mode)
         rel = float(torch.linalg.vector_norm(lhs - rhs) / torch.linalg.vector_norm(rhs))
         assert rel <= tol, (mode, rel)
+
+
+@pytest.mark.parametrize("C", [3, 16, 33, 64])
+@pytest.mark.parametrize("hot", [True, False])
+def test_every_tile_shape_same_bits(device, monkeypatch, C, hot):
+    """Every compiled launch shape -- the wave-tile kernel (256 cells per
+    workgroup, one wavefront per 64 cells) and the workgroup tiles -- at two
+    persistent grid sizes gives the default shape's grid bit for bit, in
+    every mode, with the hot pre-pass and with the multi-unit fold (hot
+    threshold raised); the fp32 grid also matches the oracle."""
+    import pygrog_amd.calib._grog as grog_mod
+    from pygrog_amd import _lib
+    from pygrog_amd.calib import GrogInterpolator
+
+    if not hot:
+        monkeypatch.setattr(grog_mod, "HOT_MIN_UNITS", 10**9)
+    rng = np.random.default_rng(100 + C)
+    shape = (80, 80)
+    coords = _radial_full(40, 160, seed=C)  # DC and the first rings hold many replicas
+    g = GrogInterpolator(shape, coords, kernel_width=3, device=device)
+    calib = (rng.standard_normal((C, 16, 16)) + 1j * rng.standard_normal((C, 16, 16))).astype(np.complex64)
+    g.calc_interp_table(calib, lamda=1e-2, quant=16)
+    assert (g.plan._compiled.hot[0]["n_hot"] > 0) == hot
+    n = coords.shape[0] * coords.shape[1]
+    y = (rng.standard_normal((C, n)) + 1j * rng.standard_normal((C, n))).astype(np.complex64)
+    yy = torch.from_numpy(y).to(device).reshape(1, 1, C, -1).contiguous()
+    p = oplan.attach_fft_plan(oplan.create_plan_single(shape, coords, None, 3, "circle"))
+    e = g._interp_idx.reshape(p["distances"].shape[:-1]).cpu().numpy()
+    ref, _ = ogrid.grid_reference(y, p, e, g._interp_kernel.cpu().numpy())
+    for mode in ("fp32", "bf16x3", "bf16"):
+        base = g.grid_device(yy, mode).clone()
+        if mode == "fp32":
+            assert ogrid.rel_l2(base.reshape(C, -1).cpu().numpy(), ref) <= 1e-5
+        for tc in _lib.tile_options(C):
+            for nb in (8, None):
+                g._tuned = {mode: (tc, nb)}
+                got = g.grid_device(yy, mode, program_cache={})
+                assert torch.equal(got, base), (mode, tc, nb)
+        g._tuned = {}
+
+
+@pytest.mark.parametrize("C", [3, 16, 40])
+def test_scatter_every_tile_shape_same_bits(device, monkeypatch, C):
+    """The SparseFFT scatter (identity operator, coil-major sources) over
+    every compiled launch shape, with a hot cell: identical bits."""
+    from pygrog_amd import _lib
+    from pygrog_amd.operator import SparseFFT
+
+    rng = np.random.default_rng(40 + C)
+    grid, image = (32, 32), (28, 28)
+    G = 32 * 32
+    idx = np.concatenate([rng.integers(0, G, 4000), np.full(900, 500), rng.integers(480, 520, 700)])
+    rng.shuffle(idx)
+    n = idx.size
+    w = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    sp = SparseFFT(grid, image, idx, w)._sp()
+    assert sp.hot[0]["n_hot"] >= 1
+    x = (rng.standard_normal((C, n)) + 1j * rng.standard_normal((C, n))).astype(np.complex64)
+    xd = torch.from_numpy(x).to(device).reshape(1, 1, C, n).contiguous()
+    base = sp.scatter(xd).clone()
+    ref = torch.zeros(C, G, dtype=torch.complex128, device=device)
+    ref.index_add_(1, sp.indices[0], xd[0, 0][:, sp.sort_perm[0]].to(torch.complex128) * sp.sqrt_w[0].double())
+    rel = torch.linalg.vector_norm(base[0, 0].to(torch.complex128) - ref) / torch.linalg.vector_norm(ref)
+    assert float(rel) <= 1e-6
+    lib = _lib.load()
+    for tc in _lib.tile_options(C):
+        monkeypatch.setattr(lib, "pg_grid_tile_cells", lambda c, tc=tc: tc)
+        assert torch.equal(sp.scatter(xd), base), tc
