@@ -820,6 +820,9 @@ __device__ __forceinline__ void load_row(const float2 *vr, int C, int cc, bool e
 #ifndef PG_WROWS_TWO
 #define PG_WROWS_TWO 0
 #endif
+#ifndef PG_WROWS_PREFETCH
+#define PG_WROWS_PREFETCH 1
+#endif
 template <int GS>
 __global__ __launch_bounds__(kGridThreads) void k_wrows(
     const pg_frame *__restrict__ frames, const uint32_t *__restrict__ tile_ptr,
@@ -863,7 +866,7 @@ __global__ __launch_bounds__(kGridThreads) void k_wrows(
     const uint32_t Ln = L + gridDim.x;
     const bool more = Ln < n_tiles;
     Head hn = h;
-    if (more) hn = head(Ln);
+    if (PG_WROWS_PREFETCH && more) hn = head(Ln);
     const pg_frame fr = frames[h.f];
     if (h.cl < fr.n_cell) {
       float2 *g = reinterpret_cast<float2 *>(fr.grid) + h.cl;
@@ -918,7 +921,7 @@ __global__ __launch_bounds__(kGridThreads) void k_wrows(
     }
     if (!more) break;
     L = Ln;
-    h = hn;
+    h = PG_WROWS_PREFETCH ? hn : head(Ln);
   }
 }
 

@@ -66,7 +66,10 @@ __global__ void k_table_pack_bf16(const float2 *__restrict__ K, int64_t nK,
   }
 }
 
-constexpr int kTilesPerWave = 8;  // 16-replica N tiles per wavefront
+#ifndef PG_APPLY_TILES
+#define PG_APPLY_TILES 8
+#endif
+constexpr int kTilesPerWave = PG_APPLY_TILES;  // 16-replica N tiles per wavefront
 
 template <int MT, int KS, int NTERMS>
 __global__ __launch_bounds__(256) void k_apply_grouped(
@@ -162,7 +165,10 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
   const int n_tiles = (int)((stop - start + 15) / 16);
   // sample-row fragments DEPTH tiles ahead (DEPTH + 1 buffers in rotation;
   // two ahead up to 32 coils, where the registers allow it)
-  constexpr int DEPTH = KS <= 2 ? 2 : 1;
+#ifndef PG_APPLY_DEPTH
+#define PG_APPLY_DEPTH 2
+#endif
+  constexpr int DEPTH = KS <= 2 ? PG_APPLY_DEPTH : 1;
   constexpr int NBUF = DEPTH + 1;
   float fr_[NBUF][KS][8];
 #pragma unroll
@@ -178,15 +184,15 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
     for (int mt = 0; mt < MT; ++mt) acc[mt] = floatx4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      u16x8_t bh, bl;
+      // hi = bf16(x), lo = bf16(x - hi), both round to nearest even: the
+      // hardware conversion (v_cvt_pk_bf16_f32) gives the bits of f2bf
+      bf16x8_t bhi, blo;
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        const unsigned short h = f2bf(cur[ks][i]);
-        bh[i] = h;
-        bl[i] = f2bf(cur[ks][i] - bf2f(h));
+        const __bf16 h = (__bf16)cur[ks][i];
+        bhi[i] = h;
+        blo[i] = (__bf16)(cur[ks][i] - (float)h);
       }
-      const bf16x8_t bhi = __builtin_bit_cast(bf16x8_t, bh);
-      const bf16x8_t blo = __builtin_bit_cast(bf16x8_t, bl);
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
         acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[mt][ks], bhi,
