@@ -201,6 +201,7 @@ __device__ __forceinline__ float2 coil_value(const pg_frame &fr, uint32_t r,
 template <int GS, int MODE>
 __device__ __forceinline__ float2 unit_partial(const pg_frame &fr, uint32_t u,
                                                int co, int C, int P) {
+#pragma clang fp contract(off)
   const uint32_t r0 = fr.unit_rep[u], r1 = fr.unit_rep[u + 1];
   float px = 0.0f, py = 0.0f;
   if (r0 >= r1 || co >= C) return make_float2(px, py);
@@ -227,6 +228,8 @@ __device__ __forceinline__ float2 unit_partial(const pg_frame &fr, uint32_t u,
     Rec nq = q;
     if (r + 1 < r1) nq = load_rec<MODE>(fr, r + 1);
     const float2 v = coil_value<GS, MODE>(fr, r, q, co, C, P);
+    // the value's last product is not contracted into the fold (every
+    // other path folds values that went through memory): see the pragma
     px = px + v.x;
     py = py + v.y;
     q = nq;
@@ -926,7 +929,8 @@ __global__ __launch_bounds__(kGridThreads) void k_wrows(
 }
 
 // Hot pre-pass 1: partial of every unit of every hot cell, in parallel
-// (grid.x over hot units, grid.y over frames).
+// (grid.x over hot units, grid.y over frames).  MODE 1 (values read from V):
+// one group of GS lanes per unit folds its rows straight from memory.
 template <int GS, int MODE>
 __global__ __launch_bounds__(kGridThreads) void k_hot_units(
     const pg_frame *__restrict__ frames, int C) {
@@ -937,6 +941,63 @@ __global__ __launch_bounds__(kGridThreads) void k_hot_units(
   if (i >= fr.n_hot_units) return;
   const float2 pv = unit_partial<GS, MODE>(fr, fr.hot_units[i], lane, C, (C + 1) / 2);
   if (lane < C) reinterpret_cast<float2 *>(fr.p_unit)[i * C + lane] = pv;
+}
+
+// Hot pre-pass 1 for the modes that evaluate replicas (0: operator apply, 2:
+// scaled source values): one workgroup per hot unit.  The unit's records go
+// to LDS, then 256 / GS groups of GS lanes (lane = output coil) evaluate its
+// replicas in parallel, each with coil_value (the chain every kernel uses),
+// into LDS rows; lane co then folds the rows in replica order --
+// ((0 + v_r0) + v_r1) + ... -- exactly as unit_partial does, so the partial
+// has the same bits while the replicas' memory round trips overlap.  A unit
+// longer than kUnitMax replicas (a unit chunk above it) falls back to
+// unit_partial.
+constexpr int kUnitMax = 32;
+
+template <int GS, int MODE>
+__global__ __launch_bounds__(kGridThreads) void k_hot_units_par(
+    const pg_frame *__restrict__ frames, int C) {
+  constexpr int NG = kGridThreads / GS;
+  constexpr int VS = GS + 1;
+  __shared__ float2 s_v[kUnitMax * VS];
+  __shared__ uint32_t s_src[kUnitMax], s_ent[kUnitMax];
+  __shared__ float s_s[kUnitMax];
+  const pg_frame fr = frames[blockIdx.y];
+  const int64_t i = blockIdx.x;
+  if (i >= fr.n_hot_units) return;  // uniform over the workgroup
+  const int tid = threadIdx.x;
+  const int co = tid % GS, g = tid / GS;
+  const int P = (C + 1) / 2;
+  const uint32_t u = fr.hot_units[i];
+  const uint32_t r0 = fr.unit_rep[u], r1 = fr.unit_rep[u + 1];
+  const int n = (int)(r1 - r0);
+  float2 *pu = reinterpret_cast<float2 *>(fr.p_unit) + i * C;
+  if (n > kUnitMax) {
+    if (g == 0 && co < C) pu[co] = unit_partial<GS, MODE>(fr, u, co, C, P);
+    return;
+  }
+  if (tid < n) {
+    s_src[tid] = gld(fr.rep_src + r0 + tid);
+    s_s[tid] = gld(fr.rep_s + r0 + tid);
+    s_ent[tid] = MODE == 0 ? gld(fr.rep_ent + r0 + tid) : 0u;
+  }
+  __syncthreads();
+  if (co < C) {
+    for (int j = g; j < n; j += NG) {
+      const Rec q{s_src[j], s_ent[j], s_s[j]};
+      s_v[j * VS + co] = coil_value<GS, MODE>(fr, r0 + j, q, co, C, P);
+    }
+  }
+  __syncthreads();
+  if (g == 0 && co < C) {
+    float px = 0.0f, py = 0.0f;
+    for (int j = 0; j < n; ++j) {
+      const float2 v = s_v[j * VS + co];
+      px = px + v.x;
+      py = py + v.y;
+    }
+    pu[co] = make_float2(px, py);
+  }
 }
 
 // Hot pre-pass 2: in-order fold of each hot cell's unit partials, written
@@ -1162,9 +1223,14 @@ static int launch_hot(hipStream_t st, const pg_frame *frames, int F, int C,
                       int64_t max_hot_units, int64_t max_hot) {
   constexpr int NG = kGridThreads / GS;
   const dim3 block(kGridThreads);
-  hipLaunchKernelGGL((k_hot_units<GS, MODE>),
-                     dim3((unsigned)ceil_div(max_hot_units, NG), (unsigned)F),
-                     block, 0, st, frames, C);
+  if (MODE == 1)
+    hipLaunchKernelGGL((k_hot_units<GS, MODE>),
+                       dim3((unsigned)ceil_div(max_hot_units, NG), (unsigned)F),
+                       block, 0, st, frames, C);
+  else
+    hipLaunchKernelGGL((k_hot_units_par<GS, MODE>),
+                       dim3((unsigned)max_hot_units, (unsigned)F), block, 0, st,
+                       frames, C);
   PG_LAUNCHED();
   hipLaunchKernelGGL((k_hot_cells<GS>),
                      dim3((unsigned)ceil_div(max_hot, NG), (unsigned)F), block,
@@ -1235,7 +1301,8 @@ static int launch_grid(const pg_frame *frames, const uint32_t *tile_ptr,
   PG_CHECK(F >= 1 && frames && tile_ptr, "grid: empty frame list");
   PG_CHECK(F < 65536, "grid: too many frames in one launch");
   PG_CHECK(n_tiles >= 0 && n_tiles < (int64_t)UINT32_MAX, "grid: bad tiles");
-  PG_CHECK(max_hot_units >= 0 && max_hot >= 0, "grid: bad hot counts");
+  PG_CHECK(max_hot_units >= 0 && max_hot >= 0 && max_hot_units < (int64_t)INT32_MAX,
+           "grid: bad hot counts");
   {
     int32_t opts[3];
     const int n = pg_grid_tile_options(C, opts, 3);
