@@ -826,6 +826,9 @@ __device__ __forceinline__ void load_row(const float2 *vr, int C, int cc, bool e
 #ifndef PG_WROWS_PREFETCH
 #define PG_WROWS_PREFETCH 1
 #endif
+#ifndef PG_WROWS_MASKED
+#define PG_WROWS_MASKED 1
+#endif
 template <int GS>
 __global__ __launch_bounds__(kGridThreads) void k_wrows(
     const pg_frame *__restrict__ frames, const uint32_t *__restrict__ tile_ptr,
@@ -892,8 +895,12 @@ __global__ __launch_bounds__(kGridThreads) void k_wrows(
 #pragma unroll 1
         for (int cc = 0; cc < C; cc += CW) {
           float2 a[CW], b[CW];
-          load_row<CW>(vb + (int64_t)r0 * C + cc, C, cc, even, a);
-          if (TWO) load_row<CW>(vb + (int64_t)r1 * C + cc, C, cc, even, b);
+#pragma unroll
+          for (int c = 0; c < CW; ++c) a[c] = b[c] = make_float2(0.0f, 0.0f);
+          // only lanes that own rows issue loads (empty cells -- half the
+          // kooshball grid -- would otherwise spend load bandwidth)
+          if (!PG_WROWS_MASKED || n >= 1) load_row<CW>(vb + (int64_t)r0 * C + cc, C, cc, even, a);
+          if (TWO && (!PG_WROWS_MASKED || n >= 2)) load_row<CW>(vb + (int64_t)r1 * C + cc, C, cc, even, b);
           float2 acc[CW];
 #pragma unroll
           for (int c = 0; c < CW; ++c) {
