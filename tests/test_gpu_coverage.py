@@ -186,3 +186,39 @@ def test_scatter_every_tile_shape_same_bits(device, monkeypatch, C):
     for tc in _lib.tile_options(C):
         monkeypatch.setattr(lib, "pg_grid_tile_cells", lambda c, tc=tc: tc)
         assert torch.equal(sp.scatter(xd), base), tc
+
+
+@pytest.mark.parametrize("C", [8, 16])
+def test_long_units_every_tile_shape(device, monkeypatch, C):
+    """Reduction units longer than a wave tile's row batch and than the
+    parallel hot pre-pass holds (unit chunk 80 instead of 32): the wave-tile
+    kernels fold such cells from memory and the pre-pass falls back to the
+    serial fold; every launch shape gives the same bits and the fp32 grid
+    matches the oracle."""
+    import pygrog_amd.calib._grog as grog_mod
+    from pygrog_amd import _lib
+    from pygrog_amd.calib import GrogInterpolator
+
+    monkeypatch.setattr(grog_mod, "UNIT_CHUNK", 80)
+    rng = np.random.default_rng(7 + C)
+    shape = (64, 64)
+    coords = _radial_full(96, 128, seed=C)  # cells near the centre hold 40-300 replicas
+    g = GrogInterpolator(shape, coords, kernel_width=3, device=device)
+    calib = (rng.standard_normal((C, 16, 16)) + 1j * rng.standard_normal((C, 16, 16))).astype(np.complex64)
+    g.calc_interp_table(calib, lamda=1e-2, quant=16)
+    cp = g.plan._compiled.cell_ptr[0].to(torch.int64)
+    assert int((cp[1:] - cp[:-1]).max()) > 80  # some cells span several long units
+    n = coords.shape[0] * coords.shape[1]
+    y = (rng.standard_normal((C, n)) + 1j * rng.standard_normal((C, n))).astype(np.complex64)
+    yy = torch.from_numpy(y).to(device).reshape(1, 1, C, -1).contiguous()
+    p = oplan.attach_fft_plan(oplan.create_plan_single(shape, coords, None, 3, "circle"))
+    e = g._interp_idx.reshape(p["distances"].shape[:-1]).cpu().numpy()
+    ref, _ = ogrid.grid_reference(y, p, e, g._interp_kernel.cpu().numpy())
+    for mode in ("fp32", "bf16x3"):
+        base = g.grid_device(yy, mode).clone()
+        if mode == "fp32":
+            assert ogrid.rel_l2(base.reshape(C, -1).cpu().numpy(), ref) <= 1e-5
+        for tc in _lib.tile_options(C):
+            g._tuned = {mode: (tc, None)}
+            assert torch.equal(g.grid_device(yy, mode, program_cache={}), base), (mode, tc)
+        g._tuned = {}
