@@ -4,7 +4,8 @@ Follows the reference algorithm (src/pygrog/calib/_grappa.py):
   * calibration normalised by its Frobenius norm                 (:275-287)
   * unit-shift operators G = X^T from the Tikhonov least squares
     min ||S X - T||^2 + lamda ||X||^2, S/T = shifted ACR rows     (:16-59, 326-368)
-  * fractional powers G^p = expm(p * logm(G)), logm = V diag(log w) V^-1
+  * fractional powers G^p = expm(p * logm(G)), logm = V diag(log w) V^-1,
+    evaluated as V diag(w^p) V^-1 for all axes in one eigendecomposition
                                                                  (:62-72, 184-208)
   * table[i1*n + i0] = Gx[i0] @ Gy[i1]            (2D)
     table[i2*n^2 + i1*n + i0] = Gx[i0] @ Gy[i1] @ Gz[i2]  (3D)   (:131-181)
@@ -29,7 +30,8 @@ the reference table pairs ``Gx`` (axis-2 operator) with ``d0``.  The default
 
 from __future__ import annotations
 
-__all__ = ["KernelTable", "KernelTableResult", "grappa_power", "lstsq", "prepare_grappa_table", "train_grappa"]
+__all__ = ["KernelTable", "KernelTableResult", "grappa_power", "grappa_powers", "lstsq", "prepare_grappa_table",
+           "train_grappa"]
 
 import math
 from dataclasses import dataclass, field
@@ -81,27 +83,43 @@ def lstsq(A, b, lamda: float = 0.0, host_steps: list | None = None):
         return sol.to(A_t.device)
 
 
-def _matrix_logm(A: torch.Tensor, host_steps: list | None = None) -> torch.Tensor:
-    """logm(A) = V diag(log w) V^-1 in complex128 (reference _grappa.py:62-72,
-    without the cast back to complex64)."""
-    A128 = A.to(torch.complex128)
+def _eig(A128: torch.Tensor, host_steps: list | None = None):
     try:
-        vals, vecs = torch.linalg.eig(A128)
+        return torch.linalg.eig(A128)
     except (RuntimeError, NotImplementedError):
         if host_steps is not None:
             host_steps.append("eig")
         vals, vecs = torch.linalg.eig(A128.cpu())
-        vals, vecs = vals.to(A.device), vecs.to(A.device)
+        return vals.to(A128.device), vecs.to(A128.device)
+
+
+def _matrix_logm(A: torch.Tensor, host_steps: list | None = None) -> torch.Tensor:
+    """logm(A) = V diag(log w) V^-1 in complex128 (reference _grappa.py:62-72,
+    without the cast back to complex64)."""
+    vals, vecs = _eig(A.to(torch.complex128), host_steps)
     return vecs @ torch.diag_embed(torch.log(vals)) @ torch.linalg.inv(vecs)
 
 
+def grappa_powers(G_units: torch.Tensor, exponents, host_steps: list | None = None) -> torch.Tensor:
+    """Fractional powers of a stack of unit operators, (A, C, C) -> (A, n, C, C)
+    complex128, one batched eigendecomposition for all of them.
+
+    The reference forms G^p = expm(p logm(G)) with logm(G) = V diag(log w)
+    V^-1 from G = V diag(w) V^-1 (_grappa.py:62-72, 184-208); that
+    exponential is V diag(exp(p log w)) V^-1, evaluated here directly (same
+    principal logarithm of the eigenvalues, no matrix exponential)."""
+    G = torch.as_tensor(G_units).to(torch.complex128)
+    exps = torch.as_tensor(exponents, dtype=torch.float32).to(G.device).to(torch.float64)
+    vals, vecs = _eig(G, host_steps)
+    vinv = torch.linalg.inv(vecs)
+    wp = torch.exp(exps.view(1, -1, 1).to(vals.dtype) * torch.log(vals).unsqueeze(1))  # (A, n, C)
+    return (vecs.unsqueeze(1) * wp.unsqueeze(-2)) @ vinv.unsqueeze(1)
+
+
 def grappa_power(G_unit: torch.Tensor, exponents, host_steps: list | None = None) -> torch.Tensor:
-    """Batched ``G^p = expm(p logm(G))`` in complex128 (reference :184-208)."""
-    G_unit = torch.as_tensor(G_unit)
-    exps = torch.as_tensor(exponents, dtype=torch.float32).to(G_unit.device)
-    lG = _matrix_logm(G_unit, host_steps)
-    lG_batch = exps.to(torch.float64).to(lG.dtype).view(-1, 1, 1) * lG.unsqueeze(0)
-    return torch.linalg.matrix_exp(lG_batch)
+    """``G^p = expm(p logm(G))`` of one unit operator for every exponent,
+    (n, C, C) complex128 (reference :184-208)."""
+    return grappa_powers(torch.as_tensor(G_unit).unsqueeze(0), exponents, host_steps)[0]
 
 
 def prepare_grappa_table(Gx, Gy, Gz, nsteps: int, ndim: int, axis_order: str = "reference") -> torch.Tensor:  # noqa: ARG001
@@ -234,10 +252,11 @@ def build_table(train_data, radius: float, precision: int | None = 1, lamda: flo
     pfac, stepsize, nsteps = quant_steps(radius, precision, quant)
     deltas = torch.arange(nsteps).float()
     deltas = (deltas - (nsteps - 1) / 2) * stepsize  # float32, as the reference
-    Gx = grappa_power(kern.x, deltas, host_steps)
-    Gy = grappa_power(kern.y, deltas, host_steps)
+    units = [kern.x, kern.y] + ([kern.z] if kern.z is not None else [])
+    powers = grappa_powers(torch.stack([torch.as_tensor(u) for u in units]), deltas, host_steps)
+    Gx, Gy = powers[0], powers[1]
     if kern.z is not None:
-        Gz = grappa_power(kern.z, deltas, host_steps)
+        Gz = powers[2]
         ndim = 3
     else:
         Gz = None
