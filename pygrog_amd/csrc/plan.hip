@@ -159,69 +159,151 @@ __global__ void k_plan_post(const uint64_t *__restrict__ keys,
   }
 }
 
-__global__ void k_plan_weights(const uint32_t *__restrict__ vals,
-                               const float *__restrict__ gauss_sorted,
-                               const int64_t *__restrict__ n_invalid,
-                               const uint32_t *__restrict__ cell_ptr,
-                               int64_t S, int64_t nper, int64_t G, int kw,
-                               int chunk, float *__restrict__ weights,
-                               float *__restrict__ sqrt_weights,
-                               uint32_t *__restrict__ rep_src,
-                               float *__restrict__ rep_s,
-                               float *__restrict__ density,
-                               uint32_t *__restrict__ unit_count,
-                               int64_t rep_stride) {
+// EGROG weights per cell, with the reference's float64 per-cell sum of the
+// Gaussian weights in natural replica order (gauss_sum.scatter_add_, the
+// stable sort keeps that order inside a cell) and the float32 density sum
+// in the same order.  A 256-thread workgroup takes 256 consecutive cells per
+// step: a thread handles its own cell when it has at most kBigCell replicas;
+// larger cells (the k-space centre: tens of thousands of replicas) are done
+// by the whole workgroup one after the other -- chunks of the cell's values
+// staged in LDS by all threads, the two ordered sums run by one thread from
+// LDS (the same additions in the same order, so the same bits), and the
+// order-free per-replica writes by all threads.
+constexpr int kBigCell = 256;
+constexpr int kPwThreads = 256;
+constexpr int kPwChunk = 2048;
+
+__global__ __launch_bounds__(kPwThreads) void k_plan_weights(
+    const uint32_t *__restrict__ vals, const float *__restrict__ gauss_sorted,
+    const int64_t *__restrict__ n_invalid, const uint32_t *__restrict__ cell_ptr,
+    int64_t S, int64_t nper, int64_t G, int kw, int chunk,
+    float *__restrict__ weights, float *__restrict__ sqrt_weights,
+    uint32_t *__restrict__ rep_src, float *__restrict__ rep_s,
+    float *__restrict__ density, uint32_t *__restrict__ unit_count,
+    int64_t rep_stride) {
+  __shared__ int64_t s_big[kPwThreads];
+  __shared__ int s_nbig;
+  __shared__ float s_val[kPwChunk];
+  __shared__ float s_denom;
+  const int tid = threadIdx.x;
   const int64_t total = S * G;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t s = t / G;
-    const int64_t c = t - s * G;
-    const uint32_t *cp = cell_ptr + s * (G + 1);
-    const int64_t r0 = cp[c], r1 = cp[c + 1];
-    const int64_t base = s * nper + n_invalid[s];
-    // gauss_sum.scatter_add_ in float64, valid replicas in natural order
-    // (the stable sort keeps that order inside a cell).  Loads go 16 at a
-    // time (independent, in flight together) and are added in order, so a
-    // cell with thousands of replicas (the k-space centre) is not a chain
-    // of dependent load latencies.
-    constexpr int KB = 16;
-    const float *gs = gauss_sorted + base;
-    double gsum = 0.0;
-    for (int64_t rb = r0; rb < r1; rb += KB) {
-      float g[KB];
+  constexpr int KB = 16;
+  for (int64_t t0 = (int64_t)blockIdx.x * kPwThreads; t0 < total;
+       t0 += (int64_t)gridDim.x * kPwThreads) {
+    if (tid == 0) s_nbig = 0;
+    __syncthreads();
+    const int64_t t = t0 + tid;
+    if (t < total) {
+      const int64_t s = t / G;
+      const int64_t c = t - s * G;
+      const uint32_t *cp = cell_ptr + s * (G + 1);
+      const int64_t r0 = cp[c], r1 = cp[c + 1];
+      const int64_t base = s * nper + n_invalid[s];
+      unit_count[t] = (uint32_t)((r1 - r0 + chunk - 1) / chunk);
+      if (r1 - r0 > kBigCell) {
+        s_big[atomicAdd(&s_nbig, 1)] = t;
+      } else {
+        // loads go 16 at a time (independent, in flight together) and are
+        // added in order
+        const float *gs = gauss_sorted + base;
+        double gsum = 0.0;
+        for (int64_t rb = r0; rb < r1; rb += KB) {
+          float g[KB];
 #pragma unroll
-      for (int k = 0; k < KB; ++k) g[k] = (rb + k < r1) ? gs[rb + k] : 0.0f;
+          for (int k = 0; k < KB; ++k) g[k] = (rb + k < r1) ? gs[rb + k] : 0.0f;
 #pragma unroll
-      for (int k = 0; k < KB; ++k)
-        if (rb + k < r1) gsum += (double)g[k];
-    }
-    // gauss_sum[...].clamp(min=1e-12).float()
-    const float denom = (float)(gsum > 1e-12 ? gsum : 1e-12);
-    float dens = 0.0f;
-    for (int64_t rb = r0; rb < r1; rb += KB) {
-      float g[KB];
-      uint32_t v[KB];
+          for (int k = 0; k < KB; ++k)
+            if (rb + k < r1) gsum += (double)g[k];
+        }
+        // gauss_sum[...].clamp(min=1e-12).float()
+        const float denom = (float)(gsum > 1e-12 ? gsum : 1e-12);
+        float dens = 0.0f;
+        for (int64_t rb = r0; rb < r1; rb += KB) {
+          float g[KB];
+          uint32_t v[KB];
 #pragma unroll
-      for (int k = 0; k < KB; ++k) {
-        g[k] = (rb + k < r1) ? gs[rb + k] : 0.0f;
-        v[k] = (rb + k < r1) ? vals[base + rb + k] : 0u;
+          for (int k = 0; k < KB; ++k) {
+            g[k] = (rb + k < r1) ? gs[rb + k] : 0.0f;
+            v[k] = (rb + k < r1) ? vals[base + rb + k] : 0u;
+          }
+#pragma unroll
+          for (int k = 0; k < KB; ++k) {
+            if (rb + k < r1) {
+              const int64_t r = rb + k;
+              const float w = g[k] / denom;
+              const float sw = sqrtf(w > 0.0f ? w : 0.0f);
+              weights[v[k]] = w;
+              sqrt_weights[base + r] = sw;
+              rep_s[s * rep_stride + r] = sw;
+              rep_src[s * rep_stride + r] = (uint32_t)(((int64_t)v[k] - s * nper) / kw);
+              dens = dens + sw * sw;
+            }
+          }
+        }
+        density[t] = dens;
       }
+    }
+    __syncthreads();
+    const int nbig = s_nbig;
+    for (int b = 0; b < nbig; ++b) {
+      const int64_t tb = s_big[b];
+      const int64_t s = tb / G;
+      const int64_t c = tb - s * G;
+      const uint32_t *cp = cell_ptr + s * (G + 1);
+      const int64_t r0 = cp[c], r1 = cp[c + 1];
+      const int64_t base = s * nper + n_invalid[s];
+      const float *gs = gauss_sorted + base;
+      // ordered float64 sum of the cell's values, chunk by chunk
+      double gsum = 0.0;
+      for (int64_t cb = r0; cb < r1; cb += kPwChunk) {
+        const int n = (int)(r1 - cb < kPwChunk ? r1 - cb : kPwChunk);
+        for (int k = tid; k < n; k += kPwThreads) s_val[k] = gs[cb + k];
+        __syncthreads();
+        if (tid == 0) {
+          for (int k0 = 0; k0 < n; k0 += KB) {
+            float g[KB];
 #pragma unroll
-      for (int k = 0; k < KB; ++k) {
-        if (rb + k < r1) {
-          const int64_t r = rb + k;
-          const float w = g[k] / denom;
+            for (int k = 0; k < KB; ++k) g[k] = (k0 + k < n) ? s_val[k0 + k] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < KB; ++k)
+              if (k0 + k < n) gsum += (double)g[k];
+          }
+        }
+        __syncthreads();
+      }
+      if (tid == 0) s_denom = (float)(gsum > 1e-12 ? gsum : 1e-12);
+      __syncthreads();
+      const float denom = s_denom;
+      // per-replica writes in parallel; sw^2 staged for the ordered density sum
+      float dens = 0.0f;
+      for (int64_t cb = r0; cb < r1; cb += kPwChunk) {
+        const int n = (int)(r1 - cb < kPwChunk ? r1 - cb : kPwChunk);
+        for (int k = tid; k < n; k += kPwThreads) {
+          const int64_t r = cb + k;
+          const uint32_t v = vals[base + r];
+          const float w = gs[r] / denom;
           const float sw = sqrtf(w > 0.0f ? w : 0.0f);
-          weights[v[k]] = w;
+          weights[v] = w;
           sqrt_weights[base + r] = sw;
           rep_s[s * rep_stride + r] = sw;
-          rep_src[s * rep_stride + r] = (uint32_t)(((int64_t)v[k] - s * nper) / kw);
-          dens = dens + sw * sw;
+          rep_src[s * rep_stride + r] = (uint32_t)(((int64_t)v - s * nper) / kw);
+          s_val[k] = sw * sw;
         }
+        __syncthreads();
+        if (tid == 0) {
+          for (int k0 = 0; k0 < n; k0 += KB) {
+            float q[KB];
+#pragma unroll
+            for (int k = 0; k < KB; ++k) q[k] = (k0 + k < n) ? s_val[k0 + k] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < KB; ++k)
+              if (k0 + k < n) dens = dens + q[k];
+          }
+        }
+        __syncthreads();
       }
+      if (tid == 0) density[tb] = dens;
     }
-    density[t] = dens;
-    unit_count[t] = (uint32_t)((r1 - r0 + chunk - 1) / chunk);
   }
 }
 
@@ -407,9 +489,8 @@ int pg_plan_weights(const uint32_t *vals_sorted, const float *gauss_sorted,
   PG_CHECK(chunk >= 1, "plan: chunk must be positive");
   const int64_t total = S * G;
   if (total == 0) return 0;
-  const int threads = 256;
-  hipLaunchKernelGGL(k_plan_weights, dim3(grid_for(total, threads)),
-                     dim3(threads), 0, as_stream(stream), vals_sorted,
+  hipLaunchKernelGGL(k_plan_weights, dim3(grid_for(total, kPwThreads)),
+                     dim3(kPwThreads), 0, as_stream(stream), vals_sorted,
                      gauss_sorted, n_invalid, cell_ptr, S, nper, G, kw, chunk,
                      weights, sqrt_weights, rep_src, rep_s, density,
                      unit_count, rep_stride);

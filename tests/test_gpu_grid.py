@@ -201,3 +201,22 @@ def test_hot_prepass_and_schedule_do_not_change_bits(device, monkeypatch):
     g_hot._transpose_all(yy, yt, None)
     g_hot.grid_program(yt, out, 1, n_blocks=8).launch("pg_grid_fp32")
     assert torch.equal(out, g_hot.grid_device(yy, "fp32"))
+
+
+def test_plan_weights_big_cells_bit_exact(device):
+    """Cells with hundreds to thousands of replicas (800 spokes through a
+    16 x 16 grid: 21 cells above 256, one above 2048) take the plan's
+    workgroup path for the per-cell sums: weights and sqrt weights match the
+    oracle and the density is bitwise the reference's ordered float32 sum."""
+    coords = _radial(800, 32, 3)
+    shape = (16, 16)
+    g = GrogInterpolator(shape, coords, kernel_width=3, device=device)
+    p = oplan.attach_fft_plan(oplan.create_plan_single(shape, coords, None, 3, "circle"))
+    cnt = np.bincount(p["indices"][p["sqrt_weights"] > 0], minlength=256)
+    assert (cnt > 256).sum() >= 10 and cnt.max() > 2048
+    np.testing.assert_array_equal(g.plan.indices.cpu().numpy(), p["indices"])
+    np.testing.assert_allclose(g.plan.weights.reshape(-1).cpu().numpy(), p["weights"].reshape(-1), rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(g.plan.sqrt_weights.cpu().numpy(), p["sqrt_weights"], rtol=1e-6, atol=1e-7)
+    dens_gpu = g.plan._compiled.density.reshape(-1).cpu().numpy()
+    p_gpu = dict(p, sqrt_weights=g.plan.sqrt_weights.cpu().numpy())
+    assert np.array_equal(dens_gpu, oplan.density(p_gpu)), "density summation order differs"
