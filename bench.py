@@ -140,7 +140,7 @@ def alg_bytes(kernel: str, st: dict) -> float:
 # kernels launched by one call of each entry point; k_hot_cells is shared by
 # the fp32 and reduce entries (same plan, same hot cells)
 ENTRY_KERNELS = {
-    "pg_grid_fp32": [r"k_hot_units<\d+, 0>", r"k_hot_cells<", r"k_tile<\d+, 0, \d+>|k_wtile<\d+, 0>"],
+    "pg_grid_fp32": [r"k_hot_units(?:_par)?<\d+, 0>", r"k_hot_cells<", r"k_tile<\d+, 0, \d+>|k_wtile<\d+, 0>"],
     "pg_grid_reduce_v": [r"k_hot_units<\d+, 1>", r"k_hot_cells<", r"k_tile<\d+, 1, \d+>|k_wtile<\d+, 1>|k_wrows<\d+>"],
     "pg_apply_grouped_bf16:1": [r"k_apply_grouped<\d+, \d+, 1>"],
     "pg_apply_grouped_bf16:3": [r"k_apply_grouped<\d+, \d+, 3>"],

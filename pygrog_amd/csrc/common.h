@@ -67,6 +67,26 @@ template <> __device__ __forceinline__ void gst<float4>(float4 *p, float4 v) {
   *(PG_GAS pg_f4n *)(uintptr_t)p = n;
 }
 
+// Streaming (non-temporal) variants for data read or written once: they do
+// not displace what should stay in the caches (the operator table).
+template <class T> __device__ __forceinline__ T gld_nt(const T *p) {
+  return __builtin_nontemporal_load((const PG_GAS T *)(uintptr_t)p);
+}
+template <> __device__ __forceinline__ float2 gld_nt<float2>(const float2 *p) {
+  const pg_f2n v = __builtin_nontemporal_load((const PG_GAS pg_f2n *)(uintptr_t)p);
+  return make_float2(v.x, v.y);
+}
+template <> __device__ __forceinline__ float4 gld_nt<float4>(const float4 *p) {
+  const pg_f4n v = __builtin_nontemporal_load((const PG_GAS pg_f4n *)(uintptr_t)p);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void gst_nt(float2 *p, float2 v) {
+  pg_f2n n;
+  n.x = v.x;
+  n.y = v.y;
+  __builtin_nontemporal_store(n, (PG_GAS pg_f2n *)(uintptr_t)p);
+}
+
 // Orders one wavefront's LDS writes before its later LDS reads (other lanes
 // of the same wavefront); no other wavefront is involved.
 __device__ __forceinline__ void wave_lds_sync() {

@@ -521,6 +521,25 @@ __global__ __launch_bounds__(kGridThreads) void k_tile(
 constexpr int kWaveCells = 64;
 constexpr int kWaveTile = (kGridThreads / kWave) * kWaveCells;
 
+// Streaming accesses of the wave-tile kernel (sample or V rows, records, cell
+// ranges, grid stores) are non-temporal: the operator table it applies is
+// read from L2 for every replica and is displaced less by data touched once
+// (measured on MI355X, 200 frames 192^2 x 16 coils: plain kernel -3 %,
+// V reduction -5 %).  The row-direct reduction keeps cached loads: there each
+// V row is read by several load instructions and non-temporal loads made it
+// 1.7x slower.
+#ifndef PG_GRID_NT
+#define PG_GRID_NT 1
+#endif
+template <class T> __device__ __forceinline__ T sld(const T *p) {
+  if constexpr (PG_GRID_NT != 0) return gld_nt(p);
+  else return gld(p);
+}
+__device__ __forceinline__ void sst(float2 *p, float2 v) {
+  if constexpr (PG_GRID_NT != 0) gst_nt(p, v);
+  else gst(p, v);
+}
+
 template <int GS> struct WaveRows {
   static constexpr int CW = GS < 32 ? GS : 32;  // output coils per pass
   static constexpr int RS = GS + 2;              // staged row stride (float2): 16-byte rows
@@ -601,8 +620,8 @@ __global__ __launch_bounds__(kGridThreads) void k_wtile(
     // the cell's row ranges; cells with rows but no ordinary rows are hot
     // (written by the pre-pass) or multi-unit (folded from memory below)
     const int64_t cl = c0 + (own ? l : 0);
-    const uint32_t vr0 = gld(fr.cell_vrow + cl), vr1 = gld(fr.cell_vrow + cl + 1);
-    const uint32_t cr0 = gld(fr.cell_row + cl), cr1 = gld(fr.cell_row + cl + 1);
+    const uint32_t vr0 = sld(fr.cell_vrow + cl), vr1 = sld(fr.cell_vrow + cl + 1);
+    const uint32_t cr0 = sld(fr.cell_row + cl), cr1 = sld(fr.cell_row + cl + 1);
     // (a cell with more ordinary rows than a batch holds -- only with a
     // unit chunk above RB -- is also folded from memory; its rows still pass
     // through the batches, so every staged row slot holds a real record)
@@ -644,9 +663,9 @@ __global__ __launch_bounds__(kGridThreads) void k_wtile(
           if (MODE != 1) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-              q0[i] = gld(fr.rep_src + r[i]);
-              q2[i] = gld(fr.rep_s + r[i]);
-              if (MODE == 0) q1[i] = gld(fr.rep_ent + r[i]);
+              q0[i] = sld(fr.rep_src + r[i]);
+              q2[i] = sld(fr.rep_s + r[i]);
+              if (MODE == 0) q1[i] = sld(fr.rep_ent + r[i]);
             }
           }
 #pragma unroll
@@ -685,9 +704,9 @@ __global__ __launch_bounds__(kGridThreads) void k_wtile(
               if (jb + i * RPI >= nk) break;
               const uint32_t q = ra[jb + j0 + i * RPI];
               if (MODE == 0)
-                v[i] = gld(reinterpret_cast<const float2 *>(fr.y) + (int64_t)q * C + cs);
+                v[i] = sld(reinterpret_cast<const float2 *>(fr.y) + (int64_t)q * C + cs);
               else if (MODE == 1)
-                v[i] = gld(reinterpret_cast<const float2 *>(fr.vbuf) + (int64_t)q * C + cs);
+                v[i] = sld(reinterpret_cast<const float2 *>(fr.vbuf) + (int64_t)q * C + cs);
               else
                 v[i] = gld(reinterpret_cast<const float2 *>(fr.y) + (int64_t)cs * fr.n_src + q);
             }
@@ -774,7 +793,7 @@ __global__ __launch_bounds__(kGridThreads) void k_wtile(
 #pragma unroll
             for (int c = 0; c < FC; ++c)
               if (cc + c8 + c < C)
-                gst(g + (int64_t)(cc + c8 + c) * fr.n_cell, make_float2(0.0f + acc[c].x, 0.0f + acc[c].y));
+                sst(g + (int64_t)(cc + c8 + c) * fr.n_cell, make_float2(0.0f + acc[c].x, 0.0f + acc[c].y));
           }
         } else if (in_b && tag == -2) {
 #pragma unroll 1
@@ -1206,7 +1225,7 @@ static int launch_tile(int64_t tile_cells, dim3 grid, hipStream_t st,
   if (tile_cells == kWaveTile) {
     // the V reduction: row-direct above 16 coils (rows of 256 bytes and
     // more), LDS-staged wave tiles below (measured on MI355X)
-    if (MODE == 1 && GS >= 32)
+    if constexpr (MODE == 1 && GS >= 32)
       hipLaunchKernelGGL((k_wrows<GS>), grid, block, 0, st, frames, tile_ptr,
                          F, C, n_tiles);
     else
@@ -1230,7 +1249,7 @@ static int launch_hot(hipStream_t st, const pg_frame *frames, int F, int C,
                       int64_t max_hot_units, int64_t max_hot) {
   constexpr int NG = kGridThreads / GS;
   const dim3 block(kGridThreads);
-  if (MODE == 1)
+  if constexpr (MODE == 1)
     hipLaunchKernelGGL((k_hot_units<GS, MODE>),
                        dim3((unsigned)ceil_div(max_hot_units, NG), (unsigned)F),
                        block, 0, st, frames, C);
