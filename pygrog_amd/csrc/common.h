@@ -86,6 +86,14 @@ __device__ __forceinline__ void gst_nt(float2 *p, float2 v) {
   n.y = v.y;
   __builtin_nontemporal_store(n, (PG_GAS pg_f2n *)(uintptr_t)p);
 }
+__device__ __forceinline__ void gst_nt(float4 *p, float4 v) {
+  pg_f4n n;
+  n.x = v.x;
+  n.y = v.y;
+  n.z = v.z;
+  n.w = v.w;
+  __builtin_nontemporal_store(n, (PG_GAS pg_f4n *)(uintptr_t)p);
+}
 
 // Orders one wavefront's LDS writes before its later LDS reads (other lanes
 // of the same wavefront); no other wavefront is involved.

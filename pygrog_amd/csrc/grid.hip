@@ -525,9 +525,10 @@ constexpr int kWaveTile = (kGridThreads / kWave) * kWaveCells;
 // ranges, grid stores) are non-temporal: the operator table it applies is
 // read from L2 for every replica and is displaced less by data touched once
 // (measured on MI355X, 200 frames 192^2 x 16 coils: plain kernel -3 %,
-// V reduction -5 %).  The row-direct reduction keeps cached loads: there each
-// V row is read by several load instructions and non-temporal loads made it
-// 1.7x slower.
+// V reduction -5 %).  The row-direct reduction keeps cached loads (each V row
+// is read by several load instructions there, and non-temporal loads made it
+// 1.7x slower) but stores the grid non-temporally (-5 %, kooshball 192^3 x 32
+// coils).
 #ifndef PG_GRID_NT
 #define PG_GRID_NT 1
 #endif
@@ -848,6 +849,9 @@ __device__ __forceinline__ void load_row(const float2 *vr, int C, int cc, bool e
 #ifndef PG_WROWS_MASKED
 #define PG_WROWS_MASKED 1
 #endif
+#ifndef PG_WROWS_NTST
+#define PG_WROWS_NTST 1
+#endif
 template <int GS>
 __global__ __launch_bounds__(kGridThreads) void k_wrows(
     const pg_frame *__restrict__ frames, const uint32_t *__restrict__ tile_ptr,
@@ -944,7 +948,11 @@ __global__ __launch_bounds__(kGridThreads) void k_wrows(
           }
 #pragma unroll
           for (int c = 0; c < CW; ++c)
-            if (cc + c < C) gst(g + (int64_t)(cc + c) * fr.n_cell, make_float2(0.0f + acc[c].x, 0.0f + acc[c].y));
+            if (cc + c < C) {
+              const float2 o = make_float2(0.0f + acc[c].x, 0.0f + acc[c].y);
+              if (PG_WROWS_NTST) gst_nt(g + (int64_t)(cc + c) * fr.n_cell, o);
+              else gst(g + (int64_t)(cc + c) * fr.n_cell, o);
+            }
         }
       }
     }
@@ -1085,7 +1093,17 @@ __global__ void k_table_pack(const float2 *__restrict__ K, int64_t nK, int C,
 // samples of one coil and writes four 16-byte pairs of coils of one sample,
 // so both sides move whole 512-byte / 256-byte runs.  Odd C or N (pairs not
 // 16-byte aligned) and ragged tiles take an element-wise path.
+// The full-tile path reads and writes non-temporally: both sides are touched
+// once, and the sample-major copy written that way is also read faster by the
+// grouped apply that follows (measured on MI355X, kooshball 192^3 x 32 coils:
+// transpose 0.70 -> 0.67 ms, apply 0.93 -> 0.91 ms).
 constexpr int kTrS = 64, kTrC = 32;
+#ifndef PG_TR_NTLD
+#define PG_TR_NTLD 1
+#endif
+#ifndef PG_TR_NTST
+#define PG_TR_NTST 1
+#endif
 
 __device__ __forceinline__ void transpose_tile(const float2 *__restrict__ y,
                                                int64_t C, int64_t N,
@@ -1099,7 +1117,8 @@ __device__ __forceinline__ void transpose_tile(const float2 *__restrict__ y,
     for (int i = 0; i < 4; ++i) {
       const int cl = t / 32 + 8 * i, sp = t % 32;
       if (c0 + cl < C) {
-        const float4 v = gld(reinterpret_cast<const float4 *>(y + (c0 + cl) * N + n0) + sp);
+        const float4 *src = reinterpret_cast<const float4 *>(y + (c0 + cl) * N + n0) + sp;
+        const float4 v = PG_TR_NTLD ? gld_nt(src) : gld(src);
         tile[cl][2 * sp] = make_float2(v.x, v.y);
         tile[cl][2 * sp + 1] = make_float2(v.z, v.w);
       }
@@ -1118,7 +1137,9 @@ __device__ __forceinline__ void transpose_tile(const float2 *__restrict__ y,
       const int sl = t / 16 + 16 * i, cp = t % 16;
       if (2 * cp < cw) {
         const float2 a = tile[2 * cp][sl], b = tile[2 * cp + 1][sl];
-        gst(reinterpret_cast<float4 *>(yt + (n0 + sl) * C + c0) + cp, make_float4(a.x, a.y, b.x, b.y));
+        float4 *dst = reinterpret_cast<float4 *>(yt + (n0 + sl) * C + c0) + cp;
+        if (PG_TR_NTST) gst_nt(dst, make_float4(a.x, a.y, b.x, b.y));
+        else gst(dst, make_float4(a.x, a.y, b.x, b.y));
       }
     }
   } else {
