@@ -120,17 +120,21 @@ def alg_bytes(kernel: str, st: dict) -> float:
         # sample row of every replica + dense grid + replica records (source,
         # entry, sqrt weight) + per-cell row offsets (2 x u32) + table once
         return 8 * C * R + 8 * C * G + 12 * R + 8 * G + 8 * C * C * K
-    if kernel == "pg_apply_grouped_bf16":
+    rb24 = 16 * ((6 * C + 15) // 16)  # a sample row of the 24-bit layout
+    if kernel in ("pg_apply_grouped_bf16", "pg_apply_grouped_bf16_s24"):
         parts = st["n_parts"]
         # sample rows + bf16 table (complex C x C per entry and part) + V rows
         # written + records (source, sqrt weight, replica id)
-        return 8 * C * R + 4 * parts * C * C * K + 8 * C * R + 12 * R
+        row = rb24 if kernel.endswith("_s24") else 8 * C
+        return row * R + 4 * parts * C * C * K + 8 * C * R + 12 * R
     if kernel == "pg_grid_reduce_v":
         return 8 * C * R + 8 * C * G + 8 * G
     if kernel == "pg_grid_scatter":
         return 8 * C * R + 8 * C * G + 8 * R + 8 * G
     if kernel in ("pg_transpose_cn_c64", "pg_transpose_frames_c64"):
         return 16 * C * N
+    if kernel in ("pg_transpose_cn_s24", "pg_transpose_frames_s24"):
+        return 8 * C * N + rb24 * N
     raise KeyError(kernel)
 
 
@@ -142,10 +146,14 @@ def alg_bytes(kernel: str, st: dict) -> float:
 ENTRY_KERNELS = {
     "pg_grid_fp32": [r"k_hot_units(?:_par)?<\d+, 0>", r"k_hot_cells<", r"k_tile<\d+, 0, \d+>|k_wtile<\d+, 0>"],
     "pg_grid_reduce_v": [r"k_hot_units<\d+, 1>", r"k_hot_cells<", r"k_tile<\d+, 1, \d+>|k_wtile<\d+, 1>|k_wrows<\d+>"],
-    "pg_apply_grouped_bf16:1": [r"k_apply_grouped<\d+, \d+, 1>"],
-    "pg_apply_grouped_bf16:3": [r"k_apply_grouped<\d+, \d+, 3>"],
-    "pg_transpose_cn_c64": [r"k_transpose_cn"],
-    "pg_transpose_frames_c64": [r"k_transpose_frames"],
+    "pg_apply_grouped_bf16:1": [r"k_apply_grouped<\d+, \d+, 1, false>"],
+    "pg_apply_grouped_bf16:3": [r"k_apply_grouped<\d+, \d+, 3, false>"],
+    "pg_apply_grouped_bf16_s24:1": [r"k_apply_grouped<\d+, \d+, 1, true>"],
+    "pg_apply_grouped_bf16_s24:3": [r"k_apply_grouped<\d+, \d+, 3, true>"],
+    "pg_transpose_cn_c64": [r"k_transpose_cn<false>"],
+    "pg_transpose_frames_c64": [r"k_transpose_frames<false>"],
+    "pg_transpose_cn_s24": [r"k_transpose_cn<true>"],
+    "pg_transpose_frames_s24": [r"k_transpose_frames<true>"],
 }
 COPY_KERNEL = r"neg_kernel"
 
@@ -665,7 +673,7 @@ def main(argv=None) -> int:
     roof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes": ab, "kernel_ms": kms}
     traffic_file = ROOT / "profiles" / f"traffic_{args.workload}.json"
-    tkey = kname + (f":{3 if head == 'bf16x3' else 1}" if kname == "pg_apply_grouped_bf16" else "")
+    tkey = kname + (f":{3 if head == 'bf16x3' else 1}" if kname.startswith("pg_apply_grouped_bf16") else "")
     if traffic_file.exists():
         tr = json.loads(traffic_file.read_text()).get("entries", {})
         if tkey in tr:

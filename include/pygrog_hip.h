@@ -347,6 +347,26 @@ int pg_transpose_frames_c64(const void *y, const int64_t *frame_off, int64_t F,
                             int64_t max_n, int64_t C, void *y_t,
                             pg_stream_t stream);
 
+/* 24-bit sample-major layout, the BF16 path's copy of the k-space: a
+ * sample's row holds its 2C values (re, im per coil), 3 bytes each -- the
+ * top three bytes of the fp32 value rounded to nearest even at bit 8 (16
+ * significant bits; NaN stays NaN, overflow rounds to infinity), packed
+ * little-endian -- and is padded to pg_s24_row_bytes(C) = 16 * ceil(6C / 16)
+ * bytes (-1 if C < 0).  The split-BF16 products use 16 bits of an input
+ * (hi = bf16(x), lo = bf16(x - hi)), so the copy holds what they read in 3/4
+ * of the bytes.  No reference counterpart: an internal format of the BF16
+ * gridding path. */
+int64_t pg_s24_row_bytes(int64_t C);
+/* pg_transpose_cn_c64 into the 24-bit layout: y_s (16-byte aligned) holds
+ * N rows of pg_s24_row_bytes(C) bytes. */
+int pg_transpose_cn_s24(const void *y, int64_t C, int64_t N, void *y_s,
+                        pg_stream_t stream);
+/* pg_transpose_frames_c64 into the 24-bit layout: frame f's rows start at
+ * byte pg_s24_row_bytes(C) * frame_off[f] of y_s (16-byte aligned). */
+int pg_transpose_frames_s24(const void *y, const int64_t *frame_off, int64_t F,
+                            int64_t max_n, int64_t C, void *y_s,
+                            pg_stream_t stream);
+
 /* Pack a (K, C, C) c64 table for the BF16 MFMA grouped GEMM: per entry and
  * part, (re, im) bf16 pairs of K[co][ci] padded to 8*ceil(2C/16) rows by
  * 16*ceil(2C/32) columns; part 0 = hi = bf16(x), part 1 = lo =
@@ -370,6 +390,15 @@ int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
                           const uint32_t *items, int64_t n_items,
                           int64_t K, int64_t C, int32_t n_terms, void *vbuf,
                           pg_stream_t stream);
+/* pg_apply_grouped_bf16 reading the 24-bit sample rows of
+ * pg_transpose_cn_s24 / pg_transpose_frames_s24 (y_s 16-byte aligned); the
+ * products and the V rows are as there.  No reference counterpart. */
+int pg_apply_grouped_bf16_s24(const void *y_s, const void *table_bf16,
+                              const uint32_t *ent_ptr, const uint32_t *ent_rep,
+                              const uint32_t *ent_src, const float *ent_s,
+                              const uint32_t *items, int64_t n_items,
+                              int64_t K, int64_t C, int32_t n_terms, void *vbuf,
+                              pg_stream_t stream);
 /* Element count (bf16) of the pg_table_pack_bf16 layout, -1 if C > 64. */
 int64_t pg_table_bf16_elems(int64_t K, int64_t C, int32_t n_parts);
 

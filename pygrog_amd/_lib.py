@@ -112,6 +112,9 @@ _SIGS = {
     "pg_unit_count": [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_transpose_cn_c64": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
     "pg_transpose_frames_c64": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p],
+    "pg_s24_row_bytes": [c_int64],
+    "pg_transpose_cn_s24": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
+    "pg_transpose_frames_s24": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p],
     "pg_vdot_c64_f64": [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p],
     "pg_vdot_workspace_bytes": [],
     "pg_sense_expand_c64": [c_void_p, c_void_p, c_int64, c_int64, ctypes.c_int32, c_void_p, c_void_p,
@@ -121,6 +124,7 @@ _SIGS = {
     "pg_table_pack_bf16": [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_table_bf16_elems": [c_int64, c_int64, c_int32],
     "pg_apply_grouped_bf16": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
+    "pg_apply_grouped_bf16_s24": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_sparse_apply_c64": [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
     "pg_csr_from_sorted": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
     "pg_csr_from_sorted_u32": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
@@ -129,6 +133,7 @@ _RESTYPES = {
     "pg_last_error": ctypes.c_char_p,
     "pg_table_bf16_elems": c_int64,
     "pg_table_packed_elems": c_int64,
+    "pg_s24_row_bytes": c_int64,
     "pg_vdot_workspace_bytes": c_int64,
 }
 

@@ -8,7 +8,8 @@ Here every frame is such an interpolator (same plan compiler, same
 quantisation, the table fitted once and shared), and one call grids all of
 them with a fixed, small number of launches whatever the frame count:
 
-  1. ``pg_transpose_frames_c64``  every frame's (C, N_f) block to (N_f, C) rows;
+  1. ``pg_transpose_frames_c64`` (``_s24`` for the BF16 modes up to 32 coils:
+     24-bit rows) every frame's (C, N_f) block to (N_f, C) rows;
   2. fp32:    ``pg_grid_fp32`` over all frames' tiles (plain per-replica apply);
      bf16(x3): ``pg_apply_grouped_bf16`` once over the replicas of all frames
               grouped by table entry (the table is shared, so groups are F
@@ -34,7 +35,8 @@ import torch
 
 from .. import _lib
 from ._grappa import build_table
-from ._grog import GridProgram, GrogInterpolator, KernelTimer, group_by_entry, tune_program
+from ._grog import (GridProgram, GrogInterpolator, KernelTimer, group_by_entry, new_sample_rows, sample_row_bytes,
+                    tune_program, uses_s24)
 
 _TUNE_CACHE: dict = {}
 
@@ -131,7 +133,8 @@ class GrogFrameBatch:
         if key in self._programs:
             return self._programs[key]
         C = self.C
-        yt = torch.empty_like(y)
+        yt = new_sample_rows(int(self.sample_off[-1]), C, uses_s24(mode, C), self.device)
+        rb = sample_row_bytes(C, yt.dtype == torch.uint8)
         tc = _lib.load().pg_grid_tile_cells(C)
         frames = []
         vbuf = None
@@ -139,7 +142,7 @@ class GrogFrameBatch:
             m = self._merged_groups()
             vbuf = torch.empty(max(m["R"], 1) * C, dtype=torch.complex64, device=self.device)
         for f, fr in enumerate(self.frames):
-            yv = yt[C * int(self.sample_off[f]):]
+            yv = yt[(rb // yt.element_size()) * int(self.sample_off[f]):]
             vv = None if vbuf is None else vbuf[C * int(self._merged["rep_off"][f]):]
             frames += fr._frames(yv, out[f], 1, vv)
         tune = self.tuned.get(mode, {})
@@ -165,25 +168,28 @@ class GrogFrameBatch:
     def _run(self, mode: str, y: torch.Tensor, out: torch.Tensor, timer: KernelTimer | None = None):
         yt, vbuf, prog = self._program(mode, y, out)
         st = _lib.stream(self.device)
+        s24 = yt.dtype == torch.uint8
+        name = "pg_transpose_frames_s24" if s24 else "pg_transpose_frames_c64"
         e0 = timer.start() if timer is not None else None
-        _lib.call("pg_transpose_frames_c64", _lib.ptr(y), _lib.ptr(self._off_dev), self.F, int(self.n_samples.max()),
+        _lib.call(name, _lib.ptr(y), _lib.ptr(self._off_dev), self.F, int(self.n_samples.max()),
                   self.C, _lib.ptr(yt), st)
         if timer is not None:
-            timer.stop("pg_transpose_frames_c64", e0)
+            timer.stop(name, e0)
         if mode == "fp32":
             prog.launch("pg_grid_fp32", timer)
             return
         m = self._merged_groups()
         n_terms = 1 if mode == "bf16" else 3
         tbl = self.frames[0]._table_bf16(2 if n_terms == 3 else 1)
+        name = "pg_apply_grouped_bf16_s24" if s24 else "pg_apply_grouped_bf16"
         e0 = timer.start() if timer is not None else None
         if m["n_items"] > 0:
             grp = m["groups"]
-            _lib.call("pg_apply_grouped_bf16", _lib.ptr(yt), _lib.ptr(tbl), _lib.ptr(grp.ent_ptr),
+            _lib.call(name, _lib.ptr(yt), _lib.ptr(tbl), _lib.ptr(grp.ent_ptr),
                       _lib.ptr(grp.ent_rep), _lib.ptr(grp.ent_src), _lib.ptr(grp.ent_s), _lib.ptr(grp.items),
                       grp.n_items, self.K, self.C, n_terms, _lib.ptr(vbuf), st)
         if timer is not None:
-            timer.stop("pg_apply_grouped_bf16", e0)
+            timer.stop(name, e0)
         prog.launch("pg_grid_reduce_v", timer)
 
     def new_output(self) -> torch.Tensor:

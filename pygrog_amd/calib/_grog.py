@@ -347,6 +347,45 @@ def group_by_entry(rep_ent: torch.Tensor, R: int, K: int, device, chunk: int, re
     return EntryGroups(ent_ptr, vals_o, items, total, ent_src, ent_s)
 
 
+# The BF16 modes can keep their sample-major copy of the k-space in the
+# library's 24-bit row format (fp32 rounded to nearest even at 16 significant
+# bits, 3 bytes a value, rows on 16-byte boundaries): the split products use
+# 16 bits of an input anyway, and the transpose writes 3/4 of the bytes.  The
+# grouped apply then reads one row per replica at random, and such a read
+# costs the whole 128-byte L2 lines the row touches (measured on MI355X: at 32
+# coils 192-byte rows fetch as much as 256-byte ones), so the format is used
+# where its rows touch no more lines on average than complex64 rows (32
+# coils: 2 and 2; 16 coils: 1.5 against 1, and the apply was slower), and up
+# to this many coils (above, the 24-bit apply kernel runs out of registers).
+S24_MAX_COILS = 32
+
+
+def _lines_per_row(row_bytes: int) -> float:
+    """128-byte lines a row of ``row_bytes`` touches, averaged over the
+    offsets rows at a stride of ``row_bytes`` take."""
+    offs = sorted({(k * row_bytes) % 128 for k in range(128)})
+    return sum((o + row_bytes + 127) // 128 for o in offs) / len(offs)
+
+
+def uses_s24(mode: str, C: int) -> bool:
+    """Whether gridding mode ``mode`` at ``C`` coils uses 24-bit sample rows."""
+    if mode == "fp32" or C > S24_MAX_COILS:
+        return False
+    return _lines_per_row(16 * ((6 * C + 15) // 16)) <= _lines_per_row(8 * C)
+
+
+def sample_row_bytes(C: int, s24: bool) -> int:
+    """Bytes of one sample's row of the sample-major copy."""
+    return int(_lib.load().pg_s24_row_bytes(C)) if s24 else 8 * C
+
+
+def new_sample_rows(n: int, C: int, s24: bool, device) -> torch.Tensor:
+    """Buffer for the sample-major copy of ``n`` samples."""
+    if s24:
+        return torch.empty(max(n, 1) * sample_row_bytes(C, True), dtype=torch.uint8, device=device)
+    return torch.empty(max(n, 1) * C, dtype=torch.complex64, device=device)
+
+
 class GridProgram:
     """A ready-to-launch fused gridding job: device frame table + tile map.
 
@@ -792,17 +831,20 @@ class GrogInterpolator:
 
     # -- gridding on device (no host transfer) --------------------------------
     def _transpose_all(self, y: torch.Tensor, yt: torch.Tensor, timer) -> None:
-        """(B, S, C, N) coil-major -> per frame (N, C) sample-major rows."""
+        """(B, S, C, N) coil-major -> per frame (N, C) sample-major rows
+        (complex64, or the 24-bit format when ``yt`` is a byte buffer)."""
         comp = self.plan._compiled
         B, S, C, N = y.shape
+        s24 = yt.dtype == torch.uint8
+        rb = sample_row_bytes(C, s24)
+        name = "pg_transpose_cn_s24" if s24 else "pg_transpose_cn_c64"
         st = _lib.stream(comp.device)
         e0 = timer.start() if timer is not None else None
         for b in range(B):
             for s in range(S):
-                _lib.call("pg_transpose_cn_c64", y[b, s].data_ptr(), C, N,
-                          yt.data_ptr() + (b * S + s) * C * N * 8, st)
+                _lib.call(name, y[b, s].data_ptr(), C, N, yt.data_ptr() + (b * S + s) * N * rb, st)
         if timer is not None:
-            timer.stop("pg_transpose_cn_c64", e0)
+            timer.stop(name, e0)
 
     def grid_device(self, y: torch.Tensor, mode: str = "fp32", out: torch.Tensor | None = None,
                     timer: KernelTimer | None = None, program_cache: dict | None = None) -> torch.Tensor:
@@ -831,7 +873,7 @@ class GrogInterpolator:
         cache = program_cache if program_cache is not None else {}
         key = (mode, y.data_ptr(), out.data_ptr(), B)
         if key not in cache:
-            yt = torch.empty(B * S * N * C, dtype=torch.complex64, device=comp.device)
+            yt = new_sample_rows(B * S * N, C, uses_s24(mode, C), comp.device)
             if mode == "fp32":
                 cache[key] = (None, yt, self.grid_program(yt, out, B, mode=mode))
             else:
@@ -847,19 +889,22 @@ class GrogInterpolator:
         groups = self._entry_groups()
         st = _lib.stream(comp.device)
         K = int(comp.table.shape[0])
+        s24 = yt.dtype == torch.uint8
+        rb = sample_row_bytes(C, s24)
+        name = "pg_apply_grouped_bf16_s24" if s24 else "pg_apply_grouped_bf16"
         e0 = timer.start() if timer is not None else None
         for b in range(B):
             for s in range(S):
                 grp = groups[s]
                 if comp.n_valid[s] == 0 or grp.n_items == 0:
                     continue
-                yf = yt.data_ptr() + (b * S + s) * C * N * 8
+                yf = yt.data_ptr() + (b * S + s) * N * rb
                 vb = vbuf.data_ptr() + (b * S + s) * self._rmax() * C * 8
-                _lib.call("pg_apply_grouped_bf16", yf, _lib.ptr(tbl), _lib.ptr(grp.ent_ptr), _lib.ptr(grp.ent_rep),
+                _lib.call(name, yf, _lib.ptr(tbl), _lib.ptr(grp.ent_ptr), _lib.ptr(grp.ent_rep),
                           _lib.ptr(grp.ent_src), _lib.ptr(grp.ent_s), _lib.ptr(grp.items), grp.n_items, K, C,
                           n_terms, vb, st)
         if timer is not None:
-            timer.stop("pg_apply_grouped_bf16", e0)
+            timer.stop(name, e0)
         prog.launch("pg_grid_reduce_v", timer)
         return out
 

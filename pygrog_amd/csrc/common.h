@@ -49,6 +49,11 @@ template <> __device__ __forceinline__ float4 gld<float4>(const float4 *p) {
   const pg_f4n v = *(const PG_GAS pg_f4n *)(uintptr_t)p;
   return make_float4(v.x, v.y, v.z, v.w);
 }
+typedef unsigned int pg_u2n __attribute__((ext_vector_type(2)));
+template <> __device__ __forceinline__ uint2 gld<uint2>(const uint2 *p) {
+  const pg_u2n v = *(const PG_GAS pg_u2n *)(uintptr_t)p;
+  return make_uint2(v.x, v.y);
+}
 template <class T> __device__ __forceinline__ void gst(T *p, T v) {
   *(PG_GAS T *)(uintptr_t)p = v;
 }
@@ -93,6 +98,41 @@ __device__ __forceinline__ void gst_nt(float4 *p, float4 v) {
   n.z = v.z;
   n.w = v.w;
   __builtin_nontemporal_store(n, (PG_GAS pg_f4n *)(uintptr_t)p);
+}
+
+// 24-bit sample format of the BF16 path's sample-major rows: the top three
+// bytes of the fp32 value rounded to nearest even at bit 8, i.e. 16
+// significant bits (NaN stays NaN, overflow rounds to infinity), packed
+// little-endian 3 bytes per value; a sample's row of 2C values (re, im per
+// coil) starts on a 16-byte boundary.  16 bits is what the split-BF16
+// products use of an input anyway: x = hi + lo with hi = bf16(x) and
+// lo = x - hi exactly representable in bf16.
+typedef unsigned int pg_u3n __attribute__((ext_vector_type(3)));
+__host__ __device__ constexpr int64_t s24_row_bytes(int64_t C) { return (6 * C + 15) / 16 * 16; }
+__device__ __forceinline__ uint32_t s24_round(float x) {
+  const uint32_t b = __float_as_uint(x);
+  if ((b & 0x7f800000u) == 0x7f800000u && (b & 0x7fffffu) != 0u) return (b >> 8) | 0x4000u;
+  return (b + 0x7fu + ((b >> 8) & 1u)) >> 8;
+}
+// values 0..3 of three packed dwords (d0 = v0 | v1 << 24, d1 = v1 >> 8 |
+// v2 << 16, d2 = v2 >> 16 | v3 << 8) as fp32 bit patterns
+__device__ __forceinline__ void s24_unpack4(uint32_t d0, uint32_t d1, uint32_t d2, float *o) {
+  o[0] = __uint_as_float(d0 << 8);
+  o[1] = __uint_as_float((d1 << 16) | ((d0 >> 16) & 0xff00u));
+  o[2] = __uint_as_float((d2 << 24) | ((d1 >> 8) & 0xffff00u));
+  o[3] = __uint_as_float(d2 & 0xffffff00u);
+}
+__device__ __forceinline__ pg_u3n s24_pack4(float a, float b, float c, float d) {
+  const uint32_t v0 = s24_round(a), v1 = s24_round(b), v2 = s24_round(c), v3 = s24_round(d);
+  pg_u3n r;
+  r.x = v0 | (v1 << 24);
+  r.y = (v1 >> 8) | (v2 << 16);
+  r.z = (v2 >> 16) | (v3 << 8);
+  return r;
+}
+__device__ __forceinline__ float s24_load(const unsigned char *p) {
+  const PG_GAS unsigned char *q = (const PG_GAS unsigned char *)(uintptr_t)p;
+  return __uint_as_float(((uint32_t)q[0] << 8) | ((uint32_t)q[1] << 16) | ((uint32_t)q[2] << 24));
 }
 
 // Orders one wavefront's LDS writes before its later LDS reads (other lanes

@@ -1105,11 +1105,15 @@ constexpr int kTrS = 64, kTrC = 32;
 #define PG_TR_NTST 1
 #endif
 
+template <bool S24>
 __device__ __forceinline__ void transpose_tile(const float2 *__restrict__ y,
                                                int64_t C, int64_t N,
-                                               float2 *__restrict__ yt,
+                                               void *__restrict__ out,
                                                int64_t n0, int64_t c0,
                                                float2 (*tile)[kTrS + 1]) {
+  float2 *yt = static_cast<float2 *>(out);
+  unsigned char *ys = static_cast<unsigned char *>(out);
+  const int64_t rb = s24_row_bytes(C);
   const int t = threadIdx.x;
   const bool full = n0 + kTrS <= N && ((C | N) & 1) == 0;
   if (full) {
@@ -1137,37 +1141,58 @@ __device__ __forceinline__ void transpose_tile(const float2 *__restrict__ y,
       const int sl = t / 16 + 16 * i, cp = t % 16;
       if (2 * cp < cw) {
         const float2 a = tile[2 * cp][sl], b = tile[2 * cp + 1][sl];
-        float4 *dst = reinterpret_cast<float4 *>(yt + (n0 + sl) * C + c0) + cp;
-        if (PG_TR_NTST) gst_nt(dst, make_float4(a.x, a.y, b.x, b.y));
-        else gst(dst, make_float4(a.x, a.y, b.x, b.y));
+        if constexpr (S24) {
+          // two coils = 12 bytes at byte 6 (c0 + 2 cp) of the row
+          PG_GAS pg_u3n *dst = (PG_GAS pg_u3n *)(uintptr_t)(ys + (n0 + sl) * rb + 6 * (c0 + 2 * cp));
+          __builtin_nontemporal_store(s24_pack4(a.x, a.y, b.x, b.y), dst);
+        } else {
+          float4 *dst = reinterpret_cast<float4 *>(yt + (n0 + sl) * C + c0) + cp;
+          if (PG_TR_NTST) gst_nt(dst, make_float4(a.x, a.y, b.x, b.y));
+          else gst(dst, make_float4(a.x, a.y, b.x, b.y));
+        }
       }
     }
   } else {
     for (int i = t; i < kTrS * kTrC; i += 256) {
       const int sl = i / kTrC, cl = i % kTrC;
-      if (cl < cw && n0 + sl < N) yt[(n0 + sl) * C + c0 + cl] = tile[cl][sl];
+      if (cl < cw && n0 + sl < N) {
+        if constexpr (S24) {
+          // one coil = 6 bytes at an even byte offset: three 16-bit stores
+          const uint32_t re = s24_round(tile[cl][sl].x), im = s24_round(tile[cl][sl].y);
+          unsigned short *d = reinterpret_cast<unsigned short *>(ys + (n0 + sl) * rb + 6 * (c0 + cl));
+          d[0] = (unsigned short)re;
+          d[1] = (unsigned short)((re >> 16) | (im << 8));
+          d[2] = (unsigned short)(im >> 8);
+        } else {
+          yt[(n0 + sl) * C + c0 + cl] = tile[cl][sl];
+        }
+      }
     }
   }
 }
 
+template <bool S24>
 __global__ __launch_bounds__(256) void k_transpose_cn(const float2 *__restrict__ y, int64_t C,
-                                                      int64_t N, float2 *__restrict__ yt) {
+                                                      int64_t N, void *__restrict__ yt) {
   __shared__ float2 tile[kTrC][kTrS + 1];
-  transpose_tile(y, C, N, yt, (int64_t)blockIdx.x * kTrS, (int64_t)blockIdx.y * kTrC, tile);
+  transpose_tile<S24>(y, C, N, yt, (int64_t)blockIdx.x * kTrS, (int64_t)blockIdx.y * kTrC, tile);
 }
 
 // Frames of uneven length back to back: frame f holds (C, N_f) coil-major
 // at sample offset off[f] (element offset C * off[f]); its (N_f, C)
 // sample-major rows go to the same offset of yt.  blockIdx.z = frame.
+template <bool S24>
 __global__ __launch_bounds__(256) void k_transpose_frames(const float2 *__restrict__ y,
                                                           const int64_t *__restrict__ off,
-                                                          int64_t C, float2 *__restrict__ yt) {
+                                                          int64_t C, void *__restrict__ yt) {
   __shared__ float2 tile[kTrC][kTrS + 1];
   const int64_t base = off[blockIdx.z];
   const int64_t N = off[blockIdx.z + 1] - base;
   const int64_t n0 = (int64_t)blockIdx.x * kTrS;
   if (n0 >= N) return;
-  transpose_tile(y + C * base, C, N, yt + C * base, n0, (int64_t)blockIdx.y * kTrC, tile);
+  void *o = S24 ? (void *)(static_cast<unsigned char *>(yt) + s24_row_bytes(C) * base)
+                : (void *)(static_cast<float2 *>(yt) + C * base);
+  transpose_tile<S24>(y + C * base, C, N, o, n0, (int64_t)blockIdx.y * kTrC, tile);
 }
 
 // out[b, co, m] = K[idx[m]][co] . y[b, :, m / kw]
@@ -1440,33 +1465,67 @@ int pg_grid_scatter(const pg_frame *frames, const uint32_t *tile_ptr,
                         max_hot_units, max_hot, stream);
 }
 
-int pg_transpose_cn_c64(const void *y, int64_t C, int64_t N, void *y_t,
-                        pg_stream_t stream) {
+static int transpose_cn(const void *y, int64_t C, int64_t N, void *y_t,
+                        bool s24, pg_stream_t stream) {
   PG_CHECK(C >= 0 && N >= 0 && C < 65535 * 32, "transpose: bad sizes");
+  PG_CHECK(!s24 || ((uintptr_t)y_t & 15) == 0, "transpose_s24: output not 16-byte aligned");
   if (C == 0 || N == 0) return 0;
   const dim3 grid((unsigned)ceil_div(N, kTrS), (unsigned)ceil_div(C, kTrC));
-  hipLaunchKernelGGL(k_transpose_cn, grid, dim3(256), 0, as_stream(stream),
-                     static_cast<const float2 *>(y), C, N,
-                     static_cast<float2 *>(y_t));
+  if (s24)
+    hipLaunchKernelGGL(k_transpose_cn<true>, grid, dim3(256), 0, as_stream(stream),
+                       static_cast<const float2 *>(y), C, N, y_t);
+  else
+    hipLaunchKernelGGL(k_transpose_cn<false>, grid, dim3(256), 0, as_stream(stream),
+                       static_cast<const float2 *>(y), C, N, y_t);
   PG_LAUNCHED();
   return 0;
 }
 
-int pg_transpose_frames_c64(const void *y, const int64_t *frame_off,
-                            int64_t F, int64_t max_n, int64_t C, void *y_t,
+static int transpose_frames(const void *y, const int64_t *frame_off, int64_t F,
+                            int64_t max_n, int64_t C, void *y_t, bool s24,
                             pg_stream_t stream) {
   PG_CHECK(F >= 0 && F < 65536 && C >= 0 && C < 65535 * 32 && max_n >= 0,
            "transpose_frames: bad sizes (F=%lld, C=%lld)", (long long)F,
            (long long)C);
   PG_CHECK(F == 0 || frame_off, "transpose_frames: null offsets");
+  PG_CHECK(!s24 || ((uintptr_t)y_t & 15) == 0, "transpose_frames_s24: output not 16-byte aligned");
   if (F == 0 || C == 0 || max_n == 0) return 0;
   const dim3 grid((unsigned)ceil_div(max_n, kTrS), (unsigned)ceil_div(C, kTrC),
                   (unsigned)F);
-  hipLaunchKernelGGL(k_transpose_frames, grid, dim3(256), 0,
-                     as_stream(stream), static_cast<const float2 *>(y),
-                     frame_off, C, static_cast<float2 *>(y_t));
+  if (s24)
+    hipLaunchKernelGGL(k_transpose_frames<true>, grid, dim3(256), 0,
+                       as_stream(stream), static_cast<const float2 *>(y),
+                       frame_off, C, y_t);
+  else
+    hipLaunchKernelGGL(k_transpose_frames<false>, grid, dim3(256), 0,
+                       as_stream(stream), static_cast<const float2 *>(y),
+                       frame_off, C, y_t);
   PG_LAUNCHED();
   return 0;
+}
+
+int pg_transpose_cn_c64(const void *y, int64_t C, int64_t N, void *y_t,
+                        pg_stream_t stream) {
+  return transpose_cn(y, C, N, y_t, false, stream);
+}
+
+int pg_transpose_frames_c64(const void *y, const int64_t *frame_off,
+                            int64_t F, int64_t max_n, int64_t C, void *y_t,
+                            pg_stream_t stream) {
+  return transpose_frames(y, frame_off, F, max_n, C, y_t, false, stream);
+}
+
+int64_t pg_s24_row_bytes(int64_t C) { return C >= 0 ? s24_row_bytes(C) : -1; }
+
+int pg_transpose_cn_s24(const void *y, int64_t C, int64_t N, void *y_s,
+                        pg_stream_t stream) {
+  return transpose_cn(y, C, N, y_s, true, stream);
+}
+
+int pg_transpose_frames_s24(const void *y, const int64_t *frame_off,
+                            int64_t F, int64_t max_n, int64_t C, void *y_s,
+                            pg_stream_t stream) {
+  return transpose_frames(y, frame_off, F, max_n, C, y_s, true, stream);
 }
 
 int pg_sparse_apply_c64(const void *y, const int64_t *interp_idx,

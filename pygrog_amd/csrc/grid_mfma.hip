@@ -71,9 +71,9 @@ __global__ void k_table_pack_bf16(const float2 *__restrict__ K, int64_t nK,
 #endif
 constexpr int kTilesPerWave = PG_APPLY_TILES;  // 16-replica N tiles per wavefront
 
-template <int MT, int KS, int NTERMS>
+template <int MT, int KS, int NTERMS, bool S24>
 __global__ __launch_bounds__(256) void k_apply_grouped(
-    const float *__restrict__ y_t, const unsigned short *__restrict__ tbl,
+    const void *__restrict__ y_t, const unsigned short *__restrict__ tbl,
     const uint32_t *__restrict__ ent_ptr, const uint32_t *__restrict__ ent_rep,
     const uint32_t *__restrict__ ent_src, const float *__restrict__ ent_s,
     const uint32_t *__restrict__ items, int64_t n_items, int64_t nK, int C,
@@ -144,9 +144,37 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
   }
   const bool vec = (twoC & 3) == 0;
   // sample-row fragments of one tile: KS x 8 floats per lane
+  // 24-bit rows stay packed until their tile is used: 8 values = 24 bytes
+  // = 6 dwords at byte 3 col0 (8-byte aligned); past the row's 2C values
+  // the bytes are zero (value 0)
+  auto load_packed = [&](int t, uint32_t (&d)[KS][6]) {
+    const bool v = start + 16 * t + j < stop;
+    const unsigned char *yr = static_cast<const unsigned char *>(y_t) + (int64_t)src[t] * s24_row_bytes(C);
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int col0 = 32 * ks + 8 * q;
+      if (v && col0 + 8 <= twoC) {
+        const uint2 *p = reinterpret_cast<const uint2 *>(yr + 3 * col0);
+        const uint2 a = gld(p), b = gld(p + 1), c = gld(p + 2);
+        d[ks][0] = a.x; d[ks][1] = a.y; d[ks][2] = b.x;
+        d[ks][3] = b.y; d[ks][4] = c.x; d[ks][5] = c.y;
+      } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          uint32_t w = 0;
+#pragma unroll
+          for (int by = 0; by < 4; ++by) {
+            const int o = 3 * col0 + 4 * k + by;
+            if (v && o < 3 * twoC) w |= (uint32_t)gld(yr + o) << (8 * by);
+          }
+          d[ks][k] = w;
+        }
+      }
+    }
+  };
   auto load_rows = [&](int t, float (&f)[KS][8]) {
     const bool v = start + 16 * t + j < stop;
-    const float *yr = y_t + (int64_t)src[t] * twoC;
+    const float *yr = static_cast<const float *>(y_t) + (int64_t)src[t] * twoC;
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       const int col0 = 32 * ks + 8 * q;
@@ -168,17 +196,38 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
 #ifndef PG_APPLY_DEPTH
 #define PG_APPLY_DEPTH 2
 #endif
-  constexpr int DEPTH = KS <= 2 ? PG_APPLY_DEPTH : 1;
+#ifndef PG_APPLY_DEPTH24
+#define PG_APPLY_DEPTH24 2
+#endif
+  constexpr int DEPTH = KS <= 2 ? (S24 ? PG_APPLY_DEPTH24 : PG_APPLY_DEPTH) : 1;
   constexpr int NBUF = DEPTH + 1;
-  float fr_[NBUF][KS][8];
+  float fr_[NBUF][KS][8];     // fp32 rows
+  uint32_t fs_[NBUF][KS][6];  // 24-bit rows, packed
+  auto load = [&](int t, int buf) {
+    if constexpr (S24) load_packed(t, fs_[buf]);
+    else load_rows(t, fr_[buf]);
+  };
 #pragma unroll
   for (int d = 0; d < DEPTH; ++d)
-    if (d < n_tiles) load_rows(d, fr_[d]);
+    if (d < n_tiles) load(d, d);
 #pragma unroll
   for (int t = 0; t < kTilesPerWave; ++t) {
     if (t >= n_tiles) break;
-    float (&cur)[KS][8] = fr_[t % NBUF];
-    if (t + DEPTH < n_tiles) load_rows(t + DEPTH, fr_[(t + DEPTH) % NBUF]);
+    float cur[KS][8];
+    if constexpr (S24) {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const uint32_t(&d)[6] = fs_[t % NBUF][ks];
+        s24_unpack4(d[0], d[1], d[2], &cur[ks][0]);
+        s24_unpack4(d[3], d[4], d[5], &cur[ks][4]);
+      }
+    } else {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) cur[ks][i] = fr_[t % NBUF][ks][i];
+    }
+    if (t + DEPTH < n_tiles) load(t + DEPTH, (t + DEPTH) % NBUF);
     floatx4_t acc[MT];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) acc[mt] = floatx4_t{0.f, 0.f, 0.f, 0.f};
@@ -255,23 +304,25 @@ static int launch_grouped(const void *y_t, const void *tbl,
                           const uint32_t *ent_ptr, const uint32_t *ent_rep,
                           const uint32_t *ent_src, const float *ent_s,
                           const uint32_t *items, int64_t n_items, int64_t K,
-                          int64_t C, int32_t n_terms, void *vbuf,
+                          int64_t C, int32_t n_terms, bool s24, void *vbuf,
                           hipStream_t st) {
   if (n_items == 0) return 0;
   PG_CHECK(items != nullptr, "grouped apply: null work list");
+  PG_CHECK(!s24 || ((uintptr_t)y_t & 15) == 0, "grouped apply: 24-bit rows not 16-byte aligned");
   const dim3 grid((unsigned)ceil_div(n_items, 4));
-  if (n_terms == 3)
-    hipLaunchKernelGGL((k_apply_grouped<MT, KS, 3>), grid, dim3(256), 0, st,
-                       static_cast<const float *>(y_t),
-                       static_cast<const unsigned short *>(tbl), ent_ptr,
-                       ent_rep, ent_src, ent_s, items, n_items, K, (int)C,
-                       static_cast<float *>(vbuf));
-  else
-    hipLaunchKernelGGL((k_apply_grouped<MT, KS, 1>), grid, dim3(256), 0, st,
-                       static_cast<const float *>(y_t),
-                       static_cast<const unsigned short *>(tbl), ent_ptr,
-                       ent_rep, ent_src, ent_s, items, n_items, K, (int)C,
-                       static_cast<float *>(vbuf));
+#define PG_APPLY_LAUNCH(NT, S)                                                   \
+  hipLaunchKernelGGL((k_apply_grouped<MT, KS, NT, S>), grid, dim3(256), 0, st,   \
+                     y_t, static_cast<const unsigned short *>(tbl), ent_ptr,      \
+                     ent_rep, ent_src, ent_s, items, n_items, K, (int)C,          \
+                     static_cast<float *>(vbuf))
+  if (n_terms == 3) {
+    if (s24) PG_APPLY_LAUNCH(3, true);
+    else PG_APPLY_LAUNCH(3, false);
+  } else {
+    if (s24) PG_APPLY_LAUNCH(1, true);
+    else PG_APPLY_LAUNCH(1, false);
+  }
+#undef PG_APPLY_LAUNCH
   PG_LAUNCHED();
   return 0;
 }
@@ -305,12 +356,12 @@ int64_t pg_table_bf16_elems(int64_t K, int64_t C, int32_t n_parts) {
   return K * n_parts * (int64_t)(8 * MT) * (16 * KS) * 2;
 }
 
-int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
-                          const uint32_t *ent_ptr, const uint32_t *ent_rep,
-                          const uint32_t *ent_src, const float *ent_s,
-                          const uint32_t *items, int64_t n_items,
-                          int64_t K, int64_t C, int32_t n_terms, void *vbuf,
-                          pg_stream_t stream) {
+static int apply_grouped(const void *y_t, const void *table_bf16,
+                         const uint32_t *ent_ptr, const uint32_t *ent_rep,
+                         const uint32_t *ent_src, const float *ent_s,
+                         const uint32_t *items, int64_t n_items, int64_t K,
+                         int64_t C, int32_t n_terms, bool s24, void *vbuf,
+                         pg_stream_t stream) {
   PG_CHECK(C >= 1 && C <= 64, "grouped apply: coil count must be in [1, 64]");
   PG_CHECK(n_terms == 1 || n_terms == 3, "grouped apply: n_terms 1 or 3");
   PG_CHECK(K >= 1, "grouped apply: empty table");
@@ -321,7 +372,7 @@ int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
   if (MT == MTV && KS == KSV)                                                \
     return launch_grouped<MTV, KSV>(y_t, table_bf16, ent_ptr, ent_rep,       \
                                     ent_src, ent_s, items, n_items, K, C,     \
-                                    n_terms, vbuf, st);
+                                    n_terms, s24, vbuf, st);
   PG_GROUPED_CASE(1, 1)
   PG_GROUPED_CASE(2, 1)
   PG_GROUPED_CASE(3, 2)
@@ -333,6 +384,26 @@ int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
 #undef PG_GROUPED_CASE
   pg::set_error("grouped apply: unsupported shape MT=%d KS=%d", MT, KS);
   return 1;
+}
+
+int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
+                          const uint32_t *ent_ptr, const uint32_t *ent_rep,
+                          const uint32_t *ent_src, const float *ent_s,
+                          const uint32_t *items, int64_t n_items,
+                          int64_t K, int64_t C, int32_t n_terms, void *vbuf,
+                          pg_stream_t stream) {
+  return apply_grouped(y_t, table_bf16, ent_ptr, ent_rep, ent_src, ent_s, items,
+                       n_items, K, C, n_terms, false, vbuf, stream);
+}
+
+int pg_apply_grouped_bf16_s24(const void *y_s, const void *table_bf16,
+                              const uint32_t *ent_ptr, const uint32_t *ent_rep,
+                              const uint32_t *ent_src, const float *ent_s,
+                              const uint32_t *items, int64_t n_items,
+                              int64_t K, int64_t C, int32_t n_terms, void *vbuf,
+                              pg_stream_t stream) {
+  return apply_grouped(y_s, table_bf16, ent_ptr, ent_rep, ent_src, ent_s, items,
+                       n_items, K, C, n_terms, true, vbuf, stream);
 }
 
 }  // extern "C"

@@ -42,7 +42,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .calib._grog import GridProgram, GrogInterpolator, _default_oversamp, _kernel_offsets
+from .calib._grog import GridProgram, GrogInterpolator, _default_oversamp, _kernel_offsets, new_sample_rows, uses_s24
 
 
 @dataclasses.dataclass
@@ -169,9 +169,11 @@ class GrogServer:
                 y = job.kspace
                 if y.device != self.device or y.dtype != torch.complex64 or not y.is_contiguous():
                     y = y.to(self.device, torch.complex64).contiguous()
-                yt = torch.empty(comp.npts * comp.C, dtype=torch.complex64, device=self.device)
+                s24 = uses_s24(self.mode, comp.C)
+                yt = new_sample_rows(comp.npts, comp.C, s24, self.device)
                 out = torch.empty(comp.C, comp.G, dtype=torch.complex64, device=self.device)
-                _lib.call("pg_transpose_cn_c64", _lib.ptr(y), comp.C, comp.npts, _lib.ptr(yt), st.cuda_stream)
+                _lib.call("pg_transpose_cn_s24" if s24 else "pg_transpose_cn_c64", _lib.ptr(y), comp.C, comp.npts,
+                          _lib.ptr(yt), st.cuda_stream)
                 vbuf = None
                 if self.mode != "fp32":
                     vbuf = torch.empty(g._rmax() * comp.C, dtype=torch.complex64, device=self.device)
@@ -179,7 +181,8 @@ class GrogServer:
                     tbl = g._table_bf16(2 if n_terms == 3 else 1)
                     grp = g._entry_groups()[0]
                     if grp.n_items > 0:
-                        _lib.call("pg_apply_grouped_bf16", _lib.ptr(yt), _lib.ptr(tbl), _lib.ptr(grp.ent_ptr),
+                        _lib.call("pg_apply_grouped_bf16_s24" if s24 else "pg_apply_grouped_bf16", _lib.ptr(yt),
+                                  _lib.ptr(tbl), _lib.ptr(grp.ent_ptr),
                                   _lib.ptr(grp.ent_rep), _lib.ptr(grp.ent_src), _lib.ptr(grp.ent_s),
                                   _lib.ptr(grp.items), grp.n_items, int(comp.table.shape[0]), comp.C, n_terms,
                                   _lib.ptr(vbuf), st.cuda_stream)

@@ -120,7 +120,7 @@ def test_grouped_modes_linear_on_kooshball(device):
         assert rel <= tol, (mode, rel)
 
 
-@pytest.mark.parametrize("C", [3, 16, 33, 64])
+@pytest.mark.parametrize("C", [3, 16, 18, 32, 33, 64])
 @pytest.mark.parametrize("hot", [True, False])
 def test_every_tile_shape_same_bits(device, monkeypatch, C, hot):
     """Every compiled launch shape -- the wave-tile kernel (256 cells per
@@ -188,7 +188,7 @@ def test_scatter_every_tile_shape_same_bits(device, monkeypatch, C):
         assert torch.equal(sp.scatter(xd), base), tc
 
 
-@pytest.mark.parametrize("C", [8, 16])
+@pytest.mark.parametrize("C", [8, 16, 32])
 def test_long_units_every_tile_shape(device, monkeypatch, C):
     """Reduction units longer than a wave tile's row batch and than the
     parallel hot pre-pass holds (unit chunk 80 instead of 32): the wave-tile
@@ -222,3 +222,131 @@ def test_long_units_every_tile_shape(device, monkeypatch, C):
             g._tuned = {mode: (tc, None)}
             assert torch.equal(g.grid_device(yy, mode, program_cache={}), base), (mode, tc)
         g._tuned = {}
+
+
+@pytest.mark.parametrize("C", [18, 32])
+def test_reduce_unaligned_v_same_bits(device, C):
+    """A caller's V buffer need only be 8-byte aligned: the grouped path's
+    V moved to an 8-byte offset reduces to the same bits (16-byte row loads
+    at 8-byte alignment), with the hot pre-pass, on 256-cell wave tiles."""
+    from pygrog_amd.calib import GrogInterpolator
+    from pygrog_amd.calib._grog import GridProgram
+
+    rng = np.random.default_rng(300 + C)
+    shape = (80, 80)
+    coords = _radial_full(40, 160, seed=C)
+    g = GrogInterpolator(shape, coords, kernel_width=3, device=device)
+    calib = (rng.standard_normal((C, 16, 16)) + 1j * rng.standard_normal((C, 16, 16))).astype(np.complex64)
+    g.calc_interp_table(calib, lamda=1e-2, quant=16)
+    assert g.plan._compiled.hot[0]["n_hot"] > 0
+    n = coords.shape[0] * coords.shape[1]
+    y = (rng.standard_normal((C, n)) + 1j * rng.standard_normal((C, n))).astype(np.complex64)
+    yy = torch.from_numpy(y).to(device).reshape(1, 1, C, -1).contiguous()
+    g._tuned = {"bf16x3": (256, None)}
+    cache = {}
+    base = g.grid_device(yy, "bf16x3", program_cache=cache).clone()
+    (vbuf, yt, _), = cache.values()
+    vf = vbuf.view(torch.float32).reshape(-1)
+    moved = torch.empty(vf.numel() + 4, dtype=torch.float32, device=device)
+    assert moved.data_ptr() % 16 == 0
+    moved[2:2 + vf.numel()] = vf
+
+    class _At8:
+        def data_ptr(self):
+            return moved.data_ptr() + 8
+
+    out = torch.full_like(base, complex(float("nan"), 0.0))
+    comp = g.plan._compiled
+    prog = GridProgram(g._frames(yt, out, 1, _At8()), comp.C, comp.G, comp.device, 256, None)
+    prog.launch("pg_grid_reduce_v")
+    torch.cuda.synchronize()
+    assert torch.equal(out, base)
+
+
+def _s24_expected(x: np.ndarray) -> np.ndarray:
+    """float32 -> the 24-bit layout's value as float32: nearest-even multiple
+    of 2^(e - 15) (e: the binade exponent, -126 for subnormals), computed in
+    float64; NaN stays NaN."""
+    x64 = x.astype(np.float64)
+    fin = np.isfinite(x64) & (x64 != 0)
+    e = np.full(x64.shape, -126.0)
+    e[fin] = np.maximum(np.floor(np.log2(np.abs(x64[fin]))), -126.0)
+    q = np.exp2(e - 15.0)
+    out = np.where(fin, np.rint(x64 / q) * q, x64)
+    with np.errstate(over="ignore"):
+        return out.astype(np.float32)
+
+
+def _s24_decode(buf: np.ndarray, N: int, C: int, rb: int) -> np.ndarray:
+    b = buf.reshape(N, rb)[:, :6 * C].reshape(N, 2 * C, 3).astype(np.uint32)
+    return ((b[..., 0] << 8) | (b[..., 1] << 16) | (b[..., 2] << 24)).view(np.float32)
+
+
+@pytest.mark.parametrize("C,N", [(32, 4096), (16, 1000), (3, 77), (40, 130)])
+def test_transpose_s24_rows(device, C, N):
+    """pg_transpose_cn_s24 (full 64 x 32 tiles and ragged ones): every value
+    of the (N, C) sample-major rows is the fp32 input rounded to nearest even
+    at 16 significant bits, stored little-endian in 3 bytes, rows of
+    pg_s24_row_bytes(C) bytes -- zeros, infinities, NaN, subnormals, ties and
+    a round-up into infinity included; pg_transpose_frames_s24 writes each
+    frame's rows at row_bytes x its sample offset."""
+    from pygrog_amd import _lib
+
+    lib = _lib.load()
+    rb = int(lib.pg_s24_row_bytes(C))
+    assert rb == 16 * ((6 * C + 15) // 16)
+    rng = np.random.default_rng(C * 7 + N)
+    re = (rng.standard_normal((C, N)) * 10.0 ** rng.integers(-30, 30, (C, N))).astype(np.float32)
+    im = rng.standard_normal((C, N)).astype(np.float32)
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1e-40, -3e-39, 3.4028235e38,
+                        np.float32(1 + 2 ** -16), np.float32(1 + 3 * 2 ** -16), np.float32(1 + 2 ** -16 + 2 ** -23),
+                        np.float32(2 - 2 ** -17)], dtype=np.float32)
+    re.reshape(-1)[:special.size] = special
+    im.reshape(-1)[-special.size:] = special
+    y = np.empty((C, N), dtype=np.complex64)  # parts set apart: 1j * inf would leak NaN into the real part
+    y.real, y.imag = re, im
+    yd = torch.from_numpy(y).to(device)
+    ys = torch.zeros(N * rb, dtype=torch.uint8, device=device)
+    _lib.call("pg_transpose_cn_s24", _lib.ptr(yd), C, N, _lib.ptr(ys), _lib.stream(device))
+    got = _s24_decode(ys.cpu().numpy(), N, C, rb)
+    want = _s24_expected(np.stack([re.T, im.T], axis=-1).reshape(N, 2 * C))
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(got[~nan].view(np.uint32), want[~nan].view(np.uint32))
+    # two frames back to back
+    n0 = N // 3
+    off = torch.tensor([0, n0, N], dtype=torch.int64, device=device)
+    yf = torch.from_numpy(np.concatenate([y[:, :n0].reshape(-1), y[:, n0:].reshape(-1)])).to(device)
+    ysf = torch.zeros(N * rb, dtype=torch.uint8, device=device)
+    _lib.call("pg_transpose_frames_s24", _lib.ptr(yf), _lib.ptr(off), 2, N - n0, C, _lib.ptr(ysf), _lib.stream(device))
+    gf = _s24_decode(ysf.cpu().numpy(), N, C, rb)
+    assert np.array_equal(np.isnan(gf), nan)
+    assert np.array_equal(gf[~nan].view(np.uint32), want[~nan].view(np.uint32))
+
+
+@pytest.mark.parametrize("C", [20, 32])
+def test_grouped_s24_same_bits_on_16bit_inputs(device, monkeypatch, C):
+    """For k-space values that already have 16 significant bits the 24-bit
+    sample rows hold them exactly and the split x = hi + lo is exact on both
+    paths, so the BF16 modes grid the same bits from 24-bit rows as from
+    complex64 rows."""
+    import pygrog_amd.calib._grog as grog_mod
+    from pygrog_amd.calib import GrogInterpolator
+
+    rng = np.random.default_rng(500 + C)
+    shape = (64, 64)
+    coords = _radial_full(48, 128, seed=C)
+    g = GrogInterpolator(shape, coords, kernel_width=3, device=device)
+    calib = (rng.standard_normal((C, 16, 16)) + 1j * rng.standard_normal((C, 16, 16))).astype(np.complex64)
+    g.calc_interp_table(calib, lamda=1e-2, quant=16)
+    n = coords.shape[0] * coords.shape[1]
+    y = (rng.standard_normal((C, n)) + 1j * rng.standard_normal((C, n))).astype(np.complex64)
+    y16 = (_s24_expected(y.real) + 1j * _s24_expected(y.imag)).astype(np.complex64)
+    yy = torch.from_numpy(y16).to(device).reshape(1, 1, C, -1).contiguous()
+    assert grog_mod.uses_s24("bf16x3", C)
+    for mode in ("bf16x3", "bf16"):
+        a = g.grid_device(yy, mode, program_cache={}).clone()
+        with monkeypatch.context() as m:
+            m.setattr(grog_mod, "S24_MAX_COILS", 0)
+            b = g.grid_device(yy, mode, program_cache={})
+        assert torch.equal(a, b), mode
