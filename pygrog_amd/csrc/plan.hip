@@ -65,6 +65,12 @@ __global__ void k_plan_targets(const float *__restrict__ coords,
   const int64_t total = S * nper;
   const int64_t G = g.gs[0] * (g.ndim > 1 ? g.gs[1] : 1) *
                     (g.ndim > 2 ? g.gs[2] : 1);
+  const int lane = threadIdx.x & (kWave - 1);
+  // invalid replicas of the element the wave is in, added to n_invalid when
+  // the wave moves on to another element and at the end (one atomic per
+  // wave and element, not per iteration: same-address atomics serialise)
+  unsigned long long run = 0;
+  int64_t run_s = -1;
   for (int64_t gi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
        gi - threadIdx.x < total; gi += (int64_t)gridDim.x * blockDim.x) {
     const bool active = gi < total;
@@ -106,14 +112,20 @@ __global__ void k_plan_targets(const float *__restrict__ coords,
                      (valid ? (uint64_t)(flat + 1) : 0ull);
       sort_val[gi] = (uint32_t)gi;
     }
-    // Invalid count per element: one atomic per (wave, element) run.
+    // lanes in lane 0's element count into the wave's run; the others (a
+    // wave straddling two elements) add directly
     const bool inv = active && !valid;
     const int64_t s0 = __shfl(s, 0, kWave);
     const unsigned long long same = __ballot(inv && s == s0);
-    const int lane = threadIdx.x & (kWave - 1);
-    if (lane == 0 && same) atomicAdd(n_invalid + s0, (unsigned long long)__popcll(same));
+    if (s0 != run_s) {
+      if (lane == 0 && run) atomicAdd(n_invalid + run_s, run);
+      run = 0;
+      run_s = s0;
+    }
+    run += (unsigned long long)__popcll(same);
     if (inv && s != s0) atomicAdd(n_invalid + s, 1ull);
   }
+  if (lane == 0 && run) atomicAdd(n_invalid + run_s, run);
 }
 
 __global__ void k_plan_post(const uint64_t *__restrict__ keys,
@@ -417,7 +429,9 @@ int pg_plan_targets(const float *coords, const float *cmax, int64_t S,
   for (int k = 0; k < 128; ++k)
     for (int d = 0; d < 3; ++d) g.off[k][d] = geom.offsets[k][d];
   const int threads = 256;
-  hipLaunchKernelGGL(k_plan_targets, dim3(grid_for(total, threads)),
+  // a persistent grid of at most 2048 workgroups (8 per CU): each wavefront
+  // walks many replicas and adds its invalid count once
+  hipLaunchKernelGGL(k_plan_targets, dim3(grid_for(total, threads, 2048)),
                      dim3(threads), 0, as_stream(stream), coords, cmax, S,
                      npts, g, target_idx, distances, gauss, sort_key, sort_val,
                      reinterpret_cast<unsigned long long *>(n_invalid));
