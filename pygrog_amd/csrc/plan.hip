@@ -128,6 +128,28 @@ __global__ void k_plan_targets(const float *__restrict__ coords,
   if (lane == 0 && run) atomicAdd(n_invalid + run_s, run);
 }
 
+// cell_ptr[lo..hi] = v for every lane's range, by the whole wavefront: a
+// lane fills a range of up to kShortGap cells itself; longer ranges (empty
+// stretches of the grid -- the corners of a 3D grid, everything past the
+// last replica -- can be most of G) are filled one after another by all 64
+// lanes with consecutive stores.  Every lane of the wavefront must call it.
+constexpr int64_t kShortGap = 8;
+__device__ __forceinline__ void wave_fill(uint32_t *cp, int64_t lo, int64_t hi, uint32_t v) {
+  const bool lng = hi - lo + 1 > kShortGap;
+  if (!lng)
+    for (int64_t c = lo; c <= hi; ++c) cp[c] = v;
+  unsigned long long m = __ballot(lng);
+  const int lane = threadIdx.x & (kWave - 1);
+  while (m) {
+    const int src = __ffsll((long long)m) - 1;
+    m &= m - 1;
+    uint32_t *p = reinterpret_cast<uint32_t *>(__shfl((long long)(uintptr_t)cp, src));
+    const int64_t l = __shfl(lo, src), h = __shfl(hi, src);
+    const uint32_t w = __shfl(v, src);
+    for (int64_t c = l + lane; c <= h; c += kWave) p[c] = w;
+  }
+}
+
 __global__ void k_plan_post(const uint64_t *__restrict__ keys,
                             const uint32_t *__restrict__ vals,
                             const float *__restrict__ gauss,
@@ -139,35 +161,51 @@ __global__ void k_plan_post(const uint64_t *__restrict__ keys,
                             float *__restrict__ gauss_sorted,
                             uint32_t *__restrict__ cell_ptr) {
   const int64_t total = S * nper;
-  for (int64_t gi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; gi < total;
-       gi += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t key = keys[gi];
-    const int64_t s = gi / nper;
-    const int64_t iloc = gi - s * nper;
-    const int64_t kloc = (int64_t)(key - (uint64_t)s * (uint64_t)(G + 1));
-    const uint32_t val = vals[gi];
-    const int64_t vloc = (int64_t)val - s * nper;
-    indices[gi] = kloc > 0 ? kloc - 1 : 0;
-    sort_perm[gi] = vloc;
-    inv_perm[s * nper + vloc] = iloc;
-    gauss_sorted[gi] = gauss[val];
-    uint32_t *cp = cell_ptr + s * (G + 1);
-    const int64_t ninv = n_invalid[s];
-    if (iloc >= ninv) {
-      const int64_t r = iloc - ninv;
-      const int64_t cell = kloc - 1;
-      int64_t prev = -1;
-      if (r > 0) {
-        const int64_t pk = (int64_t)(keys[gi - 1] - (uint64_t)s * (uint64_t)(G + 1));
-        prev = pk - 1;
+  // wave-uniform trip count (every lane takes part in wave_fill)
+  for (int64_t gi = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+       gi - (threadIdx.x & (kWave - 1)) < total; gi += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t *cp = cell_ptr;
+    int64_t lo = 0, hi = -1, lo2 = 0, hi2 = -1;  // cp[lo..hi] = v, cp[lo2..hi2] = v2
+    uint32_t v = 0, v2 = 0;
+    if (gi < total) {
+      const uint64_t key = keys[gi];
+      const int64_t s = gi / nper;
+      const int64_t iloc = gi - s * nper;
+      const int64_t kloc = (int64_t)(key - (uint64_t)s * (uint64_t)(G + 1));
+      const uint32_t val = vals[gi];
+      const int64_t vloc = (int64_t)val - s * nper;
+      indices[gi] = kloc > 0 ? kloc - 1 : 0;
+      sort_perm[gi] = vloc;
+      inv_perm[s * nper + vloc] = iloc;
+      gauss_sorted[gi] = gauss[val];
+      cp = cell_ptr + s * (G + 1);
+      const int64_t ninv = n_invalid[s];
+      if (iloc >= ninv) {
+        // cells after the previous replica's cell up to this one start here
+        const int64_t r = iloc - ninv;
+        const int64_t cell = kloc - 1;
+        int64_t prev = -1;
+        if (r > 0) {
+          const int64_t pk = (int64_t)(keys[gi - 1] - (uint64_t)s * (uint64_t)(G + 1));
+          prev = pk - 1;
+        }
+        lo = prev + 1;
+        hi = cell;
+        v = (uint32_t)r;
+        if (iloc == nper - 1) {
+          lo2 = cell + 1;
+          hi2 = G;
+          v2 = (uint32_t)(r + 1);
+        }
+      } else if (iloc == nper - 1) {
+        // no valid replica in this element
+        lo = 0;
+        hi = G;
+        v = 0u;
       }
-      for (int64_t c = prev + 1; c <= cell; ++c) cp[c] = (uint32_t)r;
-      if (iloc == nper - 1)
-        for (int64_t c = cell + 1; c <= G; ++c) cp[c] = (uint32_t)(r + 1);
-    } else if (iloc == nper - 1) {
-      // No valid replica in this element.
-      for (int64_t c = 0; c <= G; ++c) cp[c] = 0u;
     }
+    wave_fill(cp, lo, hi, v);
+    wave_fill(cp, lo2, hi2, v2);
   }
 }
 
@@ -195,8 +233,10 @@ __global__ __launch_bounds__(kPwThreads) void k_plan_weights(
     int64_t rep_stride) {
   __shared__ int64_t s_big[kPwThreads];
   __shared__ int s_nbig;
-  __shared__ float s_val[kPwChunk];
+  __shared__ __align__(16) float s_val[kPwChunk];
   __shared__ float s_denom;
+  __shared__ double s_rd[kPwThreads];
+  __shared__ int s_ri[kPwThreads];
   const int tid = threadIdx.x;
   const int64_t total = S * G;
   constexpr int KB = 16;
@@ -265,9 +305,38 @@ __global__ __launch_bounds__(kPwThreads) void k_plan_weights(
       const int64_t r0 = cp[c], r1 = cp[c + 1];
       const int64_t base = s * nper + n_invalid[s];
       const float *gs = gauss_sorted + base;
-      // ordered float64 sum of the cell's values, chunk by chunk
-      double gsum = 0.0;
-      for (int64_t cb = r0; cb < r1; cb += kPwChunk) {
+      // The float64 sum of the cell's values in replica order.  Every value
+      // is a non-negative float32, an integer multiple of 2^L (L: the
+      // smallest exponent of a least significant bit among them); while a
+      // sum stays below 2^(53+L) every partial sum, in any order, is exact.
+      // So the sum is first formed in parallel; if it is below that bound it
+      // is the ordered sum's exact value (a rounded partial sum would have
+      // been at or above the bound, and so the total too), otherwise the
+      // ordered sum is formed one value after another as before.
+      double part = 0.0;
+      int lsb = INT_MAX;
+      for (int64_t r = r0 + tid; r < r1; r += kPwThreads) {
+        const float g = gs[r];
+        part += (double)g;
+        const uint32_t e = (__float_as_uint(g) >> 23) & 0xffu;
+        if (g > 0.0f) lsb = min(lsb, (int)(e == 0u ? 1u : e) - 150);
+      }
+      s_rd[tid] = part;
+      s_ri[tid] = lsb;
+      __syncthreads();
+      for (int h = kPwThreads / 2; h > 0; h >>= 1) {
+        if (tid < h) {
+          s_rd[tid] += s_rd[tid + h];
+          s_ri[tid] = min(s_ri[tid], s_ri[tid + h]);
+        }
+        __syncthreads();
+      }
+      const double psum = s_rd[0];
+      const int plsb = s_ri[0];
+      const bool exact = plsb == INT_MAX || psum < ldexp(1.0, 53 + plsb);
+      __syncthreads();
+      double gsum = exact ? psum : 0.0;
+      for (int64_t cb = r0; !exact && cb < r1; cb += kPwChunk) {
         const int n = (int)(r1 - cb < kPwChunk ? r1 - cb : kPwChunk);
         for (int k = tid; k < n; k += kPwThreads) s_val[k] = gs[cb + k];
         __syncthreads();
@@ -303,14 +372,21 @@ __global__ __launch_bounds__(kPwThreads) void k_plan_weights(
         }
         __syncthreads();
         if (tid == 0) {
-          for (int k0 = 0; k0 < n; k0 += KB) {
-            float q[KB];
+          // the ordered float32 sum: 64 values per round of 16-byte reads
+          int k0 = 0;
+          for (; k0 + 64 <= n; k0 += 64) {
+            float4 q[16];
 #pragma unroll
-            for (int k = 0; k < KB; ++k) q[k] = (k0 + k < n) ? s_val[k0 + k] : 0.0f;
+            for (int k = 0; k < 16; ++k) q[k] = reinterpret_cast<const float4 *>(s_val + k0)[k];
 #pragma unroll
-            for (int k = 0; k < KB; ++k)
-              if (k0 + k < n) dens = dens + q[k];
+            for (int k = 0; k < 16; ++k) {
+              dens = dens + q[k].x;
+              dens = dens + q[k].y;
+              dens = dens + q[k].z;
+              dens = dens + q[k].w;
+            }
           }
+          for (; k0 < n; ++k0) dens = dens + s_val[k0];
         }
         __syncthreads();
       }
