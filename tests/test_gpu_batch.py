@@ -16,28 +16,33 @@ pytestmark = pytest.mark.gpu
 SHAPE, C, NREAD = (48, 48), 8, 96
 
 
-def _batch(device, spokes=(5, 13, 2, 9, 1, 7), seed=0):
+def _batch(device, spokes=(5, 13, 2, 9, 1, 7), seed=0, coils=C):
     from pygrog_amd import synth
     from pygrog_amd.calib import GrogFrameBatch
 
     rng = np.random.default_rng(seed)
     coords = [synth.radial_2d(n, NREAD, offset=float(rng.uniform(0, np.pi))) for n in spokes]
     b = GrogFrameBatch(SHAPE, coords, kernel_width=3, device=device)
-    calib = synth.calibration(SHAPE, C, (16, 16), seed=seed + 1, device=device)
+    calib = synth.calibration(SHAPE, coils, (16, 16), seed=seed + 1, device=device)
     b.calc_interp_table(calib, lamda=1e-3, quant=16)
-    ys = [synth.complex_gaussian((C, n * NREAD), seed=seed + 10 + f, device=device) for f, n in enumerate(spokes)]
+    ys = [synth.complex_gaussian((coils, n * NREAD), seed=seed + 10 + f, device=device) for f, n in enumerate(spokes)]
     return b, coords, ys
 
 
+@pytest.mark.parametrize("coils", [C, 20])
 @pytest.mark.parametrize("mode", ["fp32", "bf16", "bf16x3"])
-def test_batch_frames_equal_their_own_interpolator(device, mode):
-    b, coords, ys = _batch(device)
+def test_batch_frames_equal_their_own_interpolator(device, mode, coils):
+    """(20 coils: the BF16 modes go through 24-bit sample rows.)"""
+    from pygrog_amd.calib._grog import uses_s24
+
+    assert uses_s24(mode, coils) == (mode != "fp32" and coils == 20)
+    b, coords, ys = _batch(device, coils=coils)
     y = b.pack(ys)
     out = b.grid(y, mode)
     torch.cuda.synchronize()
     for f, fr in enumerate(b.frames):
-        alone = fr.grid_device(ys[f].reshape(1, 1, C, -1).contiguous(), mode)
-        assert torch.equal(out[f].reshape(C, -1), alone.reshape(C, -1)), f"frame {f} differs from its own grid"
+        alone = fr.grid_device(ys[f].reshape(1, 1, coils, -1).contiguous(), mode)
+        assert torch.equal(out[f].reshape(coils, -1), alone.reshape(coils, -1)), f"frame {f} differs from its own grid"
 
 
 def test_batch_fp32_vs_oracle(device):
@@ -51,9 +56,9 @@ def test_batch_fp32_vs_oracle(device):
         np.testing.assert_allclose(b.density(f).reshape(-1).cpu().numpy(), dens, rtol=1e-6, atol=1e-7)
 
 
-@pytest.mark.parametrize("mode", ["fp32", "bf16x3"])
-def test_batch_graph_replays_match_eager(device, mode):
-    b, _, ys = _batch(device, seed=5)
+@pytest.mark.parametrize("mode,coils", [("fp32", C), ("bf16x3", C), ("bf16x3", 20)])
+def test_batch_graph_replays_match_eager(device, mode, coils):
+    b, _, ys = _batch(device, seed=5, coils=coils)
     y = b.pack(ys)
     eager = b.grid(y, mode).clone()
     graph, out = b.capture(y, mode)
