@@ -172,6 +172,10 @@ def _grappa_op_3d(calib, lamda, host_steps):
     Tz = calib[1:, :, :, :].reshape(-1, nc)
     Ty = calib[:, 1:, :, :].reshape(-1, nc)
     Tx = calib[:, :, 1:, :].reshape(-1, nc)
+    if Sz.shape == Sy.shape == Sx.shape:
+        # a cubic region: the three systems in one batched solve
+        G = lstsq(torch.stack([Sz, Sy, Sx]), torch.stack([Tz, Ty, Tx]), lamda, host_steps).mT
+        return G[0], G[1], G[2]
     Gz = lstsq(Sz, Tz, lamda, host_steps).mT
     Gy = lstsq(Sy, Ty, lamda, host_steps).mT
     Gx = lstsq(Sx, Tx, lamda, host_steps).mT
