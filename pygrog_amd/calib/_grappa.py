@@ -56,12 +56,34 @@ def _to_dev(x, device, dtype):
     return t.to(device=device, dtype=dtype)
 
 
-def lstsq(A, b, lamda: float = 0.0, host_steps: list | None = None):
-    """Tikhonov least squares via QR of ``[A; sqrt(lamda) I] X = [b; 0]``
-    (reference _grappa.py:16-59).  Batched over leading dims."""
+# The Cartesian axis fits go through the normal equations (A^H A + lamda I)
+# X = A^H b when lamda is at least this: the calibration is scaled to unit
+# norm, so the system's condition number is at most about (1 + lamda) /
+# lamda and FP64 keeps the operators to far better than the complex64 table
+# needs (1e-15 relative to QR at lamda = 1e-3).  Below it, and for the
+# radial self-calibration (whose per-spoke operators go through a matrix
+# logarithm that amplifies any error), QR.
+NORMAL_EQ_MIN_LAMDA = 1e-8
+
+
+def lstsq(A, b, lamda: float = 0.0, host_steps: list | None = None, normal: bool = False):
+    """Tikhonov least squares ``min |A X - b|^2 + lamda |X|^2`` (reference
+    _grappa.py:16-59, which solves ``[A; sqrt(lamda) I] X = [b; 0]`` by QR).
+    Batched over leading dims.  With ``normal`` and ``lamda >=
+    NORMAL_EQ_MIN_LAMDA`` -- for a unit-norm A, whose system is then well
+    conditioned -- the Gram products are formed on A's device (two GEMMs)
+    and the C x C system is solved on the host: on a device the QR of a few
+    thousand rows by C columns costs milliseconds of solver launches
+    (measured on MI355X: 1.5 ms for a 16^3 x 16-coil region's three axes).
+    Otherwise QR, as the reference."""
     A_t = torch.as_tensor(A)
     b_t = torch.as_tensor(b)
     N = A_t.shape[-1]
+    if normal and lamda >= NORMAL_EQ_MIN_LAMDA:
+        Ah = A_t.mH
+        M = Ah @ A_t + lamda * torch.eye(N, dtype=A_t.dtype, device=A_t.device)
+        R = Ah @ b_t
+        return torch.linalg.solve(M.cpu(), R.cpu()).to(A_t.device)
     if lamda > 0:
         I_reg = math.sqrt(lamda) * torch.eye(N, dtype=A_t.dtype, device=A_t.device)
         if A_t.dim() > 2:
@@ -158,8 +180,8 @@ def _grappa_op_2d(calib, lamda, host_steps):
     Sx = calib[:-1, ...].reshape(-1, nc)
     Ty = calib[:, 1:, :].reshape(-1, nc)
     Tx = calib[1:, ...].reshape(-1, nc)
-    Gy = lstsq(Sy, Ty, lamda, host_steps).mT
-    Gx = lstsq(Sx, Tx, lamda, host_steps).mT
+    Gy = lstsq(Sy, Ty, lamda, host_steps, normal=True).mT
+    Gx = lstsq(Sx, Tx, lamda, host_steps, normal=True).mT
     return Gy, Gx
 
 
@@ -174,11 +196,11 @@ def _grappa_op_3d(calib, lamda, host_steps):
     Tx = calib[:, :, 1:, :].reshape(-1, nc)
     if Sz.shape == Sy.shape == Sx.shape:
         # a cubic region: the three systems in one batched solve
-        G = lstsq(torch.stack([Sz, Sy, Sx]), torch.stack([Tz, Ty, Tx]), lamda, host_steps).mT
+        G = lstsq(torch.stack([Sz, Sy, Sx]), torch.stack([Tz, Ty, Tx]), lamda, host_steps, normal=True).mT
         return G[0], G[1], G[2]
-    Gz = lstsq(Sz, Tz, lamda, host_steps).mT
-    Gy = lstsq(Sy, Ty, lamda, host_steps).mT
-    Gx = lstsq(Sx, Tx, lamda, host_steps).mT
+    Gz = lstsq(Sz, Tz, lamda, host_steps, normal=True).mT
+    Gy = lstsq(Sy, Ty, lamda, host_steps, normal=True).mT
+    Gx = lstsq(Sx, Tx, lamda, host_steps, normal=True).mT
     return Gz, Gy, Gx
 
 

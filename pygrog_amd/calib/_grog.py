@@ -217,7 +217,7 @@ def compile_units(cell_ptr: torch.Tensor, S: int, G: int, device, chunk: int = U
 HOT_MIN_UNITS = 2
 
 
-def hot_lists(cell_unit: torch.Tensor, S: int, G: int, device, min_units: int | None = None):
+def hot_lists(cell_unit: torch.Tensor, S: int, G: int, device, min_units: int | None = None, U: int | None = None):
     """Hot-cell work lists per stack element.
 
     Returns (unit_hot (U) int32: element-local hot-cell index of each unit or
@@ -227,7 +227,8 @@ def hot_lists(cell_unit: torch.Tensor, S: int, G: int, device, min_units: int | 
     if min_units is None:
         min_units = HOT_MIN_UNITS
     cu = cell_unit.to(torch.int64)
-    U = int(cu[-1].item())
+    if U is None:
+        U = int(cu[-1].item())
     cnt = cu[1:] - cu[:-1]
     unit_hot = torch.full((max(U, 1),), -1, dtype=torch.int32, device=device)
     per = []
@@ -241,7 +242,7 @@ def hot_lists(cell_unit: torch.Tensor, S: int, G: int, device, min_units: int | 
             ptr[1:] = torch.cumsum(lens, 0)
         HU = int(ptr[-1].item()) if H else 0
         if HU:
-            owner = torch.repeat_interleave(torch.arange(H, device=device), lens)
+            owner = torch.repeat_interleave(torch.arange(H, device=device), lens, output_size=HU)
             units = cu[hs][owner] + (torch.arange(HU, device=device) - ptr[:-1][owner])
             unit_hot[units] = owner.to(torch.int32)
             units = units.to(torch.int32)
@@ -569,7 +570,7 @@ def _create_plan(shape, coords, oversamp, kernel_width, kernel_shape, time_map, 
     del vals_s, gauss_sorted, gauss
     cell_unit, unit_cell, unit_rep = compile_units(cell_ptr, S, G, dev, UNIT_CHUNK, unit_count)
     min_units = HOT_MIN_UNITS
-    unit_hot, hot = hot_lists(cell_unit, S, G, dev, min_units)
+    unit_hot, hot = hot_lists(cell_unit, S, G, dev, min_units, U=int(unit_rep.numel()) - S)
     cell_vrow = ordinary_rows(cell_ptr, cell_unit, S, G, min_units)
     n_inv = n_invalid.cpu().tolist()
 
