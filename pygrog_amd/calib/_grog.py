@@ -332,7 +332,7 @@ def group_by_entry(rep_ent: torch.Tensor, R: int, K: int, device, chunk: int, re
     chunks = (n_e + chunk - 1) // chunk
     total = int(chunks.sum().item())
     if total:
-        ent = torch.repeat_interleave(torch.arange(K, device=device), chunks)
+        ent = torch.repeat_interleave(torch.arange(K, device=device), chunks, output_size=total)
         first = torch.cumsum(chunks, 0) - chunks
         off = (torch.arange(total, device=device) - first[ent]) * chunk
         items = torch.stack([ent, off], 1).to(torch.int32).contiguous()
@@ -755,11 +755,14 @@ class GrogInterpolator:
         (sample-major (N, C) rows for the fp32 mode) back to back."""
         comp = self.plan._compiled
         S, C, N, G = comp.S, comp.C, comp.npts, comp.G
+        # bytes of one frame's sources: complex64 rows, or the 24-bit rows
+        # of the BF16 modes (which only the reduction's frame table carries)
+        y_frame = (y.numel() * y.element_size()) // max(B * S, 1)
         frames = []
         for b in range(B):
             for s in range(S):
                 f = _lib.Frame()
-                f.y = y.data_ptr() + ((b * S + s) * C * N) * 8
+                f.y = y.data_ptr() + (b * S + s) * y_frame
                 f.grid = out.data_ptr() + ((b * S + s) * C * G) * 8
                 f.table = _lib.ptr(comp.table_packed) if comp.table_packed is not None else None
                 stride = max(comp.nper, 1) * 4
