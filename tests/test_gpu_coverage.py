@@ -350,3 +350,30 @@ def test_grouped_s24_same_bits_on_16bit_inputs(device, monkeypatch, C):
             m.setattr(grog_mod, "S24_MAX_COILS", 0)
             b = g.grid_device(yy, mode, program_cache={})
         assert torch.equal(a, b), mode
+
+
+@pytest.mark.parametrize("C", [8, 20])
+def test_grid_device_batch_of_stack_bf16(device, C):
+    """grid_device over a batch of 2 of a stacked plan of 3 elements, BF16
+    modes (20 coils: 24-bit sample rows, 8: complex64 rows): every (batch,
+    element) grid equals that element gridded alone, bit for bit."""
+    from pygrog_amd.calib import GrogInterpolator
+
+    rng = np.random.default_rng(700 + C)
+    shape = (40, 40)
+    coords = np.stack([_radial_full(6, 80, seed=10 * C + i) for i in range(3)])
+    calib = (rng.standard_normal((C, 16, 16)) + 1j * rng.standard_normal((C, 16, 16))).astype(np.complex64)
+    g = GrogInterpolator(shape, coords, kernel_width=3, device=device)
+    assert g.plan.stack_shape == (3,)
+    g.calc_interp_table(calib, lamda=1e-2, quant=16)
+    n = coords.shape[1] * coords.shape[2]
+    y = (rng.standard_normal((2, 3, C, n)) + 1j * rng.standard_normal((2, 3, C, n))).astype(np.complex64)
+    yy = torch.from_numpy(y).to(device).contiguous()
+    for mode in ("bf16x3", "bf16"):
+        got = g.grid_device(yy, mode, program_cache={})
+        for s in range(3):
+            gs = GrogInterpolator(shape, coords[s], kernel_width=3, device=device)
+            gs.calc_interp_table(calib, lamda=1e-2, quant=16)
+            for b in range(2):
+                alone = gs.grid_device(yy[b, s].reshape(1, 1, C, -1).contiguous(), mode)
+                assert torch.equal(got[b, s], alone[0, 0]), (mode, b, s)
