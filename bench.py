@@ -609,8 +609,9 @@ def main(argv=None) -> int:
         return 0
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus N>1 is not supported (see DESIGN.md, "
-                                   "'How the path would shard')", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this benchmark measures one GPU; --gpus N>1 is not supported (the "
+                                   "multi-GPU sharding of the path is designed, not implemented)",
+                          "n_gpus": args.gpus}))
         return 2
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
