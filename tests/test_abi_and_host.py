@@ -72,6 +72,16 @@ def test_library_version_and_errors():
     assert lib.pg_table_packed_elems(3, 5) == 3 * 4 * 5
     assert lib.pg_table_packed_elems(2, 33) == 2 * 32 * 33
     assert lib.pg_table_packed_elems(1, 65) == 0 and lib.pg_table_packed_elems(0, 8) == 0
+    # 24-bit sample rows: 2C values of 3 bytes, padded to whole 16-byte units
+    assert [lib.pg_s24_row_bytes(c) for c in (0, 1, 2, 3, 8, 16, 20, 32, 64)] == [0, 16, 16, 32, 48, 96, 128, 192, 384]
+    assert lib.pg_s24_row_bytes(-1) == -1
+    # the BF16 modes use them where a row touches no more 128-byte lines than
+    # a complex64 row: 32 and 20 coils yes, 16 and 8 coils no, never fp32
+    from pygrog_amd.calib._grog import uses_s24
+
+    assert uses_s24("bf16x3", 32) and uses_s24("bf16", 20)
+    assert not uses_s24("bf16x3", 16) and not uses_s24("bf16x3", 8) and not uses_s24("fp32", 32)
+    assert not uses_s24("bf16x3", 33)
 
 
 def test_kernel_offsets_and_quant_steps():
