@@ -219,6 +219,29 @@ int pg_plan_units(const uint32_t *cell_ptr, const uint32_t *cell_unit,
                   int64_t S, int64_t G, int32_t chunk, uint32_t *unit_cell,
                   uint32_t *unit_rep, pg_stream_t stream);
 
+/* Hot cells and the tile kernels' row numbering from the reduction units.
+ * cell_unit (S*G + 1) is the exclusive scan of units per cell over all
+ * elements.  A cell with nu units is hot when nu >= min_units and special
+ * (no tile-kernel rows) when hot or nu > 1.  pg_plan_hot_counts writes
+ * (S*G + 1 entries each, the last 0): a = replicas of ordinary cells (0
+ * for special ones), hf = 1 for hot cells, hn = units of hot cells.  With
+ * A, HF, HN their exclusive scans (pg_exclusive_scan_u32), pg_plan_hot_rows
+ * writes cell_vrow (S, G + 1) = A less the element's base; per element s
+ * (hb = HF[s*G], ub = HN[s*G], H_s = HF[(s+1)*G] - hb): hot_ptr[hb + s + h]
+ * for h = 0..H_s = first hot unit of the element's h-th hot cell relative
+ * to ub (h = H_s: the element's hot-unit count), hot_units[HN[cell] + k] =
+ * global unit id of the k-th unit of each hot cell, and unit_hot[u] = the
+ * element-local hot index of unit u's cell, -1 when not hot.  hot_ptr
+ * holds HF[S*G] + S entries, hot_units HN[S*G], unit_hot cell_unit[S*G].
+ * No reference counterpart: the pre-pass work lists of the fused gridding. */
+int pg_plan_hot_counts(const uint32_t *cell_ptr, const uint32_t *cell_unit,
+                       int64_t S, int64_t G, int64_t min_units, uint32_t *a,
+                       uint32_t *hf, uint32_t *hn, pg_stream_t stream);
+int pg_plan_hot_rows(const uint32_t *cell_unit, int64_t S, int64_t G,
+                     int64_t min_units, const uint32_t *A, const uint32_t *HF,
+                     const uint32_t *HN, uint32_t *cell_vrow, int32_t *unit_hot,
+                     uint32_t *hot_units, uint32_t *hot_ptr, pg_stream_t stream);
+
 /* Quantised table entry of every candidate replica (reference
  * calc_interp_table, _grog.py:406-422, generalised from 10**precision to an
  * integer quantisation q):  pfac = q, stepsize = f32(step), nsteps = q + 1,

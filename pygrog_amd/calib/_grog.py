@@ -272,6 +272,43 @@ def ordinary_rows(cell_ptr: torch.Tensor, cell_unit: torch.Tensor, S: int, G: in
     return out.to(torch.int32).contiguous()
 
 
+def plan_hot_rows(cell_ptr: torch.Tensor, cell_unit: torch.Tensor, S: int, G: int, device, U: int,
+                  min_units: int | None = None):
+    """``hot_lists`` and ``ordinary_rows`` in two device passes and three
+    scans, with one host read (the per-element hot counts, which size the
+    lists).  Returns (unit_hot, per-element hot dicts, cell_vrow), the same
+    values as those two functions."""
+    if min_units is None:
+        min_units = HOT_MIN_UNITS
+    st = _lib.stream(device)
+    n = S * G
+    a = torch.empty(n + 1, dtype=torch.int32, device=device)
+    hf = torch.empty_like(a)
+    hn = torch.empty_like(a)
+    _lib.call("pg_plan_hot_counts", _u32(cell_ptr), _u32(cell_unit), S, G, int(min_units), _u32(a), _u32(hf),
+              _u32(hn), st)
+    A, HF, HN = (_exclusive_scan(t, device) for t in (a, hf, hn))
+    idx = torch.arange(S + 1, device=device) * G
+    bases = torch.stack([HF[idx], HN[idx]]).to(torch.int64).cpu().tolist()
+    hb, ub = bases
+    cell_vrow = torch.empty(S, G + 1, dtype=torch.int32, device=device)
+    if U > 0:
+        unit_hot = torch.empty(U, dtype=torch.int32, device=device)
+    else:
+        unit_hot = torch.full((1,), -1, dtype=torch.int32, device=device)
+    hot_ptr = torch.empty(hb[S] + S, dtype=torch.int32, device=device)
+    hot_units = torch.empty(max(ub[S], 1), dtype=torch.int32, device=device)
+    _lib.call("pg_plan_hot_rows", _u32(cell_unit), S, G, int(min_units), _u32(A), _u32(HF), _u32(HN), _u32(cell_vrow),
+              _lib.ptr(unit_hot), _u32(hot_units), _u32(hot_ptr), st)
+    per = []
+    for s in range(S):
+        H, HU = hb[s + 1] - hb[s], ub[s + 1] - ub[s]
+        units = hot_units[ub[s]:ub[s] + HU] if HU else torch.zeros(1, dtype=torch.int32, device=device)
+        per.append({"hot_units": units, "hot_ptr": hot_ptr[hb[s] + s:hb[s] + s + H + 1], "n_hot": H,
+                    "n_hot_units": HU})
+    return unit_hot, per, cell_vrow
+
+
 class KernelTimer:
     """Per-kernel HIP-event timing on the launch stream: ``mark(name)``
     brackets one launch; ``durations()`` (after a synchronize) returns the
@@ -570,8 +607,7 @@ def _create_plan(shape, coords, oversamp, kernel_width, kernel_shape, time_map, 
     del vals_s, gauss_sorted, gauss
     cell_unit, unit_cell, unit_rep = compile_units(cell_ptr, S, G, dev, UNIT_CHUNK, unit_count)
     min_units = HOT_MIN_UNITS
-    unit_hot, hot = hot_lists(cell_unit, S, G, dev, min_units, U=int(unit_rep.numel()) - S)
-    cell_vrow = ordinary_rows(cell_ptr, cell_unit, S, G, min_units)
+    unit_hot, hot, cell_vrow = plan_hot_rows(cell_ptr, cell_unit, S, G, dev, int(unit_rep.numel()) - S, min_units)
     n_inv = n_invalid.cpu().tolist()
 
     if time_map is not None:

@@ -416,6 +416,81 @@ __global__ void k_plan_units(const uint32_t *__restrict__ cell_ptr,
   }
 }
 
+// Hot cells and ordinary rows from the units.  A cell with nu units is
+// hot when nu >= min_units (its units are reduced by the pre-pass); it is
+// special (left out of the tile kernels' row numbering) when it is hot or
+// has more than one unit.  Pass 1 writes, per cell t of all S*G cells (and a
+// zero at t = S*G for the exclusive scans that follow): a[t] = the cell's
+// replicas if it is ordinary, else 0; hf[t] = 1 if hot; hn[t] = nu if hot.
+__global__ void k_plan_hot_counts(const uint32_t *__restrict__ cell_ptr,
+                                  const uint32_t *__restrict__ cell_unit,
+                                  int64_t S, int64_t G, int64_t min_units,
+                                  uint32_t *__restrict__ a, uint32_t *__restrict__ hf,
+                                  uint32_t *__restrict__ hn) {
+  const int64_t total = S * G;
+  const int64_t lim = min_units > 1 ? min_units : 1;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (t == total) {
+      a[t] = hf[t] = hn[t] = 0u;
+      continue;
+    }
+    const int64_t s = t / G;
+    const int64_t c = t - s * G;
+    const uint32_t *cp = cell_ptr + s * (G + 1);
+    const int64_t nu = (int64_t)(cell_unit[t + 1] - cell_unit[t]);
+    const bool hot = nu >= min_units;
+    const bool special = nu >= lim || nu > 1;
+    a[t] = special ? 0u : cp[c + 1] - cp[c];
+    hf[t] = hot ? 1u : 0u;
+    hn[t] = hot ? (uint32_t)nu : 0u;
+  }
+}
+
+// Pass 2, from the exclusive scans A, HF, HN of pass 1 (S*G + 1 entries):
+// cell_vrow (S, G+1) = A minus the element's base; for element s with hot
+// cells h = 0..H_s-1 (cell order) the element's hot_ptr is the H_s + 1
+// entries at hot_ptr + HF[s*G] + s (unit offsets from the element's first
+// hot unit), its hot units the HN[(s+1)*G] - HN[s*G] entries at hot_units +
+// HN[s*G] (global unit ids in cell, then unit order); unit_hot[u] = the
+// element-local hot index of unit u's cell, -1 if the cell is not hot.
+__global__ void k_plan_hot_rows(const uint32_t *__restrict__ cell_unit,
+                                int64_t S, int64_t G, int64_t min_units,
+                                const uint32_t *__restrict__ A,
+                                const uint32_t *__restrict__ HF,
+                                const uint32_t *__restrict__ HN,
+                                uint32_t *__restrict__ cell_vrow,
+                                int32_t *__restrict__ unit_hot,
+                                uint32_t *__restrict__ hot_units,
+                                uint32_t *__restrict__ hot_ptr) {
+  const int64_t total = S * G;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = t / G;
+    const int64_t c = t - s * G;
+    const uint32_t a0 = A[s * G], h0 = HF[s * G], u0 = HN[s * G];
+    uint32_t *vr = cell_vrow + s * (G + 1);
+    vr[c] = A[t] - a0;
+    const uint32_t cu = cell_unit[t];
+    const int64_t nu = (int64_t)(cell_unit[t + 1] - cu);
+    if (nu >= min_units) {
+      const uint32_t h = HF[t] - h0;
+      hot_ptr[h0 + s + h] = HN[t] - u0;
+      for (int64_t k = 0; k < nu; ++k) {
+        hot_units[HN[t] + k] = cu + (uint32_t)k;
+        unit_hot[cu + k] = (int32_t)h;
+      }
+    } else {
+      for (int64_t k = 0; k < nu; ++k) unit_hot[cu + k] = -1;
+    }
+    if (c == G - 1) {
+      const int64_t e = (s + 1) * G;
+      vr[G] = A[e] - a0;
+      hot_ptr[h0 + s + (HF[e] - h0)] = HN[e] - u0;
+    }
+  }
+}
+
 __global__ void k_plan_quantize(const float *__restrict__ distances,
                                 const int64_t *__restrict__ inv_perm,
                                 const int64_t *__restrict__ n_invalid,
@@ -597,6 +672,32 @@ int pg_plan_units(const uint32_t *cell_ptr, const uint32_t *cell_unit,
   hipLaunchKernelGGL(k_plan_units, dim3(grid_for(total, threads)),
                      dim3(threads), 0, as_stream(stream), cell_ptr, cell_unit,
                      S, G, chunk, unit_cell, unit_rep);
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_plan_hot_counts(const uint32_t *cell_ptr, const uint32_t *cell_unit,
+                       int64_t S, int64_t G, int64_t min_units, uint32_t *a,
+                       uint32_t *hf, uint32_t *hn, pg_stream_t stream) {
+  PG_CHECK(S >= 1 && G >= 1, "plan_hot_counts: bad sizes");
+  const int threads = 256;
+  hipLaunchKernelGGL(k_plan_hot_counts, dim3(grid_for(S * G + 1, threads)),
+                     dim3(threads), 0, as_stream(stream), cell_ptr, cell_unit,
+                     S, G, min_units, a, hf, hn);
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_plan_hot_rows(const uint32_t *cell_unit, int64_t S, int64_t G,
+                     int64_t min_units, const uint32_t *A, const uint32_t *HF,
+                     const uint32_t *HN, uint32_t *cell_vrow, int32_t *unit_hot,
+                     uint32_t *hot_units, uint32_t *hot_ptr, pg_stream_t stream) {
+  PG_CHECK(S >= 1 && G >= 1, "plan_hot_rows: bad sizes");
+  const int threads = 256;
+  hipLaunchKernelGGL(k_plan_hot_rows, dim3(grid_for(S * G, threads)),
+                     dim3(threads), 0, as_stream(stream), cell_unit, S, G,
+                     min_units, A, HF, HN, cell_vrow, unit_hot, hot_units,
+                     hot_ptr);
   PG_LAUNCHED();
   return 0;
 }

@@ -377,3 +377,41 @@ def test_grid_device_batch_of_stack_bf16(device, C):
             for b in range(2):
                 alone = gs.grid_device(yy[b, s].reshape(1, 1, C, -1).contiguous(), mode)
                 assert torch.equal(got[b, s], alone[0, 0]), (mode, b, s)
+
+
+@pytest.mark.parametrize("case", ["2d", "3d", "stack", "nohot", "allhot"])
+def test_plan_hot_rows_match_torch_lists(device, case):
+    """The device hot lists and row numbering (pg_plan_hot_counts /
+    pg_plan_hot_rows) equal the torch formulation (hot_lists,
+    ordinary_rows) entry for entry: 2D with a hot DC cell, a 3D kooshball,
+    a stack of 3 elements, no hot cells, every occupied cell hot."""
+    import pygrog_amd.calib._grog as G
+    from pygrog_amd import synth
+    from pygrog_amd.calib import GrogInterpolator
+
+    if case == "3d":
+        g = GrogInterpolator((24, 24, 24), synth.kooshball_3d(300, 32), oversamp=(1, 1, 1), kernel_width=2,
+                             kernel_shape="sphere", device=device)
+    elif case == "stack":
+        g = GrogInterpolator((40, 40), np.stack([_radial_full(40, 96, seed=s) for s in range(3)]), kernel_width=3,
+                             device=device)
+        assert g.plan._compiled.S == 3
+    else:
+        g = GrogInterpolator((80, 80), _radial_full(40, 160, seed=1), kernel_width=3, device=device)
+    comp = g.plan._compiled
+    S, Gc = comp.S, comp.G
+    U = comp.unit_rep.numel() - S
+    mu = {"nohot": 10**9, "allhot": 1}.get(case, G.HOT_MIN_UNITS)
+    uh, per, vrow = G.plan_hot_rows(comp.cell_ptr, comp.cell_unit, S, Gc, device, U, mu)
+    uh2, per2 = G.hot_lists(comp.cell_unit, S, Gc, device, mu)
+    vrow2 = G.ordinary_rows(comp.cell_ptr, comp.cell_unit, S, Gc, mu)
+    assert torch.equal(vrow, vrow2)
+    assert torch.equal(uh, uh2)
+    total_hot = 0
+    for a, b in zip(per, per2):
+        assert (a["n_hot"], a["n_hot_units"]) == (b["n_hot"], b["n_hot_units"])
+        assert torch.equal(a["hot_ptr"], b["hot_ptr"])
+        if a["n_hot_units"]:
+            assert torch.equal(a["hot_units"], b["hot_units"])
+        total_hot += a["n_hot"]
+    assert (total_hot == 0) == (case == "nohot")
