@@ -271,6 +271,20 @@ class GrogServer:
         makespan = max(j.t_done for j in done) if done else 0.0
         lat = np.array([j.latency for j in done])
         cs = sum(j.coils * j.samples for j in done)
+        # per job kind (2D / 3D): latency and where it goes -- queued until
+        # admitted, preparing until the worker thread has enqueued the plan,
+        # table and groups (their device work may still run), then until the
+        # job's gridding batch completes
+        by_kind = {}
+        for nd in sorted({len(j.shape) for j in done}):
+            js = [j for j in done if len(j.shape) == nd]
+            lk = np.array([j.latency for j in js]) * 1e3
+            by_kind[f"{nd}D"] = {
+                "jobs": len(js), "p50_ms": float(np.percentile(lk, 50)), "p99_ms": float(np.percentile(lk, 99)),
+                "queued_ms": float(np.mean([(j.t_admit - j.arrival) * 1e3 for j in js])),
+                "prepare_ms": float(np.mean([(j.t_ready - j.t_admit) * 1e3 for j in js])),
+                "grid_ms": float(np.mean([(j.t_done - j.t_ready) * 1e3 for j in js])),
+            }
         return {
             "mode": self.mode, "jobs": len(done), "batches": len(batches),
             "mean_batch": float(np.mean(batches)) if batches else 0.0,
@@ -279,4 +293,5 @@ class GrogServer:
             "throughput_jobs_s": len(done) / makespan if makespan > 0 else 0.0,
             "coil_samples_per_s": cs / makespan if makespan > 0 else 0.0,
             "makespan_s": makespan, "hbm_budget_gb": self.hbm_budget / 1e9, "peak_admitted_gb": peak_used / 1e9,
+            "by_kind": by_kind,
         }
