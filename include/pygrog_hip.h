@@ -219,6 +219,20 @@ int pg_plan_units(const uint32_t *cell_ptr, const uint32_t *cell_unit,
                   int64_t S, int64_t G, int32_t chunk, uint32_t *unit_cell,
                   uint32_t *unit_rep, pg_stream_t stream);
 
+/* Normal-equation products of the Cartesian GRAPPA unit-operator fit
+ * (reference calib/_grappa.py:16-59 and 211-287: ridge least squares of
+ * each axis' target samples on its source samples), in FP64: calib is
+ * (C, *dims) complex128 coil-major, dims (host, ndim = 2 or 3 entries, each
+ * >= 2).  out (complex128, 2 ndim C^2 + C entries): for axis d,
+ * out[(2d) C^2 + i C + j] = sum_p conj(x_i[p]) x_j[p] and
+ * out[(2d+1) C^2 + i C + j] = sum_p conj(x_i[p]) x_j[p + e_d] over the
+ * positions p below the last slice along d; out[2 ndim C^2 + i] =
+ * sum_p |x_i[p]|^2 (real) over all positions.  The unit operator of axis d
+ * for regularisation lamda (with the calibration scaled to unit norm, as
+ * the reference does) is then ((G_d + lamda |x|^2 I)^-1 X_d)^T. */
+int pg_grappa_gram_f64(const void *calib, int64_t C, int32_t ndim,
+                       const int64_t *dims, void *out, pg_stream_t stream);
+
 /* Hot cells and the tile kernels' row numbering from the reduction units.
  * cell_unit (S*G + 1) is the exclusive scan of units per cell over all
  * elements.  A cell with nu units is hot when nu >= min_units and special

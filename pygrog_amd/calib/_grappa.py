@@ -40,6 +40,8 @@ from types import SimpleNamespace
 import numpy as np
 import torch
 
+from .. import _lib
+
 
 @dataclass
 class KernelTableResult:
@@ -231,6 +233,27 @@ def _radial_grappa_op(calib, lamda, coords, host_steps):
     return torch.linalg.matrix_exp(lGy), torch.linalg.matrix_exp(lGx)
 
 
+def _gram_fit(calib: torch.Tensor, lamda: float) -> torch.Tensor:
+    """Unit operators of a Cartesian calibration ``(C, *dims)`` complex128
+    on a device, one per axis (axis 0 first), by the normal equations: the
+    FP64 products G_d = A_d^H A_d, X_d = A_d^H T_d and |x|^2 from one kernel
+    (``pg_grappa_gram_f64``), then ((G_d + lamda |x|^2 I)^-1 X_d)^T solved on
+    the host -- the fit of the calibration scaled to unit norm, as the
+    reference does (_grappa.py:211-287), without forming the source and
+    target matrices."""
+    C = int(calib.shape[0])
+    dims = [int(n) for n in calib.shape[1:]]
+    nd = len(dims)
+    x = calib.contiguous()
+    out = torch.empty(2 * nd * C * C + C, dtype=torch.complex128, device=x.device)
+    _lib.call("pg_grappa_gram_f64", _lib.ptr(x), C, nd, _lib.dims(dims), _lib.ptr(out), _lib.stream(x.device))
+    h = out.cpu()
+    n2 = float(h[2 * nd * C * C:].real.sum())
+    M = h[:2 * nd * C * C].reshape(nd, 2, C, C)
+    eye = torch.eye(C, dtype=torch.complex128)
+    return torch.linalg.solve(M[:, 0] + lamda * n2 * eye, M[:, 1]).mT.to(x.device)
+
+
 # Calibration regions up to this many values (coils x ACR points) are fitted
 # on the host even when the table is built on a device (measured on MI355X:
 # 8 coils x 24^2, 0.25 ms on the host against 0.67 ms on the device; 16 coils
@@ -255,6 +278,11 @@ def train_grappa(train_data, lamda: float | None = None, coords=None, *, device=
                            device="cpu", host_steps=host_steps)
         return SimpleNamespace(**{k: (None if v is None else v.to(dev)) for k, v in vars(res).items()})
     calib = _to_dev(train_data, dev, torch.complex128)
+    if dev.type != "cpu" and coords is None and lamda >= NORMAL_EQ_MIN_LAMDA and calib.ndim in (3, 4):
+        ops = _gram_fit(calib, lamda)
+        if calib.ndim == 3:
+            return SimpleNamespace(x=ops[0], y=ops[1], z=None)
+        return SimpleNamespace(x=ops[2], y=ops[1], z=ops[0])
     calib = calib / torch.linalg.norm(calib)
     if coords is not None:
         c = torch.as_tensor(np.asarray(coords) if not isinstance(coords, torch.Tensor) else coords)

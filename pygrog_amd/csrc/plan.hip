@@ -491,6 +491,67 @@ __global__ void k_plan_hot_rows(const uint32_t *__restrict__ cell_unit,
   }
 }
 
+// Normal-equation products of the Cartesian GRAPPA fit, in FP64, from the
+// coil-major calibration x (C, P) complex128 over P = n0 n1 n2 positions
+// (C-order, unused dims 1).  Block (k, m): for m < 2 ndim, axis d = m / 2,
+// output element (i, j) = k of
+//   m even: G_d[i][j] = sum_p conj(x_i[p]) x_j[p]
+//   m odd:  X_d[i][j] = sum_p conj(x_i[p]) x_j[p + stride_d]
+// over the positions p not on the last slice along d (the rows of the
+// reference's source / target matrices, _grappa.py:290-368); m = 2 ndim,
+// k = i < C: sum_p |x_i[p]|^2.  Each thread sums its strided positions,
+// then a fixed tree: the same bits on every run.
+constexpr int kGramThreads = 256;
+__global__ __launch_bounds__(kGramThreads) void k_grappa_gram(
+    const double2 *__restrict__ x, int64_t C, int ndim, int64_t n0, int64_t n1,
+    int64_t n2, double2 *__restrict__ out) {
+  __shared__ double s_re[kGramThreads], s_im[kGramThreads];
+  const int m = blockIdx.y;
+  const int64_t k = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int64_t P = n0 * n1 * n2;
+  double re = 0.0, im = 0.0;
+  if (m == 2 * ndim) {
+    if (k >= C) return;
+    const double2 *xi = x + k * P;
+    for (int64_t p = tid; p < P; p += kGramThreads) {
+      const double2 a = xi[p];
+      re += a.x * a.x + a.y * a.y;
+    }
+  } else {
+    if (k >= C * C) return;
+    const int64_t i = k / C, j = k - (k / C) * C;
+    const int d = m / 2;
+    const bool cross = (m & 1) != 0;
+    const int64_t nd = d == 0 ? n0 : d == 1 ? n1 : n2;
+    const int64_t st = d == 0 ? n1 * n2 : d == 1 ? n2 : 1;  // stride of axis d
+    const double2 *xi = x + i * P;
+    const double2 *xj = x + j * P + (cross ? st : 0);
+    for (int64_t p = tid; p < P; p += kGramThreads) {
+      if ((p / st) % nd == nd - 1) continue;
+      const double2 a = xi[p], b = xj[p];
+      re += a.x * b.x + a.y * b.y;
+      im += a.x * b.y - a.y * b.x;
+    }
+  }
+  s_re[tid] = re;
+  s_im[tid] = im;
+  __syncthreads();
+  for (int h = kGramThreads / 2; h > 0; h >>= 1) {
+    if (tid < h) {
+      s_re[tid] += s_re[tid + h];
+      s_im[tid] += s_im[tid + h];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    double2 r;
+    r.x = s_re[0];
+    r.y = s_im[0];
+    out[m == 2 * ndim ? 2 * ndim * C * C + k : (int64_t)m * C * C + k] = r;
+  }
+}
+
 __global__ void k_plan_quantize(const float *__restrict__ distances,
                                 const int64_t *__restrict__ inv_perm,
                                 const int64_t *__restrict__ n_invalid,
@@ -698,6 +759,23 @@ int pg_plan_hot_rows(const uint32_t *cell_unit, int64_t S, int64_t G,
                      dim3(threads), 0, as_stream(stream), cell_unit, S, G,
                      min_units, A, HF, HN, cell_vrow, unit_hot, hot_units,
                      hot_ptr);
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_grappa_gram_f64(const void *calib, int64_t C, int32_t ndim,
+                       const int64_t *dims, void *out, pg_stream_t stream) {
+  PG_CHECK(C >= 1 && C <= 256 && (ndim == 2 || ndim == 3) && dims != nullptr,
+           "grappa_gram: bad sizes");
+  int64_t n[3] = {1, 1, 1};
+  for (int d = 0; d < ndim; ++d) {
+    PG_CHECK(dims[d] >= 2, "grappa_gram: every calibration axis needs 2 or more points");
+    n[d] = dims[d];
+  }
+  const dim3 grid((unsigned)(C * C), (unsigned)(2 * ndim + 1));
+  hipLaunchKernelGGL(k_grappa_gram, grid, dim3(kGramThreads), 0, as_stream(stream),
+                     static_cast<const double2 *>(calib), C, (int)ndim, n[0], n[1],
+                     n[2], static_cast<double2 *>(out));
   PG_LAUNCHED();
   return 0;
 }

@@ -66,3 +66,28 @@ def test_autograd_wrappers_give_adjoint_gradients(device):
     loss2 = torch.vdot(grog_backproject(yy, op).reshape(-1), z.reshape(-1)).real
     loss2.backward()
     torch.testing.assert_close(yy.grad, op.forward(z), rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("dims,C", [((48, 48), 16), ((16, 16, 16), 16), ((14, 16, 18), 12)])
+def test_cartesian_unit_operators_vs_oracle(device, dims, C):
+    """Cartesian unit operators fitted on the device -- the FP64 normal-
+    equation products from pg_grappa_gram_f64 and a host C x C solve for
+    lamda > 0, QR for lamda = 0 -- against the oracle's LAPACK ridge fit of
+    the unit-norm calibration (gelsd), on regions large enough for the
+    device path, 2D and 3D (cubic and not)."""
+    from pygrog_amd.calib import train_grappa
+
+    rng = np.random.default_rng(sum(dims) + C)
+    calib = rng.standard_normal((C, *dims)) + 1j * rng.standard_normal((C, *dims))
+    # a smooth part so the fit is not pure noise
+    grid = np.meshgrid(*[np.linspace(-1, 1, n) for n in dims], indexing="ij")
+    calib += 3 * np.exp(1j * sum((k + 1) * g for k, g in enumerate(grid)))[None] * rng.standard_normal((C,) + (1,) * len(dims))
+    for lam in (1e-3, 1e-2, 0.0):
+        ops = train_grappa(calib, lam, device=device)
+        ref = otable.unit_operators(calib, lam)
+        for ax in ("x", "y", "z"):
+            if ref[ax] is None:
+                assert ops.__dict__[ax] is None
+                continue
+            got = ops.__dict__[ax].cpu().numpy()
+            assert np.abs(got - ref[ax]).max() <= 1e-9 * np.abs(ref[ax]).max(), (lam, ax)
