@@ -50,6 +50,16 @@ def test_library_version_and_errors():
     # a host-side argument error is reported through pg_last_error (no GPU work)
     rc = lib.pg_grid_fp32(None, None, 0, 0, 1, 128, 0, 0, 0, None)
     assert rc != 0 and b"frame" in lib.pg_last_error()
+    # 24-bit sample rows must start on 16-byte boundaries (checked before any
+    # launch; the pointers are never dereferenced)
+    assert lib.pg_transpose_cn_s24(4096, 32, 100, 4104, None) != 0 and b"16-byte" in lib.pg_last_error()
+    assert lib.pg_transpose_frames_s24(4096, 4096, 1, 100, 32, 4104, None) != 0
+    assert lib.pg_apply_grouped_bf16_s24(4104, 4096, 4096, 4096, 4096, 4096, 4096, 5, 10, 32, 3, 4096, None) != 0
+    assert b"16-byte" in lib.pg_last_error()
+    # the Gram products need 2-3 calibration axes of at least 2 points
+    assert lib.pg_grappa_gram_f64(4096, 8, 3, _lib.dims([8, 1, 8]), 4096, None) != 0
+    assert b"2 or more" in lib.pg_last_error()
+    assert lib.pg_grappa_gram_f64(4096, 8, 1, _lib.dims([8]), 4096, None) != 0
     assert lib.pg_grid_tile_cells(65) == 0
     assert _lib.tile_options(65) == [] and _lib.tile_options(0) == []
     for c, gs in ((3, 4), (8, 8), (16, 16), (32, 32), (64, 64)):
