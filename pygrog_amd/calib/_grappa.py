@@ -254,13 +254,6 @@ def _gram_fit(calib: torch.Tensor, lamda: float) -> torch.Tensor:
     return torch.linalg.solve(M[:, 0] + lamda * n2 * eye, M[:, 1]).mT.to(x.device)
 
 
-# Calibration regions up to this many values (coils x ACR points) are fitted
-# on the host even when the table is built on a device (measured on MI355X:
-# 8 coils x 24^2, 0.25 ms on the host against 0.67 ms on the device; 16 coils
-# x 16^3 is faster on the device, 1.9 against 4.9 ms).
-HOST_FIT_MAX = 1 << 15
-
-
 def train_grappa(train_data, lamda: float | None = None, coords=None, *, device=None, host_steps: list | None = None) -> SimpleNamespace:
     """Unit-shift GROG operators (reference _grappa.py:211-287), FP64.
 
@@ -271,14 +264,9 @@ def train_grappa(train_data, lamda: float | None = None, coords=None, *, device=
     if lamda is None:
         lamda = 0.01 if coords is None else 0.0
     dev = torch.device(device) if device is not None else torch.device("cpu")
-    n = int(np.prod(np.shape(train_data)))
-    if dev.type != "cpu" and coords is None and n <= HOST_FIT_MAX:
-        # a small calibration: the host's QR beats the device's launches
-        res = train_grappa(train_data.cpu() if isinstance(train_data, torch.Tensor) else train_data, lamda,
-                           device="cpu", host_steps=host_steps)
-        return SimpleNamespace(**{k: (None if v is None else v.to(dev)) for k, v in vars(res).items()})
     calib = _to_dev(train_data, dev, torch.complex128)
-    if dev.type != "cpu" and coords is None and lamda >= NORMAL_EQ_MIN_LAMDA and calib.ndim in (3, 4):
+    if (dev.type != "cpu" and coords is None and lamda >= NORMAL_EQ_MIN_LAMDA and calib.ndim in (3, 4)
+            and calib.shape[0] <= 256 and min(calib.shape[1:]) >= 2):
         ops = _gram_fit(calib, lamda)
         if calib.ndim == 3:
             return SimpleNamespace(x=ops[0], y=ops[1], z=None)
