@@ -44,6 +44,10 @@ from ._grappa import build_table, quant_steps
 # of every cell independently of tiling (see grid.hip).
 UNIT_CHUNK = 32
 
+# Largest shift a replica may have along an axis (grid steps); the plan of
+# every interpolator carries it (reference _grog.py:1123-1124).
+PLAN_RADIUS = 0.5
+
 _LAZY_FIELDS = ("target_idx", "distances", "weights", "indices", "sqrt_weights", "sort_perm", "inv_perm", "time_map")
 
 
@@ -628,7 +632,7 @@ def _create_plan(shape, coords, oversamp, kernel_width, kernel_shape, time_map, 
         "inv_perm": inv_perm.reshape(s_shape + (nper,)) if s_shape else inv_perm.reshape(nper),
     }
     plan = GrogPlan(shape=shape, oversamp=oversamp, grid_shape=grid_shape, kernel_width=kernel_width,
-                    kernel_shape=kernel_shape, radius=0.5)
+                    kernel_shape=kernel_shape, radius=PLAN_RADIUS)
     plan._home = home
     plan._dev = dev_fields
     plan._compiled = SimpleNamespace(

@@ -22,26 +22,42 @@ namespace pg {
 
 // cmax[s, d] = max |coords[s, :, d]| via unsigned-bit atomicMax (values are
 // non-negative, so the IEEE bit order is the numeric order).  cmax zeroed by
-// the caller.
-__global__ void k_absmax(const float *__restrict__ coords, int64_t S,
-                         int64_t npts, int ndim, unsigned *__restrict__ cmax) {
-  const int64_t total = npts * ndim;
+// the caller.  Each thread walks whole points; the workgroup folds its
+// wavefronts' maxima in LDS and adds one atomic per axis (per-wavefront
+// atomics on the same three words had serialised at the L2).
+constexpr int kAbsmaxThreads = 256;
+__global__ __launch_bounds__(kAbsmaxThreads) void k_absmax(
+    const float *__restrict__ coords, int64_t S, int64_t npts, int ndim,
+    unsigned *__restrict__ cmax) {
   const int s = blockIdx.y;
-  const float *c = coords + s * total;
+  const float *c = coords + s * npts * ndim;
   unsigned m[3] = {0u, 0u, 0u};
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int d = (int)(t % ndim);
-    const unsigned b = __float_as_uint(fabsf(c[t]));
-    if (b > m[d]) m[d] = b;
+  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < npts;
+       p += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (d < ndim) {
+        const unsigned b = __float_as_uint(fabsf(c[p * ndim + d]));
+        m[d] = b > m[d] ? b : m[d];
+      }
+    }
   }
-  for (int d = 0; d < ndim; ++d) {
+  __shared__ unsigned s_m[3][kAbsmaxThreads / kWave];
+  const int w = threadIdx.x / kWave;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
     unsigned v = m[d];
     for (int off = 32; off > 0; off >>= 1) {
-      unsigned o = __shfl_down(v, off, kWave);
+      const unsigned o = __shfl_down(v, off, kWave);
       v = o > v ? o : v;
     }
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicMax(cmax + s * ndim + d, v);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_m[d][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < ndim) {
+    unsigned v = 0u;
+    for (int i = 0; i < kAbsmaxThreads / kWave; ++i) v = s_m[threadIdx.x][i] > v ? s_m[threadIdx.x][i] : v;
+    atomicMax(cmax + s * ndim + threadIdx.x, v);
   }
 }
 
@@ -610,9 +626,8 @@ int pg_plan_absmax(const float *coords, int64_t S, int64_t npts, int32_t ndim,
   PG_CHECK(ndim >= 1 && ndim <= 3, "plan: ndim must be 1..3");
   PG_CHECK(S >= 1 && npts >= 0 && S <= 65535, "plan: bad stack/points");
   if (npts == 0) return 0;
-  const int threads = 256;
-  dim3 grid(grid_for(npts * ndim, threads, 1024), (unsigned)S);
-  hipLaunchKernelGGL(k_absmax, grid, dim3(threads), 0, as_stream(stream),
+  dim3 grid(grid_for(npts, kAbsmaxThreads, 512), (unsigned)S);
+  hipLaunchKernelGGL(k_absmax, grid, dim3(kAbsmaxThreads), 0, as_stream(stream),
                      coords, S, npts, ndim, reinterpret_cast<unsigned *>(cmax));
   PG_LAUNCHED();
   return 0;

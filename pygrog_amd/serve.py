@@ -42,7 +42,9 @@ import numpy as np
 import torch
 
 from . import _lib
-from .calib._grog import GridProgram, GrogInterpolator, _default_oversamp, _kernel_offsets, new_sample_rows, uses_s24
+from .calib._grappa import build_table
+from .calib._grog import (PLAN_RADIUS, GridProgram, GrogInterpolator, _default_oversamp, _kernel_offsets,
+                          new_sample_rows, uses_s24)
 
 
 @dataclasses.dataclass
@@ -120,7 +122,10 @@ class GrogServer:
 
     ``hbm_budget`` bytes bound the admitted jobs' estimated footprint,
     ``slots`` the jobs in preparation at once (one stream and one worker
-    thread each), ``max_inflight`` the gridding batches on the device."""
+    thread each, plus a helper thread and stream for the table fit),
+    ``max_inflight`` the gridding batches on the device.  The worker threads
+    live as long as the server (``close()`` or a ``with`` block ends them), so
+    successive ``run`` calls reuse their per-thread library state."""
 
     def __init__(self, device=None, *, mode: str = "fp32", hbm_budget: int | None = None, slots: int = 4,
                  max_inflight: int = 2, keep_results: bool = False):
@@ -136,17 +141,49 @@ class GrogServer:
         self.max_inflight = int(max_inflight)
         self.keep_results = keep_results
         self._streams = [torch.cuda.Stream(device=self.device) for _ in range(self.slots)]
+        self._fit_streams = [torch.cuda.Stream(device=self.device) for _ in range(self.slots)]
+        self._pool = cf.ThreadPoolExecutor(max_workers=self.slots)
+        self._fit_pool = cf.ThreadPoolExecutor(max_workers=self.slots)
         self._grid_stream = torch.cuda.Stream(device=self.device)
         self._free_streams = list(range(self.slots))
         self._lock = threading.Lock()
 
+    def close(self) -> None:
+        """End the worker threads (after the work they hold)."""
+        self._pool.shutdown(wait=True)
+        self._fit_pool.shutdown(wait=True)
+
+    def __enter__(self) -> "GrogServer":
+        return self
+
+    def __exit__(self, *exc) -> None:
+        self.close()
+
     # -- preparation (worker thread, own stream) --------------------------------
+    def _fit(self, job: GridJob, slot: int):
+        """The job's operator table (FP64 fit, mostly host LAPACK work) on the
+        slot's second stream; returns the table and the event after it."""
+        fs = self._fit_streams[slot]
+        with torch.cuda.device(self.device), torch.cuda.stream(fs):
+            res = build_table(job.calib, PLAN_RADIUS, job.precision, job.lamda, quant=job.quant, device=self.device)
+            ev = torch.cuda.Event()
+            ev.record(fs)
+        return res, ev
+
     def _prepare(self, job: GridJob, slot: int) -> GridJob:
         st = self._streams[slot]
+        # the table fit runs on a helper thread (and its own stream) while this
+        # thread builds the plan; the two meet when the shifts are quantised
+        fit = self._fit_pool.submit(self._fit, job, slot)
         with torch.cuda.device(self.device), torch.cuda.stream(st):
-            g = GrogInterpolator(job.shape, job.coords, job.oversamp, job.kernel_width, job.kernel_shape,
-                                 device=self.device)
-            g.calc_interp_table(job.calib, lamda=job.lamda, precision=job.precision, quant=job.quant)
+            try:
+                g = GrogInterpolator(job.shape, job.coords, job.oversamp, job.kernel_width, job.kernel_shape,
+                                     device=self.device)
+            finally:
+                res, fev = fit.result()
+            st.wait_event(fev)
+            res.table.record_stream(st)
+            g._attach_table(res, job.precision, job.quant)
             if self.mode != "fp32":
                 g._entry_groups()
                 g._table_bf16(2 if self.mode == "bf16x3" else 1)
@@ -215,7 +252,7 @@ class GrogServer:
         inflight = []
         done = []
         batches = []
-        pool = cf.ThreadPoolExecutor(max_workers=self.slots)
+        pool = self._pool
         free_slots = list(range(self.slots))
         t0 = time.perf_counter()
         try:
@@ -267,7 +304,8 @@ class GrogServer:
                 if not progressed:
                     time.sleep(0.0002)
         finally:
-            pool.shutdown(wait=True)
+            # a failed run leaves no preparation behind
+            cf.wait(list(preparing))
         makespan = max(j.t_done for j in done) if done else 0.0
         lat = np.array([j.latency for j in done])
         cs = sum(j.coils * j.samples for j in done)

@@ -47,8 +47,8 @@ def test_serve_mix_equals_alone(device, mode):
     from pygrog_amd.serve import GrogServer
 
     jobs = _jobs(device, seed=1 + len(mode))
-    srv = GrogServer(device, mode=mode, slots=3, keep_results=True)
-    stats = srv.run(jobs, timeout=300)
+    with GrogServer(device, mode=mode, slots=3, keep_results=True) as srv:
+        stats = srv.run(jobs, timeout=300)
     assert stats["jobs"] == len(jobs) and stats["p99_ms"] >= stats["p50_ms"] > 0
     assert stats["batches"] >= 1
     for j in jobs:
@@ -62,9 +62,12 @@ def test_serve_budget_queues_and_completes(device):
 
     jobs = _jobs(device, n=8, seed=7, rate=2000.0)
     biggest = max(estimate_job_bytes(j, "fp32") for j in jobs)
-    srv = GrogServer(device, mode="fp32", hbm_budget=int(1.5 * biggest), slots=4)
-    stats = srv.run(jobs, timeout=300)
-    assert stats["jobs"] == 8
-    assert stats["peak_admitted_gb"] * 1e9 <= 1.5 * biggest + 1
+    with GrogServer(device, mode="fp32", hbm_budget=int(1.5 * biggest), slots=4) as srv:
+        stats = srv.run(jobs, timeout=300)
+        assert stats["jobs"] == 8
+        assert stats["peak_admitted_gb"] * 1e9 <= 1.5 * biggest + 1
+        # the same server (and its worker threads) serves a second stream
+        again = srv.run(_jobs(device, n=4, seed=8, rate=2000.0), timeout=300)
+        assert again["jobs"] == 4
     with pytest.raises(ValueError, match="budget"):
         GrogServer(device, mode="fp32", hbm_budget=biggest // 4).run(_jobs(device, n=3, seed=9))
