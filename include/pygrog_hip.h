@@ -366,6 +366,17 @@ int pg_gather_perm_c64(const void *grid, int64_t G, const int64_t *indices,
                        const float *weights, const int64_t *inv_perm,
                        int64_t n, int64_t B, void *out, pg_stream_t stream);
 
+/* The same gather for a whole batch of stacked plans in one launch (the
+ * reference's _fft_pad_gather_batch / stacked _forward_packed,
+ * src/pygrog/operator/_sparse_fft.py:626-680, 1050-1149): grid (B, S, C, G),
+ * indices / weights / inv_perm (S, n) (inv_perm element-local), out
+ * (B, S, C, n); out[b, s, c] is pg_gather_perm_c64 of grid[b, s] with
+ * element s's plan, bit for bit. */
+int pg_gather_perm_stack_c64(const void *grid, int64_t G, const int64_t *indices,
+                             const float *weights, const int64_t *inv_perm,
+                             int64_t n, int64_t C, int64_t S, int64_t B,
+                             void *out, pg_stream_t stream);
+
 /* unit_count[s*G + c] = ceil((cell_ptr[s][c+1] - cell_ptr[s][c]) / chunk). */
 int pg_unit_count(const uint32_t *cell_ptr, int64_t S, int64_t G,
                   int32_t chunk, uint32_t *unit_count, pg_stream_t stream);

@@ -113,6 +113,8 @@ _SIGS = {
     "pg_grid_tile_cells": [c_int64],
     "pg_grid_tile_options": [c_int64, c_void_p, ctypes.c_int32],
     "pg_gather_perm_c64": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p, c_void_p],
+    "pg_gather_perm_stack_c64": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64,
+                                 c_void_p, c_void_p],
     "pg_unit_count": [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_transpose_cn_c64": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
     "pg_transpose_frames_c64": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p],

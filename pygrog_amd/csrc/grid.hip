@@ -1222,25 +1222,30 @@ __global__ void k_sparse_apply(const float2 *__restrict__ y,
 }
 
 // out[b, m] = w[inv[m]] * grid[b, idx[inv[m]]]  (gather + unsort fused)
+// rows of n outputs; row r reads grid row r and the plan of stack element
+// (r / C) % S (indices, weights and inv_perm of that element start at
+// s * n; inv_perm holds element-local positions)
 __global__ void k_gather_perm(const float2 *__restrict__ grid, int64_t G,
                               const int64_t *__restrict__ idx,
                               const float *__restrict__ w,
                               const int64_t *__restrict__ inv, int64_t n,
-                              int64_t B, float2 *__restrict__ out) {
-  const int64_t total = B * n;
+                              int64_t C, int64_t S, int64_t rows,
+                              float2 *__restrict__ out) {
+  const int64_t total = rows * n;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = t / n;
-    const int64_t m = t - b * n;
-    const int64_t i = inv[m];
-    const int64_t k = idx[i];
-    float2 r = make_float2(0.f, 0.f);
+    const int64_t r = t / n;
+    const int64_t m = t - r * n;
+    const int64_t base = ((r / C) % S) * n;
+    const int64_t i = inv[base + m];
+    const int64_t k = idx[base + i];
+    float2 o = make_float2(0.f, 0.f);
     if (k >= 0 && k < G) {
-      const float wi = w[i];
-      const float2 g = grid[b * G + k];
-      r = make_float2(wi * g.x, wi * g.y);
+      const float wi = w[base + i];
+      const float2 g = grid[r * G + k];
+      o = make_float2(wi * g.x, wi * g.y);
     }
-    out[t] = r;
+    out[t] = o;
   }
 }
 
@@ -1555,7 +1560,25 @@ int pg_gather_perm_c64(const void *grid, int64_t G, const int64_t *indices,
   hipLaunchKernelGGL(k_gather_perm, dim3(grid_for(total, threads)),
                      dim3(threads), 0, as_stream(stream),
                      static_cast<const float2 *>(grid), G, indices, weights,
-                     inv_perm, n, B, static_cast<float2 *>(out));
+                     inv_perm, n, (int64_t)1, (int64_t)1, B,
+                     static_cast<float2 *>(out));
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_gather_perm_stack_c64(const void *grid, int64_t G, const int64_t *indices,
+                             const float *weights, const int64_t *inv_perm,
+                             int64_t n, int64_t C, int64_t S, int64_t B,
+                             void *out, pg_stream_t stream) {
+  PG_CHECK(G >= 0 && n >= 0 && C >= 0 && S >= 1 && B >= 0, "gather_perm_stack: bad sizes");
+  const int64_t total = B * S * C * n;
+  if (total == 0) return 0;
+  const int threads = 256;
+  hipLaunchKernelGGL(k_gather_perm, dim3(grid_for(total, threads)),
+                     dim3(threads), 0, as_stream(stream),
+                     static_cast<const float2 *>(grid), G, indices, weights,
+                     inv_perm, n, C, S, B * S * C,
+                     static_cast<float2 *>(out));
   PG_LAUNCHED();
   return 0;
 }

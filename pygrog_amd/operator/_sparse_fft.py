@@ -180,13 +180,12 @@ class _ScatterPlan:
         ``weights``, signed) applied."""
         B, S, C, G = kgrid.shape
         out = torch.empty(B, S, C, self.n, dtype=torch.complex64, device=self.device)
-        st = _lib.stream(self.device)
         kg = kgrid.contiguous()
-        w = self.sqrt_w if weights is None else weights
-        for b in range(B):
-            for s in range(S):
-                _lib.call("pg_gather_perm_c64", kg[b, s].data_ptr(), G, self.indices[s].data_ptr(),
-                          w[s].data_ptr(), self.inv_perm[s].data_ptr(), self.n, C, out[b, s].data_ptr(), st)
+        w = (self.sqrt_w if weights is None else weights).contiguous()
+        idx, inv = self.indices.contiguous(), self.inv_perm.contiguous()
+        # every (batch, stack element, coil) row in one launch
+        _lib.call("pg_gather_perm_stack_c64", _lib.ptr(kg), G, _lib.ptr(idx), _lib.ptr(w), _lib.ptr(inv), self.n, C,
+                  S, B, _lib.ptr(out), _lib.stream(self.device))
         return out
 
 
