@@ -688,6 +688,7 @@ class GrogInterpolator:
         self.device = self.plan._compiled.device
         _attach_fft_plan(self.plan, image_shape)
         self._programs = {}
+        self._frame_off = {}
 
     # -- metadata / convenience (reference :337-376) -------------------------
     def metadata(self):
@@ -880,13 +881,23 @@ class GrogInterpolator:
         comp = self.plan._compiled
         B, S, C, N = y.shape
         s24 = yt.dtype == torch.uint8
-        rb = sample_row_bytes(C, s24)
-        name = "pg_transpose_cn_s24" if s24 else "pg_transpose_cn_c64"
         st = _lib.stream(comp.device)
-        e0 = timer.start() if timer is not None else None
-        for b in range(B):
-            for s in range(S):
-                _lib.call(name, y[b, s].data_ptr(), C, N, yt.data_ptr() + (b * S + s) * N * rb, st)
+        F = B * S
+        if F == 1:
+            name = "pg_transpose_cn_s24" if s24 else "pg_transpose_cn_c64"
+            e0 = timer.start() if timer is not None else None
+            _lib.call(name, _lib.ptr(y), C, N, _lib.ptr(yt), st)
+        else:
+            # every (batch, stack element) block in one launch: equal-length
+            # frames back to back, frame f's rows at sample offset f * N
+            key = (F, N)
+            off = getattr(self, "_frame_off", {}).get(key)
+            if off is None:
+                off = torch.arange(F + 1, dtype=torch.int64, device=comp.device) * N
+                self._frame_off = {key: off}
+            name = "pg_transpose_frames_s24" if s24 else "pg_transpose_frames_c64"
+            e0 = timer.start() if timer is not None else None
+            _lib.call(name, _lib.ptr(y), _lib.ptr(off), F, N, C, _lib.ptr(yt), st)
         if timer is not None:
             timer.stop(name, e0)
 
