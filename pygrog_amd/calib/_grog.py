@@ -739,6 +739,7 @@ class GrogInterpolator:
         comp.rep_ent = rep_ent
         comp.interp_idx = interp_idx
         comp.entry_groups = None
+        comp.stack_groups = {}
         C = int(res.table.shape[1])
         comp.C = C
         comp.table = res.table
@@ -863,6 +864,35 @@ class GrogInterpolator:
         comp.entry_groups = groups
         return groups
 
+    def _stack_groups(self, B: int):
+        """Entry groups of every (batch, stack element) frame of a device
+        gridding call in one replica space: frame f = b * S + s owns replica
+        rows [f * rmax, f * rmax + R_s) (the rows its V block has) and
+        sample rows from f * N; the unused rows of a frame carry entry K,
+        whose work items the apply skips.  None when the indices would not
+        fit 32 bits."""
+        comp = self.plan._compiled
+        cache = getattr(comp, "stack_groups", None)
+        if cache is None:
+            cache = comp.stack_groups = {}
+        if B in cache:
+            return cache[B]
+        dev, S, N = comp.device, comp.S, comp.npts
+        K, rmax, F = int(comp.table.shape[0]), self._rmax(), B * S
+        if F * max(N, rmax) >= 2 ** 31:
+            cache[B] = None
+            return None
+        nv = torch.tensor(comp.n_valid, dtype=torch.int64, device=dev)
+        live = torch.arange(rmax, device=dev)[None, :] < nv[:, None]
+        ent = torch.where(live, comp.rep_ent[:, :rmax], K).to(torch.int32).repeat(B, 1)
+        base = (torch.arange(F, dtype=torch.int64, device=dev) * N)[:, None]
+        src = (comp.rep_src[:, :rmax].to(torch.int64).repeat(B, 1) + base).to(torch.int32)
+        sw = comp.rep_s[:, :rmax].repeat(B, 1)
+        grp = group_by_entry(ent.reshape(-1), F * rmax, K + 1, dev, self.GROUP_CHUNK, src.reshape(-1).contiguous(),
+                             sw.reshape(-1).contiguous())
+        cache[B] = grp
+        return grp
+
     def _table_bf16(self, n_parts: int):
         comp = self.plan._compiled
         if n_parts not in comp.table_bf16:
@@ -947,17 +977,25 @@ class GrogInterpolator:
         s24 = yt.dtype == torch.uint8
         rb = sample_row_bytes(C, s24)
         name = "pg_apply_grouped_bf16_s24" if s24 else "pg_apply_grouped_bf16"
+        sg = self._stack_groups(B) if B * S > 1 else None
         e0 = timer.start() if timer is not None else None
-        for b in range(B):
-            for s in range(S):
-                grp = groups[s]
-                if comp.n_valid[s] == 0 or grp.n_items == 0:
-                    continue
-                yf = yt.data_ptr() + (b * S + s) * N * rb
-                vb = vbuf.data_ptr() + (b * S + s) * self._rmax() * C * 8
-                _lib.call(name, yf, _lib.ptr(tbl), _lib.ptr(grp.ent_ptr), _lib.ptr(grp.ent_rep),
-                          _lib.ptr(grp.ent_src), _lib.ptr(grp.ent_s), _lib.ptr(grp.items), grp.n_items, K, C,
-                          n_terms, vb, st)
+        if sg is not None:
+            # every frame's replicas in one launch
+            if sg.n_items > 0:
+                _lib.call(name, yt.data_ptr(), _lib.ptr(tbl), _lib.ptr(sg.ent_ptr), _lib.ptr(sg.ent_rep),
+                          _lib.ptr(sg.ent_src), _lib.ptr(sg.ent_s), _lib.ptr(sg.items), sg.n_items, K, C, n_terms,
+                          vbuf.data_ptr(), st)
+        else:
+            for b in range(B):
+                for s in range(S):
+                    grp = groups[s]
+                    if comp.n_valid[s] == 0 or grp.n_items == 0:
+                        continue
+                    yf = yt.data_ptr() + (b * S + s) * N * rb
+                    vb = vbuf.data_ptr() + (b * S + s) * self._rmax() * C * 8
+                    _lib.call(name, yf, _lib.ptr(tbl), _lib.ptr(grp.ent_ptr), _lib.ptr(grp.ent_rep),
+                              _lib.ptr(grp.ent_src), _lib.ptr(grp.ent_s), _lib.ptr(grp.items), grp.n_items, K, C,
+                              n_terms, vb, st)
         if timer is not None:
             timer.stop(name, e0)
         prog.launch("pg_grid_reduce_v", timer)
