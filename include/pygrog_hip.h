@@ -493,6 +493,15 @@ int pg_sparse_apply_c64(const void *y, const int64_t *interp_idx,
                         const void *table, void *out, int64_t B, int64_t C,
                         int64_t N, int64_t kw, int64_t K, pg_stream_t stream);
 
+/* pg_sparse_apply_c64 for a batch of a stacked plan in one launch (the
+ * reference's per-element loop in _sparse_full, _grog.py:563-666): y
+ * (B, S, C, N), interp_idx (S, N*kw), out (B, S, C, N*kw); block (b, s) is
+ * pg_sparse_apply_c64 of y[b, s] with element s's entries, bit for bit. */
+int pg_sparse_apply_stack_c64(const void *y, const int64_t *interp_idx,
+                              const void *table, void *out, int64_t B, int64_t S,
+                              int64_t C, int64_t N, int64_t kw, int64_t K,
+                              pg_stream_t stream);
+
 /* cell_ptr (G + 1) uint32 from sorted int64 indices (n of them); indices
  * outside [0, G) are not allowed (the caller checks sortedness/range). */
 int pg_csr_from_sorted(const int64_t *indices, int64_t n, int64_t G,

@@ -132,6 +132,8 @@ _SIGS = {
     "pg_apply_grouped_bf16": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_apply_grouped_bf16_s24": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_sparse_apply_c64": [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
+    "pg_sparse_apply_stack_c64": [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64,
+                                  c_int64, c_void_p],
     "pg_csr_from_sorted": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
     "pg_csr_from_sorted_u32": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
 }

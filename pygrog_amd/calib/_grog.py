@@ -1071,13 +1071,10 @@ class GrogInterpolator:
 
         sparse = torch.empty(B, comp.S, C, comp.nper, dtype=torch.complex64, device=comp.device)
         K = int(comp.table.shape[0])
-        st = _lib.stream(comp.device)
-        for s in range(comp.S):
-            ys = y[:, s].contiguous() if comp.S > 1 else y[:, 0]
-            os_ = torch.empty(B, C, comp.nper, dtype=torch.complex64, device=comp.device)
-            _lib.call("pg_sparse_apply_c64", _lib.ptr(ys), comp.interp_idx[s].data_ptr(), _lib.ptr(comp.table),
-                      _lib.ptr(os_), B, C, comp.npts, comp.kw, K, st)
-            sparse[:, s] = os_
+        # every (batch, stack element) block in one launch
+        _lib.call("pg_sparse_apply_stack_c64", _lib.ptr(y.contiguous()), _lib.ptr(comp.interp_idx.contiguous()),
+                  _lib.ptr(comp.table), _lib.ptr(sparse), B, comp.S, C, comp.npts, comp.kw, K,
+                  _lib.stream(comp.device))
         out = sparse.reshape(*batch_shape, *s_shape, C, comp.nper).to(out_dtype).to(src_device)
         return out.numpy() if is_numpy else out
 

@@ -1196,25 +1196,27 @@ __global__ __launch_bounds__(256) void k_transpose_frames(const float2 *__restri
 }
 
 // out[b, co, m] = K[idx[m]][co] . y[b, :, m / kw]
+// rows (b, s) of B x S blocks: y (B, S, C, N), idx (S, N * kw) -- element
+// s's entries at s * N * kw --, out (B, S, C, N * kw)
 __global__ void k_sparse_apply(const float2 *__restrict__ y,
                                const int64_t *__restrict__ idx,
                                const float2 *__restrict__ K,
-                               float2 *__restrict__ out, int64_t B, int64_t C,
-                               int64_t N, int64_t kw, int64_t nK) {
+                               float2 *__restrict__ out, int64_t B, int64_t S,
+                               int64_t C, int64_t N, int64_t kw, int64_t nK) {
   const int64_t M = N * kw;
-  const int64_t total = B * C * M;
+  const int64_t total = B * S * C * M;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int64_t bc = t / M;
     const int64_t m = t - bc * M;
-    const int64_t b = bc / C;
-    const int64_t co = bc - b * C;
+    const int64_t bs = bc / C;
+    const int64_t co = bc - bs * C;
     const int64_t n = m / kw;
-    const int64_t e = idx[m];
+    const int64_t e = idx[(bs % S) * M + m];
     float2 acc = make_float2(0.f, 0.f);
     if (e >= 0 && e < nK) {
       const float2 *row = K + (e * C + co) * C;
-      const float2 *ys = y + b * C * N + n;
+      const float2 *ys = y + bs * C * N + n;
       for (int64_t ci = 0; ci < C; ++ci) cmac<float>(acc, row[ci], ys[ci * N]);
     }
     out[t] = acc;
@@ -1545,7 +1547,25 @@ int pg_sparse_apply_c64(const void *y, const int64_t *interp_idx,
                      dim3(threads), 0, as_stream(stream),
                      static_cast<const float2 *>(y), interp_idx,
                      static_cast<const float2 *>(table),
-                     static_cast<float2 *>(out), B, C, N, kw, K);
+                     static_cast<float2 *>(out), B, (int64_t)1, C, N, kw, K);
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_sparse_apply_stack_c64(const void *y, const int64_t *interp_idx,
+                              const void *table, void *out, int64_t B, int64_t S,
+                              int64_t C, int64_t N, int64_t kw, int64_t K,
+                              pg_stream_t stream) {
+  PG_CHECK(B >= 0 && S >= 1 && C >= 0 && N >= 0 && kw >= 1 && K >= 1,
+           "sparse_apply_stack: bad sizes");
+  const int64_t total = B * S * C * N * kw;
+  if (total == 0) return 0;
+  const int threads = 256;
+  hipLaunchKernelGGL(k_sparse_apply, dim3(grid_for(total, threads)),
+                     dim3(threads), 0, as_stream(stream),
+                     static_cast<const float2 *>(y), interp_idx,
+                     static_cast<const float2 *>(table),
+                     static_cast<float2 *>(out), B, S, C, N, kw, K);
   PG_LAUNCHED();
   return 0;
 }

@@ -229,3 +229,36 @@ def test_gather_perm_stack_equals_per_element(device, B, S, C, n, G):
         im = (wi * g.imag).astype(np.float32)
         exp[:, s] = np.where(ok, re + 1j * im, 0).astype(np.complex64)
     assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("B,S,C,N,kw,K", [(2, 3, 4, 50, 3, 20), (1, 1, 2, 7, 1, 5), (2, 2, 3, 0, 2, 4)])
+def test_sparse_apply_stack_equals_per_element(device, B, S, C, N, kw, K):
+    """One launch over every (batch, stack element) block gives, bit for bit,
+    the per-element operator applies (entries outside the table give 0)."""
+    from pygrog_amd import _lib
+
+    rng = np.random.default_rng(B * 100 + S * 10 + C + N)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
+    y = t((rng.standard_normal((B, S, C, N)) + 1j * rng.standard_normal((B, S, C, N))).astype(np.complex64))
+    idx = t(rng.integers(-1, K + 1, (S, N * kw)).astype(np.int64))
+    tab = t((rng.standard_normal((K, C, C)) + 1j * rng.standard_normal((K, C, C))).astype(np.complex64))
+    st = _lib.stream(device)
+    out = torch.empty(B, S, C, N * kw, dtype=torch.complex64, device=device)
+    _lib.call("pg_sparse_apply_stack_c64", _lib.ptr(y), _lib.ptr(idx), _lib.ptr(tab), _lib.ptr(out), B, S, C, N, kw,
+              K, st)
+    for s in range(S):
+        ys = y[:, s].contiguous()
+        each = torch.empty(B, C, N * kw, dtype=torch.complex64, device=device)
+        _lib.call("pg_sparse_apply_c64", _lib.ptr(ys), idx[s].data_ptr(), _lib.ptr(tab), _lib.ptr(each), B, C, N, kw,
+                  K, st)
+        assert torch.equal(out[:, s], each), s
+    if N:
+        e = idx.cpu().numpy()
+        ref = np.zeros((B, S, C, N * kw), np.complex128)
+        yh, th = y.cpu().numpy().astype(np.complex128), tab.cpu().numpy().astype(np.complex128)
+        for s in range(S):
+            for m in range(N * kw):
+                if 0 <= e[s, m] < K:
+                    ref[:, s, :, m] = yh[:, s, :, m // kw] @ th[e[s, m]].T
+        got = out.cpu().numpy()
+        assert np.abs(got - ref).max() <= 1e-5 * max(np.abs(ref).max(), 1.0)
