@@ -68,6 +68,14 @@ int pg_scatter_add_c64(void *grid, int64_t grid_size, const void *data,
 int pg_scatter_add_c128(void *grid, int64_t grid_size, const void *data,
                         const int64_t *indices, const double *weights,
                         int64_t n, pg_stream_t stream);
+/* pg_scatter_add_c64 on `rows` grids at once: grid (rows, grid_size) and
+ * data (rows, n) c64 share indices and weights; row r is
+ * pg_scatter_add_c64(grid[r], data[r]) (same atomics).  Serves the
+ * per-coil loop of the shot-wise gridding (reference _grog.py:437-555,
+ * interpolate with shot_index). */
+int pg_scatter_add_rows_c64(void *grid, int64_t grid_size, const void *data,
+                            const int64_t *indices, const float *weights,
+                            int64_t n, int64_t rows, pg_stream_t stream);
 
 /* Same contract as pg_scatter_add_*; `bin_starts` (n_bin_starts int64, the
  * searchsorted bin edges of the reference) is validated but not needed by

@@ -82,6 +82,7 @@ _SIGS = {
     "pg_grog_interpolate_c128": [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_void_p],
     "pg_scatter_add_c64": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p],
     "pg_scatter_add_c128": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p],
+    "pg_scatter_add_rows_c64": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p],
     "pg_scatter_add_binned_c64": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int64, c_void_p],
     "pg_scatter_add_binned_c128": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int64, c_void_p],
     "pg_gather_c64": [c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p, c_void_p],

@@ -1100,9 +1100,9 @@ class GrogInterpolator:
         _lib.call("pg_sparse_apply_c64", _lib.ptr(y), _lib.ptr(e_shot), _lib.ptr(comp.table), _lib.ptr(applied), 1, C,
                   npts, kw, int(comp.table.shape[0]), st)
         out = torch.zeros(C, comp.G, dtype=torch.complex64, device=dev)
-        for c in range(C):
-            _lib.call("pg_scatter_add_c64", out[c].data_ptr(), comp.G, applied[0, c].data_ptr(), _lib.ptr(t_shot),
-                      _lib.ptr(w_shot), npts * kw, st)
+        # every coil in one launch
+        _lib.call("pg_scatter_add_rows_c64", _lib.ptr(out), comp.G, _lib.ptr(applied), _lib.ptr(t_shot),
+                  _lib.ptr(w_shot), npts * kw, C, st)
         return out.to(data.device)
 
     def collect_shots(self) -> torch.Tensor:

@@ -106,11 +106,14 @@ __device__ __forceinline__ void run_reduce_store(int64_t key, Vals<R, NV> val,
 }
 
 // grid[idx[i]] += w[i] * data[i]; complex (NV = 2) with real weights.
+// blockIdx.y selects a row: grid (rows, G) and data (rows, n) share idx, w.
 template <class R>
 __global__ void k_scatter_add(R *__restrict__ grid, int64_t G,
                               const R *__restrict__ data,
                               const int64_t *__restrict__ idx,
                               const R *__restrict__ w, int64_t n) {
+  grid += (int64_t)blockIdx.y * 2 * G;
+  data += (int64_t)blockIdx.y * 2 * n;
   const int64_t nwaves_total = (int64_t)gridDim.x * (blockDim.x / kWave);
   const int64_t wave0 = blockIdx.x * (int64_t)(blockDim.x / kWave) +
                         threadIdx.x / kWave;
@@ -364,6 +367,22 @@ int pg_scatter_add_c64(void *grid, int64_t grid_size, const void *data,
                        pg_stream_t stream) {
   return scatter_impl<float>(grid, grid_size, data, indices, weights, n,
                              stream);
+}
+int pg_scatter_add_rows_c64(void *grid, int64_t grid_size, const void *data,
+                            const int64_t *indices, const float *weights,
+                            int64_t n, int64_t rows, pg_stream_t stream) {
+  PG_CHECK(grid_size >= 0 && n >= 0 && rows >= 0 && rows <= 65535,
+           "scatter_add_rows: bad sizes");
+  if (n == 0 || rows == 0) return 0;
+  PG_CHECK(grid && data && indices && weights, "scatter_add_rows: null pointer");
+  const int threads = 256;
+  hipLaunchKernelGGL(k_scatter_add<float>,
+                     dim3(grid_for(n, threads, 16384), (unsigned)rows),
+                     dim3(threads), 0, as_stream(stream),
+                     static_cast<float *>(grid), grid_size,
+                     static_cast<const float *>(data), indices, weights, n);
+  PG_LAUNCHED();
+  return 0;
 }
 int pg_scatter_add_c128(void *grid, int64_t grid_size, const void *data,
                         const int64_t *indices, const double *weights,

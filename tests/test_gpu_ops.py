@@ -262,3 +262,32 @@ def test_sparse_apply_stack_equals_per_element(device, B, S, C, N, kw, K):
                     ref[:, s, :, m] = yh[:, s, :, m // kw] @ th[e[s, m]].T
         got = out.cpu().numpy()
         assert np.abs(got - ref).max() <= 1e-5 * max(np.abs(ref).max(), 1.0)
+
+
+@pytest.mark.parametrize("rows,n,G", [(5, 3000, 257), (1, 10, 4), (3, 0, 8)])
+def test_scatter_add_rows_equals_per_row(device, rows, n, G):
+    """Several grids sharing indices and weights in one launch: each row as
+    pg_scatter_add_c64 on its own (atomic adds, so equal to float32
+    rounding) and as a float64 np.add.at (indices outside the grid skipped)."""
+    from pygrog_amd import _lib
+
+    rng = np.random.default_rng(rows * 7 + n)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)
+    d = (rng.standard_normal((rows, n)) + 1j * rng.standard_normal((rows, n))).astype(np.complex64)
+    idx = rng.integers(-3, G + 3, n).astype(np.int64)
+    w = rng.random(n, dtype=np.float32)
+    d_d, idx_d, w_d = t(d), t(idx), t(w)
+    st = _lib.stream(device)
+    got = torch.zeros(rows, G, dtype=torch.complex64, device=device)
+    _lib.call("pg_scatter_add_rows_c64", _lib.ptr(got), G, _lib.ptr(d_d), _lib.ptr(idx_d), _lib.ptr(w_d), n, rows, st)
+    each = torch.zeros(rows, G, dtype=torch.complex64, device=device)
+    for r in range(rows):
+        _lib.call("pg_scatter_add_c64", each[r].data_ptr(), G, d_d[r].data_ptr(), _lib.ptr(idx_d), _lib.ptr(w_d), n, st)
+    g, e = got.cpu().numpy(), each.cpu().numpy()
+    scale = max(np.abs(e).max() if e.size else 0.0, 1.0)
+    assert np.abs(g - e).max() <= 1e-5 * scale
+    ref = np.zeros((rows, G), np.complex128)
+    ok = (idx >= 0) & (idx < G)
+    for r in range(rows):
+        np.add.at(ref[r], idx[ok], (w[ok].astype(np.float64) * d[r][ok]))
+    assert np.abs(g - ref).max() <= 1e-5 * scale
