@@ -60,6 +60,17 @@ def test_library_version_and_errors():
     assert lib.pg_grappa_gram_f64(4096, 8, 3, _lib.dims([8, 1, 8]), 4096, None) != 0
     assert b"2 or more" in lib.pg_last_error()
     assert lib.pg_grappa_gram_f64(4096, 8, 1, _lib.dims([8]), 4096, None) != 0
+    # the batched gather / apply / scatter entries reject bad sizes before any
+    # launch; empty work returns 0 without touching the pointers
+    assert lib.pg_gather_perm_stack_c64(4096, 16, 4096, 4096, 4096, 10, 2, 0, 1, 4096, None) != 0
+    assert b"gather_perm_stack" in lib.pg_last_error()
+    assert lib.pg_gather_perm_stack_c64(4096, 16, 4096, 4096, 4096, 0, 2, 3, 1, 4096, None) == 0
+    assert lib.pg_sparse_apply_stack_c64(4096, 4096, 4096, 4096, 1, 2, 3, 10, 0, 5, None) != 0
+    assert b"sparse_apply_stack" in lib.pg_last_error()
+    assert lib.pg_sparse_apply_stack_c64(4096, 4096, 4096, 4096, 0, 2, 3, 10, 1, 5, None) == 0
+    assert lib.pg_scatter_add_rows_c64(4096, 16, 4096, 4096, 4096, 10, 70000, None) != 0
+    assert b"scatter_add_rows" in lib.pg_last_error()
+    assert lib.pg_scatter_add_rows_c64(4096, 16, 4096, 4096, 4096, 10, 0, None) == 0
     assert lib.pg_grid_tile_cells(65) == 0
     assert _lib.tile_options(65) == [] and _lib.tile_options(0) == []
     for c, gs in ((3, 4), (8, 8), (16, 16), (32, 32), (64, 64)):
