@@ -30,8 +30,19 @@ def sparse_reference(y: np.ndarray, interp_idx: np.ndarray, table: np.ndarray, k
     return np.ascontiguousarray(out[:, 0, :].T)
 
 
-def grid_reference(y, plan: dict, interp_idx, table) -> tuple[np.ndarray, np.ndarray]:
-    """-> (grid (C, G) complex64, density (G,) float32)."""
+def grid_reference(y, plan: dict, interp_idx, table, cells: np.ndarray | None = None):
+    """-> (grid (C, G) complex64, density (G,) float32).
+
+    With ``cells``: only those cells, -> (C, len(cells)) complex64 (columns
+    in the given order).  Only the replicas landing there are formed, with
+    the same per-replica arithmetic, and each cell is summed over its
+    replicas in the same (sorted) order as the full flow, so the columns
+    equal the full grid's bit for bit."""
+    if cells is not None:
+        cells = np.asarray(cells, dtype=np.int64)
+        if np.unique(cells).size != cells.size:
+            raise ValueError("cells must be distinct")
+        return _grid_reference_cells(y, plan, interp_idx, table, cells)
     kw = plan["offsets"].shape[0]
     sparse = sparse_reference(np.asarray(y, dtype=np.complex64), interp_idx, table, kw)
     pre_w = plan["sqrt_weights"][plan["inv_perm"]].astype(np.complex64)
@@ -48,10 +59,37 @@ def grid_reference(y, plan: dict, interp_idx, table) -> tuple[np.ndarray, np.nda
     return grid, density(plan)
 
 
-def grid_fp64(y, plan: dict, interp_idx, table64, cells: np.ndarray | None = None) -> np.ndarray:
+def _grid_reference_cells(y, plan: dict, interp_idx, table, cells: np.ndarray) -> np.ndarray:
+    kw = plan["offsets"].shape[0]
+    G = int(np.prod(plan["grid_shape"]))
+    pos = np.full(G, -1, dtype=np.int64)
+    pos[cells] = np.arange(len(cells))
+    ind = plan["indices"].reshape(-1)
+    q = np.flatnonzero(pos[ind] >= 0)  # sorted positions, ascending = the full flow's order
+    r = plan["sort_perm"].reshape(-1)[q]  # replica (n * kw + k)
+    e = np.asarray(interp_idx).reshape(-1)[r]
+    y = np.asarray(y)
+    rows = np.ascontiguousarray(y[:, r // kw].T.astype(np.complex64))[:, None, :]
+    applied = ops.interpolate(rows, e, np.asarray(table, dtype=np.complex64))[:, 0, :]  # (n, C)
+    sw = plan["sqrt_weights"].reshape(-1)[q]
+    # pre_weights[r] = sqrt_weights[inv_perm[r]] = sqrt_weights[q]
+    data = (applied * sw.astype(np.complex64)[:, None]).astype(np.complex64)
+    C = y.shape[0]
+    out = np.zeros((C, len(cells)), dtype=np.complex64)
+    col = pos[ind[q]]
+    for c in range(C):
+        g = np.zeros(len(cells), dtype=np.complex64)
+        ops.scatter_add(g, data[:, c], col, sw)
+        out[c] = g
+    return out
+
+
+def grid_fp64(y, plan: dict, interp_idx, table64, cells: np.ndarray | None = None,
+              chunk: int = 4096) -> np.ndarray:
     """FP64 oracle grid (C, G); with ``cells`` only those cells are computed
-    (columns in the given order, shape (C, len(cells)))."""
-    y = np.asarray(y, dtype=np.complex128)
+    (columns in the given order, shape (C, len(cells))).  ``y`` may be
+    complex64; it is widened per replica."""
+    y = np.asarray(y)
     C = y.shape[0]
     kw = plan["offsets"].shape[0]
     tflat = plan["target_idx"].reshape(-1)
@@ -60,6 +98,8 @@ def grid_fp64(y, plan: dict, interp_idx, table64, cells: np.ndarray | None = Non
     valid = np.nonzero(tflat >= 0)[0]
     G = int(np.prod(plan["grid_shape"]))
     if cells is not None:
+        if np.unique(cells).size != np.asarray(cells).size:
+            raise ValueError("cells must be distinct")
         pos = np.full(G, -1, dtype=np.int64)
         pos[np.asarray(cells)] = np.arange(len(cells))
         valid = valid[pos[tflat[valid]] >= 0]
@@ -69,19 +109,21 @@ def grid_fp64(y, plan: dict, interp_idx, table64, cells: np.ndarray | None = Non
         col = tflat[valid]
         ncol = G
     src = valid // kw
+    ev = e[valid]
+    wv = w[valid]
+    T = table64
+    # V[r] = w_r K64[e_r] y[:, src_r], in chunks of replicas (only the
+    # selected source columns are widened to complex128)
+    V = np.empty((len(valid), C), dtype=np.complex128)
+    for r0 in range(0, len(valid), chunk):
+        r1 = min(r0 + chunk, len(valid))
+        Tk = np.asarray(T[ev[r0:r1]], dtype=np.complex128)
+        yk = np.asarray(y[:, src[r0:r1]], dtype=np.complex128).T
+        V[r0:r1] = np.einsum("rij,rj->ri", Tk, yk) * wv[r0:r1, None]
     out = np.zeros((C, ncol), dtype=np.complex128)
-    T = np.asarray(table64, dtype=np.complex128)
-    order = np.argsort(e[valid], kind="stable")
-    ev = e[valid][order]
-    bounds = np.flatnonzero(np.diff(ev)) + 1
-    starts = np.concatenate([[0], bounds])
-    ends = np.concatenate([bounds, [len(ev)]])
-    for s0, s1 in zip(starts, ends):
-        sel = order[s0:s1]
-        V = (T[ev[s0]] @ y[:, src[sel]]) * w[valid][sel][None, :]
-        for c in range(C):
-            out[c] += np.bincount(col[sel], weights=V[c].real, minlength=ncol) + 1j * np.bincount(
-                col[sel], weights=V[c].imag, minlength=ncol)
+    for c in range(C):
+        out[c] = np.bincount(col, weights=V[:, c].real, minlength=ncol) + 1j * np.bincount(
+            col, weights=V[:, c].imag, minlength=ncol)
     return out
 
 

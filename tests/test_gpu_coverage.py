@@ -51,10 +51,12 @@ def test_radial_256_8coil_full_size(device):
     assert ogrid.rel_l2(g.grid_device(yy, "bf16").reshape(C, -1).cpu().numpy(), g64) <= 5e-3
 
 
-@pytest.mark.parametrize("C", [1, 3, 17, 33, 64])
+@pytest.mark.parametrize("C", [1, 3, 16, 17, 32, 33, 64])
 def test_coil_counts_every_kernel_shape(device, C):
     """Every coil group (4, 8, 16, 32, 64 lanes) and MFMA shape, odd and
-    even coil counts (unaligned rows), against the oracle."""
+    even coil counts (unaligned rows), against the oracle; 16 and 32 coils
+    are the grouped apply's complex64-row (MT=2, KS=1) and 24-bit-row
+    (MT=4, KS=2) instances of the benchmark workloads."""
     from pygrog_amd.calib import GrogInterpolator
 
     rng = np.random.default_rng(C)

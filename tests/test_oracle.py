@@ -226,6 +226,36 @@ def test_unit_operator_exact_recovery():
     np.testing.assert_allclose(ops_["x"], G, atol=1e-9)
 
 
+def test_unit_operator_exact_recovery_3d_axis_labels():
+    """3D calibration generated by three known commuting shift operators,
+    c[:, i, j, k] = A0^i A1^j A2^k v: lamda = 0 recovers the operator of
+    calibration axis 2 as "x", of axis 1 as "y" and of axis 0 as "z" (the
+    reference's labelling, src/pygrog/calib/_grappa.py:347-368); a swapped
+    or transposed result fails (the three operators differ and are not
+    symmetric)."""
+    rng = np.random.default_rng(15)
+    C = 4
+    Q, _ = np.linalg.qr(_rand_c(rng, C, C))
+    P = Q @ np.diag(rng.uniform(0.8, 1.3, C)) @ Q.conj().T  # not unitary: A^T != A^H-based forms
+    Pi = np.linalg.inv(P)
+    A = [P @ np.diag(np.exp(1j * rng.uniform(-0.4, 0.4, C))) @ Pi for _ in range(3)]
+    v = _rand_c(rng, C)
+    n0, n1, n2 = 5, 6, 7
+    calib = np.zeros((C, n0, n1, n2), np.complex128)
+    for i in range(n0):
+        for j in range(n1):
+            for k in range(n2):
+                calib[:, i, j, k] = (np.linalg.matrix_power(A[0], i) @ np.linalg.matrix_power(A[1], j)
+                                     @ np.linalg.matrix_power(A[2], k) @ v)
+    ops_ = otable.unit_operators(calib, 0.0)
+    np.testing.assert_allclose(ops_["x"], A[2], atol=1e-8)
+    np.testing.assert_allclose(ops_["y"], A[1], atol=1e-8)
+    np.testing.assert_allclose(ops_["z"], A[0], atol=1e-8)
+    for a in range(3):
+        assert np.abs(A[a] - A[a].T).max() > 1e-3
+        assert np.abs(A[a] - A[(a + 1) % 3]).max() > 1e-3
+
+
 def test_table_identity_semigroup_and_order():
     rng = np.random.default_rng(6)
     calib = _rand_c(rng, 3, 10, 10)
@@ -275,9 +305,36 @@ def test_grid_fp64_matches_brute_force():
     np.testing.assert_allclose(out, brute, rtol=1e-12, atol=1e-14)
     sub = ogrid.grid_fp64(y, p, e, t, cells=np.array([tf[tf >= 0][0], 7]))
     np.testing.assert_allclose(sub, brute[:, [tf[tf >= 0][0], 7]], rtol=1e-12, atol=1e-14)
+    # replicas taken in several chunks, complex64 input widened per replica
+    np.testing.assert_allclose(ogrid.grid_fp64(y, p, e, t, chunk=7), brute, rtol=1e-12, atol=1e-14)
+    y32 = y.astype(np.complex64)
+    np.testing.assert_allclose(ogrid.grid_fp64(y32, p, e, t, chunk=5),
+                               ogrid.grid_fp64(y32.astype(np.complex128), p, e, t), rtol=1e-13, atol=1e-15)
     gr, dens = ogrid.grid_reference(y.astype(np.complex64), p, e, t.astype(np.complex64))
     assert ogrid.rel_l2(gr, brute) < 1e-6
     np.testing.assert_allclose(dens[dens > 0], 1.0, rtol=1e-6)
+    # the cell-subset reference flow gives the full flow's columns bit for bit
+    cells = np.array([tf[tf >= 0][3], 7, tf[tf >= 0][0], 399, tf[tf >= 0][-1]])
+    sub = ogrid.grid_reference(y.astype(np.complex64), p, e, t.astype(np.complex64), cells=cells)
+    assert sub.dtype == np.complex64 and np.array_equal(sub, gr[:, cells])
+
+
+def test_grid_reference_cells_bitwise_on_hot_cells():
+    """Many replicas per cell (radial centre, summation order matters in
+    float32): the subset flow still equals the full flow bit for bit."""
+    rng = np.random.default_rng(9)
+    r = (np.arange(64) - 32) / 64
+    th = np.deg2rad(111.246117975) * np.arange(40)
+    coords = np.stack([np.outer(np.cos(th), r), np.outer(np.sin(th), r)], -1).astype(np.float32)
+    p = oplan.attach_fft_plan(oplan.create_plan_single((32, 32), coords, None, 3, "circle"))
+    C = 3
+    t, _, _ = otable.table(_rand_c(rng, C, 8, 8), 0.01, 1, 16)
+    e = oplan.quantize(p["distances"], t.shape[0], quant=16)
+    y = _rand_c(rng, C, 40 * 64).astype(np.complex64)
+    full, _ = ogrid.grid_reference(y, p, e, t.astype(np.complex64))
+    cells = np.concatenate([np.arange(16 * 32 + 12, 16 * 32 + 20), rng.choice(512, 50, replace=False)])
+    sub = ogrid.grid_reference(y, p, e, t.astype(np.complex64), cells=cells)
+    assert np.array_equal(sub, full[:, cells])
 
 
 def test_radial_unit_operators_recover_known_generators():
