@@ -27,15 +27,22 @@ pass to ``GrogInterpolator(shape, coords, ...)``,
 A job's result is bitwise the one its own interpolator produces alone with
 the same mode: every cell folds its replicas in a fixed order whatever
 launch the job shares.
+
+Failures stay with their request.  A malformed job (k-space that does not
+match its trajectory or calibration, a stacked trajectory, too many coils)
+and a job whose estimate exceeds the whole HBM budget are rejected before
+admission; a job whose preparation raises, or whose buffers cannot be
+allocated when its batch is launched, is marked failed and its budget and
+slot are released.  Every other job is served as if it had not been there.
+Each job records ``status`` ("done", "rejected" or "failed") and ``error``.
 """
 
 from __future__ import annotations
 
-__all__ = ["GridJob", "GrogServer", "estimate_job_bytes", "poisson_arrivals"]
+__all__ = ["GridJob", "GrogServer", "estimate_job_bytes", "poisson_arrivals", "serve_loop", "validate_job"]
 
 import concurrent.futures as cf
 import dataclasses
-import threading
 import time
 
 import numpy as np
@@ -44,12 +51,12 @@ import torch
 from . import _lib
 from .calib._grappa import build_table
 from .calib._grog import (PLAN_RADIUS, GridProgram, GrogInterpolator, _default_oversamp, _kernel_offsets,
-                          new_sample_rows, uses_s24)
+                          sample_row_bytes, uses_s24)
 
 
-@dataclasses.dataclass
+@dataclasses.dataclass(eq=False)
 class GridJob:
-    """One gridding request.  ``kspace`` (C, N) complex64 and ``calib``
+    """One gridding request (compared and hashed by identity).  ``kspace`` (C, N) complex64 and ``calib``
     (C, *acr) are device tensors (the request payload, staged on the GPU);
     ``coords`` is the trajectory (..., ndim) on the host."""
 
@@ -70,6 +77,8 @@ class GridJob:
     t_ready: float | None = None
     t_done: float | None = None
     est_bytes: int = 0
+    status: str | None = None
+    error: str | None = None
     result: torch.Tensor | None = None
     interp: GrogInterpolator | None = None
     _ready_event: torch.cuda.Event | None = None
@@ -117,6 +126,154 @@ def poisson_arrivals(n: int, rate: float, seed: int) -> np.ndarray:
     return np.cumsum(rng.exponential(1.0 / rate, size=n))
 
 
+def validate_job(job: GridJob) -> str | None:
+    """Why ``job`` cannot be served, or None.  Checks what the gridding
+    launch assumes: one trajectory (no stack), k-space (C, N) complex with
+    N the trajectory's sample count and C the calibration's coil count,
+    at most 64 coils, a known kernel shape."""
+    try:
+        shape = tuple(int(s) for s in job.shape)
+    except (TypeError, ValueError):
+        return f"grid shape {job.shape!r} is not a tuple of integers"
+    ndim = len(shape)
+    if ndim not in (2, 3) or min(shape) < 1:
+        return f"grid shape {shape} must have 2 or 3 positive axes"
+    coords = np.asarray(job.coords)
+    if coords.ndim < 2 or coords.shape[-1] != ndim:
+        return f"coords shape {coords.shape} does not end in the grid's {ndim} axes"
+    if coords.ndim > 3:
+        return (f"coords shape {coords.shape} is a stack of trajectories; a serving job grids one "
+                "trajectory (submit one job per stack element)")
+    n = int(np.prod(coords.shape[:-1]))
+    y = job.kspace
+    if not isinstance(y, torch.Tensor) or y.ndim != 2 or not y.is_complex():
+        return "kspace must be a complex tensor of shape (coils, samples)"
+    C = int(y.shape[0])
+    if int(y.shape[1]) != n:
+        return f"kspace has {int(y.shape[1])} samples per coil but the trajectory has {n}"
+    if not isinstance(job.calib, torch.Tensor) or job.calib.ndim != ndim + 1:
+        return f"calib must be a tensor (coils, *acr) with {ndim} spatial axes"
+    if int(job.calib.shape[0]) != C:
+        return f"kspace has {C} coils but the calibration has {int(job.calib.shape[0])}"
+    if not 1 <= C <= 64:
+        return f"fused gridding supports 1 to 64 coils, got {C}"
+    try:
+        kw = len(_kernel_offsets(ndim, int(job.kernel_width), job.kernel_shape))
+    except (ValueError, TypeError) as exc:
+        return f"kernel: {exc}"
+    if kw == 0:
+        return f"kernel: width {job.kernel_width} gives no neighbours"
+    return None
+
+
+def serve_loop(jobs: list, *, budget: int, slots: int, max_inflight: int, estimate, prepare, launch,
+               submit, timeout: float = 600.0, on_retire=None) -> dict:
+    """The scheduling loop of a serving run, with the device work passed in.
+
+    ``estimate(job)`` -> bytes; ``prepare(job, slot)`` runs through
+    ``submit(fn, *args)`` (an executor's submit) and may raise: the job
+    fails alone.  ``launch(batch)`` -> ``(handle, keep, failed)`` starts one
+    gridding batch (``handle.query()`` is true once it has completed) and
+    returns the jobs it could not launch, ``{job: message}``; ``keep`` is
+    held until completion.  ``on_retire(job)`` runs for each completed job.
+    Jobs that fail validation or exceed the whole budget are rejected
+    before admission.  Returns the bookkeeping lists (done, rejected,
+    failed, batch sizes, peak admitted bytes, makespan); each job's
+    ``status`` and ``error`` are set."""
+    t0 = time.perf_counter()
+    pending, rejected = [], []
+    for j in sorted(jobs, key=lambda j: j.arrival):
+        j.status, j.error = None, None
+        why = validate_job(j)
+        if why is None:
+            j.est_bytes = int(estimate(j))
+            if j.est_bytes > budget:
+                why = (f"job {j.job_id} needs ~{j.est_bytes / 1e9:.2f} GB, above the budget "
+                       f"{budget / 1e9:.2f} GB")
+        if why is not None:
+            j.status, j.error = "rejected", why
+            rejected.append(j)
+        else:
+            pending.append(j)
+    n_served = len(pending)
+    used = 0
+    peak_used = 0
+    preparing = {}
+    ready, inflight, done, failed, batches = [], [], [], [], []
+    free_slots = list(range(slots))
+
+    def fail(job, msg):
+        nonlocal used
+        job.status, job.error = "failed", msg
+        job.t_done = None
+        job.result = None
+        job.interp = None
+        used -= job.est_bytes
+        failed.append(job)
+
+    try:
+        while len(done) + len(failed) < n_served:
+            now = time.perf_counter() - t0
+            if now > timeout:
+                raise RuntimeError(f"serving run exceeded {timeout:.0f} s")
+            progressed = False
+            # admit in arrival order while budget and a slot are free
+            while pending and pending[0].arrival <= now and free_slots and used + pending[0].est_bytes <= budget:
+                job = pending.pop(0)
+                slot = free_slots.pop(0)
+                used += job.est_bytes
+                peak_used = max(peak_used, used)
+                job.t_admit = now
+                preparing[submit(prepare, job, slot)] = (job, slot)
+                progressed = True
+            # collect finished preparations; a preparation error fails its job only
+            for fut in [f for f in preparing if f.done()]:
+                job, slot = preparing.pop(fut)
+                free_slots.append(slot)
+                exc = fut.exception()
+                if exc is not None:
+                    fail(job, f"preparation failed: {type(exc).__name__}: {exc}")
+                else:
+                    job.t_ready = time.perf_counter() - t0
+                    ready.append(job)
+                progressed = True
+            # continuous batching: everything ready joins the next launch
+            if ready and len(inflight) < max_inflight:
+                batch, ready = ready, []
+                handle, keep, bad = launch(batch)
+                for job, msg in bad.items():
+                    fail(job, msg)
+                batch = [j for j in batch if j not in bad]
+                if batch:
+                    inflight.append((handle, batch, keep))
+                    batches.append(len(batch))
+                progressed = True
+            # retire completed batches
+            still = []
+            for handle, batch, keep in inflight:
+                if handle.query():
+                    t = time.perf_counter() - t0
+                    for job in batch:
+                        job.t_done = t
+                        job.status = "done"
+                        used -= job.est_bytes
+                        if on_retire is not None:
+                            on_retire(job)
+                        done.append(job)
+                    progressed = True
+                else:
+                    still.append((handle, batch, keep))
+            inflight = still
+            if not progressed:
+                time.sleep(0.0002)
+    finally:
+        # a failed run leaves no preparation behind
+        cf.wait(list(preparing))
+    makespan = max(j.t_done for j in done) if done else 0.0
+    return {"done": done, "rejected": rejected, "failed": failed, "batches": batches, "peak_used": peak_used,
+            "makespan": makespan}
+
+
 class GrogServer:
     """Serve ``GridJob`` requests on one HIP device (see the module notes).
 
@@ -145,8 +302,6 @@ class GrogServer:
         self._pool = cf.ThreadPoolExecutor(max_workers=self.slots)
         self._fit_pool = cf.ThreadPoolExecutor(max_workers=self.slots)
         self._grid_stream = torch.cuda.Stream(device=self.device)
-        self._free_streams = list(range(self.slots))
-        self._lock = threading.Lock()
 
     def close(self) -> None:
         """End the worker threads (after the work they hold)."""
@@ -193,143 +348,122 @@ class GrogServer:
         job._ready_event = ev
         return job
 
+    # -- device buffers -------------------------------------------------------------
+    def _alloc(self, n: int, dtype) -> torch.Tensor:
+        """A device buffer of ``n`` elements for a job's gridding (the hook
+        a test replaces to inject an allocation failure)."""
+        return torch.empty(n, dtype=dtype, device=self.device)
+
     # -- one gridding batch on the grid stream ---------------------------------
+    def _stage_job(self, job: GridJob, st, groups: dict, keep: list) -> None:
+        """Transpose (and for the BF16 modes apply) one job on the grid
+        stream and add its frames to the batch's reduction groups."""
+        g = job.interp
+        comp = g.plan._compiled
+        y = job.kspace
+        if comp.S != 1 or tuple(y.shape) != (comp.C, comp.npts):
+            raise ValueError(f"kspace shape {tuple(y.shape)} does not match the plan ({comp.C}, {comp.npts}) "
+                             f"of {comp.S} trajectory")
+        if y.device != self.device or y.dtype != torch.complex64 or not y.is_contiguous():
+            y = y.to(self.device, torch.complex64).contiguous()
+        s24 = uses_s24(self.mode, comp.C)
+        yt = self._alloc(max(comp.npts, 1) * sample_row_bytes(comp.C, s24), torch.uint8)
+        out = self._alloc(comp.C * comp.G, torch.complex64).reshape(comp.C, comp.G)
+        vbuf = None
+        if self.mode != "fp32":
+            vbuf = self._alloc(g._rmax() * comp.C, torch.complex64)
+        _lib.call("pg_transpose_cn_s24" if s24 else "pg_transpose_cn_c64", _lib.ptr(y), comp.C, comp.npts,
+                  _lib.ptr(yt), st.cuda_stream)
+        if self.mode != "fp32":
+            n_terms = 1 if self.mode == "bf16" else 3
+            tbl = g._table_bf16(2 if n_terms == 3 else 1)
+            grp = g._entry_groups()[0]
+            if grp.n_items > 0:
+                _lib.call("pg_apply_grouped_bf16_s24" if s24 else "pg_apply_grouped_bf16", _lib.ptr(yt),
+                          _lib.ptr(tbl), _lib.ptr(grp.ent_ptr),
+                          _lib.ptr(grp.ent_rep), _lib.ptr(grp.ent_src), _lib.ptr(grp.ent_s),
+                          _lib.ptr(grp.items), grp.n_items, int(comp.table.shape[0]), comp.C, n_terms,
+                          _lib.ptr(vbuf), st.cuda_stream)
+        frames = g._frames(yt, out, 1, vbuf)
+        groups.setdefault((comp.C, comp.G), []).extend(frames)
+        keep += [y, yt, out, vbuf]
+        job.result = out
+
     def _launch_batch(self, jobs: list) -> tuple:
+        """Enqueue one gridding batch; -> (completion event, buffers to keep,
+        {job: error} for the jobs that could not be staged)."""
         st = self._grid_stream
         keep = []
         groups = {}
+        bad = {}
         with torch.cuda.device(self.device), torch.cuda.stream(st):
             for job in jobs:
                 st.wait_event(job._ready_event)
-                g = job.interp
-                comp = g.plan._compiled
-                y = job.kspace
-                if y.device != self.device or y.dtype != torch.complex64 or not y.is_contiguous():
-                    y = y.to(self.device, torch.complex64).contiguous()
-                s24 = uses_s24(self.mode, comp.C)
-                yt = new_sample_rows(comp.npts, comp.C, s24, self.device)
-                out = torch.empty(comp.C, comp.G, dtype=torch.complex64, device=self.device)
-                _lib.call("pg_transpose_cn_s24" if s24 else "pg_transpose_cn_c64", _lib.ptr(y), comp.C, comp.npts,
-                          _lib.ptr(yt), st.cuda_stream)
-                vbuf = None
-                if self.mode != "fp32":
-                    vbuf = torch.empty(g._rmax() * comp.C, dtype=torch.complex64, device=self.device)
-                    n_terms = 1 if self.mode == "bf16" else 3
-                    tbl = g._table_bf16(2 if n_terms == 3 else 1)
-                    grp = g._entry_groups()[0]
-                    if grp.n_items > 0:
-                        _lib.call("pg_apply_grouped_bf16_s24" if s24 else "pg_apply_grouped_bf16", _lib.ptr(yt),
-                                  _lib.ptr(tbl), _lib.ptr(grp.ent_ptr),
-                                  _lib.ptr(grp.ent_rep), _lib.ptr(grp.ent_src), _lib.ptr(grp.ent_s),
-                                  _lib.ptr(grp.items), grp.n_items, int(comp.table.shape[0]), comp.C, n_terms,
-                                  _lib.ptr(vbuf), st.cuda_stream)
-                frames = g._frames(yt, out, 1, vbuf)
-                groups.setdefault((comp.C, comp.G), []).extend(frames)
-                keep += [y, yt, out, vbuf]
-                job.result = out
+                try:
+                    self._stage_job(job, st, groups, keep)
+                except (RuntimeError, ValueError, MemoryError) as exc:
+                    # nothing of this job reaches the shared launch; what it
+                    # already enqueued touches only its own buffers
+                    bad[job] = f"launch failed: {type(exc).__name__}: {exc}"
+                    job.result = None
             for (C, G), frames in groups.items():
                 prog = GridProgram(frames, C, G, self.device, _lib.load().pg_grid_tile_cells(C))
                 prog.launch("pg_grid_fp32" if self.mode == "fp32" else "pg_grid_reduce_v")
                 keep.append(prog)
             ev = torch.cuda.Event()
             ev.record(st)
-        return ev, keep
+        return ev, keep, bad
 
     # -- the loop ------------------------------------------------------------------
     def run(self, jobs: list, *, timeout: float = 600.0) -> dict:
         """Serve ``jobs`` (arrival times relative to the start of the run)
-        and return latency / throughput statistics."""
-        pending = sorted(jobs, key=lambda j: j.arrival)
-        for j in pending:
-            j.est_bytes = estimate_job_bytes(j, self.mode)
-            if j.est_bytes > self.hbm_budget:
-                raise ValueError(f"job {j.job_id} needs ~{j.est_bytes / 1e9:.2f} GB, above the budget "
-                                 f"{self.hbm_budget / 1e9:.2f} GB")
-        used = 0
-        peak_used = 0
-        preparing = {}
-        ready = []
-        inflight = []
-        done = []
-        batches = []
-        pool = self._pool
-        free_slots = list(range(self.slots))
-        t0 = time.perf_counter()
-        try:
-            while len(done) < len(jobs):
-                now = time.perf_counter() - t0
-                if now > timeout:
-                    raise RuntimeError(f"serving run exceeded {timeout:.0f} s")
-                progressed = False
-                # admit in arrival order while budget and a slot are free
-                while pending and pending[0].arrival <= now and free_slots and used + pending[0].est_bytes <= self.hbm_budget:
-                    job = pending.pop(0)
-                    slot = free_slots.pop(0)
-                    used += job.est_bytes
-                    peak_used = max(peak_used, used)
-                    job.t_admit = now
-                    preparing[pool.submit(self._prepare, job, slot)] = (job, slot)
-                    progressed = True
-                # collect finished preparations
-                for fut in [f for f in preparing if f.done()]:
-                    job, slot = preparing.pop(fut)
-                    fut.result()  # re-raise a preparation error
-                    job.t_ready = time.perf_counter() - t0
-                    free_slots.append(slot)
-                    ready.append(job)
-                    progressed = True
-                # continuous batching: everything ready joins the next launch
-                if ready and len(inflight) < self.max_inflight:
-                    batch, ready = ready, []
-                    ev, keep = self._launch_batch(batch)
-                    inflight.append((ev, batch, keep))
-                    batches.append(len(batch))
-                    progressed = True
-                # retire completed batches
-                still = []
-                for ev, batch, keep in inflight:
-                    if ev.query():
-                        t = time.perf_counter() - t0
-                        for job in batch:
-                            job.t_done = t
-                            used -= job.est_bytes
-                            if not self.keep_results:
-                                job.result = None
-                            job.interp = None
-                            done.append(job)
-                        progressed = True
-                    else:
-                        still.append((ev, batch, keep))
-                inflight = still
-                if not progressed:
-                    time.sleep(0.0002)
-        finally:
-            # a failed run leaves no preparation behind
-            cf.wait(list(preparing))
-        makespan = max(j.t_done for j in done) if done else 0.0
-        lat = np.array([j.latency for j in done])
-        cs = sum(j.coils * j.samples for j in done)
-        # per job kind (2D / 3D): latency and where it goes -- queued until
-        # admitted, preparing until the worker thread has enqueued the plan,
-        # table and groups (their device work may still run), then until the
-        # job's gridding batch completes
-        by_kind = {}
-        for nd in sorted({len(j.shape) for j in done}):
-            js = [j for j in done if len(j.shape) == nd]
-            lk = np.array([j.latency for j in js]) * 1e3
-            by_kind[f"{nd}D"] = {
-                "jobs": len(js), "p50_ms": float(np.percentile(lk, 50)), "p99_ms": float(np.percentile(lk, 99)),
-                "queued_ms": float(np.mean([(j.t_admit - j.arrival) * 1e3 for j in js])),
-                "prepare_ms": float(np.mean([(j.t_ready - j.t_admit) * 1e3 for j in js])),
-                "grid_ms": float(np.mean([(j.t_done - j.t_ready) * 1e3 for j in js])),
-            }
-        return {
-            "mode": self.mode, "jobs": len(done), "batches": len(batches),
-            "mean_batch": float(np.mean(batches)) if batches else 0.0,
-            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-            "mean_ms": float(lat.mean() * 1e3), "max_ms": float(lat.max() * 1e3),
-            "throughput_jobs_s": len(done) / makespan if makespan > 0 else 0.0,
-            "coil_samples_per_s": cs / makespan if makespan > 0 else 0.0,
-            "makespan_s": makespan, "hbm_budget_gb": self.hbm_budget / 1e9, "peak_admitted_gb": peak_used / 1e9,
-            "by_kind": by_kind,
+        and return latency / throughput statistics.  Rejected and failed
+        jobs are listed with their errors and left out of the latency
+        figures."""
+
+        def retire(job):
+            if not self.keep_results:
+                job.result = None
+            job.interp = None
+
+        r = serve_loop(jobs, budget=self.hbm_budget, slots=self.slots, max_inflight=self.max_inflight,
+                       estimate=lambda j: estimate_job_bytes(j, self.mode), prepare=self._prepare,
+                       launch=self._launch_batch, submit=self._pool.submit, timeout=timeout, on_retire=retire)
+        return summarize(r, self.mode, self.hbm_budget)
+
+
+def summarize(r: dict, mode: str, budget: int) -> dict:
+    """Latency / throughput statistics of a ``serve_loop`` result."""
+    done, batches, makespan = r["done"], r["batches"], r["makespan"]
+    lat = np.array([j.latency for j in done]) if done else np.zeros(0)
+    cs = sum(j.coils * j.samples for j in done)
+    # per job kind (2D / 3D): latency and where it goes -- queued until
+    # admitted, preparing until the worker thread has enqueued the plan,
+    # table and groups (their device work may still run), then until the
+    # job's gridding batch completes
+    by_kind = {}
+    for nd in sorted({len(j.shape) for j in done}):
+        js = [j for j in done if len(j.shape) == nd]
+        lk = np.array([j.latency for j in js]) * 1e3
+        by_kind[f"{nd}D"] = {
+            "jobs": len(js), "p50_ms": float(np.percentile(lk, 50)), "p99_ms": float(np.percentile(lk, 99)),
+            "queued_ms": float(np.mean([(j.t_admit - j.arrival) * 1e3 for j in js])),
+            "prepare_ms": float(np.mean([(j.t_ready - j.t_admit) * 1e3 for j in js])),
+            "grid_ms": float(np.mean([(j.t_done - j.t_ready) * 1e3 for j in js])),
         }
+
+    def pct(q):
+        return float(np.percentile(lat, q) * 1e3) if lat.size else 0.0
+
+    return {
+        "mode": mode, "jobs": len(done), "batches": len(batches),
+        "rejected": [(j.job_id, j.error) for j in r["rejected"]],
+        "failed": [(j.job_id, j.error) for j in r["failed"]],
+        "mean_batch": float(np.mean(batches)) if batches else 0.0,
+        "p50_ms": pct(50), "p99_ms": pct(99),
+        "mean_ms": float(lat.mean() * 1e3) if lat.size else 0.0, "max_ms": float(lat.max() * 1e3) if lat.size else 0.0,
+        "throughput_jobs_s": len(done) / makespan if makespan > 0 else 0.0,
+        "coil_samples_per_s": cs / makespan if makespan > 0 else 0.0,
+        "makespan_s": makespan, "hbm_budget_gb": budget / 1e9, "peak_admitted_gb": r["peak_used"] / 1e9,
+        "by_kind": by_kind,
+    }

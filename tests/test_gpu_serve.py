@@ -3,6 +3,8 @@ and 3D jobs, each with its own table, admitted by HBM budget and gridded
 in shared launches; every job's result equals the same job gridded alone,
 bitwise, in every mode."""
 
+import dataclasses
+
 import numpy as np
 import pytest
 import torch
@@ -69,5 +71,61 @@ def test_serve_budget_queues_and_completes(device):
         # the same server (and its worker threads) serves a second stream
         again = srv.run(_jobs(device, n=4, seed=8, rate=2000.0), timeout=300)
         assert again["jobs"] == 4
-    with pytest.raises(ValueError, match="budget"):
-        GrogServer(device, mode="fp32", hbm_budget=biggest // 4).run(_jobs(device, n=3, seed=9))
+    # a budget below every job: each is rejected alone, the run still ends
+    with GrogServer(device, mode="fp32", hbm_budget=biggest // 4) as srv:
+        st = srv.run(_jobs(device, n=3, seed=9), timeout=60)
+    assert st["jobs"] == 0 and len(st["rejected"]) == 3 and all("budget" in e for _, e in st["rejected"])
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3"])
+def test_serve_failures_isolated(device, monkeypatch, mode):
+    """Malformed requests (k-space of the wrong length, a stacked
+    trajectory), an over-budget job, a preparation that raises and an
+    injected allocation failure each fail their own job only; every other
+    job's result equals its run alone, bitwise."""
+    from pygrog_amd import synth
+    from pygrog_amd.serve import GrogServer, estimate_job_bytes
+
+    jobs = _jobs(device, n=10, seed=21, rate=1000.0)
+    jobs[1].kspace = jobs[1].kspace[:, :-3].contiguous()
+    jobs[4].coords = np.stack([jobs[4].coords, jobs[4].coords])
+    jobs[6].job_id = 106      # its table fit raises
+    jobs[8].job_id = 108      # its buffers fail to allocate
+    budget = 3 * max(estimate_job_bytes(j, mode) for j in jobs if j.job_id not in (1, 4))
+    big = synth.complex_gaussian((4, 300000), seed=3, device=device)
+    jobs.append(dataclasses.replace(jobs[2], kspace=big, coords=synth.kooshball_3d(1000, 300), job_id=99,
+                                    arrival=0.0))
+    with GrogServer(device, mode=mode, hbm_budget=budget, slots=3, keep_results=True) as srv:
+        real_alloc = srv._alloc
+        current = {}
+
+        def alloc(n, dtype):
+            if current.get("job") == 108:
+                raise RuntimeError("injected: hipMalloc out of memory")
+            return real_alloc(n, dtype)
+
+        real_stage = srv._stage_job
+
+        def stage(job, st, groups, keep):
+            current["job"] = job.job_id
+            return real_stage(job, st, groups, keep)
+
+        real_fit = srv._fit
+
+        def fit(job, slot):
+            if job.job_id == 106:
+                raise RuntimeError("injected: table fit failed")
+            return real_fit(job, slot)
+
+        monkeypatch.setattr(srv, "_fit", fit)
+        monkeypatch.setattr(srv, "_alloc", alloc)
+        monkeypatch.setattr(srv, "_stage_job", stage)
+        stats = srv.run(jobs, timeout=300)
+    status = {j.job_id: j.status for j in jobs}
+    assert {i for i, s in status.items() if s == "rejected"} == {1, 4, 99}, stats["rejected"]
+    assert {i for i, s in status.items() if s == "failed"} == {106, 108}, stats["failed"]
+    assert stats["jobs"] == 6
+    for j in jobs:
+        if j.status == "done":
+            ref = _alone(j, mode, device)
+            assert torch.equal(j.result.reshape(-1), ref.reshape(-1)), f"job {j.job_id} differs from its run alone"
