@@ -3,9 +3,9 @@ src/pygrog/_toep/_grog_toep.py).
 
 ``A^H A x = sum_c conj(S_c) crop(IFFT(PSF * FFT(pad(S_c x))))``, with the PSF
 on the (oversampled) GROG grid: no 2x padding, because GROG samples sit on
-the grid.  PSF = sum of squared sqrt-weights per cell, built once by the
-``psf_scatter_scalar`` HIP kernel (SparseFFT) or taken from the plan density
-(MaskedFFT).  All coils (and stack elements) go through one batched
+the grid.  PSF = sum of squared sqrt-weights per cell: the density of a
+MaskedFFT or of the compiled plan behind a plan-backed SparseFFT, otherwise
+built once by the order-fixed ``psf_scatter_scalar`` HIP kernel.  All coils (and stack elements) go through one batched
 FFT / multiply / IFFT on the GPU.
 """
 
@@ -34,9 +34,15 @@ class GrogToeplitzOp:
         self._pad_slices = sparse_fft._pad_slices
         self.stack_shape = tuple(getattr(sparse_fft, "stack_shape", ()) or ())
         S = int(np.prod(self.stack_shape)) if self.stack_shape else 1
+        comp = getattr(getattr(sparse_fft, "_plan", None), "_compiled", None)
         if getattr(sparse_fft, "density", None) is not None:
             dens = torch.as_tensor(sparse_fft.density).to(self.device).float()
             psf = dens.reshape(S, *self.grid_shape).clone()
+        elif comp is not None and getattr(comp, "density", None) is not None and comp.density.numel() == S * int(
+                np.prod(self.grid_shape)):
+            # a compiled GROG plan already holds sum(sqrt_w^2) per cell, folded
+            # in the plan's sorted order from zero: the scalar PSF, bit for bit
+            psf = comp.density.to(self.device).reshape(S, *self.grid_shape).clone()
         else:
             ext = _lib.get_torch_ext()
             G = int(np.prod(self.grid_shape))

@@ -1100,9 +1100,15 @@ class GrogInterpolator:
         _lib.call("pg_sparse_apply_c64", _lib.ptr(y), _lib.ptr(e_shot), _lib.ptr(comp.table), _lib.ptr(applied), 1, C,
                   npts, kw, int(comp.table.shape[0]), st)
         out = torch.zeros(C, comp.G, dtype=torch.complex64, device=dev)
+        # sorted by cell, replicas of a cell in natural order (a stable sort):
+        # the scatter then folds each cell in the order of the reference's
+        # serial scatter_add_, the same bits on every call
+        t_sorted, order = torch.sort(t_shot, stable=True)
+        applied = applied[:, :, order].contiguous()
+        w_sorted = w_shot[order].contiguous()
         # every coil in one launch
-        _lib.call("pg_scatter_add_rows_c64", _lib.ptr(out), comp.G, _lib.ptr(applied), _lib.ptr(t_shot),
-                  _lib.ptr(w_shot), npts * kw, C, st)
+        _lib.call("pg_scatter_add_rows_c64", _lib.ptr(out), comp.G, _lib.ptr(applied), _lib.ptr(t_sorted),
+                  _lib.ptr(w_sorted), npts * kw, C, st)
         return out.to(data.device)
 
     def collect_shots(self) -> torch.Tensor:

@@ -3,8 +3,9 @@
 Each ``.hip`` file is compiled to an object in parallel, then linked into
 ``pygrog_amd/_lib/libpygrog_hip.so``.  The plan compiler is built with
 ``-ffp-contract=off`` so its float32 arithmetic is rounded op by op (the
-bit-exact target/shift/table-entry contract); every file is built with packed
-FP32 instructions disabled.
+bit-exact target/shift/table-entry contract), and so are the reference's
+seven ops (each product and sum rounded as in the reference's CPU loops);
+every file is built with packed FP32 instructions disabled.
 
 Usage: ``python -m pygrog_amd.csrc.build [--jobs N]``.
 """
@@ -28,7 +29,7 @@ LIB_PATH = LIB_DIR / "libpygrog_hip.so"
 
 ARCH = "gfx950"
 SOURCES = {
-    "ops_ref.hip": [],
+    "ops_ref.hip": ["-ffp-contract=off"],
     "plan.hip": ["-ffp-contract=off"],
     "grid.hip": [],
     "grid_mfma.hip": [],

@@ -4,16 +4,15 @@
 //   grog_interpolate   one thread per (row, output coil); the input row is
 //                      shared by the C threads of a row (one transaction per
 //                      coil), the operator row streams per thread.
-//   scatter_add(_binned), psf_scatter_scalar
-//                      one element per lane; runs of equal indices inside a
-//                      wavefront are reduced with a segmented shuffle scan
-//                      and written with one hardware atomic per run, so the
-//                      radial DC run costs one atomic per 64 samples instead
-//                      of one per sample (the reference CUDA path serialises
-//                      on it, sparse_ops_cuda.cu:27-63).
+//   scatter_add(_binned), psf_scatter_scalar, psf_scatter_outer(_basis)
+//                      order-fixed: the first element of each run of equal
+//                      sorted indices folds the run in input order from the
+//                      cell's value, as the reference's CPU loops do, and
+//                      commits with a compare-and-swap (see below); one
+//                      thread per sample (per sample and PSF element for the
+//                      outer products).  Bitwise the reference's CPU result
+//                      for sorted indices, run to run.
 //   gather             one element per thread.
-//   psf_scatter_outer(_basis)
-//                      one sample per thread, M*M complex atomics.
 #include <cstring>
 
 #include "common.h"
@@ -57,7 +56,21 @@ __global__ void k_grog_interp(const v2_t<T> *__restrict__ data,
 }
 
 // ---------------------------------------------------------------------------
-// Wavefront segmented reduction + one atomic per run.
+// Order-fixed accumulation for the in-place ops (scatter_add, the PSFs).
+//
+// The reference's CPU ops take sorted indices and add each sample into its
+// cell in input order, starting from the cell's current value
+// (sparse_ops_cpu_impl.inl:77-120, toep_psf.cpp:58-118: a clean thread
+// partition keeps every cell on one thread).  Here the first element of
+// every run of equal consecutive indices folds the whole run the same way,
+// from the value it reads, and commits the result with a compare-and-swap.
+// With sorted indices a cell is one run, so the swap always succeeds and the
+// cell holds exactly the reference's sum, whatever the launch shape.  If the
+// swap fails, another run of the same cell wrote it (unsorted input): the
+// run's partial sum, folded from zero, is then added atomically -- the sum
+// is right, its rounding order is not fixed (as with the reference's CUDA
+// atomics).  The file is built with -ffp-contract=off: every product and
+// sum rounds separately, as in the reference's loops.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void atomic_add_val(float *p, float v) {
   unsafeAtomicAdd(p, v);
@@ -66,47 +79,60 @@ __device__ __forceinline__ void atomic_add_val(double *p, double v) {
   unsafeAtomicAdd(p, v);
 }
 
-// NV = number of scalar components per element (2 complex, 1 real).
-template <class R, int NV>
-struct Vals {
-  R v[NV];
-};
+// the cell's current value, read at device scope (not a stale cached copy)
+template <class R>
+__device__ __forceinline__ R coherent_load(R *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
-template <class R, int NV>
-__device__ __forceinline__ void run_reduce_store(int64_t key, Vals<R, NV> val,
-                                                 R *grid) {
-  const int lane = threadIdx.x & (kWave - 1);
-  // Head flags -> segment ids (inclusive prefix count of heads).
-  int64_t prev = __shfl_up(key, 1, kWave);
-  int head = (lane == 0 || prev != key) ? 1 : 0;
-  int seg = head;
+__device__ __forceinline__ bool cas_commit(float *p, float seen, float v) {
+  const unsigned int e = __float_as_uint(seen);
+  return atomicCAS(reinterpret_cast<unsigned int *>(p), e, __float_as_uint(v)) == e;
+}
+__device__ __forceinline__ bool cas_commit(double *p, double seen, double v) {
+  const unsigned long long e = (unsigned long long)__double_as_longlong(seen);
+  return atomicCAS(reinterpret_cast<unsigned long long *>(p), e,
+                   (unsigned long long)__double_as_longlong(v)) == e;
+}
+
+// Run heads: the first element of a run of equal in-range indices.
+__device__ __forceinline__ bool run_head(const int64_t *__restrict__ idx,
+                                         int64_t i, int64_t G, int64_t &k) {
+  k = idx[i];
+  if (k < 0 || k >= G) return false;
+  return i == 0 || idx[i - 1] != k;
+}
+__device__ __forceinline__ int64_t run_end(const int64_t *__restrict__ idx,
+                                           int64_t i, int64_t n, int64_t k) {
+  int64_t e = i + 1;
+  while (e < n && idx[e] == k) ++e;
+  return e;
+}
+
+// Fold NV components over the run [i, e): acc[v] = acc[v] + term(j, v) in j
+// order from the cell's values at p[0..NV), commit each with a swap, and on a
+// lost swap add the partial instead.
+template <class R, int NV, class Term>
+__device__ __forceinline__ void fold_commit(R *p, int64_t i, int64_t e, Term term) {
+  R seen[NV], acc[NV];
 #pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    int o = __shfl_up(seg, off, kWave);
-    if (lane >= off) seg += o;
+  for (int v = 0; v < NV; ++v) acc[v] = seen[v] = coherent_load(p + v);
+  for (int64_t j = i; j < e; ++j) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = acc[v] + term(j, v);
   }
-  // Hillis-Steele inclusive segmented scan of the values.
 #pragma unroll
-  for (int off = 1; off < kWave; off <<= 1) {
-    int oseg = __shfl_up(seg, off, kWave);
-    Vals<R, NV> o;
-#pragma unroll
-    for (int j = 0; j < NV; ++j) o.v[j] = __shfl_up(val.v[j], off, kWave);
-    if (lane >= off && oseg == seg) {
-#pragma unroll
-      for (int j = 0; j < NV; ++j) val.v[j] = o.v[j] + val.v[j];
+  for (int v = 0; v < NV; ++v) {
+    if (!cas_commit(p + v, seen[v], acc[v])) {
+      R part = R(0);
+      for (int64_t j = i; j < e; ++j) part = part + term(j, v);
+      atomic_add_val(p + v, part);
     }
-  }
-  int nseg = __shfl_down(seg, 1, kWave);
-  const bool tail = (lane == kWave - 1) || (nseg != seg);
-  if (tail && key >= 0) {
-#pragma unroll
-    for (int j = 0; j < NV; ++j) atomic_add_val(grid + key * NV + j, val.v[j]);
   }
 }
 
-// grid[idx[i]] += w[i] * data[i]; complex (NV = 2) with real weights.
-// blockIdx.y selects a row: grid (rows, G) and data (rows, n) share idx, w.
+// grid[idx[i]] += w[i] * data[i]; complex with real weights.  blockIdx.y
+// selects a row: grid (rows, G) and data (rows, n) share idx and w.
 template <class R>
 __global__ void k_scatter_add(R *__restrict__ grid, int64_t G,
                               const R *__restrict__ data,
@@ -114,26 +140,13 @@ __global__ void k_scatter_add(R *__restrict__ grid, int64_t G,
                               const R *__restrict__ w, int64_t n) {
   grid += (int64_t)blockIdx.y * 2 * G;
   data += (int64_t)blockIdx.y * 2 * n;
-  const int64_t nwaves_total = (int64_t)gridDim.x * (blockDim.x / kWave);
-  const int64_t wave0 = blockIdx.x * (int64_t)(blockDim.x / kWave) +
-                        threadIdx.x / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
-  for (int64_t base = wave0 * kWave; base < n; base += nwaves_total * kWave) {
-    const int64_t i = base + lane;
-    int64_t key = -1;
-    Vals<R, 2> v;
-    v.v[0] = R(0);
-    v.v[1] = R(0);
-    if (i < n) {
-      const int64_t k = idx[i];
-      if (k >= 0 && k < G) {
-        key = k;
-        const R wi = w[i];
-        v.v[0] = wi * data[2 * i];
-        v.v[1] = wi * data[2 * i + 1];
-      }
-    }
-    run_reduce_store<R, 2>(key, v, grid);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t k;
+    if (!run_head(idx, i, G, k)) continue;
+    const int64_t e = run_end(idx, i, n, k);
+    fold_commit<R, 2>(grid + 2 * k, i, e,
+                      [&](int64_t j, int v) { return w[j] * data[2 * j + v]; });
   }
 }
 
@@ -142,23 +155,12 @@ template <class R>
 __global__ void k_psf_scalar(R *__restrict__ psf, int64_t G,
                              const int64_t *__restrict__ idx,
                              const R *__restrict__ w, int64_t n) {
-  const int64_t nwaves_total = (int64_t)gridDim.x * (blockDim.x / kWave);
-  const int64_t wave0 = blockIdx.x * (int64_t)(blockDim.x / kWave) +
-                        threadIdx.x / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
-  for (int64_t base = wave0 * kWave; base < n; base += nwaves_total * kWave) {
-    const int64_t i = base + lane;
-    int64_t key = -1;
-    Vals<R, 1> v;
-    v.v[0] = R(0);
-    if (i < n) {
-      const int64_t k = idx[i];
-      if (k >= 0 && k < G) {
-        key = k;
-        v.v[0] = w[i];
-      }
-    }
-    run_reduce_store<R, 1>(key, v, psf);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t k;
+    if (!run_head(idx, i, G, k)) continue;
+    const int64_t e = run_end(idx, i, n, k);
+    fold_commit<R, 1>(psf + k, i, e, [&](int64_t j, int) { return w[j]; });
   }
 }
 
@@ -182,32 +184,34 @@ __global__ void k_gather(const R *__restrict__ grid, int64_t G,
   }
 }
 
-// psf[idx, l, l'] += w * conj(b_l) * b_l' (reference toep_psf.cpp:124-175).
+// psf[idx, l, l'] += (conj(b_l) * w) * b_l' (reference toep_psf.cpp:124-175):
+// one thread per (sample, l, l'); the run head's threads fold the run.
 template <class R>
 __global__ void k_psf_outer(R *__restrict__ psf, int64_t G, int64_t M,
                             const int64_t *__restrict__ idx,
                             const R *__restrict__ w,
                             const R *__restrict__ basis, int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t k = idx[i];
-    if (k < 0 || k >= G) continue;
-    const R wi = w[i];
-    const R *b = basis + 2 * i * M;
-    R *p = psf + 2 * k * M * M;
-    for (int64_t l = 0; l < M; ++l) {
-      // cb = conj(b_l) * w
-      const R cr = b[2 * l] * wi, ci = -b[2 * l + 1] * wi;
-      for (int64_t lp = 0; lp < M; ++lp) {
-        const R br = b[2 * lp], bi = b[2 * lp + 1];
-        atomic_add_val(p + 2 * (l * M + lp), cr * br - ci * bi);
-        atomic_add_val(p + 2 * (l * M + lp) + 1, cr * bi + ci * br);
-      }
-    }
+  const int64_t MM = M * M;
+  const int64_t total = n * MM;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / MM;
+    const int64_t el = t - i * MM;
+    int64_t k;
+    if (!run_head(idx, i, G, k)) continue;
+    const int64_t e = run_end(idx, i, n, k);
+    const int64_t l = el / M, lp = el - l * M;
+    fold_commit<R, 2>(psf + 2 * (k * MM + el), i, e, [&](int64_t j, int v) {
+      const R *b = basis + 2 * j * M;
+      const R cr = b[2 * l] * w[j], ci = -b[2 * l + 1] * w[j];
+      const R br = b[2 * lp], bi = b[2 * lp + 1];
+      return v == 0 ? cr * br - ci * bi : cr * bi + ci * br;
+    });
   }
 }
 
-// psf[idx, k, k'] += w * conj(B[k,t]) * B[k',t] (toep_psf.cpp:181-237).
+// psf[idx, k, k'] += (conj(B[k,t]) * w) * B[k',t] (toep_psf.cpp:181-237);
+// samples whose time index is out of range add nothing.
 template <class R>
 __global__ void k_psf_outer_basis(R *__restrict__ psf, int64_t G, int64_t K,
                                   const int64_t *__restrict__ idx,
@@ -215,21 +219,47 @@ __global__ void k_psf_outer_basis(R *__restrict__ psf, int64_t G, int64_t K,
                                   const R *__restrict__ basis, int64_t T,
                                   const int64_t *__restrict__ tidx,
                                   int64_t n) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t g = idx[i];
-    if (g < 0 || g >= G) continue;
-    const int64_t t = tidx[i];
-    if (t < 0 || t >= T) continue;
-    const R wi = w[i];
-    R *p = psf + 2 * g * K * K;
-    for (int64_t k = 0; k < K; ++k) {
-      const R cr = basis[2 * (k * T + t)] * wi;
-      const R ci = -basis[2 * (k * T + t) + 1] * wi;
-      for (int64_t kp = 0; kp < K; ++kp) {
-        const R br = basis[2 * (kp * T + t)], bi = basis[2 * (kp * T + t) + 1];
-        atomic_add_val(p + 2 * (k * K + kp), cr * br - ci * bi);
-        atomic_add_val(p + 2 * (k * K + kp) + 1, cr * bi + ci * br);
+  const int64_t KK = K * K;
+  const int64_t total = n * KK;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t / KK;
+    const int64_t el = t - i * KK;
+    int64_t g;
+    if (!run_head(idx, i, G, g)) continue;
+    const int64_t e = run_end(idx, i, n, g);
+    const int64_t a = el / K, ap = el - a * K;
+    R *p = psf + 2 * (g * KK + el);
+    // a sample with its time index out of range is skipped, not added as 0
+    // (0 + x and x differ for x = -0)
+    R seen[2], acc[2];
+    acc[0] = seen[0] = coherent_load(p);
+    acc[1] = seen[1] = coherent_load(p + 1);
+    auto term = [&](int64_t j, R &re, R &im) {
+      const int64_t tt = tidx[j];
+      if (tt < 0 || tt >= T) return false;
+      const R cr = basis[2 * (a * T + tt)] * w[j];
+      const R ci = -basis[2 * (a * T + tt) + 1] * w[j];
+      const R br = basis[2 * (ap * T + tt)], bi = basis[2 * (ap * T + tt) + 1];
+      re = cr * br - ci * bi;
+      im = cr * bi + ci * br;
+      return true;
+    };
+    for (int64_t j = i; j < e; ++j) {
+      R re, im;
+      if (term(j, re, im)) {
+        acc[0] = acc[0] + re;
+        acc[1] = acc[1] + im;
+      }
+    }
+    for (int v = 0; v < 2; ++v) {
+      if (!cas_commit(p + v, seen[v], acc[v])) {
+        R part = R(0);
+        for (int64_t j = i; j < e; ++j) {
+          R re, im;
+          if (term(j, re, im)) part = part + (v == 0 ? re : im);
+        }
+        atomic_add_val(p + v, part);
       }
     }
   }
