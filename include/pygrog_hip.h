@@ -492,6 +492,22 @@ int pg_vdot_c64_f64(const void *a, const void *b, int64_t n, void *workspace,
                     double *out, pg_stream_t stream);
 int64_t pg_vdot_workspace_bytes(void);
 
+/* pg_vdot_c64_f64 without a workspace and without allocating: the same
+ * kernels and the same bits, with the block partials in a 16 KiB array the
+ * library holds in device memory (one per GPU, kept until the process ends;
+ * its address is looked up once per device, with that device current).  No
+ * counterpart in the reference: for callers that cannot allocate (code under
+ * HIP graph capture, tight solver or serving loops).  n is limited to
+ * PG_VDOT_NOALLOC_MAX_N (larger n is rejected with an error); the scratch
+ * need does not grow with n (at most 1024 partials).  Every call on one GPU
+ * shares that GPU's array and leaves its last contents there: calls that
+ * overlap on one GPU (different streams, or host threads) race -- serialise
+ * them, e.g. on one stream.  The pointers must belong to the current
+ * device. */
+#define PG_VDOT_NOALLOC_MAX_N ((int64_t)1 << 40)
+int pg_vdot_c64_f64_noalloc(const void *a, const void *b, int64_t n,
+                            double *out, pg_stream_t stream);
+
 /* interpolate(ret_image=False) fused: out[b, co, n*kw + k] =
  *   K[interp_idx[n, k]] @ y[b, :, n]  for every candidate replica
  * (including invalid ones, as the reference does).  y (B, C, N) c64,

@@ -123,6 +123,7 @@ _SIGS = {
     "pg_transpose_cn_s24": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
     "pg_transpose_frames_s24": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p],
     "pg_vdot_c64_f64": [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p],
+    "pg_vdot_c64_f64_noalloc": [c_void_p, c_void_p, c_int64, c_void_p, c_void_p],
     "pg_vdot_workspace_bytes": [],
     "pg_sense_expand_c64": [c_void_p, c_void_p, c_int64, c_int64, ctypes.c_int32, c_void_p, c_void_p,
                             ctypes.c_int32, c_void_p, c_void_p],

@@ -18,6 +18,8 @@
 //            (coils in order); without maps out[b, c, p] = sign * s(p + off) * full[b, c, p + off]
 // with s(g) = (-1)^(g0 + g1 + g2) when modulate, else 1, and off = (grid -
 // image) // 2 per axis (the reference's centred crop / pad).
+#include <mutex>
+
 #include "common.h"
 
 namespace pg {
@@ -234,27 +236,73 @@ __global__ void k_vdot_final(const double2 *__restrict__ part, int nb,
 
 }  // namespace pg
 
-extern "C" {
+namespace pg {
 
-int64_t pg_vdot_workspace_bytes(void) { return (int64_t)kDotBlocks * 16; }
+// Block partials of pg_vdot_c64_f64_noalloc: the library's own device memory
+// (one copy per GPU, held until the process ends) instead of a caller
+// workspace.  Host code never uses the symbol itself as an address: the
+// device address of each GPU's copy comes from hipGetSymbolAddress, taken
+// once per device with that device current.
+static __device__ __align__(256) double2 vdot_scratch[kDotBlocks];
 
-int pg_vdot_c64_f64(const void *a, const void *b, int64_t n, void *workspace,
-                    double *out, pg_stream_t stream) {
-  PG_CHECK(n >= 0 && out && workspace, "vdot: bad arguments");
-  PG_CHECK(n == 0 || (a && b), "vdot: null vectors");
-  const hipStream_t st = as_stream(stream);
+constexpr int kMaxDevices = 64;
+static std::mutex g_vdot_scratch_mu;
+static void *g_vdot_scratch_ptr[kMaxDevices] = {nullptr};
+
+static int vdot_scratch_address(void **out) {
+  int dev = -1;
+  PG_HIP(hipGetDevice(&dev));
+  PG_CHECK(dev >= 0 && dev < kMaxDevices, "vdot_noalloc: device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_vdot_scratch_mu);
+  if (g_vdot_scratch_ptr[dev] == nullptr) {
+    void *p = nullptr;
+    PG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(vdot_scratch)));
+    PG_CHECK(p != nullptr, "vdot_noalloc: no device address for the scratch array");
+    g_vdot_scratch_ptr[dev] = p;
+  }
+  *out = g_vdot_scratch_ptr[dev];
+  return 0;
+}
+
+static int vdot_launch(const void *a, const void *b, int64_t n, void *part,
+                       double *out, hipStream_t st) {
   int nb = (int)((n + kDotThreads - 1) / kDotThreads);
   if (nb > kDotBlocks) nb = kDotBlocks;
   if (nb < 1) nb = 1;
   hipLaunchKernelGGL(k_vdot_partial, dim3(nb), dim3(kDotThreads), 0, st,
                      static_cast<const float2 *>(a),
                      static_cast<const float2 *>(b), n,
-                     static_cast<double2 *>(workspace));
+                     static_cast<double2 *>(part));
   PG_LAUNCHED();
   hipLaunchKernelGGL(k_vdot_final, dim3(1), dim3(64), 0, st,
-                     static_cast<const double2 *>(workspace), nb, out);
+                     static_cast<const double2 *>(part), nb, out);
   PG_LAUNCHED();
   return 0;
+}
+
+}  // namespace pg
+
+extern "C" {
+
+int64_t pg_vdot_workspace_bytes(void) { return (int64_t)kDotBlocks * 16; }
+
+int pg_vdot_c64_f64_noalloc(const void *a, const void *b, int64_t n,
+                            double *out, pg_stream_t stream) {
+  PG_CHECK(n >= 0 && out, "vdot_noalloc: bad arguments");
+  PG_CHECK(n <= PG_VDOT_NOALLOC_MAX_N,
+           "vdot_noalloc: n = %lld exceeds the maximum %lld of the allocation-free entry",
+           (long long)n, (long long)PG_VDOT_NOALLOC_MAX_N);
+  PG_CHECK(n == 0 || (a && b), "vdot_noalloc: null vectors");
+  void *part = nullptr;
+  if (int rc = vdot_scratch_address(&part)) return rc;
+  return vdot_launch(a, b, n, part, out, as_stream(stream));
+}
+
+int pg_vdot_c64_f64(const void *a, const void *b, int64_t n, void *workspace,
+                    double *out, pg_stream_t stream) {
+  PG_CHECK(n >= 0 && out && workspace, "vdot: bad arguments");
+  PG_CHECK(n == 0 || (a && b), "vdot: null vectors");
+  return vdot_launch(a, b, n, workspace, out, as_stream(stream));
 }
 
 }  // extern "C"
