@@ -58,9 +58,14 @@ int pg_grog_interpolate_c128(const void *data, const int64_t *indexes,
                              pg_stream_t stream);
 
 /* grid[indices[i]] += weights[i] * data[i], in place; indices outside
- * [0, grid_size) are skipped.  Correct for any index order; equal indices
- * that are adjacent are reduced in registers first, so sorted input costs
- * one atomic per run per wavefront.
+ * [0, grid_size) are skipped.  Order-fixed: the first element of each run of
+ * equal consecutive indices adds the run's terms one by one, in input order,
+ * to the cell's current value (each product and sum rounded separately) and
+ * commits the result with a compare-and-swap.  With sorted indices -- the
+ * reference CPU op's contract -- that is exactly the reference's arithmetic
+ * (sparse_ops_cpu_impl.inl:77-120), bit for bit on every call.  Unsorted
+ * input stays correct: a run that loses the swap to another run of the same
+ * cell adds its partial sum atomically (order then not fixed).
  * Replaces scatter_add (csrc/torch/sparse_ops.cpp:233-267). */
 int pg_scatter_add_c64(void *grid, int64_t grid_size, const void *data,
                        const int64_t *indices, const float *weights,
@@ -68,18 +73,21 @@ int pg_scatter_add_c64(void *grid, int64_t grid_size, const void *data,
 int pg_scatter_add_c128(void *grid, int64_t grid_size, const void *data,
                         const int64_t *indices, const double *weights,
                         int64_t n, pg_stream_t stream);
-/* pg_scatter_add_c64 on `rows` grids at once: grid (rows, grid_size) and
- * data (rows, n) c64 share indices and weights; row r is
- * pg_scatter_add_c64(grid[r], data[r]) (same atomics).  Serves the
- * per-coil loop of the shot-wise gridding (reference _grog.py:437-555,
- * interpolate with shot_index). */
+/* pg_scatter_add_c64 on `rows` grids at once (rows <= 65535): grid (rows,
+ * grid_size) and data (rows, n) share indices and weights; row r is
+ * pg_scatter_add_c64(grid[r], data[r]), the same bits.  Serves the per-coil
+ * loops of the shot-wise gridding (reference _grog.py:697-731) and of
+ * interpolate(grid=True) (_grog.py:763-821) in complex128. */
 int pg_scatter_add_rows_c64(void *grid, int64_t grid_size, const void *data,
                             const int64_t *indices, const float *weights,
                             int64_t n, int64_t rows, pg_stream_t stream);
+int pg_scatter_add_rows_c128(void *grid, int64_t grid_size, const void *data,
+                             const int64_t *indices, const double *weights,
+                             int64_t n, int64_t rows, pg_stream_t stream);
 
 /* Same contract as pg_scatter_add_*; `bin_starts` (n_bin_starts int64, the
  * searchsorted bin edges of the reference) is validated but not needed by
- * the wavefront-run reduction.
+ * the run fold.
  * Replaces scatter_add_binned (csrc/torch/sparse_ops.cpp:299-340). */
 int pg_scatter_add_binned_c64(void *grid, int64_t grid_size, const void *data,
                               const int64_t *indices, const float *weights,
@@ -101,7 +109,10 @@ int pg_gather_c128(const void *grid, int64_t grid_size,
                    const int64_t *indices, const double *weights, int64_t n,
                    void *out, pg_stream_t stream);
 
-/* psf[indices[i]] += w_sq[i], in place (real PSF).
+/* psf[indices[i]] += w_sq[i], in place (real PSF).  The three PSF builders
+ * fold like pg_scatter_add_c64: for sorted indices each cell (and each PSF
+ * element) equals the reference's in-order CPU sum bit for bit
+ * (toep_psf.cpp:58-237); unsorted input stays correct, order not fixed.
  * Replaces psf_scatter_scalar (csrc/torch/toep_psf.cpp:242-274). */
 int pg_psf_scatter_scalar_f32(float *psf, int64_t grid_size,
                               const int64_t *indices, const float *w_sq,

@@ -22,37 +22,42 @@ import numpy as np
 from . import ops
 
 
-def sparse_reference(y: np.ndarray, interp_idx: np.ndarray, table: np.ndarray, kw: int) -> np.ndarray:
-    """y (C, N) -> sparse (C, N*kw): every replica (n, k) gets K[interp_idx] @ y[:, n]."""
+def sparse_reference(y: np.ndarray, interp_idx: np.ndarray, table: np.ndarray, kw: int,
+                     dtype=np.complex64) -> np.ndarray:
+    """y (C, N) -> sparse (C, N*kw): every replica (n, k) gets K[interp_idx] @ y[:, n]
+    (complex64, or complex128 with ``dtype``)."""
     C, N = y.shape
     rep = np.repeat(y.T[:, None, :], kw, axis=1).reshape(N * kw, 1, C)
-    out = ops.interpolate(rep.astype(np.complex64), interp_idx.reshape(-1), table.astype(np.complex64))
+    out = ops.interpolate(rep.astype(dtype), interp_idx.reshape(-1), table.astype(dtype))
     return np.ascontiguousarray(out[:, 0, :].T)
 
 
-def grid_reference(y, plan: dict, interp_idx, table, cells: np.ndarray | None = None):
-    """-> (grid (C, G) complex64, density (G,) float32).
+def grid_reference(y, plan: dict, interp_idx, table, cells: np.ndarray | None = None, dtype=np.complex64):
+    """-> (grid (C, G), density (G,) float32), the grid in ``dtype``
+    (complex64, or complex128: the same flow with float64 weights, as the
+    reference's ops dispatch on the data type).
 
-    With ``cells``: only those cells, -> (C, len(cells)) complex64 (columns
-    in the given order).  Only the replicas landing there are formed, with
-    the same per-replica arithmetic, and each cell is summed over its
-    replicas in the same (sorted) order as the full flow, so the columns
-    equal the full grid's bit for bit."""
+    With ``cells`` (complex64 only): only those cells, -> (C, len(cells))
+    complex64 (columns in the given order).  Only the replicas landing there
+    are formed, with the same per-replica arithmetic, and each cell is
+    summed over its replicas in the same (sorted) order as the full flow, so
+    the columns equal the full grid's bit for bit."""
     if cells is not None:
         cells = np.asarray(cells, dtype=np.int64)
         if np.unique(cells).size != cells.size:
             raise ValueError("cells must be distinct")
         return _grid_reference_cells(y, plan, interp_idx, table, cells)
     kw = plan["offsets"].shape[0]
-    sparse = sparse_reference(np.asarray(y, dtype=np.complex64), interp_idx, table, kw)
-    pre_w = plan["sqrt_weights"][plan["inv_perm"]].astype(np.complex64)
-    sparse_dc = (sparse * pre_w[None, :]).astype(np.complex64)
+    rt = np.float32 if dtype == np.complex64 else np.float64
+    sparse = sparse_reference(np.asarray(y, dtype=dtype), interp_idx, table, kw, dtype)
+    pre_w = plan["sqrt_weights"][plan["inv_perm"]].astype(dtype)
+    sparse_dc = (sparse * pre_w[None, :]).astype(dtype)
     G = int(np.prod(plan["grid_shape"]))
     C = sparse.shape[0]
-    grid = np.zeros((C, G), dtype=np.complex64)
+    grid = np.zeros((C, G), dtype=dtype)
     for c in range(C):
-        g = np.zeros(G, dtype=np.complex64)
-        ops.scatter_add(g, sparse_dc[c][plan["sort_perm"]], plan["indices"], plan["sqrt_weights"])
+        g = np.zeros(G, dtype=dtype)
+        ops.scatter_add(g, sparse_dc[c][plan["sort_perm"]], plan["indices"], plan["sqrt_weights"].astype(rt))
         grid[c] = g
     from .plan import density
 

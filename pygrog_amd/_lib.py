@@ -6,7 +6,8 @@ The reference binds seven ops through a pybind11 module ``_pygrog_torch``
 Here the same names and argument meanings are served by :class:`TorchExt`,
 whose methods validate shapes and dtypes with the reference's messages and then
 call the C ABI of ``include/pygrog_hip.h`` on HIP device tensors, enqueued on
-torch's current stream.
+torch's current stream; ``get_torch_ext()`` returns them registered as torch
+custom operators (``pygrog_amd._torch_ops``).
 
 There is no CPU implementation behind these ops: if the shared library is
 missing, or a tensor is not on a HIP device, the call raises.
@@ -83,6 +84,7 @@ _SIGS = {
     "pg_scatter_add_c64": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p],
     "pg_scatter_add_c128": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p],
     "pg_scatter_add_rows_c64": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p],
+    "pg_scatter_add_rows_c128": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_void_p],
     "pg_scatter_add_binned_c64": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int64, c_void_p],
     "pg_scatter_add_binned_c128": [c_void_p, c_int64, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_int64, c_void_p],
     "pg_gather_c64": [c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p, c_void_p],
@@ -408,10 +410,15 @@ class TorchExt:
 _ext = TorchExt()
 
 
-def get_torch_ext() -> TorchExt:
-    """Return the op facade after making sure the HIP library loads."""
+def get_torch_ext():
+    """The drop-in for the reference's ``_get_torch_ext()``: the seven ops as
+    torch custom operators (``torch.ops.pygrog_amd``, registered on first
+    use; see ``pygrog_amd._torch_ops``), after making sure the HIP library
+    loads."""
     load()
-    return _ext
+    from . import _torch_ops
+
+    return _torch_ops.ops()
 
 
 # Workspace helper for the two-phase hipCUB entry points.

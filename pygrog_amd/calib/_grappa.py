@@ -51,6 +51,10 @@ class KernelTableResult:
     stepsize: float
     host_steps: list = field(default_factory=list)
     table64: torch.Tensor | None = None  # complex128 table (kept on request)
+    # the FP64 per-axis powers the table is assembled from (small: ndim x
+    # nsteps x C x C), kept so the complex128 table can be rebuilt on demand
+    powers64: tuple | None = None
+    axis_order: str = "reference"
 
 
 def _to_dev(x, device, dtype):
@@ -326,7 +330,8 @@ def build_table(train_data, radius: float, precision: int | None = 1, lamda: flo
         ndim = 2
     t64 = prepare_grappa_table(Gx, Gy, Gz, nsteps, ndim, axis_order=axis_order)
     t32 = t64.to(torch.complex64).contiguous()
-    return KernelTableResult(t32, nsteps, ndim, stepsize, host_steps, t64 if keep_fp64 else None)
+    return KernelTableResult(t32, nsteps, ndim, stepsize, host_steps, t64 if keep_fp64 else None,
+                             powers64=(Gx, Gy, Gz), axis_order=axis_order)
 
 
 def KernelTable(train_data, radius: float, precision: int = 1, lamda: float = 0.01, *, quant=None, device=None,

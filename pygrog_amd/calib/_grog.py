@@ -744,6 +744,9 @@ class GrogInterpolator:
         comp.C = C
         comp.table = res.table
         comp.table64 = res.table64
+        comp.powers64 = getattr(res, "powers64", None)
+        comp.table_nsteps = int(nsteps)
+        comp.table_axis_order = getattr(res, "axis_order", "reference")
         comp.table_host_steps = res.host_steps
         if C <= 64:
             if packed is None:
@@ -1043,6 +1046,25 @@ class GrogInterpolator:
             raise ValueError(f"data has {C} coils but the table was trained for {comp.C}")
         B = int(np.prod(batch_shape)) if batch_shape else 1
         s_shape = tuple(plan.stack_shape)
+        if data_t.dtype == torch.complex128:
+            # complex128 data is computed in complex128, with the FP64 table
+            if mode != "fp32":
+                raise ValueError(f"mode {mode!r} computes in complex64; complex128 data is gridded in "
+                                 "complex128 (mode='fp32')")
+            out = self._interpolate_c128(data_t, batch_shape, B, C, ret_image, grid)
+            if grid:
+                kspace, mplan = out
+                if is_numpy:
+                    from ..operator._masked_fft import MaskedFFTPlan
+
+                    return np.asarray(kspace.cpu()), MaskedFFTPlan(
+                        grid_shape=mplan.grid_shape, image_shape=mplan.image_shape, stack_shape=s_shape,
+                        mask=np.asarray(mplan.mask.cpu()), density=np.asarray(mplan.density.cpu()))
+                mplan.mask = mplan.mask.to(src_device)
+                mplan.density = mplan.density.to(src_device)
+                return kspace.to(src_device), mplan
+            out = out.to(src_device)
+            return out.numpy() if is_numpy else out
         y = self._to_device_c64(data_t, B, comp.S, C)
 
         if ret_image or grid:
@@ -1078,6 +1100,83 @@ class GrogInterpolator:
         out = sparse.reshape(*batch_shape, *s_shape, C, comp.nper).to(out_dtype).to(src_device)
         return out.numpy() if is_numpy else out
 
+    # -- complex128 (the reference keeps the data's precision end to end,
+    # csrc/torch/grog_interp.cpp:112-149, calib/_grappa.py:62-72) ----------------
+    def _table64(self) -> torch.Tensor:
+        """The (K, C, C) complex128 table: kept at fit time or assembled
+        (once) from the FP64 per-axis powers of the fit."""
+        from ._grappa import prepare_grappa_table
+
+        comp = self.plan._compiled
+        if getattr(comp, "table64", None) is None:
+            if getattr(comp, "powers64", None) is None:
+                raise RuntimeError("complex128 data needs the FP64 table: this table was set without its FP64 "
+                                   "powers (fit it with calc_interp_table)")
+            Gx, Gy, Gz = comp.powers64
+            comp.table64 = prepare_grappa_table(Gx, Gy, Gz, comp.table_nsteps, comp.ndim,
+                                                axis_order=comp.table_axis_order).to(comp.device).contiguous()
+        return comp.table64
+
+    def _apply_c128(self, y: torch.Tensor, s: int) -> torch.Tensor:
+        """Every candidate replica of stack element s, in complex128, the
+        reference flow (_grog.py:668-695): y (B, C, N) -> (B, C, N * kw)."""
+        comp = self.plan._compiled
+        B, C, N = y.shape
+        kw = comp.kw
+        rows = y.permute(2, 0, 1).repeat_interleave(kw, dim=0).contiguous()  # (N * kw, B, C)
+        T = self._table64()
+        out = torch.empty_like(rows)
+        idx = comp.interp_idx[s].contiguous()
+        _lib.call("pg_grog_interpolate_c128", _lib.ptr(rows), _lib.ptr(idx), _lib.ptr(T), _lib.ptr(out), N * kw, B,
+                  C, int(T.shape[0]), _lib.stream(comp.device))
+        return out.permute(1, 2, 0)
+
+    def _interpolate_c128(self, data: torch.Tensor, batch_shape, B: int, C: int, ret_image: bool, grid: bool):
+        plan = self.plan
+        comp = plan._compiled
+        dev = comp.device
+        S, G = comp.S, comp.G
+        s_shape = tuple(plan.stack_shape)
+        y = data.to(dev).reshape(B, S, C, comp.npts)
+        sparse = torch.empty(B, S, C, comp.nper, dtype=torch.complex128, device=dev)
+        for s in range(S):
+            sparse[:, s] = self._apply_c128(y[:, s].contiguous(), s)
+        if not (grid or ret_image):
+            return sparse.reshape(*batch_shape, *s_shape, C, comp.nper)
+        # pre-weights, sorted order, then the order-fixed scatter of every
+        # (batch, coil) row with the sqrt weights (_grog.py:763-821)
+        idx = self.plan._dev["indices"].reshape(S, comp.nper)
+        sqw = self.plan._dev["sqrt_weights"].reshape(S, comp.nper).double()
+        sp = self.plan._dev["sort_perm"].reshape(S, comp.nper)
+        ip = self.plan._dev["inv_perm"].reshape(S, comp.nper)
+        g = torch.zeros(B, S, C, G, dtype=torch.complex128, device=dev)
+        st = _lib.stream(dev)
+        for s in range(S):
+            pre_w = sqw[s][ip[s]].to(torch.complex128)
+            rows = (sparse[:, s] * pre_w)[:, :, sp[s]].reshape(B * C, comp.nper).contiguous()
+            gs = torch.zeros(B * C, G, dtype=torch.complex128, device=dev)
+            w = sqw[s].contiguous()
+            ix = idx[s].contiguous()
+            for r0 in range(0, B * C, 65535):
+                r1 = min(r0 + 65535, B * C)
+                _lib.call("pg_scatter_add_rows_c128", _lib.ptr(gs[r0:r1]), G, _lib.ptr(rows[r0:r1]), _lib.ptr(ix),
+                          _lib.ptr(w), comp.nper, r1 - r0, st)
+            g[:, s] = gs.reshape(B, C, G)
+        gshape = tuple(plan.grid_shape)
+        if ret_image:
+            from .._base._fftc import ifft
+            from .._utils import resize
+
+            img = ifft(g.reshape(B, S, C, *gshape), axes=tuple(range(-len(gshape), 0)))
+            img = resize(img, (B, S, C, *plan.image_shape))
+            return img.abs().square().sum(2).sqrt().reshape(*batch_shape, *s_shape, *plan.image_shape)
+        from ..operator._masked_fft import MaskedFFTPlan
+
+        density = comp.density.reshape(*s_shape, *gshape)
+        mplan = MaskedFFTPlan(grid_shape=gshape, image_shape=tuple(plan.image_shape), stack_shape=s_shape,
+                              mask=(density > 0).float(), density=density)
+        return g.reshape(*batch_shape, *s_shape, C, *gshape), mplan
+
     def __call__(self, data, shot_index=None, ret_image=False, grid=False):
         return self.interpolate(data, shot_index, ret_image, grid)
 
@@ -1094,6 +1193,21 @@ class GrogInterpolator:
         kw = comp.kw
         npts = t_shot.numel() // kw
         C = int(data.shape[0])
+        if data.dtype == torch.complex128:
+            # the reference shot flow in complex128 (operator, weight, serial
+            # scatter in natural order within a cell)
+            y = data.to(dev).reshape(C, npts).contiguous()
+            rows = y.T.repeat_interleave(kw, dim=0).reshape(npts * kw, 1, C).contiguous()
+            T = self._table64()
+            applied = torch.empty_like(rows)
+            _lib.call("pg_grog_interpolate_c128", _lib.ptr(rows), _lib.ptr(e_shot), _lib.ptr(T), _lib.ptr(applied),
+                      npts * kw, 1, C, int(T.shape[0]), _lib.stream(dev))
+            t_sorted, order = torch.sort(t_shot, stable=True)
+            vals = applied[:, 0, :].T[:, order].contiguous()
+            out = torch.zeros(C, comp.G, dtype=torch.complex128, device=dev)
+            _lib.call("pg_scatter_add_rows_c128", _lib.ptr(out), comp.G, _lib.ptr(vals), _lib.ptr(t_sorted),
+                      _lib.ptr(w_shot[order].double().contiguous()), npts * kw, C, _lib.stream(dev))
+            return out.to(data.device)
         y = data.to(dev).to(torch.complex64).reshape(1, C, npts).contiguous()
         applied = torch.empty(1, C, npts * kw, dtype=torch.complex64, device=dev)
         st = _lib.stream(dev)

@@ -414,6 +414,22 @@ int pg_scatter_add_rows_c64(void *grid, int64_t grid_size, const void *data,
   PG_LAUNCHED();
   return 0;
 }
+int pg_scatter_add_rows_c128(void *grid, int64_t grid_size, const void *data,
+                             const int64_t *indices, const double *weights,
+                             int64_t n, int64_t rows, pg_stream_t stream) {
+  PG_CHECK(grid_size >= 0 && n >= 0 && rows >= 0 && rows <= 65535,
+           "scatter_add_rows: bad sizes");
+  if (n == 0 || rows == 0) return 0;
+  PG_CHECK(grid && data && indices && weights, "scatter_add_rows: null pointer");
+  const int threads = 256;
+  hipLaunchKernelGGL(k_scatter_add<double>,
+                     dim3(grid_for(n, threads, 16384), (unsigned)rows),
+                     dim3(threads), 0, as_stream(stream),
+                     static_cast<double *>(grid), grid_size,
+                     static_cast<const double *>(data), indices, weights, n);
+  PG_LAUNCHED();
+  return 0;
+}
 int pg_scatter_add_c128(void *grid, int64_t grid_size, const void *data,
                         const int64_t *indices, const double *weights,
                         int64_t n, pg_stream_t stream) {

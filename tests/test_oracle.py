@@ -365,3 +365,23 @@ def test_radial_unit_operators_recover_known_generators():
     ops = otable.radial_unit_operators(calib, coords, 0.0)
     np.testing.assert_allclose(ops["x"], sla.expm(A.T), atol=1e-8)
     np.testing.assert_allclose(ops["y"], sla.expm(B.T), atol=1e-8)
+
+
+def test_grid_reference_complex128_flow():
+    """The complex128 reference flow equals brute force in float64 with the
+    plan's sqrt weights (pre-weight times scatter weight) to rounding."""
+    rng = np.random.default_rng(10)
+    coords = rng.uniform(-0.5, 0.5, (5, 6, 2)).astype(np.float32)
+    p = oplan.attach_fft_plan(oplan.create_plan_single((10, 10), coords, 2.0, 2, "circle"))
+    C = 2
+    t, _, _ = otable.table(_rand_c(rng, C, 8, 8), 0.01, 1)
+    e = oplan.quantize(p["distances"], t.shape[0])
+    y = _rand_c(rng, C, 30)
+    g128, _ = ogrid.grid_reference(y, p, e, t, dtype=np.complex128)
+    assert g128.dtype == np.complex128
+    brute = np.zeros((C, 400), np.complex128)
+    sw = p["sqrt_weights"].astype(np.float64)
+    for q in range(p["indices"].size):
+        r = p["sort_perm"][q]
+        brute[:, p["indices"][q]] += sw[q] * sw[q] * (t[e.reshape(-1)[r]] @ y[:, r // 3])
+    np.testing.assert_allclose(g128, brute, rtol=1e-12, atol=1e-14)
