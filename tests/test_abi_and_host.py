@@ -263,6 +263,39 @@ def test_polynomial_preconditioner_applies_horner():
         PolynomialPreconditioner(op, degree=-1)
 
 
+def test_lanczos_spectrum_bound():
+    """The spectrum bound of the preconditioner (Lanczos, float64 inner
+    products): never below the largest eigenvalue of a Hermitian T, equal
+    to it once the Krylov space is exhausted; the coefficients then come
+    from that bracket."""
+    from pygrog_amd import PolynomialPreconditioner
+    from pygrog_amd._solve._polynomial import _lanczos_upper_bound
+
+    rng = np.random.default_rng(8)
+    n = 40
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    lam = np.sort(rng.uniform(0.1, 5.0, n))
+    T = torch.from_numpy(Q @ np.diag(lam) @ Q.conj().T)
+    x0 = torch.from_numpy(rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    bounds = [_lanczos_upper_bound(lambda v: T @ v, x0, k) for k in (2, 5, 10, n)]
+    assert all(b >= lam[-1] * (1 - 1e-12) for b in bounds)
+    assert abs(bounds[-1] - lam[-1]) <= 1e-8 * lam[-1]
+    assert _lanczos_upper_bound(lambda v: 0 * v, x0, 3) == 0.0
+
+    class Op:
+        image_shape = (n,)
+        smaps = torch.ones(1)
+
+        def normal(self, v):
+            return T.to(v.dtype) @ v
+
+    pc = PolynomialPreconditioner(Op(), degree=3, n_power_iter=n)
+    assert pc.spectrum[0] == 0.0 and abs(pc.spectrum[1] - lam[-1]) <= 1e-5 * lam[-1]
+    from pygrog_amd._solve._polynomial import _l2_optimal_coeffs
+
+    assert pc.coeffs == _l2_optimal_coeffs(3, 0.0, pc.spectrum[1])
+
+
 def test_resize_and_centered_fft_roundtrip():
     from pygrog_amd._base import fft, ifft
     from pygrog_amd._utils import resize
