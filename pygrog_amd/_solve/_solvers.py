@@ -83,6 +83,25 @@ def cg(op, b, *, damp: float = 0.0, x0=None, max_iter: int = 20, tol: float = 1e
     p = z.clone()
     rz_old = _dot(r, z, dt)
     r0 = _norm(r, dt) + 1e-30
+    # On a HIP device the residual test of iteration k is read only after
+    # iteration k + 1 has been enqueued: its ratio goes to pinned host memory
+    # asynchronously, so the host never waits for an idle device.  If
+    # iteration k had converged, its saved iterate is returned and the one
+    # enqueued after it is dropped: the same result and stopping rule as
+    # testing every iteration synchronously.
+    pipelined = x.is_cuda and isinstance(r0, torch.Tensor) and r0.is_cuda
+    if pipelined:
+        rel_host = torch.empty(2, dtype=torch.float64, pin_memory=True)
+        pending = None  # (k, x_k, event, slot)
+
+    def _settle(item):
+        k_, x_, ev_, slot_ = item
+        ev_.synchronize()
+        rel_ = float(rel_host[slot_])
+        if callback is not None:
+            callback(k_, x_, rel_)
+        return rel_ < tol
+
     for k in range(max_iter):
         Ap = AHA(p)
         if damp != 0.0:
@@ -93,16 +112,28 @@ def cg(op, b, *, damp: float = 0.0, x0=None, max_iter: int = 20, tol: float = 1e
         r = r - a * Ap
         # ||r||^2 as <r, r>: the residual test, and <r, z> itself when z = r
         rr = _dot(r, r, dt)
-        rel = (torch.sqrt(rr) / r0).item()
-        if callback is not None:
-            callback(k, x, rel)
-        if rel < tol:
-            break
+        if pipelined:
+            slot = k % 2
+            rel_host[slot].copy_(torch.sqrt(rr) / r0, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record()
+            if pending is not None and _settle(pending):
+                return pending[1]
+            pending = (k, x, ev, slot)
+        else:
+            rel = float(torch.sqrt(rr) / r0)
+            if callback is not None:
+                callback(k, x, rel)
+            if rel < tol:
+                break
         z = preconditioner.apply(r) if preconditioner is not None else r
         rz_new = _dot(r, z, dt) if preconditioner is not None else rr
         beta = rz_new / (rz_old + 1e-30)
         p = z + beta.to(a.dtype) * p
         rz_old = rz_new
+    if pipelined and pending is not None:
+        _settle(pending)
+        return pending[1]
     return x
 
 

@@ -440,6 +440,36 @@ def test_cg_fp64_scalars_vs_oracle_cg(device):
     assert _rel(_np(x), ref) <= 1e-5
 
 
+def test_cg_stops_like_the_synchronous_rule(device):
+    """The device CG reads each iteration's residual test one iteration
+    late (no host wait per iteration) but stops where testing every
+    iteration stops: the callbacks arrive in order, the first one under
+    tol is the last, and the result equals a run of exactly that many
+    iterations, bit for bit."""
+    from pygrog_amd._solve import cg
+    from pygrog_amd.operator import SparseFFT
+
+    rng = np.random.default_rng(22)
+    grid, image, n, C = (40, 40), (32, 32), 2500, 3
+    idx = rng.integers(0, 40 * 40, size=n)
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    sm = (rng.standard_normal((C, *image)) + 1j * rng.standard_normal((C, *image))).astype(np.complex64) * 0.5
+    b = torch.as_tensor((rng.standard_normal((C, n)) + 1j * rng.standard_normal((C, n))).astype(np.complex64))
+    op = SparseFFT(grid, image, idx, w, smaps=torch.as_tensor(sm).to(device), device=device, toeplitz=True)
+    b = b.to(device)
+    seen = []
+    x = cg(op, b, damp=1e-2, max_iter=60, tol=3e-3, callback=lambda k, xk, rel: seen.append((k, rel, xk)))
+    ks = [k for k, _, _ in seen]
+    assert ks == list(range(len(seen))) and 2 < len(seen) < 60
+    assert seen[-1][1] < 3e-3 and all(rel >= 3e-3 for _, rel, _ in seen[:-1])
+    assert torch.equal(x, seen[-1][2])
+    x_fixed = cg(op, b, damp=1e-2, max_iter=len(seen), tol=0.0)
+    assert torch.equal(x, x_fixed)
+    # max_iter reached without convergence: the last iterate
+    x3 = cg(op, b, damp=1e-2, max_iter=2, tol=0.0)
+    assert torch.equal(x3, cg(op, b, damp=1e-2, max_iter=2, tol=1e-30))
+
+
 # ---------------------------------------------------------------------------
 # GrogInterpolator (reference tests/test_grog_interpolator.py)
 # ---------------------------------------------------------------------------
