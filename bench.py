@@ -268,27 +268,18 @@ def summarize_pmc(fetch_csv: str, write_csv: str, copy_bytes: int, workload: str
 # ---------------------------------------------------------------------------
 # reference CPU baseline (bounded sample)
 # ---------------------------------------------------------------------------
-def cpu_baseline(wl, n_spokes: int, threads: int) -> dict:
-    """Time the reference's CPU flow of interpolate(grid=True)
-    (src/pygrog/calib/_grog.py:668-695, 736-821 on csrc/torch CPU ops)
-    on the first ``n_spokes`` spokes."""
+def _reference_flow_cpu(tgt, w, e, kernel, data, kw: int, G: int, threads: int) -> tuple:
+    """The reference's CPU flow of interpolate(grid=True) for one trajectory
+    (src/pygrog/calib/_grog.py:668-695, 736-821 on csrc/torch CPU ops), on
+    ``threads`` host threads; -> (seconds, kind).  tgt, w, e: the replicas'
+    targets, plan weights and table entries in natural order; data (C, n)."""
     import oracle
 
     ref = oracle.ref_ext()
-    kind = "reference"
-    if ref is None:
-        kind = "port"
-    g = wl["g"]
-    comp = g.plan._compiled
-    kw, C = comp.kw, wl["C"]
-    n = n_spokes * wl["n_read"]
+    kind = "reference" if ref is not None else "port"
+    C, n = data.shape
     nper = n * kw
-    tgt = comp.target_flat.reshape(-1)[:nper].cpu()
-    w = g.plan._dev["weights"].reshape(-1)[:nper].cpu()
-    e = comp.interp_idx.reshape(-1)[:nper].cpu()
-    kernel = comp.table.cpu()
-    data = wl["y"][0, 0, :, :n].cpu().contiguous()
-    # the sample's FFT plan, as _attach_fft_plan builds it (untimed setup)
+    # the FFT plan, as _attach_fft_plan builds it (untimed setup)
     idx = tgt.clone()
     sqrt_w = torch.sqrt(w.clamp(min=0.0))
     bad = idx < 0
@@ -300,8 +291,6 @@ def cpu_baseline(wl, n_spokes: int, threads: int) -> dict:
     inv_perm = torch.empty_like(sort_perm)
     inv_perm[sort_perm] = torch.arange(nper)
     pre_w = sqw[inv_perm]
-    G = comp.G
-    torch.set_num_threads(threads)
 
     def run():
         # _sparse_full_single: replicate, transpose, apply, permute back
@@ -331,10 +320,56 @@ def cpu_baseline(wl, n_spokes: int, threads: int) -> dict:
                 ops.scatter_add(gc_, sc.numpy(), sidx.numpy(), sqw.numpy())
         return grid
 
-    t0 = time.perf_counter()
-    run()
-    t1 = time.perf_counter()
-    return {"seconds": t1 - t0, "coil_samples": n * C, "kind": kind, "spokes": n_spokes}
+    prev = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    try:
+        t0 = time.perf_counter()
+        run()
+        return time.perf_counter() - t0, kind
+    finally:
+        torch.set_num_threads(prev)
+
+
+def cpu_baseline(wl, n_spokes: int, threads: int) -> dict:
+    """The reference CPU flow on the first ``n_spokes`` spokes of a
+    single-trajectory workload."""
+    g = wl["g"]
+    comp = g.plan._compiled
+    kw, C = comp.kw, wl["C"]
+    n = n_spokes * wl["n_read"]
+    nper = n * kw
+    tgt = comp.target_flat.reshape(-1)[:nper].cpu()
+    w = g.plan._dev["weights"].reshape(-1)[:nper].cpu()
+    e = comp.interp_idx.reshape(-1)[:nper].cpu()
+    data = wl["y"][0, 0, :, :n].cpu().contiguous()
+    sec, kind = _reference_flow_cpu(tgt, w, e, comp.table.cpu(), data, kw, comp.G, threads)
+    return {"seconds": sec, "coil_samples": n * C, "kind": kind, "spokes": n_spokes}
+
+
+def scaled_cpu_baseline(run_sample, total_units: int, first: int, target_s: float) -> tuple:
+    """Run ``run_sample(k)`` (the first k units of the workload) and grow k
+    (at most twice) until it is about ``target_s`` seconds of CPU work;
+    -> (result dict of the last run, k)."""
+    k = max(1, min(first, total_units))
+    res = run_sample(k)
+    for _ in range(2):
+        if res["seconds"] >= 0.6 * target_s or k >= total_units:
+            break
+        rate = k / max(res["seconds"], 1e-9)
+        k = int(min(total_units, max(k + 1, target_s * rate)))
+        res = run_sample(k)
+    return res, k
+
+
+def single_cpu_baseline(wl, threads: int, target_s: float) -> dict:
+    """The reference CPU flow on a bounded spoke sample of a
+    single-trajectory workload, as a ``cpu_baseline`` record."""
+    first = 100 if wl["spokes"] > 1000 else wl["spokes"]
+    res, n_sp = scaled_cpu_baseline(lambda k: cpu_baseline(wl, k, threads), wl["spokes"], first, target_s)
+    return {"value": res["coil_samples"] / res["seconds"], "unit": "coil-samples/s", "cores": threads,
+            "kind": res["kind"],
+            "sample": f"first {n_sp} of {wl['spokes']} spokes ({n_sp * wl['n_read']} samples x {wl['C']} coils), "
+                      f"reference interpolate(grid=True) flow, {res['seconds']:.2f} s"}
 
 
 # ---------------------------------------------------------------------------
@@ -372,6 +407,34 @@ def frames_fp64_rel(b, y, out) -> float:
         num += float(torch.linalg.vector_norm(out[f].reshape(C, -1).to(torch.complex128) - ref) ** 2)
         den += float(torch.linalg.vector_norm(ref) ** 2)
     return (num / den) ** 0.5
+
+
+def frames_cpu_baseline(b, y, threads: int, target_s: float) -> dict:
+    """The reference CPU flow frame after frame (each frame one
+    single-trajectory interpolate(grid=True)) on the first frames of the
+    batch, as a ``cpu_baseline`` record."""
+    C = b.C
+
+    def run(k):
+        sec, kind, cs = 0.0, "reference", 0
+        for f in range(k):
+            fr = b.frames[f]
+            comp = fr.plan._compiled
+            n = int(b.n_samples[f])
+            data = y[C * int(b.sample_off[f]):C * int(b.sample_off[f + 1])].reshape(C, n).cpu()
+            t, kind = _reference_flow_cpu(comp.target_flat.reshape(-1).cpu(),
+                                          fr.plan._dev["weights"].reshape(-1).cpu(),
+                                          comp.interp_idx.reshape(-1).cpu(), comp.table.cpu(), data, comp.kw,
+                                          comp.G, threads)
+            sec += t
+            cs += n * C
+        return {"seconds": sec, "coil_samples": cs, "kind": kind}
+
+    res, k = scaled_cpu_baseline(run, b.F, 4, target_s)
+    return {"value": res["coil_samples"] / res["seconds"], "unit": "coil-samples/s", "cores": threads,
+            "kind": res["kind"],
+            "sample": f"first {k} of {b.F} frames ({res['coil_samples']} coil-samples), reference "
+                      f"interpolate(grid=True) flow per frame, {res['seconds']:.2f} s"}
 
 
 def run_frames2d(args, dev) -> dict:
@@ -440,6 +503,11 @@ def run_frames2d(args, dev) -> dict:
                                                        or modes[m]["rel_l2_vs_fp64"] <= 1e-3)]
     best = max(bf, key=lambda m: modes[m]["coil_samples_per_s"]) if bf else head
     hm = modes[best]
+    cpu = cpu1 = None
+    if not args.no_cpu_baseline:
+        cpu = frames_cpu_baseline(b, y, len(os.sched_getaffinity(0)), args.cpu_seconds / 2)
+        cpu1 = frames_cpu_baseline(b, y, 1, args.cpu_seconds / 4)
+        log(f"frames2d cpu baseline: {cpu}; 1 thread: {cpu1}")
     return {
         "metric": METRIC, "value": hm["coil_samples_per_s"], "unit": "coil-samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": hm["ms_per_step"], "step_ms": hm["step_ms"], "higher_is_better": True,
@@ -449,7 +517,7 @@ def run_frames2d(args, dev) -> dict:
                    "grid": list(b.grid_shape), "spokes_min_max": [int(spokes.min()), int(spokes.max())],
                    "table_entries": stats["K"], "replicas_valid": stats["R"]},
         "modes": modes, "roofline": roof, "roofline_step": roof_steps, "roofline_all": all_roof,
-        "cpu_baseline": None, "setup_s": setup_s,
+        "cpu_baseline": cpu, "cpu_baseline_1thread": cpu1, "setup_s": setup_s,
     }
 
 
@@ -531,19 +599,23 @@ CG_DESC = ("SENSE CG, 30 iterations, SparseFFT with Toeplitz PSF normal operator
 
 
 class _Normal64:
-    """The SENSE Toeplitz normal equations of a SparseFFT in complex128
-    (torch on the device): the FP64 reference of the CG benchmark."""
+    """The SENSE Toeplitz normal equations of a SparseFFT (grid = image) in
+    torch: complex128 on the device is the FP64 reference of the CG
+    benchmark; complex64 on the host restates the reference's CPU solve
+    (src/pygrog/_toep/_grog_toep.py:167-209 on torch.fft)."""
 
-    def __init__(self, op, dev):
+    def __init__(self, op, dev, dtype=torch.complex128):
         sp = op._sp()
+        rt = torch.float64 if dtype == torch.complex128 else torch.float32
+        self.dtype = dtype
         self.shape = tuple(op.image_shape)
         self.grid = tuple(op.grid_shape)
-        self.idx = sp.indices[0]
-        self.sqw = sp.sqrt_w[0].to(torch.float64)
-        self.perm = sp.sort_perm[0]
-        self.smaps = op.smaps.to(dev, torch.complex128)
+        self.idx = sp.indices[0].to(dev)
+        self.sqw = sp.sqrt_w[0].to(dev, rt)
+        self.perm = sp.sort_perm[0].to(dev)
+        self.smaps = op.smaps.to(dev, dtype)
         G = int(np.prod(self.grid))
-        self.psf = torch.zeros(G, dtype=torch.float64, device=dev).index_add_(0, self.idx, self.sqw * self.sqw)
+        self.psf = torch.zeros(G, dtype=rt, device=dev).index_add_(0, self.idx, self.sqw * self.sqw)
         self.psf = self.psf.reshape(self.grid)
 
     def _fft(self, x, inv=False):
@@ -552,9 +624,9 @@ class _Normal64:
         return torch.fft.fftshift(f(torch.fft.ifftshift(x, dim=ax), dim=ax, norm="ortho"), dim=ax)
 
     def adjoint(self, b):
-        b = b.reshape(b.shape[0], -1).to(torch.complex128)
+        b = b.reshape(b.shape[0], -1).to(self.dtype)
         G = int(np.prod(self.grid))
-        g = torch.zeros(b.shape[0], G, dtype=torch.complex128, device=b.device)
+        g = torch.zeros(b.shape[0], G, dtype=self.dtype, device=b.device)
         g.index_add_(1, self.idx, b[:, self.perm] * self.sqw)
         img = self._fft(g.reshape(-1, *self.grid), inv=True)
         return (img * self.smaps.conj()).sum(0)
@@ -562,6 +634,42 @@ class _Normal64:
     def normal(self, x):
         k = self._fft(self.smaps * x) * self.psf
         return (self._fft(k, inv=True) * self.smaps.conj()).sum(0)
+
+
+def cg_cpu_baseline(op, b, threads: int, iters: int) -> dict:
+    """The reference's CPU solve restated in torch on ``threads`` host
+    threads: Toeplitz normal operator on torch.fft, CG with complex64
+    vdot scalars and a host read per iteration (src/pygrog/_solve/
+    _solvers.py:49-139); ``iters`` iterations timed."""
+    prev = torch.get_num_threads()
+    torch.set_num_threads(threads)
+    try:
+        return _cg_cpu(op, b, threads, iters)
+    finally:
+        torch.set_num_threads(prev)
+
+
+def _cg_cpu(op, b, threads: int, iters: int) -> dict:
+    cpu = torch.device("cpu")
+    opc = _Normal64(op, cpu, torch.complex64)
+    rhs = opc.adjoint(b.to(cpu))
+    x = torch.zeros_like(rhs)
+    r = rhs.clone()
+    p = r.clone()
+    rr = torch.vdot(r.reshape(-1), r.reshape(-1)).real
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        Ap = opc.normal(p)
+        alpha = rr / torch.vdot(p.reshape(-1), Ap.reshape(-1)).real
+        x = x + alpha * p
+        r = r - alpha * Ap
+        rr_new = torch.vdot(r.reshape(-1), r.reshape(-1)).real
+        float(torch.sqrt(rr_new))  # the reference's per-iteration residual read
+        p = r + (rr_new / rr) * p
+        rr = rr_new
+    sec = time.perf_counter() - t0
+    return {"value": sec * 1e3 / iters, "unit": "ms/iteration", "cores": threads, "kind": "port",
+            "sample": f"{iters} CG iterations of the full problem, complex64 on torch.fft (host), {sec:.2f} s"}
 
 
 def run_cg(args, dev) -> dict:
@@ -616,71 +724,23 @@ def run_cg(args, dev) -> dict:
     res = {"workload": CG_DESC, "iterations": iters, "solve_ms": t * 1e3, "ms_per_iteration": t * 1e3 / iters,
            "rel_l2_vs_fp64_cg": res_rel, "rel_l2_vs_fp64_cg_damp_1e-2": rel(xd, xd64),
            "operator_rel_l2_vs_fp64": op_rel, "runs": len(times)}
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cg_cpu_baseline(op, b, len(os.sched_getaffinity(0)), iters)
+        res["cpu_baseline_1thread"] = cg_cpu_baseline(op, b, 1, 3)
+        log(f"cg cpu baseline: {res['cpu_baseline']}; 1 thread: {res['cpu_baseline_1thread']}")
     log(f"cg: {t * 1e3:.1f} ms for {iters} iterations ({t * 1e3 / iters:.2f} ms/iter), rel-L2 vs FP64 CG "
         f"{res_rel:.2e} (damp 1e-2: {res['rel_l2_vs_fp64_cg_damp_1e-2']:.2e}), operators vs FP64 {op_rel}")
     return res
 
 
-# ---------------------------------------------------------------------------
-def main(argv=None) -> int:
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="kooshball3d", choices=["kooshball3d", "radial2d", "frames2d", "serve", "cg"])
-    ap.add_argument("--serve-jobs", type=int, default=48, help="jobs per serving run (0: skip serving)")
-    ap.add_argument("--no-extras", action="store_true",
-                    help="default workload only: skip the 200-frame batch and SENSE-CG legs")
-    ap.add_argument("--modes", default="fp32,bf16,bf16x3")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work (s)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-check", action="store_true", help="skip the FP64 rel-L2 check")
-    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
-                    help="run one plain device copy (float32 negation) of BYTES first: the PMC scale factor")
-    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
-                    help="turn two rocprofv3 --pmc counter_collection.csv files into profiles/traffic_<workload>.json "
-                         "(no GPU needed)")
-    args = ap.parse_args(argv)
-
-    if args.summarize_pmc:
-        if args.copy_calibration <= 0:
-            raise SystemExit("--summarize-pmc needs --copy-calibration BYTES of the profiled run")
-        res = summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.copy_calibration, args.workload)
-        path = ROOT / "profiles" / f"traffic_{args.workload}.json"
-        path.write_text(json.dumps(res, indent=1) + "\n")
-        print(json.dumps(res["entries"]))
-        return 0
-
-    if args.gpus != 1:
-        print(json.dumps({"error": "this benchmark measures one GPU; --gpus N>1 is not supported (the "
-                                   "multi-GPU sharding of the path is designed, not implemented)",
-                          "n_gpus": args.gpus}))
-        return 2
-    if not torch.cuda.is_available():
-        print(json.dumps({"error": "no HIP device"}))
-        return 2
+def measure_single(name: str, args, dev) -> dict:
+    """Every gridding mode of one single-trajectory workload: timed steps,
+    per-kernel times, rel-L2 against FP64, step and kernel rooflines and
+    the CPU baselines."""
     from pygrog_amd.calib._grog import KernelTimer
 
-    dev = torch.device("cuda:0")
-    if args.copy_calibration > 0:
-        src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        torch.neg(src, out=dst)
-        torch.cuda.synchronize()
-        del src, dst
-    if args.workload == "frames2d":
-        line = run_frames2d(args, dev)
-        print(json.dumps(line))
-        return 0
-    if args.workload == "cg":
-        print(json.dumps({"metric": METRIC, "cg": run_cg(args, dev)}))
-        return 0
-    if args.workload == "serve":
-        res = run_serving(dev, max(args.serve_jobs, 4))
-        print(json.dumps({"metric": METRIC, "serving": res}))
-        return 0
     t_setup = time.perf_counter()
-    wl = make_workload(args.workload, dev)
+    wl = make_workload(name, dev)
     g = wl["g"]
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t_setup
@@ -732,7 +792,7 @@ def main(argv=None) -> int:
     achieved = ab / (kms * 1e-3) / 1e9
     kroof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
              "frac": achieved / HBM_PEAK_GBS, "traffic": None, "alg_bytes": ab, "kernel_ms": kms}
-    traffic_file = ROOT / "profiles" / f"traffic_{args.workload}.json"
+    traffic_file = ROOT / "profiles" / f"traffic_{name}.json"
     tkey = traffic_key(kname, head)
     if traffic_file.exists():
         tr = json.loads(traffic_file.read_text()).get("entries", {})
@@ -748,29 +808,82 @@ def main(argv=None) -> int:
         stats["n_parts"] = 2 if m == "bf16x3" else 1
         all_roof[m] = {k: {"GB/s": alg_bytes(k, stats) / (t * 1e-3) / 1e9, "ms": t} for k, t in v["kernel_ms"].items()}
 
-    cpu = None
+    cpu = cpu1 = None
     if not args.no_cpu_baseline:
-        threads = len(os.sched_getaffinity(0))
-        n_sp = max(1, min(wl["spokes"], 100 if wl["spokes"] > 1000 else wl["spokes"]))
-        res = cpu_baseline(wl, n_sp, threads)
-        # rescale (at most twice) until the sample is about --cpu-seconds of work
-        for _ in range(2):
-            if res["seconds"] >= 0.6 * args.cpu_seconds or n_sp >= wl["spokes"]:
-                break
-            rate = res["coil_samples"] / res["seconds"]
-            n_sp = int(min(wl["spokes"], max(n_sp + 1, args.cpu_seconds * rate / (wl["n_read"] * wl["C"]))))
-            res = cpu_baseline(wl, n_sp, threads)
-        cpu = {"value": res["coil_samples"] / res["seconds"], "unit": "coil-samples/s", "cores": threads,
-               "kind": res["kind"],
-               "sample": f"first {n_sp} of {wl['spokes']} spokes ({n_sp * wl['n_read']} samples x {wl['C']} coils), "
-                         f"reference interpolate(grid=True) flow, {res['seconds']:.2f} s"}
-        log(f"cpu baseline: {cpu}")
+        cpu = single_cpu_baseline(wl, len(os.sched_getaffinity(0)), args.cpu_seconds)
+        cpu1 = single_cpu_baseline(wl, 1, args.cpu_seconds / 3)
+        log(f"cpu baseline: {cpu}; 1 thread: {cpu1}")
 
-    hbm_peak_gb = torch.cuda.max_memory_allocated(dev) / 1e9
     wl_info = {k: wl[k] for k in ("desc", "N", "C", "shape")}
+    return {"wl_info": wl_info, "stats": stats, "modes": modes, "head": head, "roofline": roof,
+            "roofline_all": all_roof, "cpu_baseline": cpu, "cpu_baseline_1thread": cpu1, "setup_s": setup_s}
+
+
+# ---------------------------------------------------------------------------
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="kooshball3d", choices=["kooshball3d", "radial2d", "frames2d", "serve", "cg"])
+    ap.add_argument("--serve-jobs", type=int, default=48, help="jobs per serving run (0: skip serving)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="default workload only: skip the 200-frame batch and SENSE-CG legs")
+    ap.add_argument("--modes", default="fp32,bf16,bf16x3")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work (s)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 rel-L2 check")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="run one plain device copy (float32 negation) of BYTES first: the PMC scale factor")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="turn two rocprofv3 --pmc counter_collection.csv files into profiles/traffic_<workload>.json "
+                         "(no GPU needed)")
+    args = ap.parse_args(argv)
+
+    if args.summarize_pmc:
+        if args.copy_calibration <= 0:
+            raise SystemExit("--summarize-pmc needs --copy-calibration BYTES of the profiled run")
+        res = summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1], args.copy_calibration, args.workload)
+        path = ROOT / "profiles" / f"traffic_{args.workload}.json"
+        path.write_text(json.dumps(res, indent=1) + "\n")
+        print(json.dumps(res["entries"]))
+        return 0
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark measures one GPU; --gpus N>1 is not supported (the "
+                                   "multi-GPU sharding of the path is designed, not implemented)",
+                          "n_gpus": args.gpus}))
+        return 2
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        return 2
+
+    dev = torch.device("cuda:0")
+    if args.copy_calibration > 0:
+        src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
+    if args.workload == "frames2d":
+        line = run_frames2d(args, dev)
+        print(json.dumps(line))
+        return 0
+    if args.workload == "cg":
+        print(json.dumps({"metric": METRIC, "cg": run_cg(args, dev)}))
+        return 0
+    if args.workload == "serve":
+        res = run_serving(dev, max(args.serve_jobs, 4))
+        print(json.dumps({"metric": METRIC, "serving": res}))
+        return 0
+    r = measure_single(args.workload, args, dev)
+    modes, head, stats, wl_info = r["modes"], r["head"], r["stats"], r["wl_info"]
+    hm = modes[head]
+    roof, all_roof, cpu, cpu1, setup_s = (r["roofline"], r["roofline_all"], r["cpu_baseline"],
+                                          r["cpu_baseline_1thread"], r["setup_s"])
+    hbm_peak_gb = torch.cuda.max_memory_allocated(dev) / 1e9
     serving = None
     if args.serve_jobs > 0:
-        del wl, g, comp, out, ref64, cache, y
         import gc
 
         gc.collect()
@@ -779,9 +892,14 @@ def main(argv=None) -> int:
     extras = {}
     if not args.no_extras and args.workload == "kooshball3d":
         torch.cuda.empty_cache()
+        r1 = measure_single("radial2d", args, dev)
+        extras["radial2d"] = {"value": r1["modes"][r1["head"]]["coil_samples_per_s"], "mode": r1["head"],
+                              "config": r1["wl_info"]["desc"], "modes": r1["modes"], "roofline": r1["roofline"],
+                              "cpu_baseline": r1["cpu_baseline"], "cpu_baseline_1thread": r1["cpu_baseline_1thread"]}
+        torch.cuda.empty_cache()
         fr = run_frames2d(args, dev)
         extras["frames2d"] = {k: fr[k] for k in ("value", "ms_per_step", "config", "modes", "roofline",
-                                                 "roofline_step")}
+                                                 "roofline_step", "cpu_baseline", "cpu_baseline_1thread")}
         torch.cuda.empty_cache()
         extras["sense_cg"] = run_cg(args, dev)
 
@@ -806,6 +924,7 @@ def main(argv=None) -> int:
         "roofline": roof,
         "roofline_all": all_roof,
         "cpu_baseline": cpu,
+        "cpu_baseline_1thread": cpu1,
         "setup_s": setup_s,
         "hbm_peak_gb": hbm_peak_gb,
     }
