@@ -526,7 +526,7 @@ def run_frames2d(args, dev) -> dict:
 # ---------------------------------------------------------------------------
 SERVE_DESC = ("Poisson stream mixing 2D 256^2 8-coil golden-angle radial jobs (40-400 spokes x 512; 3x3 kernel, q=1/32) "
               "and 3D 128^3 16-coil golden-means kooshball jobs (2k-10k spokes x 256; 2-wide sphere, q=1/10), "
-              "each with its own ACR-fit table (lamda=1e-3); HBM-budget admission, 4 preparation slots "
+              "each with its own ACR-fit table (lamda=1e-3); request payloads (trajectory, ACR, k-space) staged on the device; HBM-budget admission, 2 preparation slots "
               "(plan and table fit overlapped on two threads and streams each), continuous batching of the gridding "
               "launches; one long-lived server per mode, warmed by an untimed first stream")
 
@@ -549,6 +549,9 @@ def make_serve_jobs(dev, n_jobs: int, seed: int = 5):
             calib = synth.calibration(shape, C, (24, 24), seed=seed + i, device=dev)
             kw = dict(kernel_width=3, quant=32)
         y = synth.complex_gaussian((C, coords.shape[0] * coords.shape[1]), seed=seed + 1000 + i, device=dev)
+        # the request payload -- trajectory, calibration and k-space -- is
+        # staged on the device before the stream starts
+        coords = torch.from_numpy(coords).to(dev)
         jobs.append(GridJob(shape=shape, coords=coords, calib=calib, kspace=y, job_id=i, lamda=1e-3, **kw))
     return jobs
 

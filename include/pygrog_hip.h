@@ -275,6 +275,37 @@ int pg_plan_hot_rows(const uint32_t *cell_unit, int64_t S, int64_t G,
                      const uint32_t *HN, uint32_t *cell_vrow, int32_t *unit_hot,
                      uint32_t *hot_units, uint32_t *hot_ptr, pg_stream_t stream);
 
+/* The whole plan compiler in one call (the steps above, in order: absmax,
+ * targets, stable key sort, post, weights, units, hot counts, scans, hot
+ * rows), for callers that want no per-step orchestration (the serving
+ * loop's preparation threads).  pg_plan_compile_layout gives the byte
+ * offsets of the PG_PLAN_FIELDS arrays inside one output workspace (in
+ * this order: target_idx i64 (S, nper), weights f32 (S, nper), indices
+ * i64, sqrt_weights f32, sort_perm i64, inv_perm i64, cell_ptr u32
+ * (S, G + 1), rep_src u32 (S, max(nper, 1)), rep_s f32 (S, max(nper, 1)),
+ * density f32 (S, G), cell_unit u32 (S G + 1), unit_cell u32 (Umax),
+ * unit_rep u32 (Umax + S), unit_hot i32 (Umax), hot_units u32 (Umax),
+ * hot_ptr u32 (S G + S), cell_vrow u32 (S, G + 1), n_invalid i64 (S),
+ * summary i64 (3 S + 3); Umax = max(S nper, 1) bounds the units), its
+ * size and the size of the scratch workspace (free once the stream has run
+ * the call).  pg_plan_compile enqueues everything on `stream`, writes the
+ * candidates' shifts to `distances` (S, nper, ndim) f32 (a buffer of its
+ * own, so it can be released alone), then synchronises the stream once and
+ * fills the host array summary[3 S + 3] = [n_invalid[s] for s < S, U (the
+ * unit count), hot-cell bases HF[s G] for s <= S, hot-unit bases HN[s G]
+ * for s <= S] (the scans of pg_plan_hot_rows).  The arrays are those the
+ * separate entry points produce, bit for bit.
+ * No reference counterpart beyond the steps it runs. */
+#define PG_PLAN_FIELDS 19
+int pg_plan_compile_layout(int64_t S, int64_t npts, int32_t ndim, int32_t kw,
+                           int64_t G, int64_t *offsets, int64_t *out_bytes,
+                           int64_t *tmp_bytes);
+int pg_plan_compile(const float *coords, int64_t S, int64_t npts,
+                    pg_plan_geom geom, int64_t G, int32_t chunk,
+                    int64_t min_units, void *out_ws, int64_t out_ws_bytes,
+                    void *tmp_ws, int64_t tmp_ws_bytes, float *distances,
+                    int64_t *summary, pg_stream_t stream);
+
 /* Quantised table entry of every candidate replica (reference
  * calc_interp_table, _grog.py:406-422, generalised from 10**precision to an
  * integer quantisation q):  pfac = q, stepsize = f32(step), nsteps = q + 1,

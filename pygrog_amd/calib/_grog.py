@@ -220,6 +220,10 @@ def compile_units(cell_ptr: torch.Tensor, S: int, G: int, device, chunk: int = U
 # pre-pass kernels instead of inside their tile.
 HOT_MIN_UNITS = 2
 
+# arrays of one compiled plan's workspace (pg_plan_compile_layout)
+PLAN_FIELDS = 19
+_ITEMSIZE = {torch.int64: 8, torch.int32: 4, torch.float32: 4}
+
 
 def hot_lists(cell_unit: torch.Tensor, S: int, G: int, device, min_units: int | None = None, U: int | None = None):
     """Hot-cell work lists per stack element.
@@ -274,43 +278,6 @@ def ordinary_rows(cell_ptr: torch.Tensor, cell_unit: torch.Tensor, S: int, G: in
     out = torch.zeros(S, G + 1, dtype=torch.int64, device=cell_ptr.device)
     out[:, 1:] = torch.cumsum(vc, 1)
     return out.to(torch.int32).contiguous()
-
-
-def plan_hot_rows(cell_ptr: torch.Tensor, cell_unit: torch.Tensor, S: int, G: int, device, U: int,
-                  min_units: int | None = None):
-    """``hot_lists`` and ``ordinary_rows`` in two device passes and three
-    scans, with one host read (the per-element hot counts, which size the
-    lists).  Returns (unit_hot, per-element hot dicts, cell_vrow), the same
-    values as those two functions."""
-    if min_units is None:
-        min_units = HOT_MIN_UNITS
-    st = _lib.stream(device)
-    n = S * G
-    a = torch.empty(n + 1, dtype=torch.int32, device=device)
-    hf = torch.empty_like(a)
-    hn = torch.empty_like(a)
-    _lib.call("pg_plan_hot_counts", _u32(cell_ptr), _u32(cell_unit), S, G, int(min_units), _u32(a), _u32(hf),
-              _u32(hn), st)
-    A, HF, HN = (_exclusive_scan(t, device) for t in (a, hf, hn))
-    idx = torch.arange(S + 1, device=device) * G
-    bases = torch.stack([HF[idx], HN[idx]]).to(torch.int64).cpu().tolist()
-    hb, ub = bases
-    cell_vrow = torch.empty(S, G + 1, dtype=torch.int32, device=device)
-    if U > 0:
-        unit_hot = torch.empty(U, dtype=torch.int32, device=device)
-    else:
-        unit_hot = torch.full((1,), -1, dtype=torch.int32, device=device)
-    hot_ptr = torch.empty(hb[S] + S, dtype=torch.int32, device=device)
-    hot_units = torch.empty(max(ub[S], 1), dtype=torch.int32, device=device)
-    _lib.call("pg_plan_hot_rows", _u32(cell_unit), S, G, int(min_units), _u32(A), _u32(HF), _u32(HN), _u32(cell_vrow),
-              _lib.ptr(unit_hot), _u32(hot_units), _u32(hot_ptr), st)
-    per = []
-    for s in range(S):
-        H, HU = hb[s + 1] - hb[s], ub[s + 1] - ub[s]
-        units = hot_units[ub[s]:ub[s] + HU] if HU else torch.zeros(1, dtype=torch.int32, device=device)
-        per.append({"hot_units": units, "hot_ptr": hot_ptr[hb[s] + s:hb[s] + s + H + 1], "n_hot": H,
-                    "n_hot_units": HU})
-    return unit_hot, per, cell_vrow
 
 
 class KernelTimer:
@@ -568,51 +535,56 @@ def _create_plan(shape, coords, oversamp, kernel_width, kernel_shape, time_map, 
     geom = _geom(shape, grid_shape, offsets)
 
     cd = c.reshape(S, npts, ndim).to(dev).contiguous()
-    cmax = torch.zeros(S, ndim, dtype=torch.float32, device=dev)
-    _lib.call("pg_plan_absmax", _lib.ptr(cd), S, npts, ndim, _lib.ptr(cmax), st)
-
-    total = S * nper
-    target_idx = torch.empty(S, nper, dtype=torch.int64, device=dev)
+    # the whole plan compiler in one library call (the calling thread holds
+    # no interpreter lock meanwhile); every array is a view of one workspace
+    offs = (ctypes.c_int64 * PLAN_FIELDS)()
+    ob, tb = ctypes.c_int64(0), ctypes.c_int64(0)
+    _lib.call("pg_plan_compile_layout", S, npts, ndim, kw, G, ctypes.addressof(offs), ctypes.addressof(ob),
+              ctypes.addressof(tb))
+    ws = torch.empty(int(ob.value), dtype=torch.uint8, device=dev)
+    tmp = torch.empty(int(tb.value), dtype=torch.uint8, device=dev)
     distances = torch.empty(S, nper, ndim, dtype=torch.float32, device=dev)
-    gauss = torch.empty(S, nper, dtype=torch.float32, device=dev)
-    keys = torch.empty(total, dtype=torch.int64, device=dev)
-    vals = torch.empty(total, dtype=torch.int32, device=dev)
-    n_invalid = torch.zeros(S, dtype=torch.int64, device=dev)
-    _lib.call("pg_plan_targets", _lib.ptr(cd), _lib.ptr(cmax), S, npts, geom, _lib.ptr(target_idx), _lib.ptr(distances),
-              _lib.ptr(gauss), _lib.ptr(keys), _lib.ptr(vals), _lib.ptr(n_invalid), st)
+    summary = np.zeros(3 * S + 3, dtype=np.int64)
+    _lib.call("pg_plan_compile", _lib.ptr(cd), S, npts, geom, G, UNIT_CHUNK, int(HOT_MIN_UNITS), _lib.ptr(ws),
+              int(ob.value), _lib.ptr(tmp), int(tb.value), _lib.ptr(distances), summary.ctypes.data, st)
+    del tmp  # the call synchronised its stream
+    n_inv = [int(v) for v in summary[:S]]
+    U = int(summary[S])
+    hb = [int(v) for v in summary[S + 1:2 * S + 2]]
+    ub = [int(v) for v in summary[2 * S + 2:3 * S + 3]]
 
-    keys_s = torch.empty_like(keys)
-    vals_s = torch.empty_like(vals)
-    end_bit = max(1, int(S * (G + 1) - 1).bit_length())
-    _lib.run_with_workspace("pg_sort_pairs_u64", dev, _lib.ptr(keys), _lib.ptr(keys_s), _lib.ptr(vals), _lib.ptr(vals_s),
-                            total, end_bit, st)
-    del keys, vals
+    def field(i, dtype, *shape):
+        n = 1
+        for v in shape:
+            n *= int(v)
+        start = int(offs[i])
+        return ws[start:start + n * _ITEMSIZE[dtype]].view(dtype).reshape(shape)
 
-    indices = torch.empty(S, nper, dtype=torch.int64, device=dev)
-    sort_perm = torch.empty(S, nper, dtype=torch.int64, device=dev)
-    inv_perm = torch.empty(S, nper, dtype=torch.int64, device=dev)
-    gauss_sorted = torch.empty(total, dtype=torch.float32, device=dev)
-    cell_ptr = torch.empty(S, G + 1, dtype=torch.int32, device=dev)
-    if nper == 0:
-        cell_ptr.zero_()
-    _lib.call("pg_plan_post", _lib.ptr(keys_s), _lib.ptr(vals_s), _lib.ptr(gauss), _lib.ptr(n_invalid), S, nper, G,
-              _lib.ptr(indices), _lib.ptr(sort_perm), _lib.ptr(inv_perm), _lib.ptr(gauss_sorted), _lib.ptr(cell_ptr), st)
-    del keys_s
-
-    weights = torch.zeros(S, nper, dtype=torch.float32, device=dev)
-    sqrt_weights = torch.zeros(S, nper, dtype=torch.float32, device=dev)
-    rep_src = torch.zeros(S, max(nper, 1), dtype=torch.int32, device=dev)
-    rep_s = torch.zeros(S, max(nper, 1), dtype=torch.float32, device=dev)
-    density = torch.empty(S, G, dtype=torch.float32, device=dev)
-    unit_count = torch.zeros(S * G + 1, dtype=torch.int32, device=dev)
-    _lib.call("pg_plan_weights", _lib.ptr(vals_s), _lib.ptr(gauss_sorted), _lib.ptr(n_invalid), _lib.ptr(cell_ptr), S,
-              nper, G, kw, UNIT_CHUNK, _lib.ptr(weights), _lib.ptr(sqrt_weights), _lib.ptr(rep_src), _lib.ptr(rep_s),
-              _lib.ptr(density), _lib.ptr(unit_count), max(nper, 1), st)
-    del vals_s, gauss_sorted, gauss
-    cell_unit, unit_cell, unit_rep = compile_units(cell_ptr, S, G, dev, UNIT_CHUNK, unit_count)
-    min_units = HOT_MIN_UNITS
-    unit_hot, hot, cell_vrow = plan_hot_rows(cell_ptr, cell_unit, S, G, dev, int(unit_rep.numel()) - S, min_units)
-    n_inv = n_invalid.cpu().tolist()
+    i32 = torch.int32
+    target_idx = field(0, torch.int64, S, nper)
+    weights = field(1, torch.float32, S, nper)
+    indices = field(2, torch.int64, S, nper)
+    sqrt_weights = field(3, torch.float32, S, nper)
+    sort_perm = field(4, torch.int64, S, nper)
+    inv_perm = field(5, torch.int64, S, nper)
+    cell_ptr = field(6, i32, S, G + 1)
+    rep_src = field(7, i32, S, max(nper, 1))
+    rep_s = field(8, torch.float32, S, max(nper, 1))
+    density = field(9, torch.float32, S, G)
+    cell_unit = field(10, i32, S * G + 1)
+    unit_cell = field(11, i32, max(U, 1))
+    unit_rep = field(12, i32, U + S)
+    unit_hot = field(13, i32, max(U, 1))
+    hot_units_all = field(14, i32, max(ub[S], 1))
+    hot_ptr_all = field(15, i32, hb[S] + S)
+    cell_vrow = field(16, i32, S, G + 1)
+    n_invalid = field(17, torch.int64, S)
+    hot = []
+    for s in range(S):
+        H, HU = hb[s + 1] - hb[s], ub[s + 1] - ub[s]
+        units = hot_units_all[ub[s]:ub[s] + HU] if HU else torch.zeros(1, dtype=i32, device=dev)
+        hot.append({"hot_units": units, "hot_ptr": hot_ptr_all[hb[s] + s:hb[s] + s + H + 1], "n_hot": H,
+                    "n_hot_units": HU})
 
     if time_map is not None:
         tm = torch.as_tensor(np.asarray(time_map, dtype=np.float32) if not isinstance(time_map, torch.Tensor) else time_map)

@@ -31,6 +31,7 @@ ARCH = "gfx950"
 SOURCES = {
     "ops_ref.hip": ["-ffp-contract=off"],
     "plan.hip": ["-ffp-contract=off"],
+    "plan_compile.hip": [],
     "grid.hip": [],
     "grid_mfma.hip": [],
     "sense.hip": [],

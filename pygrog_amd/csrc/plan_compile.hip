@@ -1,0 +1,292 @@
+// The GROG plan compiler behind one call: every kernel, sort and scan of a
+// plan (reference GrogInterpolator.__init__ -> _create_plan ->
+// _create_plan_single -> _attach_fft_plan, src/pygrog/calib/_grog.py:302-331,
+// 918-1234) enqueued in order on one stream, with all arrays carved from two
+// caller-provided workspaces and one small host read at the end.  A Python
+// orchestration of the same steps spends a millisecond of host time per
+// plan in allocations and calls under the interpreter lock; here the host
+// work is a few microseconds and the calling thread releases the lock for
+// the whole call.
+//
+// Data-dependent sizes (reduction units, hot cells) are bounded above --
+// units <= valid replicas <= S * nper, hot cells <= S * G -- so nothing
+// needs to be read back before the end; the summary then gives the exact
+// counts.
+#include <cstring>
+#include <utility>
+
+#include "common.h"
+
+namespace pg {
+
+namespace {
+
+// Field ids of the output workspace (offsets written by the layout query).
+enum PlanField {
+  kTargetIdx = 0,   // (S, nper) int64
+  kWeights,         // (S, nper) f32
+  kIndices,         // (S, nper) int64
+  kSqrtWeights,     // (S, nper) f32
+  kSortPerm,        // (S, nper) int64
+  kInvPerm,         // (S, nper) int64
+  kCellPtr,         // (S, G + 1) uint32
+  kRepSrc,          // (S, Rmax) uint32
+  kRepS,            // (S, Rmax) f32
+  kDensity,         // (S, G) f32
+  kCellUnit,        // (S G + 1) uint32
+  kUnitCell,        // (Umax) uint32
+  kUnitRep,         // (Umax + S) uint32
+  kUnitHot,         // (Umax) int32
+  kHotUnits,        // (Umax) uint32
+  kHotPtr,          // (S G + S) uint32
+  kCellVrow,        // (S, G + 1) uint32
+  kNInvalid,        // (S) int64
+  kSummary,         // (3 S + 3) int64, device copy of the summary
+  kNumFields
+};
+
+struct Sizes {
+  int64_t S, npts, ndim, kw, G, nper, rmax, umax, ncell;
+};
+
+Sizes sizes_of(int64_t S, int64_t npts, int64_t ndim, int64_t kw, int64_t G) {
+  Sizes z;
+  z.S = S;
+  z.npts = npts;
+  z.ndim = ndim;
+  z.kw = kw;
+  z.G = G;
+  z.nper = npts * kw;
+  z.rmax = z.nper > 0 ? z.nper : 1;
+  z.umax = S * z.nper > 0 ? S * z.nper : 1;
+  z.ncell = S * G;
+  return z;
+}
+
+constexpr int64_t kAlign = 256;
+int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+// Element bytes of each output field.
+void field_bytes(const Sizes &z, int64_t *b) {
+  const int64_t rep = z.S * z.nper;
+  b[kTargetIdx] = 8 * rep;
+  b[kWeights] = 4 * rep;
+  b[kIndices] = 8 * rep;
+  b[kSqrtWeights] = 4 * rep;
+  b[kSortPerm] = 8 * rep;
+  b[kInvPerm] = 8 * rep;
+  b[kCellPtr] = 4 * z.S * (z.G + 1);
+  b[kRepSrc] = 4 * z.S * z.rmax;
+  b[kRepS] = 4 * z.S * z.rmax;
+  b[kDensity] = 4 * z.ncell;
+  b[kCellUnit] = 4 * (z.ncell + 1);
+  b[kUnitCell] = 4 * z.umax;
+  b[kUnitRep] = 4 * (z.umax + z.S);
+  b[kUnitHot] = 4 * z.umax;
+  b[kHotUnits] = 4 * z.umax;
+  b[kHotPtr] = 4 * (z.ncell + z.S);
+  b[kCellVrow] = 4 * z.S * (z.G + 1);
+  b[kNInvalid] = 8 * z.S;
+  b[kSummary] = 8 * (3 * z.S + 3);
+}
+
+// Scratch fields, in carve order.
+enum TmpField {
+  kCmax = 0, kGauss, kKeys, kVals, kKeysS, kValsS, kGaussSorted, kUnitCount,
+  kA, kHF, kHN, kAs, kHFs, kHNs, kSortWs, kScanWs, kNumTmp
+};
+
+int tmp_bytes(const Sizes &z, int64_t *b) {
+  const int64_t rep = z.S * z.nper;
+  b[kCmax] = 4 * z.S * z.ndim;
+  b[kGauss] = 4 * rep;
+  b[kKeys] = 8 * rep;
+  b[kVals] = 4 * rep;
+  b[kKeysS] = 8 * rep;
+  b[kValsS] = 4 * rep;
+  b[kGaussSorted] = 4 * rep;
+  for (int f : {kUnitCount, kA, kHF, kHN, kAs, kHFs, kHNs}) b[f] = 4 * (z.ncell + 1);
+  size_t sort_ws = 0, scan_ws = 0;
+  if (rep > 0) {
+    const int end_bit = 64 - __builtin_clzll((unsigned long long)(z.S * (z.G + 1) - 1) | 1ull);
+    if (pg_sort_pairs_u64(nullptr, &sort_ws, nullptr, nullptr, nullptr, nullptr, rep,
+                          end_bit > 0 ? end_bit : 1, nullptr))
+      return 1;
+  }
+  if (pg_exclusive_scan_u32(nullptr, &scan_ws, nullptr, nullptr, z.ncell, nullptr)) return 1;
+  b[kSortWs] = (int64_t)sort_ws;
+  b[kScanWs] = (int64_t)scan_ws;
+  return 0;
+}
+
+// summary = [n_invalid[0..S), U, HF[s G] for s = 0..S, HN[s G] for s = 0..S]
+__global__ void k_plan_summary(const int64_t *__restrict__ n_invalid,
+                               const uint32_t *__restrict__ cell_unit,
+                               const uint32_t *__restrict__ HF,
+                               const uint32_t *__restrict__ HN, int64_t S,
+                               int64_t G, int64_t *__restrict__ out) {
+  for (int64_t i = threadIdx.x; i < 3 * S + 3; i += blockDim.x) {
+    int64_t v;
+    if (i < S) v = n_invalid[i];
+    else if (i == S) v = cell_unit[S * G];
+    else if (i <= 2 * S + 1) v = HF[(i - S - 1) * G];
+    else v = HN[(i - 2 * S - 2) * G];
+    out[i] = v;
+  }
+}
+
+__global__ void k_fill_u32(uint32_t *__restrict__ p, int64_t n, uint32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+}  // namespace
+
+}  // namespace pg
+
+using namespace pg;
+
+extern "C" {
+
+int pg_plan_compile_layout(int64_t S, int64_t npts, int32_t ndim, int32_t kw,
+                           int64_t G, int64_t *offsets, int64_t *out_bytes,
+                           int64_t *tmp_bytes_out) {
+  PG_CHECK(S >= 1 && S <= 65535 && npts >= 0 && ndim >= 1 && ndim <= 3 &&
+               kw >= 1 && kw <= 128 && G >= 1,
+           "plan_compile: bad sizes");
+  PG_CHECK(S * npts * kw < (int64_t)UINT32_MAX && S * (G + 1) < (int64_t)UINT32_MAX,
+           "plan_compile: too many candidate replicas or cells");
+  PG_CHECK(offsets && out_bytes && tmp_bytes_out, "plan_compile: null output");
+  const Sizes z = sizes_of(S, npts, ndim, kw, G);
+  int64_t b[kNumFields];
+  field_bytes(z, b);
+  int64_t off = 0;
+  for (int f = 0; f < kNumFields; ++f) {
+    offsets[f] = off;
+    off += align_up(b[f] > 0 ? b[f] : 1);
+  }
+  *out_bytes = off;
+  int64_t t[kNumTmp];
+  if (tmp_bytes(z, t)) return 1;
+  int64_t toff = 0;
+  for (int f = 0; f < kNumTmp; ++f) toff += align_up(t[f] > 0 ? t[f] : 1);
+  *tmp_bytes_out = toff;
+  return 0;
+}
+
+int pg_plan_compile(const float *coords, int64_t S, int64_t npts,
+                    pg_plan_geom geom, int64_t G, int32_t chunk,
+                    int64_t min_units, void *out_ws, int64_t out_ws_bytes,
+                    void *tmp_ws, int64_t tmp_ws_bytes, float *distances,
+                    int64_t *summary, pg_stream_t stream) {
+  PG_CHECK(chunk >= 1, "plan_compile: unit chunk must be positive");
+  PG_CHECK(out_ws && tmp_ws && summary, "plan_compile: null workspace");
+  int64_t offs[kNumFields], ob = 0, tb = 0;
+  if (int rc = pg_plan_compile_layout(S, npts, geom.ndim, geom.kw, G, offs, &ob, &tb)) return rc;
+  PG_CHECK(out_ws_bytes >= ob && tmp_ws_bytes >= tb,
+           "plan_compile: workspaces of %lld / %lld bytes, %lld / %lld needed",
+           (long long)out_ws_bytes, (long long)tmp_ws_bytes, (long long)ob, (long long)tb);
+  const Sizes z = sizes_of(S, npts, geom.ndim, geom.kw, G);
+  PG_CHECK(z.nper == 0 || (coords && distances), "plan_compile: null coordinates");
+  const hipStream_t st = as_stream(stream);
+  char *o = static_cast<char *>(out_ws);
+  auto O = [&](int f) { return static_cast<void *>(o + offs[f]); };
+  int64_t tbytes[kNumTmp];
+  if (tmp_bytes(z, tbytes)) return 1;
+  void *T[kNumTmp];
+  {
+    char *t = static_cast<char *>(tmp_ws);
+    int64_t toff = 0;
+    for (int f = 0; f < kNumTmp; ++f) {
+      T[f] = t + toff;
+      toff += align_up(tbytes[f] > 0 ? tbytes[f] : 1);
+    }
+  }
+  const int64_t rep = S * z.nper;
+  auto *target_idx = static_cast<int64_t *>(O(kTargetIdx));
+  auto *weights = static_cast<float *>(O(kWeights));
+  auto *indices = static_cast<int64_t *>(O(kIndices));
+  auto *sqrt_w = static_cast<float *>(O(kSqrtWeights));
+  auto *sort_perm = static_cast<int64_t *>(O(kSortPerm));
+  auto *inv_perm = static_cast<int64_t *>(O(kInvPerm));
+  auto *cell_ptr = static_cast<uint32_t *>(O(kCellPtr));
+  auto *rep_src = static_cast<uint32_t *>(O(kRepSrc));
+  auto *rep_s = static_cast<float *>(O(kRepS));
+  auto *density = static_cast<float *>(O(kDensity));
+  auto *cell_unit = static_cast<uint32_t *>(O(kCellUnit));
+  auto *unit_cell = static_cast<uint32_t *>(O(kUnitCell));
+  auto *unit_rep = static_cast<uint32_t *>(O(kUnitRep));
+  auto *unit_hot = static_cast<int32_t *>(O(kUnitHot));
+  auto *hot_units = static_cast<uint32_t *>(O(kHotUnits));
+  auto *hot_ptr = static_cast<uint32_t *>(O(kHotPtr));
+  auto *cell_vrow = static_cast<uint32_t *>(O(kCellVrow));
+  auto *n_invalid = static_cast<int64_t *>(O(kNInvalid));
+  auto *summary_dev = static_cast<int64_t *>(O(kSummary));
+  auto *cmax = static_cast<float *>(T[kCmax]);
+  auto *gauss = static_cast<float *>(T[kGauss]);
+  auto *keys = static_cast<uint64_t *>(T[kKeys]);
+  auto *vals = static_cast<uint32_t *>(T[kVals]);
+  auto *keys_s = static_cast<uint64_t *>(T[kKeysS]);
+  auto *vals_s = static_cast<uint32_t *>(T[kValsS]);
+  auto *gauss_sorted = static_cast<float *>(T[kGaussSorted]);
+  auto *unit_count = static_cast<uint32_t *>(T[kUnitCount]);
+  const int64_t nc1 = z.ncell + 1;
+
+  // zero what the kernels accumulate into or only partly write
+  PG_HIP(hipMemsetAsync(cmax, 0, tbytes[kCmax], st));
+  PG_HIP(hipMemsetAsync(n_invalid, 0, 8 * S, st));
+  PG_HIP(hipMemsetAsync(weights, 0, 4 * rep, st));
+  PG_HIP(hipMemsetAsync(sqrt_w, 0, 4 * rep, st));
+  PG_HIP(hipMemsetAsync(rep_src, 0, 4 * S * z.rmax, st));
+  PG_HIP(hipMemsetAsync(rep_s, 0, 4 * S * z.rmax, st));
+  PG_HIP(hipMemsetAsync(unit_count, 0, 4 * nc1, st));
+  if (z.nper == 0) PG_HIP(hipMemsetAsync(cell_ptr, 0, 4 * S * (G + 1), st));
+
+  if (int rc = pg_plan_absmax(coords, S, npts, geom.ndim, cmax, stream)) return rc;
+  if (int rc = pg_plan_targets(coords, cmax, S, npts, geom, target_idx, distances, gauss, keys, vals,
+                               n_invalid, stream))
+    return rc;
+  if (rep > 0) {
+    const int end_bit = 64 - __builtin_clzll((unsigned long long)(S * (G + 1) - 1) | 1ull);
+    size_t ws = (size_t)tbytes[kSortWs];
+    if (int rc = pg_sort_pairs_u64(T[kSortWs], &ws, keys, keys_s, vals, vals_s, rep, end_bit > 0 ? end_bit : 1,
+                                   stream))
+      return rc;
+  }
+  if (int rc = pg_plan_post(keys_s, vals_s, gauss, n_invalid, S, z.nper, G, indices, sort_perm, inv_perm,
+                            gauss_sorted, cell_ptr, stream))
+    return rc;
+  if (int rc = pg_plan_weights(vals_s, gauss_sorted, n_invalid, cell_ptr, S, z.nper, G, geom.kw, chunk, weights,
+                               sqrt_w, rep_src, rep_s, density, unit_count, z.rmax, stream))
+    return rc;
+  size_t scan_ws = (size_t)tbytes[kScanWs];
+  if (int rc = pg_exclusive_scan_u32(T[kScanWs], &scan_ws, unit_count, cell_unit, z.ncell, stream)) return rc;
+  if (int rc = pg_plan_units(cell_ptr, cell_unit, S, G, chunk, unit_cell, unit_rep, stream)) return rc;
+  auto *a = static_cast<uint32_t *>(T[kA]);
+  auto *hf = static_cast<uint32_t *>(T[kHF]);
+  auto *hn = static_cast<uint32_t *>(T[kHN]);
+  auto *A = static_cast<uint32_t *>(T[kAs]);
+  auto *HF = static_cast<uint32_t *>(T[kHFs]);
+  auto *HN = static_cast<uint32_t *>(T[kHNs]);
+  if (int rc = pg_plan_hot_counts(cell_ptr, cell_unit, S, G, min_units, a, hf, hn, stream)) return rc;
+  for (auto pr : {std::make_pair(a, A), std::make_pair(hf, HF), std::make_pair(hn, HN)}) {
+    scan_ws = (size_t)tbytes[kScanWs];
+    if (int rc = pg_exclusive_scan_u32(T[kScanWs], &scan_ws, pr.first, pr.second, z.ncell, stream)) return rc;
+  }
+  // units of no hot cell stay -1 (pg_plan_hot_rows writes every unit of the
+  // bound when U < Umax only up to U; the rest is never read)
+  hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(z.umax, 256)), dim3(256), 0, st,
+                     reinterpret_cast<uint32_t *>(unit_hot), z.umax, 0xFFFFFFFFu);
+  PG_LAUNCHED();
+  if (int rc = pg_plan_hot_rows(cell_unit, S, G, min_units, A, HF, HN, cell_vrow, unit_hot, hot_units, hot_ptr,
+                                stream))
+    return rc;
+  hipLaunchKernelGGL(k_plan_summary, dim3(1), dim3(256), 0, st, n_invalid, cell_unit, HF, HN, S, G, summary_dev);
+  PG_LAUNCHED();
+  PG_HIP(hipMemcpyAsync(summary, summary_dev, 8 * (3 * S + 3), hipMemcpyDeviceToHost, st));
+  PG_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+}  // extern "C"

@@ -56,9 +56,10 @@ from .calib._grog import (PLAN_RADIUS, GridProgram, GrogInterpolator, _default_o
 
 @dataclasses.dataclass(eq=False)
 class GridJob:
-    """One gridding request (compared and hashed by identity).  ``kspace`` (C, N) complex64 and ``calib``
-    (C, *acr) are device tensors (the request payload, staged on the GPU);
-    ``coords`` is the trajectory (..., ndim) on the host."""
+    """One gridding request (compared and hashed by identity).  ``kspace``
+    (C, N) complex64 and ``calib`` (C, *acr) are device tensors (the request
+    payload, staged on the GPU); ``coords`` is the trajectory (..., ndim),
+    float32, a host array or a device tensor staged with the payload."""
 
     shape: tuple
     coords: np.ndarray
@@ -112,7 +113,8 @@ def estimate_job_bytes(job: GridJob, mode: str) -> int:
     else:
         nsteps = 10 ** job.precision + 1
     K = nsteps ** ndim
-    per_rep = 8 + 4 * ndim + 4 + 8 + 4 + 8 + 8 + 4 * 3 + 8 + 16   # plan fields + records + sort buffers
+    # plan fields + records + sort buffers + the unit arrays' upper bound
+    per_rep = 8 + 4 * ndim + 4 + 8 + 4 + 8 + 8 + 4 * 3 + 8 + 16 + 16
     plan = N * kw * per_rep + G * (4 * 4 + 4 + 4)
     table = K * C * C * (8 + 8 + 8 + 2 * 8 * 2)  # c64, fp64 pair, fp32 packed, bf16 hi/lo real form
     data = 2 * 8 * C * N + 8 * C * G
@@ -138,13 +140,13 @@ def validate_job(job: GridJob) -> str | None:
     ndim = len(shape)
     if ndim not in (2, 3) or min(shape) < 1:
         return f"grid shape {shape} must have 2 or 3 positive axes"
-    coords = np.asarray(job.coords)
-    if coords.ndim < 2 or coords.shape[-1] != ndim:
-        return f"coords shape {coords.shape} does not end in the grid's {ndim} axes"
-    if coords.ndim > 3:
-        return (f"coords shape {coords.shape} is a stack of trajectories; a serving job grids one "
+    cshape = tuple(int(v) for v in (job.coords.shape if hasattr(job.coords, "shape") else np.shape(job.coords)))
+    if len(cshape) < 2 or cshape[-1] != ndim:
+        return f"coords shape {cshape} does not end in the grid's {ndim} axes"
+    if len(cshape) > 3:
+        return (f"coords shape {cshape} is a stack of trajectories; a serving job grids one "
                 "trajectory (submit one job per stack element)")
-    n = int(np.prod(coords.shape[:-1]))
+    n = int(np.prod(cshape[:-1]))
     y = job.kspace
     if not isinstance(y, torch.Tensor) or y.ndim != 2 or not y.is_complex():
         return "kspace must be a complex tensor of shape (coils, samples)"
@@ -279,12 +281,15 @@ class GrogServer:
 
     ``hbm_budget`` bytes bound the admitted jobs' estimated footprint,
     ``slots`` the jobs in preparation at once (one stream and one worker
-    thread each, plus a helper thread and stream for the table fit),
+    thread each, plus a helper thread and stream for the table fit; the
+    preparations' host work shares the interpreter lock, and on MI355X two
+    slots gave both higher saturation throughput and a lower tail than
+    four),
     ``max_inflight`` the gridding batches on the device.  The worker threads
     live as long as the server (``close()`` or a ``with`` block ends them), so
     successive ``run`` calls reuse their per-thread library state."""
 
-    def __init__(self, device=None, *, mode: str = "fp32", hbm_budget: int | None = None, slots: int = 4,
+    def __init__(self, device=None, *, mode: str = "fp32", hbm_budget: int | None = None, slots: int = 2,
                  max_inflight: int = 2, keep_results: bool = False):
         if mode not in ("fp32", "bf16", "bf16x3"):
             raise ValueError(f"unknown gridding mode {mode!r}")

@@ -382,15 +382,18 @@ def test_grid_device_batch_of_stack_bf16(device, C):
 
 
 @pytest.mark.parametrize("case", ["2d", "3d", "stack", "nohot", "allhot"])
-def test_plan_hot_rows_match_torch_lists(device, case):
-    """The device hot lists and row numbering (pg_plan_hot_counts /
-    pg_plan_hot_rows) equal the torch formulation (hot_lists,
-    ordinary_rows) entry for entry: 2D with a hot DC cell, a 3D kooshball,
-    a stack of 3 elements, no hot cells, every occupied cell hot."""
+def test_plan_hot_rows_match_torch_lists(device, monkeypatch, case):
+    """The native plan compiler's reduction units, hot lists and row
+    numbering (pg_plan_compile) equal the torch formulation
+    (compile_units, hot_lists, ordinary_rows) entry for entry: 2D with a hot
+    DC cell, a 3D kooshball, a stack of 3 elements, no hot cells, every
+    occupied cell hot."""
     import pygrog_amd.calib._grog as G
     from pygrog_amd import synth
     from pygrog_amd.calib import GrogInterpolator
 
+    mu = {"nohot": 10**9, "allhot": 1}.get(case, G.HOT_MIN_UNITS)
+    monkeypatch.setattr(G, "HOT_MIN_UNITS", mu)
     if case == "3d":
         g = GrogInterpolator((24, 24, 24), synth.kooshball_3d(300, 32), oversamp=(1, 1, 1), kernel_width=2,
                              kernel_shape="sphere", device=device)
@@ -402,15 +405,15 @@ def test_plan_hot_rows_match_torch_lists(device, case):
         g = GrogInterpolator((80, 80), _radial_full(40, 160, seed=1), kernel_width=3, device=device)
     comp = g.plan._compiled
     S, Gc = comp.S, comp.G
-    U = comp.unit_rep.numel() - S
-    mu = {"nohot": 10**9, "allhot": 1}.get(case, G.HOT_MIN_UNITS)
-    uh, per, vrow = G.plan_hot_rows(comp.cell_ptr, comp.cell_unit, S, Gc, device, U, mu)
+    cu, uc, ur = G.compile_units(comp.cell_ptr, S, Gc, device, G.UNIT_CHUNK)
+    assert torch.equal(comp.cell_unit, cu) and torch.equal(comp.unit_rep, ur)
+    assert torch.equal(comp.unit_cell[:uc.numel()], uc) or uc.numel() == 1
     uh2, per2 = G.hot_lists(comp.cell_unit, S, Gc, device, mu)
     vrow2 = G.ordinary_rows(comp.cell_ptr, comp.cell_unit, S, Gc, mu)
-    assert torch.equal(vrow, vrow2)
-    assert torch.equal(uh, uh2)
+    assert torch.equal(comp.cell_vrow, vrow2)
+    assert torch.equal(comp.unit_hot, uh2)
     total_hot = 0
-    for a, b in zip(per, per2):
+    for a, b in zip(comp.hot, per2):
         assert (a["n_hot"], a["n_hot_units"]) == (b["n_hot"], b["n_hot_units"])
         assert torch.equal(a["hot_ptr"], b["hot_ptr"])
         if a["n_hot_units"]:
