@@ -261,18 +261,58 @@ __device__ __forceinline__ float2 unit_partial(const pg_frame &fr, uint32_t u,
 constexpr int kRowBytes = 16384;
 
 // Cell with several units that is not reduced by the hot pre-pass: fold its
-// unit partials for one coil in canonical order, straight from memory.
+// unit partials for one coil in canonical order, straight from memory, with
+// every loop rolled: one replica, one input coil at a time (the chain of
+// apply_row: vx += Kr yr, vx -= Ki yi, vy += Kr yi, vy += Ki yr in ascending
+// input coil, then (v * s) * s).  The tile kernels take this path only for
+// plans whose pre-pass threshold leaves multi-unit cells out of the hot list;
+// an unrolled copy would multiply their code size and scalar register demand.
 template <int GS, int MODE>
-__device__ __forceinline__ float2 multi_unit_coil(const pg_frame *__restrict__ frames,
-                                               int f, uint32_t ua, uint32_t ub,
-                                               int co, int C) {
-  const pg_frame fr = frames[f];
-  const int P = (C + 1) / 2;
+__device__ __forceinline__ float2 multi_unit_coil(const pg_frame &fr, uint32_t ua,
+                                                      uint32_t ub, int co, int C) {
+#pragma clang fp contract(off)
+  constexpr int PM = GS / 2;
   float cx = 0.0f, cy = 0.0f;
+#pragma unroll 1
   for (uint32_t u = ua; u < ub; ++u) {
-    const float2 p = unit_partial<GS, MODE>(fr, u, co, C, P);
-    cx = cx + p.x;
-    cy = cy + p.y;
+    const uint32_t r0 = gld(fr.unit_rep + u), r1 = gld(fr.unit_rep + u + 1);
+    float px = 0.0f, py = 0.0f;
+#pragma unroll 1
+    for (uint32_t r = r0; r < r1; ++r) {
+      float vx = 0.0f, vy = 0.0f;
+      if (MODE == 1) {
+        const float2 v = gld(reinterpret_cast<const float2 *>(fr.vbuf) + (uint64_t)r * (uint32_t)C + co);
+        vx = v.x;
+        vy = v.y;
+      } else if (MODE == 2) {
+        const float s = gld(fr.rep_s + r);
+        const float2 a = gld(reinterpret_cast<const float2 *>(fr.y) + (uint64_t)co * fr.n_src + gld(fr.rep_src + r));
+        vx = a.x * s;
+        vy = a.y * s;
+      } else {
+        const float s = gld(fr.rep_s + r);
+        const float2 *yr = reinterpret_cast<const float2 *>(fr.y) + (uint64_t)gld(fr.rep_src + r) * (uint32_t)C;
+        const float2 *tb = reinterpret_cast<const float2 *>(fr.table) +
+                           2 * ((uint64_t)gld(fr.rep_ent + r) * PM * (uint32_t)C + co);
+#pragma unroll 1
+        for (int ci = 0; ci < C; ++ci) {
+          const float2 k = gld(tb + 2 * (ci >> 1) * C + (ci & 1));
+          const float2 yv = gld(yr + ci);
+          vx = fmaf(k.x, yv.x, vx);
+          vx = fmaf(-k.y, yv.y, vx);
+          vy = fmaf(k.x, yv.y, vy);
+          vy = fmaf(k.y, yv.x, vy);
+        }
+        vx = vx * s;
+        vy = vy * s;
+        vx = vx * s;
+        vy = vy * s;
+      }
+      px = px + vx;
+      py = py + vy;
+    }
+    cx = cx + px;
+    cy = cy + py;
   }
   return make_float2(cx, cy);
 }
@@ -482,7 +522,7 @@ __global__ __launch_bounds__(kGridThreads) void k_tile(
       for (int k = 0; k < CPT; ++k)
         if (co0 + k < C)
           g[(int64_t)(co0 + k) * fr.n_cell] =
-              multi_unit_coil<GS, MODE>(frames, f, cu_lo[t_own], cu_hi[t_own], co0 + k, C);
+              multi_unit_coil<GS, MODE>(fr, cu_lo[t_own], cu_hi[t_own], co0 + k, C);
     } else if (own && tag[t_own] == -1) {
       float2 *g = reinterpret_cast<float2 *>(fr.grid) + cell0 + t_own;
 #pragma unroll
@@ -686,33 +726,46 @@ __global__ __launch_bounds__(kGridThreads) void k_wtile(
         wave_lds_sync();
         // S: stage the rows: lane = (row, coil) with SW coils per row (all GS
         // coils for MODE 0, zero padded past C; the pass's CW coils else),
-        // up to 16 loads per lane issued before their LDS writes.  Record
+        // NS loads per lane issued before their LDS writes.  Record
         // slots past the batch hold valid indices (reset per tile), so the
         // loads of lanes past the batch are in bounds; they store nothing.
         {
           constexpr int SW = MODE == 0 ? GS : CW;
           constexpr int RPI = kWave / SW;       // rows per load instruction
-          constexpr int NB = 16 * RPI;          // rows per load round
+          // loads per lane and round: 8 for the plain apply at 16 coils
+          // (fewer live registers: four waves per SIMD instead of three,
+          // -5 % on MI355X, 200 frames 192^2 x 16 coils), 16 otherwise
+          constexpr int NS = (MODE == 0 && GS == 16) ? 8 : 16;
+          constexpr int NB = NS * RPI;          // rows per load round
           const int c = l % SW;
           const int j0 = l / SW;
           const int cg = MODE == 0 ? c : cc + c;  // coil of this lane's values
           const bool cv = cg < C;
           const int cs = cv ? cg : 0;
+          // this lane's first value of every row (MODE 2: its coil's row)
+          const float2 *ybase =
+              MODE == 1 ? reinterpret_cast<const float2 *>(fr.vbuf) + cs
+              : MODE == 0 ? reinterpret_cast<const float2 *>(fr.y) + cs
+                          : reinterpret_cast<const float2 *>(fr.y) + (uint64_t)cs * fr.n_src;
           for (int jb = 0; jb < nk; jb += NB) {
-            float2 v[16];
+            // the round's record indices first (all LDS reads in flight
+            // together), then its loads
+            uint32_t qi[NS];
+            float2 v[NS];
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-              if (jb + i * RPI >= nk) break;
-              const uint32_t q = ra[jb + j0 + i * RPI];
-              if (MODE == 0)
-                v[i] = sld(reinterpret_cast<const float2 *>(fr.y) + (int64_t)q * C + cs);
-              else if (MODE == 1)
-                v[i] = sld(reinterpret_cast<const float2 *>(fr.vbuf) + (int64_t)q * C + cs);
-              else
-                v[i] = gld(reinterpret_cast<const float2 *>(fr.y) + (int64_t)cs * fr.n_src + q);
+            for (int i = 0; i < NS; ++i) {
+              const int jj = jb + j0 + i * RPI;
+              qi[i] = ra[jj < RB ? jj : RB - 1];
             }
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
+            for (int i = 0; i < NS; ++i) {
+              if (jb + i * RPI < nk) {
+                if (MODE == 2) v[i] = gld(ybase + qi[i]);
+                else v[i] = sld(ybase + (uint64_t)qi[i] * (uint32_t)C);
+              }
+            }
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
               if (jb + i * RPI >= nk) break;
               const int j = jb + j0 + i * RPI;
               if (j < nk) {
@@ -745,8 +798,8 @@ __global__ __launch_bounds__(kGridThreads) void k_wtile(
               const int j2 = j1 + NG;
               const float4 *y2 = reinterpret_cast<const float4 *>(&rows[j2 * RS]);
               float4 k1[PM], k2[PM];
-              load_pairs<PM>(tb + (int64_t)rb[j1] * PM * C, C, k1);
-              load_pairs<PM>(tb + (int64_t)rb[j2] * PM * C, C, k2);
+              load_pairs<PM>(tb + (uint64_t)rb[j1] * (uint32_t)(PM * C), C, k1);
+              load_pairs<PM>(tb + (uint64_t)rb[j2] * (uint32_t)(PM * C), C, k2);
               float bx = 0.0f, by = 0.0f;
               dot_pairs<PM>(k1, y1, ax, ay);
               dot_pairs<PM>(k2, y2, bx, by);
@@ -756,7 +809,7 @@ __global__ __launch_bounds__(kGridThreads) void k_wtile(
               if (cv && j2 < nk) rows[j2 * RS + co] = make_float2(bx * s2, by * s2);
             } else {
               constexpr int PB = PM < 16 ? PM : 16;
-              const float4 *t1 = tb + (int64_t)rb[j1] * PM * C;
+              const float4 *t1 = tb + (uint64_t)rb[j1] * (uint32_t)(PM * C);
 #pragma unroll
               for (int p0 = 0; p0 < PM; p0 += PB) {
                 float4 k1[PB];
@@ -800,7 +853,7 @@ __global__ __launch_bounds__(kGridThreads) void k_wtile(
 #pragma unroll 1
           for (int c = 0; c < CW; ++c)
             if (cc + c < C)
-              gst(g + (int64_t)(cc + c) * fr.n_cell, multi_unit_coil<GS, MODE>(frames, f, ua, ub, cc + c, C));
+              gst(g + (int64_t)(cc + c) * fr.n_cell, multi_unit_coil<GS, MODE>(fr, ua, ub, cc + c, C));
         }
         wave_lds_sync();
         t0 = t1;
@@ -905,7 +958,7 @@ __global__ __launch_bounds__(kGridThreads) void k_wrows(
         if (!(fr.n_hot > 0 && gld(fr.unit_hot + ua) >= 0)) {
 #pragma unroll 1
           for (int c = 0; c < C; ++c)
-            gst(g + (int64_t)c * fr.n_cell, multi_unit_coil<GS, 1>(frames, h.f, ua, ub, c, C));
+            gst(g + (int64_t)c * fr.n_cell, multi_unit_coil<GS, 1>(fr, ua, ub, c, C));
         }
       } else {
         const uint32_t n = h.cr1 - h.cr0;
