@@ -420,11 +420,11 @@ class GrogServer:
         return ev, keep, bad
 
     # -- the loop ------------------------------------------------------------------
-    def run(self, jobs: list, *, timeout: float = 600.0) -> dict:
+    def run(self, jobs: list, *, timeout: float = 600.0, raw: bool = False) -> dict:
         """Serve ``jobs`` (arrival times relative to the start of the run)
-        and return latency / throughput statistics.  Rejected and failed
-        jobs are listed with their errors and left out of the latency
-        figures."""
+        and return latency / throughput statistics (``raw``: the loop's
+        bookkeeping, for ``summarize_runs``).  Rejected and failed jobs are
+        listed with their errors and left out of the latency figures."""
 
         def retire(job):
             if not self.keep_results:
@@ -434,7 +434,23 @@ class GrogServer:
         r = serve_loop(jobs, budget=self.hbm_budget, slots=self.slots, max_inflight=self.max_inflight,
                        estimate=lambda j: estimate_job_bytes(j, self.mode), prepare=self._prepare,
                        launch=self._launch_batch, submit=self._pool.submit, timeout=timeout, on_retire=retire)
-        return summarize(r, self.mode, self.hbm_budget)
+        return r if raw else summarize(r, self.mode, self.hbm_budget)
+
+
+def summarize_runs(runs: list, mode: str, budget: int) -> dict:
+    """Statistics of several separate streams pooled: every completed job's
+    latency in one distribution, throughput over the summed run times,
+    plus each stream's own p99."""
+    pooled = {"done": [j for r in runs for j in r["done"]],
+              "rejected": [j for r in runs for j in r["rejected"]],
+              "failed": [j for r in runs for j in r["failed"]],
+              "batches": [b for r in runs for b in r["batches"]],
+              "peak_used": max((r["peak_used"] for r in runs), default=0),
+              "makespan": sum(r["makespan"] for r in runs)}
+    out = summarize(pooled, mode, budget)
+    out["streams"] = len(runs)
+    out["p99_ms_per_stream"] = [summarize(r, mode, budget)["p99_ms"] for r in runs]
+    return out
 
 
 def summarize(r: dict, mode: str, budget: int) -> dict:
