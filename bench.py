@@ -529,7 +529,8 @@ SERVE_DESC = ("Poisson stream mixing 2D 256^2 8-coil golden-angle radial jobs (4
               "each with its own ACR-fit table (lamda=1e-3); request payloads (trajectory, ACR, k-space) staged on the device; HBM-budget admission, 2 preparation slots "
               "(plan and table fit overlapped on two threads and streams each), continuous batching of the gridding "
               "launches; one long-lived server per mode, warmed by an untimed first stream; latency per offered rate "
-              "pooled over 3 Poisson streams of the jobs (different arrival draws)")
+              "pooled over 3 Poisson streams of the jobs (different arrival draws), offered at fractions of the median "
+              "of 3 saturation runs")
 
 
 def make_serve_jobs(dev, n_jobs: int, seed: int = 5):
@@ -575,7 +576,11 @@ def run_serving(dev, n_jobs: int, modes=("fp32", "bf16x3"), loads=(0.3, 0.5, 0.7
         # threads are warmed by an untimed first stream
         srv = GrogServer(dev, mode=mode)
         srv.run([dataclasses.replace(j, arrival=0.0) for j in jobs[:8]])
-        sat = srv.run([dataclasses.replace(j, arrival=0.0) for j in jobs])
+        # saturation: the median of three all-at-once runs (one run varies by
+        # +-15 %, and the offered rates below are fractions of it)
+        sats = [srv.run([dataclasses.replace(j, arrival=0.0) for j in jobs]) for _ in range(3)]
+        sat = sorted(sats, key=lambda x: x["throughput_jobs_s"])[1]
+        sat["throughput_jobs_s_runs"] = [x["throughput_jobs_s"] for x in sats]
         mu = sat["throughput_jobs_s"]
         res = {"saturation": sat, "rates": {}}
         for frac in loads:

@@ -306,6 +306,24 @@ int pg_plan_compile(const float *coords, int64_t S, int64_t npts,
                     void *tmp_ws, int64_t tmp_ws_bytes, float *distances,
                     int64_t *summary, pg_stream_t stream);
 
+/* Replicas grouped by table entry for pg_apply_grouped_bf16* (no reference
+ * counterpart: the grouped path's preparation).  For the R replicas with
+ * entries rep_ent[0..R) < K: ent_rep = the replica ids in a stable sort by
+ * entry, ent_ptr (K + 1) each entry's range of positions, items the work list
+ * of (entry, first offset) pairs of chunks of `chunk` replicas in entry
+ * order, ent_src / ent_s each position's source and weight; *n_items (host)
+ * = the work list's length, written after the call has synchronised
+ * `stream`.  pg_group_by_entry_layout gives the work list's capacity and the
+ * temporary workspace's size (1 <= R < 2^31). */
+int pg_group_by_entry_layout(int64_t R, int64_t K, int64_t chunk, int64_t *items_max,
+                             int64_t *tmp_bytes);
+int pg_group_by_entry(const uint32_t *rep_ent, const uint32_t *rep_src,
+                      const float *rep_s, int64_t R, int64_t K, int64_t chunk,
+                      uint32_t *ent_ptr, uint32_t *ent_rep, uint32_t *ent_src,
+                      float *ent_s, uint32_t *items, int64_t items_max,
+                      void *tmp, int64_t tmp_bytes, int64_t *n_items,
+                      pg_stream_t stream);
+
 /* Quantised table entry of every candidate replica (reference
  * calc_interp_table, _grog.py:406-422, generalised from 10**precision to an
  * integer quantisation q):  pfac = q, stepsize = f32(step), nsteps = q + 1,

@@ -136,6 +136,9 @@ _SIGS = {
                              ctypes.c_int32, ctypes.c_float, c_void_p, c_void_p],
     "pg_table_pack_bf16": [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_table_bf16_elems": [c_int64, c_int64, c_int32],
+    "pg_group_by_entry_layout": [c_int64, c_int64, c_int64, c_void_p, c_void_p],
+    "pg_group_by_entry": [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
+                          c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p],
     "pg_apply_grouped_bf16": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_apply_grouped_bf16_s24": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_sparse_apply_c64": [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],

@@ -128,10 +128,11 @@ def _matrix_logm(A: torch.Tensor, host_steps: list | None = None) -> torch.Tenso
     return vecs @ torch.diag_embed(torch.log(vals)) @ torch.linalg.inv(vecs)
 
 
-def grappa_powers(G_units: torch.Tensor, exponents, host_steps: list | None = None) -> torch.Tensor:  # noqa: ARG001
+def grappa_powers(G_units: torch.Tensor, exponents, host_steps: list | None = None,  # noqa: ARG001
+                  device=None) -> torch.Tensor:
     """Fractional powers of a stack of unit operators, (A, C, C) -> (A, n, C, C)
-    complex128 on the operators' device, one batched eigendecomposition for
-    all of them.
+    complex128 on ``device`` (default: the operators' device), one batched
+    eigendecomposition for all of them.
 
     The reference forms G^p = expm(p logm(G)) with logm(G) = V diag(log w)
     V^-1 from G = V diag(w) V^-1 (_grappa.py:62-72, 184-208); that
@@ -149,7 +150,7 @@ def grappa_powers(G_units: torch.Tensor, exponents, host_steps: list | None = No
     vals, vecs = torch.linalg.eig(Gh)
     vinv = torch.linalg.inv(vecs)
     wp = torch.exp(exps.view(1, -1, 1).to(vals.dtype) * torch.log(vals).unsqueeze(1))  # (A, n, C)
-    return ((vecs.unsqueeze(1) * wp.unsqueeze(-2)) @ vinv.unsqueeze(1)).to(G.device)
+    return ((vecs.unsqueeze(1) * wp.unsqueeze(-2)) @ vinv.unsqueeze(1)).to(device if device is not None else G.device)
 
 
 def grappa_power(G_unit: torch.Tensor, exponents, host_steps: list | None = None) -> torch.Tensor:
@@ -237,14 +238,15 @@ def _radial_grappa_op(calib, lamda, coords, host_steps):
     return torch.linalg.matrix_exp(lGy), torch.linalg.matrix_exp(lGx)
 
 
-def _gram_fit(calib: torch.Tensor, lamda: float) -> torch.Tensor:
+def _gram_fit(calib: torch.Tensor, lamda: float, on_host: bool = False) -> torch.Tensor:
     """Unit operators of a Cartesian calibration ``(C, *dims)`` complex128
     on a device, one per axis (axis 0 first), by the normal equations: the
     FP64 products G_d = A_d^H A_d, X_d = A_d^H T_d and |x|^2 from one kernel
     (``pg_grappa_gram_f64``), then ((G_d + lamda |x|^2 I)^-1 X_d)^T solved on
     the host -- the fit of the calibration scaled to unit norm, as the
     reference does (_grappa.py:211-287), without forming the source and
-    target matrices."""
+    target matrices.  ``on_host``: leave the operators on the host (where the
+    table build takes their eigendecompositions next)."""
     C = int(calib.shape[0])
     dims = [int(n) for n in calib.shape[1:]]
     nd = len(dims)
@@ -255,23 +257,26 @@ def _gram_fit(calib: torch.Tensor, lamda: float) -> torch.Tensor:
     n2 = float(h[2 * nd * C * C:].real.sum())
     M = h[:2 * nd * C * C].reshape(nd, 2, C, C)
     eye = torch.eye(C, dtype=torch.complex128)
-    return torch.linalg.solve(M[:, 0] + lamda * n2 * eye, M[:, 1]).mT.to(x.device)
+    ops = torch.linalg.solve(M[:, 0] + lamda * n2 * eye, M[:, 1]).mT
+    return ops if on_host else ops.to(x.device)
 
 
-def train_grappa(train_data, lamda: float | None = None, coords=None, *, device=None, host_steps: list | None = None) -> SimpleNamespace:
+def train_grappa(train_data, lamda: float | None = None, coords=None, *, device=None, host_steps: list | None = None,
+                 host_units: bool = False) -> SimpleNamespace:
     """Unit-shift GROG operators (reference _grappa.py:211-287), FP64.
 
     Cartesian calibration ``(coils, *2 or 3 spatial dims)``, or with
     ``coords`` (spokes, readout, 2) radial self-calibration from
     ``(coils, spokes, readout)`` data (lamda defaults to 0 then, as in the
-    reference)."""
+    reference).  ``host_units``: the normal-equation fit may leave the
+    operators on the host."""
     if lamda is None:
         lamda = 0.01 if coords is None else 0.0
     dev = torch.device(device) if device is not None else torch.device("cpu")
     calib = _to_dev(train_data, dev, torch.complex128)
     if (dev.type != "cpu" and coords is None and lamda >= NORMAL_EQ_MIN_LAMDA and calib.ndim in (3, 4)
             and calib.shape[0] <= 256 and min(calib.shape[1:]) >= 2):
-        ops = _gram_fit(calib, lamda)
+        ops = _gram_fit(calib, lamda, on_host=host_units)
         if calib.ndim == 3:
             return SimpleNamespace(x=ops[0], y=ops[1], z=None)
         return SimpleNamespace(x=ops[2], y=ops[1], z=ops[0])
@@ -315,12 +320,15 @@ def build_table(train_data, radius: float, precision: int | None = 1, lamda: flo
                 device=None, axis_order: str = "reference", keep_fp64: bool = False) -> KernelTableResult:
     """Operator table on ``device`` (complex64 result of an FP64 build)."""
     host_steps: list = []
-    kern = train_grappa(train_data, lamda, device=device, host_steps=host_steps)
+    # the unit operators stay on the host when the fit leaves them there: their
+    # eigendecompositions run there, and only the powers go to the device
+    kern = train_grappa(train_data, lamda, device=device, host_steps=host_steps, host_units=True)
     pfac, stepsize, nsteps = quant_steps(radius, precision, quant)
     deltas = torch.arange(nsteps).float()
     deltas = (deltas - (nsteps - 1) / 2) * stepsize  # float32, as the reference
     units = [kern.x, kern.y] + ([kern.z] if kern.z is not None else [])
-    powers = grappa_powers(torch.stack([torch.as_tensor(u) for u in units]), deltas, host_steps)
+    dev = torch.device(device) if device is not None else units[0].device
+    powers = grappa_powers(torch.stack([torch.as_tensor(u) for u in units]), deltas, host_steps, device=dev)
     Gx, Gy = powers[0], powers[1]
     if kern.z is not None:
         Gz = powers[2]

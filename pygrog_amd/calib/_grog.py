@@ -314,7 +314,7 @@ class EntryGroups(NamedTuple):
 
     ent_ptr: torch.Tensor   # (K + 1) int32: group of entry e = positions [ent_ptr[e], ent_ptr[e+1])
     ent_rep: torch.Tensor   # (R) int32: replica (cell-sorted index) at each position
-    items: torch.Tensor     # (n_items, 2) int32: (entry, first offset) work chunks
+    items: torch.Tensor     # (>= n_items, 2) int32: (entry, first offset) work chunks, the first n_items used
     n_items: int
     ent_src: torch.Tensor   # (R) int32: source sample of each position's replica
     ent_s: torch.Tensor     # (R) float32: sqrt weight of each position's replica
@@ -325,35 +325,28 @@ def group_by_entry(rep_ent: torch.Tensor, R: int, K: int, device, chunk: int, re
     """Replicas of each table entry (a stable radix sort of ``rep_ent[:R]``),
     the work list of chunks of ``chunk`` replicas, and each replica's source
     and weight gathered into that order (so the apply reads its records
-    contiguously instead of gathering them)."""
-    st = _lib.stream(device)
-    keys = rep_ent[:max(R, 1)].to(torch.int32).contiguous()
-    vals = torch.arange(max(R, 1), dtype=torch.int32, device=device)
-    keys_o = torch.empty_like(keys)
-    vals_o = torch.empty_like(vals)
-    if R > 0:
-        _lib.run_with_workspace("pg_sort_pairs_u32", device, _lib.ptr(keys), _lib.ptr(keys_o), _lib.ptr(vals),
-                                _lib.ptr(vals_o), R, max(1, (K - 1).bit_length()), st)
+    contiguously instead of gathering them): one library call
+    (``pg_group_by_entry``), one small host read."""
+    if R <= 0:
+        z = torch.zeros(1, dtype=torch.int32, device=device)
+        return EntryGroups(torch.zeros(K + 1, dtype=torch.int32, device=device), z, torch.zeros(1, 2, dtype=torch.int32,
+                           device=device), 0, z.clone(), torch.zeros(1, dtype=torch.float32, device=device))
+    keys = rep_ent[:R].to(torch.int32).contiguous()
+    src = rep_src[:R].to(torch.int32).contiguous()
+    sw = rep_s[:R].to(torch.float32).contiguous()
+    im, tb = ctypes.c_int64(0), ctypes.c_int64(0)
+    _lib.call("pg_group_by_entry_layout", R, K, chunk, ctypes.addressof(im), ctypes.addressof(tb))
     ent_ptr = torch.empty(K + 1, dtype=torch.int32, device=device)
-    _lib.call("pg_csr_from_sorted_u32", _lib.ptr(keys_o), R, K, _lib.ptr(ent_ptr), st)
-    n_e = (ent_ptr[1:] - ent_ptr[:-1]).to(torch.int64)
-    chunks = (n_e + chunk - 1) // chunk
-    total = int(chunks.sum().item())
-    if total:
-        ent = torch.repeat_interleave(torch.arange(K, device=device), chunks, output_size=total)
-        first = torch.cumsum(chunks, 0) - chunks
-        off = (torch.arange(total, device=device) - first[ent]) * chunk
-        items = torch.stack([ent, off], 1).to(torch.int32).contiguous()
-    else:
-        items = torch.zeros(1, 2, dtype=torch.int32, device=device)
-    if R > 0:
-        order = vals_o[:R].to(torch.int64)
-        ent_src = rep_src[:R][order].to(torch.int32).contiguous()
-        ent_s = rep_s[:R][order].contiguous()
-    else:
-        ent_src = torch.zeros(1, dtype=torch.int32, device=device)
-        ent_s = torch.zeros(1, dtype=torch.float32, device=device)
-    return EntryGroups(ent_ptr, vals_o, items, total, ent_src, ent_s)
+    ent_rep = torch.empty(R, dtype=torch.int32, device=device)
+    ent_src = torch.empty(R, dtype=torch.int32, device=device)
+    ent_s = torch.empty(R, dtype=torch.float32, device=device)
+    items = torch.empty(int(im.value), 2, dtype=torch.int32, device=device)
+    tmp = torch.empty(int(tb.value), dtype=torch.uint8, device=device)
+    n = ctypes.c_int64(0)
+    _lib.call("pg_group_by_entry", _lib.ptr(keys), _lib.ptr(src), _lib.ptr(sw), R, K, chunk, _lib.ptr(ent_ptr),
+              _lib.ptr(ent_rep), _lib.ptr(ent_src), _lib.ptr(ent_s), _lib.ptr(items), int(im.value), _lib.ptr(tmp),
+              int(tb.value), ctypes.addressof(n), _lib.stream(device))
+    return EntryGroups(ent_ptr, ent_rep, items, int(n.value), ent_src, ent_s)
 
 
 # The BF16 modes can keep their sample-major copy of the k-space in the

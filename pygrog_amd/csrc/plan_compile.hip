@@ -135,6 +135,40 @@ __global__ void k_plan_summary(const int64_t *__restrict__ n_invalid,
   }
 }
 
+__global__ void k_iota_u32(uint32_t *__restrict__ p, int64_t n) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
+    p[t] = (uint32_t)t;
+}
+
+// chunks of `chunk` replicas of each entry; cnt[K] = 0 (the scan's last input)
+__global__ void k_entry_chunks(const uint32_t *__restrict__ ent_ptr, int64_t K, int64_t chunk,
+                               uint32_t *__restrict__ cnt) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e <= K; e += (int64_t)gridDim.x * blockDim.x)
+    cnt[e] = e < K ? (uint32_t)(((int64_t)(ent_ptr[e + 1] - ent_ptr[e]) + chunk - 1) / chunk) : 0u;
+}
+
+// work item first[e] + c = (entry e, offset c * chunk)
+__global__ void k_entry_items(const uint32_t *__restrict__ first, const uint32_t *__restrict__ cnt, int64_t K,
+                              int64_t chunk, uint32_t *__restrict__ items) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < K; e += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t f = first[e], n = cnt[e];
+    for (uint32_t c = 0; c < n; ++c) {
+      items[2 * (int64_t)(f + c)] = (uint32_t)e;
+      items[2 * (int64_t)(f + c) + 1] = (uint32_t)(c * chunk);
+    }
+  }
+}
+
+__global__ void k_gather_records(const uint32_t *__restrict__ ent_rep, const uint32_t *__restrict__ rep_src,
+                                 const float *__restrict__ rep_s, int64_t R, uint32_t *__restrict__ ent_src,
+                                 float *__restrict__ ent_s) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < R; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t r = ent_rep[i];
+    ent_src[i] = rep_src[r];
+    ent_s[i] = rep_s[r];
+  }
+}
+
 __global__ void k_fill_u32(uint32_t *__restrict__ p, int64_t n, uint32_t v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -286,6 +320,80 @@ int pg_plan_compile(const float *coords, int64_t S, int64_t npts,
   PG_LAUNCHED();
   PG_HIP(hipMemcpyAsync(summary, summary_dev, 8 * (3 * S + 3), hipMemcpyDeviceToHost, st));
   PG_HIP(hipStreamSynchronize(st));
+  return 0;
+}
+
+// bits of the entries 0 .. K-1 (at least one)
+static int entry_bits(int64_t K) { return 64 - __builtin_clzll((unsigned long long)(K - 1) | 1ull); }
+
+// Replicas grouped by table entry for the grouped apply, in one call: a
+// stable radix sort of the entries (values = replica ids), each entry's range
+// of replicas (CSR, K + 1), the work list of (entry, first offset) chunks of
+// `chunk` replicas in entry order, and each replica's source and weight
+// gathered into entry order.  Temporary arrays come from one caller
+// workspace (pg_group_by_entry_layout); the work-list length is copied to
+// the host at the end (the stream is synchronised).
+int pg_group_by_entry_layout(int64_t R, int64_t K, int64_t chunk, int64_t *items_max,
+                             int64_t *tmp_bytes) {
+  PG_CHECK(R >= 1 && R < (int64_t)INT32_MAX && K >= 1 && K < (int64_t)INT32_MAX - 1 && chunk >= 1,
+           "group_by_entry: bad sizes R=%lld K=%lld chunk=%lld", (long long)R, (long long)K,
+           (long long)chunk);
+  PG_CHECK(items_max && tmp_bytes, "group_by_entry: null output");
+  size_t sort_ws = 0, scan_ws = 0;
+  if (int rc = pg_sort_pairs_u32(nullptr, &sort_ws, nullptr, nullptr, nullptr, nullptr, R, entry_bits(K), nullptr))
+    return rc;
+  if (int rc = pg_exclusive_scan_u32(nullptr, &scan_ws, nullptr, nullptr, K, nullptr)) return rc;
+  auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
+  *items_max = R / chunk + K + 1;
+  *tmp_bytes = up((int64_t)sort_ws) + up((int64_t)scan_ws) + 2 * up(4 * R) + 2 * up(4 * (K + 1));
+  return 0;
+}
+
+int pg_group_by_entry(const uint32_t *rep_ent, const uint32_t *rep_src, const float *rep_s, int64_t R,
+                      int64_t K, int64_t chunk, uint32_t *ent_ptr, uint32_t *ent_rep, uint32_t *ent_src,
+                      float *ent_s, uint32_t *items, int64_t items_max, void *tmp, int64_t tmp_bytes,
+                      int64_t *n_items, pg_stream_t stream) {
+  int64_t im = 0, tb = 0;
+  if (int rc = pg_group_by_entry_layout(R, K, chunk, &im, &tb)) return rc;
+  PG_CHECK(items_max >= im && tmp_bytes >= tb, "group_by_entry: work list or workspace too small");
+  PG_CHECK(rep_ent && rep_src && rep_s && ent_ptr && ent_rep && ent_src && ent_s && items && tmp && n_items,
+           "group_by_entry: null pointer");
+  const hipStream_t st = as_stream(stream);
+  size_t sort_ws = 0, scan_ws = 0;
+  const int end_bit = entry_bits(K);
+  if (int rc = pg_sort_pairs_u32(nullptr, &sort_ws, nullptr, nullptr, nullptr, nullptr, R, end_bit, nullptr))
+    return rc;
+  if (int rc = pg_exclusive_scan_u32(nullptr, &scan_ws, nullptr, nullptr, K, nullptr)) return rc;
+  auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
+  auto *base = static_cast<unsigned char *>(tmp);
+  void *sort_t = base;
+  base += up((int64_t)sort_ws);
+  void *scan_t = base;
+  base += up((int64_t)scan_ws);
+  auto *iota = reinterpret_cast<uint32_t *>(base);
+  base += up(4 * R);
+  auto *keys_o = reinterpret_cast<uint32_t *>(base);
+  base += up(4 * R);
+  auto *cnt = reinterpret_cast<uint32_t *>(base);
+  base += up(4 * (K + 1));
+  auto *first = reinterpret_cast<uint32_t *>(base);
+
+  hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(R, 256)), dim3(256), 0, st, iota, R);
+  PG_LAUNCHED();
+  if (int rc = pg_sort_pairs_u32(sort_t, &sort_ws, rep_ent, keys_o, iota, ent_rep, R, end_bit, stream)) return rc;
+  if (int rc = pg_csr_from_sorted_u32(keys_o, R, K, ent_ptr, stream)) return rc;
+  hipLaunchKernelGGL(k_entry_chunks, dim3(grid_for(K + 1, 256)), dim3(256), 0, st, ent_ptr, K, chunk, cnt);
+  PG_LAUNCHED();
+  if (int rc = pg_exclusive_scan_u32(scan_t, &scan_ws, cnt, first, K, stream)) return rc;
+  hipLaunchKernelGGL(k_entry_items, dim3(grid_for(K, 256)), dim3(256), 0, st, first, cnt, K, chunk, items);
+  PG_LAUNCHED();
+  hipLaunchKernelGGL(k_gather_records, dim3(grid_for(R, 256)), dim3(256), 0, st, ent_rep, rep_src, rep_s, R,
+                     ent_src, ent_s);
+  PG_LAUNCHED();
+  uint32_t total = 0;
+  PG_HIP(hipMemcpyAsync(&total, first + K, 4, hipMemcpyDeviceToHost, st));
+  PG_HIP(hipStreamSynchronize(st));
+  *n_items = total;
   return 0;
 }
 

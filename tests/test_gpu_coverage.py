@@ -420,3 +420,36 @@ def test_plan_hot_rows_match_torch_lists(device, monkeypatch, case):
             assert torch.equal(a["hot_units"], b["hot_units"])
         total_hot += a["n_hot"]
     assert (total_hot == 0) == (case == "nohot")
+
+
+@pytest.mark.parametrize("R,K,chunk", [(1, 1, 128), (1000, 7, 128), (250000, 1089, 128), (100000, 68921, 128),
+                                       (5000, 3, 1), (70000, 2, 128)])
+def test_group_by_entry_matches_torch(device, R, K, chunk):
+    """pg_group_by_entry (one call) against the torch formulation of the same
+    contract: replicas in a stable sort by entry, each entry's range, the
+    work list of (entry, offset) chunks in entry order, the gathered sources
+    and weights -- equal entry for entry; skewed entry counts included."""
+    from pygrog_amd.calib._grog import group_by_entry
+
+    g = torch.Generator(device="cpu").manual_seed(R + K)
+    # a few entries carry most replicas, as the zero-shift entries do
+    heavy = torch.randint(0, K, (max(K // 50, 1),), generator=g)
+    ent = torch.where(torch.rand(R, generator=g) < 0.5, heavy[torch.randint(0, heavy.numel(), (R,), generator=g)],
+                      torch.randint(0, K, (R,), generator=g)).to(torch.int32)
+    src = torch.randint(0, 10**6, (R,), generator=g, dtype=torch.int32)
+    sw = torch.rand(R, generator=g)
+    grp = group_by_entry(ent.to(device), R, K, device, chunk, src.to(device), sw.to(device))
+    order = torch.sort(ent.to(torch.int64), stable=True).indices
+    counts = torch.bincount(ent.to(torch.int64), minlength=K)
+    ptr = torch.cat([torch.zeros(1, dtype=torch.int64), torch.cumsum(counts, 0)])
+    chunks = (counts + chunk - 1) // chunk
+    total = int(chunks.sum())
+    e = torch.repeat_interleave(torch.arange(K), chunks)
+    first = torch.cumsum(chunks, 0) - chunks
+    off = (torch.arange(total) - first[e]) * chunk
+    assert grp.n_items == total
+    assert torch.equal(grp.ent_ptr.cpu().to(torch.int64), ptr)
+    assert torch.equal(grp.ent_rep.cpu().to(torch.int64), order)
+    assert torch.equal(grp.items[:total].cpu().to(torch.int64), torch.stack([e, off], 1))
+    assert torch.equal(grp.ent_src.cpu(), src[order])
+    assert torch.equal(grp.ent_s.cpu(), sw[order])
