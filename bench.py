@@ -594,7 +594,8 @@ def run_serving(dev, n_jobs: int, modes=("fp32", "bf16x3"), loads=(0.3, 0.5, 0.7
             res["rates"][f"{frac:.2f}"] = st
             log(f"serve {mode} @ {lam:.2f} jobs/s ({frac:.0%} of saturation): p50 {st['p50_ms']:.1f} ms, "
                 f"p99 {st['p99_ms']:.1f} ms (per stream {[round(v, 1) for v in st['p99_ms_per_stream']]}), "
-                f"{st['throughput_jobs_s']:.2f} jobs/s, mean batch {st['mean_batch']:.2f}")
+                f"{st['throughput_jobs_s']:.2f} jobs/s, mean batch {st['mean_batch']:.2f}, "
+                f"gc pauses {st['gc_pauses']} (max {st['gc_pause_ms_max']:.1f} ms)")
         srv.close()
         kept = [k for k, st in res["rates"].items() if st["throughput_jobs_s"] >= 0.95 * st["rate_jobs_s"]]
         res["sustained"] = max(kept, key=float) if kept else min(res["rates"], key=float)

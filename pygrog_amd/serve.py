@@ -43,6 +43,7 @@ __all__ = ["GridJob", "GrogServer", "estimate_job_bytes", "poisson_arrivals", "s
 
 import concurrent.futures as cf
 import dataclasses
+import gc
 import time
 
 import numpy as np
@@ -431,9 +432,23 @@ class GrogServer:
                 job.result = None
             job.interp = None
 
-        r = serve_loop(jobs, budget=self.hbm_budget, slots=self.slots, max_inflight=self.max_inflight,
-                       estimate=lambda j: estimate_job_bytes(j, self.mode), prepare=self._prepare,
-                       launch=self._launch_batch, submit=self._pool.submit, timeout=timeout, on_retire=retire)
+        # garbage-collector pauses during the run, reported with its figures
+        pauses, t_gc = [], {}
+
+        def on_gc(phase, info):
+            if phase == "start":
+                t_gc["t"] = time.perf_counter()
+            elif "t" in t_gc:
+                pauses.append((info.get("generation", -1), time.perf_counter() - t_gc.pop("t")))
+
+        gc.callbacks.append(on_gc)
+        try:
+            r = serve_loop(jobs, budget=self.hbm_budget, slots=self.slots, max_inflight=self.max_inflight,
+                           estimate=lambda j: estimate_job_bytes(j, self.mode), prepare=self._prepare,
+                           launch=self._launch_batch, submit=self._pool.submit, timeout=timeout, on_retire=retire)
+        finally:
+            gc.callbacks.remove(on_gc)
+        r["gc_pauses"] = pauses
         return r if raw else summarize(r, self.mode, self.hbm_budget)
 
 
@@ -446,7 +461,8 @@ def summarize_runs(runs: list, mode: str, budget: int) -> dict:
               "failed": [j for r in runs for j in r["failed"]],
               "batches": [b for r in runs for b in r["batches"]],
               "peak_used": max((r["peak_used"] for r in runs), default=0),
-              "makespan": sum(r["makespan"] for r in runs)}
+              "makespan": sum(r["makespan"] for r in runs),
+              "gc_pauses": [p for r in runs for p in r.get("gc_pauses", ())]}
     out = summarize(pooled, mode, budget)
     out["streams"] = len(runs)
     out["p99_ms_per_stream"] = [summarize(r, mode, budget)["p99_ms"] for r in runs]
@@ -487,4 +503,6 @@ def summarize(r: dict, mode: str, budget: int) -> dict:
         "coil_samples_per_s": cs / makespan if makespan > 0 else 0.0,
         "makespan_s": makespan, "hbm_budget_gb": budget / 1e9, "peak_admitted_gb": r["peak_used"] / 1e9,
         "by_kind": by_kind,
+        "gc_pauses": len(r.get("gc_pauses", ())),
+        "gc_pause_ms_max": max((d for _, d in r.get("gc_pauses", ())), default=0.0) * 1e3,
     }
