@@ -9,9 +9,11 @@
  *     element counts.  Complex arrays are interleaved (re, im) pairs:
  *     "c64" = float2, "c128" = double2.
  *   - `stream` is a hipStream_t (NULL = legacy default stream).  All work is
- *     enqueued asynchronously on it; no entry point synchronises the host,
- *     and no entry point allocates device memory: outputs and workspaces
- *     are caller-provided.
+ *     enqueued asynchronously on it; the entry points that end with a small
+ *     host read (pg_plan_compile, pg_group_by_entry) synchronise that stream
+ *     and say so, no other entry point waits on the device, and no entry
+ *     point allocates device memory: outputs and workspaces are
+ *     caller-provided.
  *
  * The first block mirrors the seven ops of the reference's pybind module
  * `_pygrog_torch` (csrc/torch/module.cpp:54-87), one entry per dtype.  The
@@ -551,6 +553,21 @@ int pg_sense_combine_c64(const void *full, const void *smaps, int64_t B,
 int pg_vdot_c64_f64(const void *a, const void *b, int64_t n, void *workspace,
                     double *out, pg_stream_t stream);
 int64_t pg_vdot_workspace_bytes(void);
+
+/* One iteration of the CG solver's vector and scalar work (cg with FP64
+ * scalars and no preconditioner, src/pygrog/_solve/_solvers.py:49-139),
+ * fused: sc (FP64, device) holds [0] rz, [1] alpha, [2] beta, [3] r0,
+ * [4] rel, [5] rr.  pg_cg_step_xr: pAp = Re<p, Ap>, alpha = rz / (pAp +
+ * 1e-30), x_out = x_in + f32(alpha) p, r -= f32(alpha) Ap, rr = Re<r, r>,
+ * rel = sqrt(rr) / r0, beta = rr / (rz + 1e-30), rz = rr (workspace of
+ * pg_vdot_workspace_bytes()).  pg_cg_step_p: p = r + f32(beta) p.  The same
+ * roundings and sum orders as the separate operations (pg_vdot_c64_f64 for
+ * the inner products), so the iterates equal theirs bit for bit. */
+int pg_cg_step_xr(const void *p, const void *ap, const void *x_in, void *x_out,
+                  void *r, int64_t n, double *sc, void *workspace,
+                  pg_stream_t stream);
+int pg_cg_step_p(const void *r, void *p, int64_t n, const double *sc,
+                 pg_stream_t stream);
 
 /* pg_vdot_c64_f64 without a workspace and without allocating: the same
  * kernels and the same bits, with the block partials in a 16 KiB array the

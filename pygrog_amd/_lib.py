@@ -136,6 +136,8 @@ _SIGS = {
                              ctypes.c_int32, ctypes.c_float, c_void_p, c_void_p],
     "pg_table_pack_bf16": [c_void_p, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_table_bf16_elems": [c_int64, c_int64, c_int32],
+    "pg_cg_step_xr": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p],
+    "pg_cg_step_p": [c_void_p, c_void_p, c_int64, c_void_p, c_void_p],
     "pg_group_by_entry_layout": [c_int64, c_int64, c_int64, c_void_p, c_void_p],
     "pg_group_by_entry": [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
                           c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p],

@@ -102,6 +102,10 @@ def cg(op, b, *, damp: float = 0.0, x0=None, max_iter: int = 20, tol: float = 1e
             callback(k_, x_, rel_)
         return rel_ < tol
 
+    if (pipelined and preconditioner is None and dt == torch.float64 and x.dtype == torch.complex64
+            and p.dtype == torch.complex64 and x.numel() > 0):
+        return _cg_fused(AHA, x, r, p, rz_old, r0, damp, max_iter, tol, callback, rel_host)
+
     for k in range(max_iter):
         Ap = AHA(p)
         if damp != 0.0:
@@ -135,6 +139,61 @@ def cg(op, b, *, damp: float = 0.0, x0=None, max_iter: int = 20, tol: float = 1e
         _settle(pending)
         return pending[1]
     return x
+
+
+def _cg_fused(AHA, x, r, p, rz_old, r0, damp, max_iter, tol, callback, rel_host):
+    """The loop of ``cg`` for complex64 vectors on a HIP device with FP64
+    scalars and no preconditioner: each iteration's vector and scalar work in
+    two library calls (``pg_cg_step_xr``: <p, Ap>, alpha, x' = x + a p,
+    r -= a Ap, <r, r>, the residual ratio and beta; ``pg_cg_step_p``:
+    p = r + b p) with the scalars kept on the device.  The arithmetic is the
+    generic loop's, rounding for rounding, so the iterates are its bit for
+    bit; the residual test is read one iteration late as there.  Two x
+    buffers alternate, so iterate k is intact when its test is read."""
+    from .. import _lib
+
+    dev = x.device
+    st = _lib.stream(dev)
+    shape = x.shape
+    n = x.numel()
+    xs = [x.reshape(-1).contiguous().clone(), torch.empty(n, dtype=x.dtype, device=dev)]
+    r = r.reshape(-1).contiguous().clone()
+    p = p.reshape(-1).contiguous().clone()
+    sc = torch.zeros(8, dtype=torch.float64, device=dev)
+    sc[0] = rz_old
+    sc[3] = r0
+    ws = torch.empty(int(_lib.load().pg_vdot_workspace_bytes()), dtype=torch.uint8, device=dev)
+    pending = None  # (k, x_k, event, slot)
+
+    def settle(item):
+        k_, x_, ev_, slot_ = item
+        ev_.synchronize()
+        rel_ = float(rel_host[slot_])
+        if callback is not None:
+            callback(k_, x_.reshape(shape), rel_)
+        return rel_ < tol
+
+    cur = 0
+    for k in range(max_iter):
+        Ap = AHA(p.reshape(shape))
+        if damp != 0.0:
+            Ap = Ap + damp * p.reshape(shape)
+        Ap = Ap.reshape(-1).contiguous()
+        _lib.call("pg_cg_step_xr", _lib.ptr(p), _lib.ptr(Ap), _lib.ptr(xs[cur]), _lib.ptr(xs[1 - cur]), _lib.ptr(r),
+                  n, _lib.ptr(sc), _lib.ptr(ws), st)
+        cur = 1 - cur
+        slot = k % 2
+        rel_host[slot].copy_(sc[4], non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        if pending is not None and settle(pending):
+            return pending[1].reshape(shape)
+        pending = (k, xs[cur], ev, slot)
+        _lib.call("pg_cg_step_p", _lib.ptr(r), _lib.ptr(p), n, _lib.ptr(sc), st)
+    if pending is not None:
+        settle(pending)
+        return pending[1].reshape(shape)
+    return xs[cur].reshape(shape)
 
 
 def lsmr(op, b, *, damp: float = 0.0, x0=None, max_iter: int = 20, atol: float = 1e-6, btol: float = 1e-6,

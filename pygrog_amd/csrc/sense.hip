@@ -190,11 +190,40 @@ int pg_sense_combine_c64(const void *full, const void *smaps, int64_t B,
 // ---------------------------------------------------------------------------
 // FP64-accumulated inner product of complex64 vectors (the CG / LSMR scalars):
 // out[0] + i out[1] = sum_i conj(a_i) b_i, summed in double, deterministic
-// (fixed per-block ranges, fixed tree, then the block partials in order).
+// (fixed per-block ranges, fixed tree; the block partials by a fixed tree).
 namespace pg {
 
 constexpr int kDotThreads = 256;
 constexpr int kDotBlocks = 1024;
+
+// conj(x) y accumulated in double, each product and sum rounded on its own
+// (no contraction), so every kernel that forms these sums over the same
+// elements in the same order gets the same bits
+__device__ __forceinline__ void dot_acc(double &re, double &im, float2 x, float2 y) {
+#pragma clang fp contract(off)
+  re = re + ((double)x.x * (double)y.x + (double)x.y * (double)y.y);
+  im = im + ((double)x.x * (double)y.y - (double)x.y * (double)y.x);
+}
+
+// Block b of nb owns elements [b per, min((b + 1) per, n)), per = ceil(n / nb);
+// thread t adds elements t, t + 256, ... of its block's range in order; the
+// block's 256 sums then go through a fixed tree.
+__host__ __device__ __forceinline__ int dot_blocks(int64_t n) {
+  int64_t nb = (n + kDotThreads - 1) / kDotThreads;
+  if (nb > kDotBlocks) nb = kDotBlocks;
+  return nb < 1 ? 1 : (int)nb;
+}
+
+__device__ __forceinline__ void block_tree(double *sr, double *si) {
+  __syncthreads();
+  for (int s = kDotThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sr[threadIdx.x] += sr[threadIdx.x + s];
+      si[threadIdx.x] += si[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+}
 
 __global__ __launch_bounds__(kDotThreads) void k_vdot_partial(
     const float2 *__restrict__ a, const float2 *__restrict__ b, int64_t n,
@@ -204,34 +233,120 @@ __global__ __launch_bounds__(kDotThreads) void k_vdot_partial(
   const int64_t i0 = (int64_t)blockIdx.x * per;
   const int64_t i1 = i0 + per < n ? i0 + per : n;
   double re = 0.0, im = 0.0;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += kDotThreads) {
-    const float2 x = a[i], y = b[i];
-    re += (double)x.x * (double)y.x + (double)x.y * (double)y.y;
-    im += (double)x.x * (double)y.y - (double)x.y * (double)y.x;
-  }
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kDotThreads) dot_acc(re, im, a[i], b[i]);
   sr[threadIdx.x] = re;
   si[threadIdx.x] = im;
-  __syncthreads();
-  for (int s = kDotThreads / 2; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      sr[threadIdx.x] += sr[threadIdx.x + s];
-      si[threadIdx.x] += si[threadIdx.x + s];
-    }
-    __syncthreads();
-  }
+  block_tree(sr, si);
   if (threadIdx.x == 0) part[blockIdx.x] = make_double2(sr[0], si[0]);
 }
 
-__global__ void k_vdot_final(const double2 *__restrict__ part, int nb,
-                             double *__restrict__ out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// The block partials summed by one workgroup: thread t adds partials t,
+// t + 256, ... in order, then the same fixed tree as k_vdot_partial (every
+// load in flight at once; one thread walking the partials took a memory
+// round trip per partial: 34 us for 400 of them).
+__global__ __launch_bounds__(kDotThreads) void k_vdot_final(const double2 *__restrict__ part, int nb,
+                                                            double *__restrict__ out) {
+  __shared__ double sr[kDotThreads], si[kDotThreads];
   double re = 0.0, im = 0.0;
-  for (int i = 0; i < nb; ++i) {
-    re += part[i].x;
-    im += part[i].y;
+  for (int i = threadIdx.x; i < nb; i += kDotThreads) {
+    const double2 v = part[i];
+    re += v.x;
+    im += v.y;
   }
-  out[0] = re;
-  out[1] = im;
+  sr[threadIdx.x] = re;
+  si[threadIdx.x] = im;
+  block_tree(sr, si);
+  if (threadIdx.x == 0) {
+    out[0] = sr[0];
+    out[1] = si[0];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One CG iteration's vector and scalar work, fused (cg with FP64 scalars and
+// no preconditioner): the same arithmetic as the host loop -- pAp = Re<p, Ap>,
+// alpha = rz / (pAp + 1e-30), a = f32(alpha), x' = x + a p, r = r - a Ap,
+// rr = Re<r, r>, rel = sqrt(rr) / r0, beta = rr / (rz + 1e-30), rz = rr,
+// p = r + f32(beta) p -- with every product and sum rounded on its own as the
+// separate torch operations round them, and the sums in the FP64 dot's block
+// decomposition and trees, so the iterates are the host loop's bit for bit.
+// sc (FP64, device): [0] rz, [1] alpha, [2] beta, [3] r0, [4] rel, [5] rr.
+enum CgScalar { kRz = 0, kAlpha, kBeta, kR0, kRel, kRr };
+
+// block partials -> Re of the sum, by the same tree as k_vdot_final
+__device__ __forceinline__ double reduce_parts_re(const double2 *__restrict__ part, int nb, double *sr, double *si) {
+  double re = 0.0, im = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kDotThreads) {
+    const double2 v = part[i];
+    re += v.x;
+    im += v.y;
+  }
+  sr[threadIdx.x] = re;
+  si[threadIdx.x] = im;
+  block_tree(sr, si);
+  return sr[0];
+}
+
+__global__ __launch_bounds__(kDotThreads) void k_cg_alpha(const double2 *__restrict__ part, int nb,
+                                                          double *__restrict__ sc) {
+#pragma clang fp contract(off)
+  __shared__ double sr[kDotThreads], si[kDotThreads];
+  const double pap = reduce_parts_re(part, nb, sr, si);
+  if (threadIdx.x == 0) sc[kAlpha] = sc[kRz] / (pap + 1e-30);
+}
+
+// x_out = x_in + a p, r = r - a Ap, and the block partials of <r, r>
+__global__ __launch_bounds__(kDotThreads) void k_cg_xr(const float2 *__restrict__ p, const float2 *__restrict__ ap,
+                                                       const float2 *__restrict__ x_in, float2 *__restrict__ x_out,
+                                                       float2 *__restrict__ r, int64_t n,
+                                                       const double *__restrict__ sc, double2 *__restrict__ part) {
+#pragma clang fp contract(off)
+  __shared__ double sr[kDotThreads], si[kDotThreads];
+  const float a = __double2float_rn(sc[kAlpha]);
+  const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * per;
+  const int64_t i1 = i0 + per < n ? i0 + per : n;
+  double re = 0.0, im = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += kDotThreads) {
+    const float2 pv = p[i], av = ap[i], xv = x_in[i];
+    float2 rv = r[i];
+    // plain operators under this kernel's contract(off): the products and
+    // sums are rounded separately, as the separate torch operations round
+    // them (an inlined helper's operators would not be covered)
+    x_out[i] = make_float2(xv.x + a * pv.x, xv.y + a * pv.y);
+    rv = make_float2(rv.x - a * av.x, rv.y - a * av.y);
+    r[i] = rv;
+    dot_acc(re, im, rv, rv);
+  }
+  sr[threadIdx.x] = re;
+  si[threadIdx.x] = im;
+  block_tree(sr, si);
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(sr[0], si[0]);
+}
+
+__global__ __launch_bounds__(kDotThreads) void k_cg_beta(const double2 *__restrict__ part, int nb,
+                                                         double *__restrict__ sc) {
+#pragma clang fp contract(off)
+  __shared__ double sr[kDotThreads], si[kDotThreads];
+  const double rr = reduce_parts_re(part, nb, sr, si);
+  if (threadIdx.x == 0) {
+    const double rz = sc[kRz];
+    sc[kRr] = rr;
+    sc[kRel] = sqrt(rr) / sc[kR0];
+    sc[kBeta] = rr / (rz + 1e-30);
+    sc[kRz] = rr;
+  }
+}
+
+// p = r + f32(beta) p
+__global__ void k_cg_p(const float2 *__restrict__ r, float2 *__restrict__ p, int64_t n,
+                       const double *__restrict__ sc) {
+#pragma clang fp contract(off)
+  const float b = __double2float_rn(sc[kBeta]);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float2 pv = p[i], rv = r[i];
+    p[i] = make_float2(rv.x + b * pv.x, rv.y + b * pv.y);
+  }
 }
 
 }  // namespace pg
@@ -266,15 +381,13 @@ static int vdot_scratch_address(void **out) {
 
 static int vdot_launch(const void *a, const void *b, int64_t n, void *part,
                        double *out, hipStream_t st) {
-  int nb = (int)((n + kDotThreads - 1) / kDotThreads);
-  if (nb > kDotBlocks) nb = kDotBlocks;
-  if (nb < 1) nb = 1;
+  const int nb = dot_blocks(n);
   hipLaunchKernelGGL(k_vdot_partial, dim3(nb), dim3(kDotThreads), 0, st,
                      static_cast<const float2 *>(a),
                      static_cast<const float2 *>(b), n,
                      static_cast<double2 *>(part));
   PG_LAUNCHED();
-  hipLaunchKernelGGL(k_vdot_final, dim3(1), dim3(64), 0, st,
+  hipLaunchKernelGGL(k_vdot_final, dim3(1), dim3(kDotThreads), 0, st,
                      static_cast<const double2 *>(part), nb, out);
   PG_LAUNCHED();
   return 0;
@@ -285,6 +398,35 @@ static int vdot_launch(const void *a, const void *b, int64_t n, void *part,
 extern "C" {
 
 int64_t pg_vdot_workspace_bytes(void) { return (int64_t)kDotBlocks * 16; }
+
+int pg_cg_step_xr(const void *p, const void *ap, const void *x_in, void *x_out, void *r, int64_t n,
+                  double *sc, void *workspace, pg_stream_t stream) {
+  PG_CHECK(n >= 1 && p && ap && x_in && x_out && r && sc && workspace, "cg_step_xr: bad arguments");
+  PG_CHECK(x_in != x_out, "cg_step_xr: x_in and x_out must differ");
+  const hipStream_t st = as_stream(stream);
+  const int nb = dot_blocks(n);
+  auto *part = static_cast<double2 *>(workspace);
+  hipLaunchKernelGGL(k_vdot_partial, dim3(nb), dim3(kDotThreads), 0, st, static_cast<const float2 *>(p),
+                     static_cast<const float2 *>(ap), n, part);
+  PG_LAUNCHED();
+  hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kDotThreads), 0, st, part, nb, sc);
+  PG_LAUNCHED();
+  hipLaunchKernelGGL(k_cg_xr, dim3(nb), dim3(kDotThreads), 0, st, static_cast<const float2 *>(p),
+                     static_cast<const float2 *>(ap), static_cast<const float2 *>(x_in),
+                     static_cast<float2 *>(x_out), static_cast<float2 *>(r), n, sc, part);
+  PG_LAUNCHED();
+  hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kDotThreads), 0, st, part, nb, sc);
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_cg_step_p(const void *r, void *p, int64_t n, const double *sc, pg_stream_t stream) {
+  PG_CHECK(n >= 1 && r && p && sc, "cg_step_p: bad arguments");
+  hipLaunchKernelGGL(k_cg_p, dim3(grid_for(n, 256)), dim3(256), 0, as_stream(stream),
+                     static_cast<const float2 *>(r), static_cast<float2 *>(p), n, sc);
+  PG_LAUNCHED();
+  return 0;
+}
 
 int pg_vdot_c64_f64_noalloc(const void *a, const void *b, int64_t n,
                             double *out, pg_stream_t stream) {

@@ -470,6 +470,37 @@ def test_cg_stops_like_the_synchronous_rule(device):
     assert torch.equal(x3, cg(op, b, damp=1e-2, max_iter=2, tol=1e-30))
 
 
+@pytest.mark.parametrize("damp", [0.0, 1e-2])
+def test_cg_fused_step_equals_generic_loop(device, damp):
+    """The fused device CG step (pg_cg_step_xr / pg_cg_step_p, taken without
+    a preconditioner) gives the generic loop's iterates bit for bit; an
+    identity preconditioner runs the generic loop with the same arithmetic
+    (z = r, <r, z> = <r, r>)."""
+    from pygrog_amd._solve import cg
+    from pygrog_amd.operator import SparseFFT
+
+    class _Identity:
+        def apply(self, r):
+            return r
+
+    rng = np.random.default_rng(23)
+    grid, image, n, C = (48, 48), (40, 40), 4000, 4
+    idx = rng.integers(0, 48 * 48, size=n)
+    w = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    sm = (rng.standard_normal((C, *image)) + 1j * rng.standard_normal((C, *image))).astype(np.complex64) * 0.5
+    b = torch.as_tensor((rng.standard_normal((C, n)) + 1j * rng.standard_normal((C, n))).astype(np.complex64))
+    op = SparseFFT(grid, image, idx, w, smaps=torch.as_tensor(sm).to(device), device=device, toeplitz=True)
+    b = b.to(device)
+    fused, generic = [], []
+    x_f = cg(op, b, damp=damp, max_iter=15, tol=0.0, callback=lambda k, xk, rel: fused.append((k, rel, xk.clone())))
+    x_g = cg(op, b, damp=damp, max_iter=15, tol=0.0, preconditioner=_Identity(),
+             callback=lambda k, xk, rel: generic.append((k, rel, xk.clone())))
+    assert torch.equal(x_f, x_g)
+    assert len(fused) == len(generic) == 15
+    for (kf, rf, xf), (kg, rg, xg) in zip(fused, generic):
+        assert kf == kg and rf == rg and torch.equal(xf, xg)
+
+
 # ---------------------------------------------------------------------------
 # GrogInterpolator (reference tests/test_grog_interpolator.py)
 # ---------------------------------------------------------------------------
