@@ -55,6 +55,9 @@ class GrogToeplitzOp:
             psf = psf.reshape(S, *self.grid_shape)
         self.psf = psf if self.stack_shape else psf[0]
         self._psf_c = psf.to(torch.complex64 if psf.dtype == torch.float32 else torch.complex128)
+        # the fast path runs both transforms unnormalised and applies their
+        # 1/N here, in the one multiply between them
+        self._psf_n = (psf.to(torch.float32) / float(np.prod(self.grid_shape))).to(torch.complex64)
 
     def __call__(self, image: torch.Tensor) -> torch.Tensor:
         sf = self.sparse_fft
@@ -76,13 +79,14 @@ class GrogToeplitzOp:
         C = int(sf.smaps.shape[0]) if sf.smaps is not None else int(x.shape[-len(self.image_shape) - 1])
         if ctype == torch.complex64 and even_grid(self.grid_shape):
             # (-1)^n sign + pad in the expand pass; the k-space signs of the two
-            # centred transforms cancel around the real PSF; crop + (-1)^n +
-            # conj(S) coil sum in the combine pass
+            # centred transforms cancel around the real PSF, which also carries
+            # both transforms' scale; crop + (-1)^n + conj(S) coil sum in the
+            # combine pass
             padded = expand(xd.reshape(B * S, *(() if sf.smaps is not None else (C,)), *self.image_shape),
                             sf.smaps, B * S, C, self.image_shape, self.grid_shape)
-            k = torch.fft.fftn(padded, dim=self.fft_axes, norm="ortho").reshape(B, S, C, *self.grid_shape)
-            k *= self._psf_c.to(ctype).reshape(1, S, 1, *self.grid_shape)
-            full = torch.fft.ifftn(k.reshape(B * S, C, *self.grid_shape), dim=self.fft_axes, norm="ortho")
+            k = torch.fft.fftn(padded, dim=self.fft_axes, norm="backward").reshape(B, S, C, *self.grid_shape)
+            k *= self._psf_n.reshape(1, S, 1, *self.grid_shape)
+            full = torch.fft.ifftn(k.reshape(B * S, C, *self.grid_shape), dim=self.fft_axes, norm="forward")
             out = combine(full, sf.smaps, B * S, C, self.image_shape, self.grid_shape, 1.0)
             single = tuple(self.image_shape) if sf.smaps is not None else (C, *self.image_shape)
             return out.reshape(*B_shape, *s_shape, *single).to(src)

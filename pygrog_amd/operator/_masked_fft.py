@@ -13,6 +13,8 @@ from __future__ import annotations
 
 __all__ = ["MaskedFFT", "MaskedFFTPlan"]
 
+import math
+
 import numpy as np
 import torch
 
@@ -107,17 +109,21 @@ class MaskedFFT(SolveMixin):
             self._mask_dev = self.mask.to(self._comp_device).reshape(S, *self.grid_shape)
         return self._mask_dev
 
-    def _signed_mask(self, S: int, half: bool) -> torch.Tensor:
-        """mask * (-1)^(k0 + k1 + ...) (* prod (-1)^(N/2) when ``half``):
-        the k-space side of the centred transforms folded into the mask."""
-        key = ("signed", half)
+    def _signed_mask(self, S: int, half: bool, scale: float = 1.0) -> torch.Tensor:
+        """mask * (-1)^(k0 + k1 + ...) (* prod (-1)^(N/2) when ``half``,
+        * ``scale``): the k-space side of the centred transforms folded into
+        the mask."""
+        key = ("signed", half, float(scale))
         cache = self.__dict__.setdefault("_signed_cache", {})
         if key not in cache:
             m = self._mask_s(S).to(torch.float32)
             sg = grid_signs(self.grid_shape, m.device)
             if half:
                 sg = sg * half_sign(self.grid_shape)
-            cache[key] = (m * sg).contiguous()
+            w = m * sg
+            if scale != 1.0:
+                w = w * scale
+            cache[key] = w.contiguous()
         return cache[key]
 
     def _stack_mask(self, s_flat_idx: int):
@@ -150,10 +156,12 @@ class MaskedFFT(SolveMixin):
         ctype = dtype if dtype.is_complex else torch.complex64
         xd = x.to(d).to(ctype).reshape(B, S, C, *self.grid_shape)
         if ctype == torch.complex64 and even_grid(self.grid_shape):
-            # signed mask, plain IFFT, one pass for sign + crop + conj(S) sum
+            # signed mask, plain unnormalised IFFT, one pass for sign, scale,
+            # crop and the conj(S) sum
             masked = xd * self._signed_mask(S, False).reshape(1, S, 1, *self.grid_shape)
-            full = torch.fft.ifftn(masked.reshape(B * S, C, *self.grid_shape), dim=self.fft_axes, norm="ortho")
-            img = combine(full, self.smaps, B * S, C, self.image_shape, self.grid_shape, half_sign(self.grid_shape))
+            full = torch.fft.ifftn(masked.reshape(B * S, C, *self.grid_shape), dim=self.fft_axes, norm="forward")
+            img = combine(full, self.smaps, B * S, C, self.image_shape, self.grid_shape,
+                          half_sign(self.grid_shape) / math.sqrt(float(np.prod(self.grid_shape))))
             single = tuple(self.image_shape) if self.smaps is not None else (C, *self.image_shape)
             return img.reshape(*B_shape, *self.stack_shape, *single).to(src_device)
         m = self._mask_s(S).to(torch.float64 if ctype == torch.complex128 else torch.float32)
@@ -181,11 +189,13 @@ class MaskedFFT(SolveMixin):
         xd = x.to(d).to(ctype)
         C = int(self.smaps.shape[0]) if self.smaps is not None else int(x.shape[-len(self.image_shape) - 1])
         if ctype == torch.complex64 and even_grid(self.grid_shape):
-            # one pass for S_c x + zero pad + sign, plain FFT, signed mask
+            # one pass for S_c x + zero pad + sign, plain unnormalised FFT,
+            # signed mask carrying the 1/sqrt(N)
             padded = expand(xd.reshape(B * S, *(() if self.smaps is not None else (C,)), *self.image_shape),
                             self.smaps, B * S, C, self.image_shape, self.grid_shape)
-            k = torch.fft.fftn(padded, dim=self.fft_axes, norm="ortho").reshape(B, S, C, *self.grid_shape)
-            k = k * self._signed_mask(S, True).reshape(1, S, 1, *self.grid_shape)
+            k = torch.fft.fftn(padded, dim=self.fft_axes, norm="backward").reshape(B, S, C, *self.grid_shape)
+            scale = 1.0 / math.sqrt(float(np.prod(self.grid_shape)))
+            k = k * self._signed_mask(S, True, scale).reshape(1, S, 1, *self.grid_shape)
             return k.reshape(*B_shape, *self.stack_shape, C, *self.grid_shape).to(src_device)
         if self.smaps is not None:
             sm = self.smaps.to(d, ctype)

@@ -21,6 +21,7 @@ from __future__ import annotations
 __all__ = ["SparseFFT", "gather", "scatter_add"]
 
 import functools
+import math
 
 import numpy as np
 import torch
@@ -110,18 +111,23 @@ class _ScatterPlan:
         self.rep_src = self.sort_perm.to(torch.int32).contiguous()
         self._signed = {}
 
-    def signed_weights(self, grid_shape, half: bool) -> torch.Tensor:
+    def signed_weights(self, grid_shape, half: bool, scale: float = 1.0) -> torch.Tensor:
         """sqrt weights times (-1)^(k0 + k1 + ...) of each sample's cell
-        (times prod (-1)^(N/2) when ``half``): the k-space side of the
-        centred transforms, folded into the scatter / gather weights."""
-        if half not in self._signed:
+        (times prod (-1)^(N/2) when ``half``, times ``scale``): the k-space
+        side of the centred transforms, folded into the scatter / gather
+        weights."""
+        key = (half, float(scale))
+        if key not in self._signed:
             from ._centred import cell_signs, half_sign
 
             sg = cell_signs(self.indices, grid_shape)
             if half:
                 sg = sg * half_sign(grid_shape)
-            self._signed[half] = (self.sqrt_w * sg).contiguous()
-        return self._signed[half]
+            w = self.sqrt_w * sg
+            if scale != 1.0:
+                w = w * scale
+            self._signed[key] = w.contiguous()
+        return self._signed[key]
 
     def frames(self, y_ptr: int, grid_ptr: int, B: int, C: int, y_stride_s: int | None = None,
                rep_s: torch.Tensor | None = None):
@@ -325,9 +331,11 @@ class SparseFFT(SolveMixin):
         if even_grid(self.grid_shape):
             # scatter with (-1)^k-signed weights, plain IFFT, then one pass for
             # the (-1)^(n - M) sign, the crop and the conj(S) coil sum
+            # the IFFT runs unnormalised: its 1/sqrt(N) rides on that pass too
             g = sp.scatter(xd, rep_s=sp.signed_weights(self.grid_shape, False))
-            full = torch.fft.ifftn(g.reshape(B * S, C, *self.grid_shape), dim=self.fft_axes, norm="ortho")
-            img = combine(full, self.smaps, B * S, C, self.image_shape, self.grid_shape, half_sign(self.grid_shape))
+            full = torch.fft.ifftn(g.reshape(B * S, C, *self.grid_shape), dim=self.fft_axes, norm="forward")
+            img = combine(full, self.smaps, B * S, C, self.image_shape, self.grid_shape,
+                          half_sign(self.grid_shape) / math.sqrt(self.grid_size))
             single = tuple(self.image_shape) if self.smaps is not None else (C, *self.image_shape)
         else:
             g = sp.scatter(xd)
@@ -372,12 +380,13 @@ class SparseFFT(SolveMixin):
         else:
             C = int(x.shape[-len(self.image_shape) - 1])
         if even_grid(self.grid_shape):
-            # one pass for S_c x, the zero pad and the (-1)^n sign; plain FFT;
-            # the (-1)^(k - M) sign rides on the gather weights
+            # one pass for S_c x, the zero pad and the (-1)^n sign; plain,
+            # unnormalised FFT; the (-1)^(k - M) sign and the 1/sqrt(N) ride on
+            # the gather weights
             padded = expand(xd.reshape(B * S, *(() if self.smaps is not None else (C,)), *self.image_shape),
                             self.smaps, B * S, C, self.image_shape, self.grid_shape)
-            k = torch.fft.fftn(padded, dim=self.fft_axes, norm="ortho").reshape(B, S, C, self.grid_size)
-            out = sp.gather(k, sp.signed_weights(self.grid_shape, True))
+            k = torch.fft.fftn(padded, dim=self.fft_axes, norm="backward").reshape(B, S, C, self.grid_size)
+            out = sp.gather(k, sp.signed_weights(self.grid_shape, True, 1.0 / math.sqrt(self.grid_size)))
         else:
             if self.smaps is not None:
                 sm = self.smaps.to(sp.device, torch.complex64)
