@@ -12,7 +12,10 @@ A step grids the device-resident k-space (1, 32, 7.68 M) into a dense
 (``GrogInterpolator.grid_device``).  Modes:
   fp32    plain fused kernel (per-replica operator apply, FP32);
   bf16    grouped complex GEMM on BF16 MFMA, FP32 accumulate (1 product);
-  bf16x3  grouped, split-BF16 (3 products, ~FP32 accuracy).
+  bf16x3  grouped, split-BF16 (3 products, ~FP32 accuracy);
+  bf16x3c bf16x3 over compact block-floating-point sample rows (15-bit
+          mantissas with an exponent per 4 coils: a 32-coil row is one
+          128-byte line; rel-L2 about 5e-5 vs FP64 on Gaussian k-space).
 ``value`` is the fastest BF16-MFMA mode whose rel-L2 vs the FP64 reference is
 <= 1e-3, in coil-samples/s = samples * coils / step time.
 
@@ -134,11 +137,12 @@ def alg_bytes(kernel: str, st: dict) -> float:
         # entry, sqrt weight) + per-cell row offsets (2 x u32) + table once
         return 8 * C * R + 8 * C * G + 12 * R + 8 * G + 8 * C * C * K
     rb24 = 16 * ((6 * C + 15) // 16)  # a sample row of the 24-bit layout
-    if kernel in ("pg_apply_grouped_bf16", "pg_apply_grouped_bf16_s24"):
+    rbfp = 16 * ((C + 3) // 4)  # a sample row of the block-floating-point layout
+    if kernel in ("pg_apply_grouped_bf16", "pg_apply_grouped_bf16_s24", "pg_apply_grouped_bf16_bfp"):
         parts = st["n_parts"]
         # sample rows + bf16 table (complex C x C per entry and part) + V rows
         # written + records (source, sqrt weight, replica id)
-        row = rb24 if kernel.endswith("_s24") else 8 * C
+        row = rb24 if kernel.endswith("_s24") else rbfp if kernel.endswith("_bfp") else 8 * C
         return row * R + 4 * parts * C * C * K + 8 * C * R + 12 * R
     if kernel == "pg_grid_reduce_v":
         return 8 * C * R + 8 * C * G + 8 * G
@@ -148,6 +152,8 @@ def alg_bytes(kernel: str, st: dict) -> float:
         return 16 * C * N
     if kernel in ("pg_transpose_cn_s24", "pg_transpose_frames_s24"):
         return 8 * C * N + rb24 * N
+    if kernel in ("pg_transpose_cn_bfp", "pg_transpose_frames_bfp"):
+        return 8 * C * N + rbfp * N
     raise KeyError(kernel)
 
 
@@ -162,9 +168,14 @@ def step_alg_bytes(st: dict) -> float:
     return 8 * C * N + 8 * C * G + 16 * R + 8 * C * C * K
 
 
+def n_parts(mode: str) -> int:
+    """bf16 table parts a gridding mode reads (hi, or hi and lo)."""
+    return 2 if mode in ("bf16x3", "bf16x3c") else 1
+
+
 def traffic_key(kernel: str, mode: str) -> str:
     if kernel.startswith("pg_apply_grouped_bf16"):
-        return kernel + f":{3 if mode == 'bf16x3' else 1}"
+        return kernel + f":{3 if n_parts(mode) == 2 else 1}"
     return kernel
 
 
@@ -194,14 +205,17 @@ def step_roofline(stats: dict, mode_res: dict, mode: str, traffic_file) -> dict:
 ENTRY_KERNELS = {
     "pg_grid_fp32": [r"k_hot_units(?:_par)?<\d+, 0>", r"k_hot_cells<", r"k_tile<\d+, 0, \d+>|k_wtile<\d+, 0>"],
     "pg_grid_reduce_v": [r"k_hot_units<\d+, 1>", r"k_hot_cells<", r"k_tile<\d+, 1, \d+>|k_wtile<\d+, 1>|k_wrows<\d+>"],
-    "pg_apply_grouped_bf16:1": [r"k_apply_grouped<\d+, \d+, 1, false>"],
-    "pg_apply_grouped_bf16:3": [r"k_apply_grouped<\d+, \d+, 3, false>"],
-    "pg_apply_grouped_bf16_s24:1": [r"k_apply_grouped<\d+, \d+, 1, true>"],
-    "pg_apply_grouped_bf16_s24:3": [r"k_apply_grouped<\d+, \d+, 3, true>"],
-    "pg_transpose_cn_c64": [r"k_transpose_cn<false>"],
-    "pg_transpose_frames_c64": [r"k_transpose_frames<false>"],
-    "pg_transpose_cn_s24": [r"k_transpose_cn<true>"],
-    "pg_transpose_frames_s24": [r"k_transpose_frames<true>"],
+    "pg_apply_grouped_bf16:1": [r"k_apply_grouped<\d+, \d+, 1, 0>"],
+    "pg_apply_grouped_bf16:3": [r"k_apply_grouped<\d+, \d+, 3, 0>"],
+    "pg_apply_grouped_bf16_s24:1": [r"k_apply_grouped<\d+, \d+, 1, 1>"],
+    "pg_apply_grouped_bf16_s24:3": [r"k_apply_grouped<\d+, \d+, 3, 1>"],
+    "pg_apply_grouped_bf16_bfp:3": [r"k_apply_grouped<\d+, \d+, 3, 2>"],
+    "pg_transpose_cn_c64": [r"k_transpose_cn<0>"],
+    "pg_transpose_frames_c64": [r"k_transpose_frames<0>"],
+    "pg_transpose_cn_s24": [r"k_transpose_cn<1>"],
+    "pg_transpose_frames_s24": [r"k_transpose_frames<1>"],
+    "pg_transpose_cn_bfp": [r"k_transpose_cn<2>"],
+    "pg_transpose_frames_bfp": [r"k_transpose_frames<2>"],
 }
 COPY_KERNEL = r"neg_kernel"
 
@@ -479,7 +493,7 @@ def run_frames2d(args, dev) -> dict:
             f"kernels {kt}, tune {tune}")
     all_roof = {}
     for m, v in modes.items():
-        stats["n_parts"] = 2 if m == "bf16x3" else 1
+        stats["n_parts"] = n_parts(m)
         all_roof[m] = {}
         for k, t in v["kernel_ms"].items():
             all_roof[m][k] = {"GB/s": alg_bytes(k, stats) / (t * 1e-3) / 1e9, "ms": t}
@@ -803,7 +817,7 @@ def measure_single(name: str, args, dev) -> dict:
     head = max(bf, key=lambda m: modes[m]["coil_samples_per_s"]) if bf else next(iter(modes))
     hm = modes[head]
     kname, kms = max(hm["kernel_ms"].items(), key=lambda kv: kv[1])
-    stats["n_parts"] = 2 if head == "bf16x3" else 1
+    stats["n_parts"] = n_parts(head)
     ab = alg_bytes(kname, stats)
     achieved = ab / (kms * 1e-3) / 1e9
     kroof = {"kernel": kname, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -821,7 +835,7 @@ def measure_single(name: str, args, dev) -> dict:
     roof["dominant_kernel"] = kroof
     all_roof = {}
     for m, v in modes.items():
-        stats["n_parts"] = 2 if m == "bf16x3" else 1
+        stats["n_parts"] = n_parts(m)
         all_roof[m] = {k: {"GB/s": alg_bytes(k, stats) / (t * 1e-3) / 1e9, "ms": t} for k, t in v["kernel_ms"].items()}
 
     cpu = cpu1 = None
@@ -845,7 +859,7 @@ def main(argv=None) -> int:
     ap.add_argument("--serve-jobs", type=int, default=144, help="jobs per serving run (0: skip serving)")
     ap.add_argument("--no-extras", action="store_true",
                     help="default workload only: skip the 200-frame batch and SENSE-CG legs")
-    ap.add_argument("--modes", default="fp32,bf16,bf16x3")
+    ap.add_argument("--modes", default="fp32,bf16,bf16x3,bf16x3c")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline work (s)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 rel-L2 check")

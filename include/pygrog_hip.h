@@ -485,6 +485,31 @@ int pg_transpose_frames_s24(const void *y, const int64_t *frame_off, int64_t F,
                             int64_t max_n, int64_t C, void *y_s,
                             pg_stream_t stream);
 
+/* Block-floating-point sample-major layout ("bfp15"), the compact copy of
+ * the k-space for the split-BF16 products: a sample's 2C values (re, im per
+ * coil) in groups of 8 (4 coils; the last group zero-padded), 16 bytes a
+ * group, pg_bfp_row_bytes(C) = 16 * ceil(C / 4) bytes a row (-1 if C < 0):
+ * a 32-coil row is one 128-byte line.  A group is 8 little-endian 16-bit
+ * fields; bits 1..15 of field i are a two's-complement mantissa m_i in
+ * [-16384, 16383], bit 0 of field i is bit i of the group's exponent E, and
+ * value_i = m_i * 2^(E - 140).  The encoder takes M = max |x_i| over the
+ * group, s = max(floor(log2 M) - 13, -140) (+1 when M * 2^-s rounds past
+ * 16383; s = -140 for M = 0), m_i = x_i * 2^-s rounded to nearest even: every
+ * value is within M * 2^-14 of its input.  A group holding a NaN or an
+ * infinity decodes to -infinity.  No reference counterpart: an internal
+ * format of the BF16 gridding path (lossy; the bf16x3 products read every
+ * stored value exactly). */
+int64_t pg_bfp_row_bytes(int64_t C);
+/* pg_transpose_cn_c64 into the bfp15 layout: y_b (16-byte aligned) holds N
+ * rows of pg_bfp_row_bytes(C) bytes. */
+int pg_transpose_cn_bfp(const void *y, int64_t C, int64_t N, void *y_b,
+                        pg_stream_t stream);
+/* pg_transpose_frames_c64 into the bfp15 layout: frame f's rows start at
+ * byte pg_bfp_row_bytes(C) * frame_off[f] of y_b (16-byte aligned). */
+int pg_transpose_frames_bfp(const void *y, const int64_t *frame_off, int64_t F,
+                            int64_t max_n, int64_t C, void *y_b,
+                            pg_stream_t stream);
+
 /* Pack a (K, C, C) c64 table for the BF16 MFMA grouped GEMM: per entry and
  * part, (re, im) bf16 pairs of K[co][ci] padded to 8*ceil(2C/16) rows by
  * 16*ceil(2C/32) columns; part 0 = hi = bf16(x), part 1 = lo =
@@ -512,6 +537,15 @@ int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
  * pg_transpose_cn_s24 / pg_transpose_frames_s24 (y_s 16-byte aligned); the
  * products and the V rows are as there.  No reference counterpart. */
 int pg_apply_grouped_bf16_s24(const void *y_s, const void *table_bf16,
+                              const uint32_t *ent_ptr, const uint32_t *ent_rep,
+                              const uint32_t *ent_src, const float *ent_s,
+                              const uint32_t *items, int64_t n_items,
+                              int64_t K, int64_t C, int32_t n_terms, void *vbuf,
+                              pg_stream_t stream);
+/* pg_apply_grouped_bf16 reading the bfp15 sample rows of
+ * pg_transpose_cn_bfp / pg_transpose_frames_bfp (y_b 16-byte aligned); the
+ * products and the V rows are as there.  No reference counterpart. */
+int pg_apply_grouped_bf16_bfp(const void *y_b, const void *table_bf16,
                               const uint32_t *ent_ptr, const uint32_t *ent_rep,
                               const uint32_t *ent_src, const float *ent_s,
                               const uint32_t *items, int64_t n_items,

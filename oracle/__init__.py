@@ -12,6 +12,8 @@ Contents
   * ``table``: FP64 (complex128) GRAPPA operator table with numpy/scipy.
   * ``grid``:  the reference's ``interpolate(grid=True)`` flow on the C ops,
                and an FP64 gridding oracle.
+  * ``rows``:  what the grouped path's lossy sample-row formats (24-bit,
+               block floating point) must hold, in numpy.
   * ``ref_ext()``: the reference's own CPU ops (``csrc/torch``) compiled in
                place into ``oracle/_ref`` by ``make -C oracle ref``, when built.
 """

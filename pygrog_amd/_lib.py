@@ -124,6 +124,9 @@ _SIGS = {
     "pg_s24_row_bytes": [c_int64],
     "pg_transpose_cn_s24": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
     "pg_transpose_frames_s24": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p],
+    "pg_bfp_row_bytes": [c_int64],
+    "pg_transpose_cn_bfp": [c_void_p, c_int64, c_int64, c_void_p, c_void_p],
+    "pg_transpose_frames_bfp": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p],
     "pg_vdot_c64_f64": [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p],
     "pg_vdot_c64_f64_noalloc": [c_void_p, c_void_p, c_int64, c_void_p, c_void_p],
     "pg_plan_compile_layout": [c_int64, c_int64, c_int32, c_int32, c_int64, c_void_p, c_void_p, c_void_p],
@@ -143,6 +146,7 @@ _SIGS = {
                           c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p],
     "pg_apply_grouped_bf16": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_apply_grouped_bf16_s24": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
+    "pg_apply_grouped_bf16_bfp": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_sparse_apply_c64": [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
     "pg_sparse_apply_stack_c64": [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64,
                                   c_int64, c_void_p],
@@ -154,6 +158,7 @@ _RESTYPES = {
     "pg_table_bf16_elems": c_int64,
     "pg_table_packed_elems": c_int64,
     "pg_s24_row_bytes": c_int64,
+    "pg_bfp_row_bytes": c_int64,
     "pg_vdot_workspace_bytes": c_int64,
 }
 

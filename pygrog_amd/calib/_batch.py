@@ -9,9 +9,10 @@ quantisation, the table fitted once and shared), and one call grids all of
 them with a fixed, small number of launches whatever the frame count:
 
   1. ``pg_transpose_frames_c64`` (``_s24`` for the BF16 modes up to 32 coils:
-     24-bit rows) every frame's (C, N_f) block to (N_f, C) rows;
+     24-bit rows; ``_bfp`` for bf16x3c: block-floating-point rows) every
+     frame's (C, N_f) block to (N_f, C) rows;
   2. fp32:    ``pg_grid_fp32`` over all frames' tiles (plain per-replica apply);
-     bf16(x3): ``pg_apply_grouped_bf16`` once over the replicas of all frames
+     bf16(x3, x3c): ``pg_apply_grouped_bf16`` once over the replicas of all frames
               grouped by table entry (the table is shared, so groups are F
               times larger than one frame's), then ``pg_grid_reduce_v`` over
               all frames' tiles.
@@ -35,8 +36,8 @@ import torch
 
 from .. import _lib
 from ._grappa import build_table
-from ._grog import (GridProgram, GrogInterpolator, KernelTimer, group_by_entry, new_sample_rows, sample_row_bytes,
-                    tune_program, uses_s24)
+from ._grog import (GRID_MODES, GridProgram, GrogInterpolator, KernelTimer, apply_entry, group_by_entry, mode_terms,
+                    new_sample_rows, row_format, row_format_of, sample_row_bytes, transpose_entry, tune_program)
 
 _TUNE_CACHE: dict = {}
 
@@ -133,8 +134,8 @@ class GrogFrameBatch:
         if key in self._programs:
             return self._programs[key]
         C = self.C
-        yt = new_sample_rows(int(self.sample_off[-1]), C, uses_s24(mode, C), self.device)
-        rb = sample_row_bytes(C, yt.dtype == torch.uint8)
+        yt = new_sample_rows(int(self.sample_off[-1]), C, row_format(mode, C), self.device)
+        rb = sample_row_bytes(C, row_format_of(yt))
         tc = _lib.load().pg_grid_tile_cells(C)
         frames = []
         vbuf = None
@@ -153,7 +154,7 @@ class GrogFrameBatch:
     def _check(self, y: torch.Tensor, out: torch.Tensor, mode: str):
         if self.C is None:
             raise RuntimeError("GRAPPA kernels not set. Call calc_interp_table() first.")
-        if mode not in ("fp32", "bf16", "bf16x3"):
+        if mode not in GRID_MODES:
             raise ValueError(f"unknown gridding mode {mode!r}")
         if self.C > 64:
             raise RuntimeError(f"fused gridding supports up to 64 coils, got {self.C}")
@@ -168,8 +169,8 @@ class GrogFrameBatch:
     def _run(self, mode: str, y: torch.Tensor, out: torch.Tensor, timer: KernelTimer | None = None):
         yt, vbuf, prog = self._program(mode, y, out)
         st = _lib.stream(self.device)
-        s24 = yt.dtype == torch.uint8
-        name = "pg_transpose_frames_s24" if s24 else "pg_transpose_frames_c64"
+        fmt = row_format_of(yt)
+        name = transpose_entry(fmt, True)
         e0 = timer.start() if timer is not None else None
         _lib.call(name, _lib.ptr(y), _lib.ptr(self._off_dev), self.F, int(self.n_samples.max()),
                   self.C, _lib.ptr(yt), st)
@@ -179,9 +180,9 @@ class GrogFrameBatch:
             prog.launch("pg_grid_fp32", timer)
             return
         m = self._merged_groups()
-        n_terms = 1 if mode == "bf16" else 3
+        n_terms = mode_terms(mode)
         tbl = self.frames[0]._table_bf16(2 if n_terms == 3 else 1)
-        name = "pg_apply_grouped_bf16_s24" if s24 else "pg_apply_grouped_bf16"
+        name = apply_entry(fmt)
         e0 = timer.start() if timer is not None else None
         if m["n_items"] > 0:
             grp = m["groups"]

@@ -371,21 +371,65 @@ def _lines_per_row(row_bytes: int) -> float:
 
 def uses_s24(mode: str, C: int) -> bool:
     """Whether gridding mode ``mode`` at ``C`` coils uses 24-bit sample rows."""
-    if mode == "fp32" or C > S24_MAX_COILS:
+    if mode in ("fp32", "bf16x3c") or C > S24_MAX_COILS:
         return False
     return _lines_per_row(16 * ((6 * C + 15) // 16)) <= _lines_per_row(8 * C)
 
 
-def sample_row_bytes(C: int, s24: bool) -> int:
+# Gridding modes: the plain fused FP32 kernel, and the grouped BF16-MFMA
+# path with one product (bf16), three split products (bf16x3) over rows
+# holding 16 significant bits of every value (24-bit or complex64 rows), or
+# three split products over block-floating-point rows (bf16x3c: 15-bit
+# mantissas with an exponent per 4 coils, 16 * ceil(C / 4) bytes a row --
+# one 128-byte line at 32 coils; each value within 2^-14 of its 4-coil
+# group's largest magnitude).
+GRID_MODES = ("fp32", "bf16", "bf16x3", "bf16x3c")
+
+
+def mode_terms(mode: str) -> int:
+    """Split-BF16 products of a grouped mode."""
+    return 1 if mode == "bf16" else 3
+
+
+def row_format(mode: str, C: int) -> str:
+    """Sample-row format of ``mode`` at ``C`` coils: "c64", "s24" or "bfp"."""
+    if mode == "bf16x3c":
+        return "bfp"
+    return "s24" if uses_s24(mode, C) else "c64"
+
+
+_ROW_DTYPE = {"c64": torch.complex64, "s24": torch.uint8, "bfp": torch.int16}
+_ROW_TRANSPOSE = {"c64": "_c64", "s24": "_s24", "bfp": "_bfp"}
+_ROW_APPLY = {"c64": "", "s24": "_s24", "bfp": "_bfp"}
+
+
+def row_format_of(yt: torch.Tensor) -> str:
+    """Format of a sample-row buffer made by ``new_sample_rows``."""
+    return {v: k for k, v in _ROW_DTYPE.items()}[yt.dtype]
+
+
+def transpose_entry(fmt: str, frames: bool) -> str:
+    return ("pg_transpose_frames" if frames else "pg_transpose_cn") + _ROW_TRANSPOSE[fmt]
+
+
+def apply_entry(fmt: str) -> str:
+    return "pg_apply_grouped_bf16" + _ROW_APPLY[fmt]
+
+
+def sample_row_bytes(C: int, fmt: str) -> int:
     """Bytes of one sample's row of the sample-major copy."""
-    return int(_lib.load().pg_s24_row_bytes(C)) if s24 else 8 * C
+    if fmt == "s24":
+        return int(_lib.load().pg_s24_row_bytes(C))
+    if fmt == "bfp":
+        return int(_lib.load().pg_bfp_row_bytes(C))
+    return 8 * C
 
 
-def new_sample_rows(n: int, C: int, s24: bool, device) -> torch.Tensor:
-    """Buffer for the sample-major copy of ``n`` samples."""
-    if s24:
-        return torch.empty(max(n, 1) * sample_row_bytes(C, True), dtype=torch.uint8, device=device)
-    return torch.empty(max(n, 1) * C, dtype=torch.complex64, device=device)
+def new_sample_rows(n: int, C: int, fmt: str, device) -> torch.Tensor:
+    """Buffer for the sample-major copy of ``n`` samples in format ``fmt``."""
+    dt = _ROW_DTYPE[fmt]
+    per = sample_row_bytes(C, fmt) // torch.empty(0, dtype=dt).element_size()
+    return torch.empty(max(n, 1) * per, dtype=dt, device=device)
 
 
 class GridProgram:
@@ -878,11 +922,11 @@ class GrogInterpolator:
         (complex64, or the 24-bit format when ``yt`` is a byte buffer)."""
         comp = self.plan._compiled
         B, S, C, N = y.shape
-        s24 = yt.dtype == torch.uint8
+        fmt = row_format_of(yt)
         st = _lib.stream(comp.device)
         F = B * S
         if F == 1:
-            name = "pg_transpose_cn_s24" if s24 else "pg_transpose_cn_c64"
+            name = transpose_entry(fmt, False)
             e0 = timer.start() if timer is not None else None
             _lib.call(name, _lib.ptr(y), C, N, _lib.ptr(yt), st)
         else:
@@ -893,7 +937,7 @@ class GrogInterpolator:
             if off is None:
                 off = torch.arange(F + 1, dtype=torch.int64, device=comp.device) * N
                 self._frame_off = {key: off}
-            name = "pg_transpose_frames_s24" if s24 else "pg_transpose_frames_c64"
+            name = transpose_entry(fmt, True)
             e0 = timer.start() if timer is not None else None
             _lib.call(name, _lib.ptr(y), _lib.ptr(off), F, N, C, _lib.ptr(yt), st)
         if timer is not None:
@@ -907,14 +951,17 @@ class GrogInterpolator:
         (one transpose pass), so a replica reads one contiguous row.
         mode "fp32": plain fused kernel (per-replica operator apply).
         "bf16" / "bf16x3": grouped complex GEMM on BF16 MFMA (1 or 3 split
-        products, FP32 accumulate), then the same deterministic reduction.
+        products, FP32 accumulate), then the same deterministic reduction;
+        "bf16x3c": bf16x3 over block-floating-point sample rows (a 4-coil
+        group's values within 2^-14 of its largest magnitude; half the
+        bytes of complex64 rows).
         """
         if not self._interpolator_set:
             raise RuntimeError("GRAPPA kernels not set. Call calc_interp_table() first.")
         comp = self.plan._compiled
         if comp.C > 64:
             raise RuntimeError(f"fused gridding supports up to 64 coils, got {comp.C}")
-        if mode not in ("fp32", "bf16", "bf16x3"):
+        if mode not in GRID_MODES:
             raise ValueError(f"unknown gridding mode {mode!r}")
         B, S, C, N = y.shape
         if (S, C, N) != (comp.S, comp.C, comp.npts):
@@ -926,7 +973,7 @@ class GrogInterpolator:
         cache = program_cache if program_cache is not None else {}
         key = (mode, y.data_ptr(), out.data_ptr(), B)
         if key not in cache:
-            yt = new_sample_rows(B * S * N, C, uses_s24(mode, C), comp.device)
+            yt = new_sample_rows(B * S * N, C, row_format(mode, C), comp.device)
             if mode == "fp32":
                 cache[key] = (None, yt, self.grid_program(yt, out, B, mode=mode))
             else:
@@ -937,14 +984,14 @@ class GrogInterpolator:
         if mode == "fp32":
             prog.launch("pg_grid_fp32", timer)
             return out
-        n_terms = 1 if mode == "bf16" else 3
+        n_terms = mode_terms(mode)
         tbl = self._table_bf16(2 if n_terms == 3 else 1)
         groups = self._entry_groups()
         st = _lib.stream(comp.device)
         K = int(comp.table.shape[0])
-        s24 = yt.dtype == torch.uint8
-        rb = sample_row_bytes(C, s24)
-        name = "pg_apply_grouped_bf16_s24" if s24 else "pg_apply_grouped_bf16"
+        fmt = row_format_of(yt)
+        rb = sample_row_bytes(C, fmt)
+        name = apply_entry(fmt)
         sg = self._stack_groups(B) if B * S > 1 else None
         e0 = timer.start() if timer is not None else None
         if sg is not None:

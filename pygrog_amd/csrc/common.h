@@ -54,6 +54,11 @@ template <> __device__ __forceinline__ uint2 gld<uint2>(const uint2 *p) {
   const pg_u2n v = *(const PG_GAS pg_u2n *)(uintptr_t)p;
   return make_uint2(v.x, v.y);
 }
+typedef unsigned int pg_u4n __attribute__((ext_vector_type(4)));
+template <> __device__ __forceinline__ uint4 gld<uint4>(const uint4 *p) {
+  const pg_u4n v = *(const PG_GAS pg_u4n *)(uintptr_t)p;
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
 template <class T> __device__ __forceinline__ void gst(T *p, T v) {
   *(PG_GAS T *)(uintptr_t)p = v;
 }
@@ -133,6 +138,68 @@ __device__ __forceinline__ pg_u3n s24_pack4(float a, float b, float c, float d) 
 __device__ __forceinline__ float s24_load(const unsigned char *p) {
   const PG_GAS unsigned char *q = (const PG_GAS unsigned char *)(uintptr_t)p;
   return __uint_as_float(((uint32_t)q[0] << 8) | ((uint32_t)q[1] << 16) | ((uint32_t)q[2] << 24));
+}
+
+// Sample-row formats of the grouped path's sample-major copy.
+constexpr int kRowC64 = 0;  // complex64, 8 C bytes a row
+constexpr int kRowS24 = 1;  // 24-bit values (above), 16 * ceil(6 C / 16) bytes
+constexpr int kRowBfp = 2;  // block floating point (below), 16 * ceil(C / 4) bytes
+
+// Block-floating-point rows ("bfp15"): a row's 2C values (re, im per coil)
+// in groups of 8 (4 coils; the last group zero-padded), 16 bytes a group
+// (a 32-coil row is one 128-byte line).  A group holds 8 little-endian
+// 16-bit fields f_i; bits 1..15 of f_i are a two's-complement mantissa
+// m_i in [-16384, 16383] and bit 0 of f_i is bit i of the group's 8-bit
+// exponent E; value_i = m_i * 2^(E - 140).  Encoding: M = max |x_i|; the
+// scale s = max(floor(log2 M) - 13, -140), plus 1 if M * 2^-s rounds past
+// 16383 (s = -140 when M = 0); m_i = x_i * 2^-s rounded to nearest even.
+// So |error| <= 2^(s-1) <= M * 2^-14 for every value of a group.  Every
+// finite fp32 input has such an s in [-140, 115] (inputs within 2^-15 of
+// FLT_MAX round to infinity); a group holding a NaN or an infinity is
+// stored as E = 255, m_i = -16384 and decodes to -infinity.  A split-BF16
+// product reads x = hi + lo exactly from such a value (|m| < 2^14: hi the
+// top 8 significant bits, lo the rest).
+__host__ __device__ constexpr int64_t bfp_row_bytes(int64_t C) { return (C + 3) / 4 * 16; }
+constexpr int kBfpBias = 140;
+__device__ __forceinline__ uint4 bfp_encode8(const float (&x)[8]) {
+  float M = 0.0f;
+  bool finite = true;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    M = fmaxf(M, fabsf(x[i]));
+    finite = finite && __builtin_isfinite(x[i]);
+  }
+  int s = -kBfpBias;
+  if (M > 0.0f) {
+    s = __builtin_amdgcn_frexp_expf(M) - 1 - 13;  // M in [2^(s+13), 2^(s+14))
+    if (s < -kBfpBias) s = -kBfpBias;
+    if (__builtin_amdgcn_ldexpf(M, -s) >= 16383.5f) s += 1;
+  }
+  uint32_t f[8];
+  if (finite) {
+    const uint32_t E = (uint32_t)(s + kBfpBias);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int m = (int)rintf(__builtin_amdgcn_ldexpf(x[i], -s));
+      f[i] = (((uint32_t)m & 0x7fffu) << 1) | ((E >> i) & 1u);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) f[i] = 0x8000u | 1u;  // E = 255, m = -16384
+  }
+  return make_uint4(f[0] | (f[1] << 16), f[2] | (f[3] << 16), f[4] | (f[5] << 16), f[6] | (f[7] << 16));
+}
+__device__ __forceinline__ void bfp_decode8(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3, float *o) {
+  const uint32_t t0 = d0 & 0x10001u, t1 = d1 & 0x10001u, t2 = d2 & 0x10001u, t3 = d3 & 0x10001u;
+  const uint32_t E = ((t0 | (t0 >> 15)) & 3u) | (((t1 | (t1 >> 15)) & 3u) << 2) |
+                     (((t2 | (t2 >> 15)) & 3u) << 4) | (((t3 | (t3 >> 15)) & 3u) << 6);
+  const int s = (int)E - kBfpBias;
+  const uint32_t d[4] = {d0, d1, d2, d3};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    o[2 * k] = __builtin_amdgcn_ldexpf((float)((int)(d[k] << 16) >> 17), s);
+    o[2 * k + 1] = __builtin_amdgcn_ldexpf((float)((int)d[k] >> 17), s);
+  }
 }
 
 // Orders one wavefront's LDS writes before its later LDS reads (other lanes

@@ -1158,7 +1158,7 @@ constexpr int kTrS = 64, kTrC = 32;
 #define PG_TR_NTST 1
 #endif
 
-template <bool S24>
+template <int RF>
 __device__ __forceinline__ void transpose_tile(const float2 *__restrict__ y,
                                                int64_t C, int64_t N,
                                                void *__restrict__ out,
@@ -1166,7 +1166,7 @@ __device__ __forceinline__ void transpose_tile(const float2 *__restrict__ y,
                                                float2 (*tile)[kTrS + 1]) {
   float2 *yt = static_cast<float2 *>(out);
   unsigned char *ys = static_cast<unsigned char *>(out);
-  const int64_t rb = s24_row_bytes(C);
+  const int64_t rb = RF == kRowS24 ? s24_row_bytes(C) : bfp_row_bytes(C);
   const int t = threadIdx.x;
   const bool full = n0 + kTrS <= N && ((C | N) & 1) == 0;
   if (full) {
@@ -1188,13 +1188,43 @@ __device__ __forceinline__ void transpose_tile(const float2 *__restrict__ y,
   }
   __syncthreads();
   const int64_t cw = C - c0 < kTrC ? C - c0 : kTrC;  // coils of this tile
+  if constexpr (RF == kRowBfp) {
+    // one (sample, 4-coil group) per thread and round: 8 consecutive
+    // threads write one sample's 128 bytes (32 coils), rows back to back
+    const int ng = (int)((cw + 3) / 4);
+#pragma unroll
+    for (int i = 0; i < kTrS * kTrC / 4 / 256; ++i) {
+      const int idx = t + 256 * i;
+      const int sl = idx / (kTrC / 4), gi = idx % (kTrC / 4);
+      if (gi < ng && n0 + sl < N) {
+        float x[8];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const bool v = 4 * gi + k < cw;
+          const float2 a = v ? tile[4 * gi + k][sl] : make_float2(0.0f, 0.0f);
+          x[2 * k] = a.x;
+          x[2 * k + 1] = a.y;
+        }
+        const uint4 w = bfp_encode8(x);
+        PG_GAS pg_u4n *dst = (PG_GAS pg_u4n *)(uintptr_t)(ys + (n0 + sl) * rb + 16 * (c0 / 4 + gi));
+        pg_u4n n;
+        n.x = w.x;
+        n.y = w.y;
+        n.z = w.z;
+        n.w = w.w;
+        if (PG_TR_NTST) __builtin_nontemporal_store(n, dst);
+        else *dst = n;
+      }
+    }
+    return;
+  }
   if (full) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int sl = t / 16 + 16 * i, cp = t % 16;
       if (2 * cp < cw) {
         const float2 a = tile[2 * cp][sl], b = tile[2 * cp + 1][sl];
-        if constexpr (S24) {
+        if constexpr (RF == kRowS24) {
           // two coils = 12 bytes at byte 6 (c0 + 2 cp) of the row
           PG_GAS pg_u3n *dst = (PG_GAS pg_u3n *)(uintptr_t)(ys + (n0 + sl) * rb + 6 * (c0 + 2 * cp));
           __builtin_nontemporal_store(s24_pack4(a.x, a.y, b.x, b.y), dst);
@@ -1209,7 +1239,7 @@ __device__ __forceinline__ void transpose_tile(const float2 *__restrict__ y,
     for (int i = t; i < kTrS * kTrC; i += 256) {
       const int sl = i / kTrC, cl = i % kTrC;
       if (cl < cw && n0 + sl < N) {
-        if constexpr (S24) {
+        if constexpr (RF == kRowS24) {
           // one coil = 6 bytes at an even byte offset: three 16-bit stores
           const uint32_t re = s24_round(tile[cl][sl].x), im = s24_round(tile[cl][sl].y);
           unsigned short *d = reinterpret_cast<unsigned short *>(ys + (n0 + sl) * rb + 6 * (c0 + cl));
@@ -1224,17 +1254,17 @@ __device__ __forceinline__ void transpose_tile(const float2 *__restrict__ y,
   }
 }
 
-template <bool S24>
+template <int RF>
 __global__ __launch_bounds__(256) void k_transpose_cn(const float2 *__restrict__ y, int64_t C,
                                                       int64_t N, void *__restrict__ yt) {
   __shared__ float2 tile[kTrC][kTrS + 1];
-  transpose_tile<S24>(y, C, N, yt, (int64_t)blockIdx.x * kTrS, (int64_t)blockIdx.y * kTrC, tile);
+  transpose_tile<RF>(y, C, N, yt, (int64_t)blockIdx.x * kTrS, (int64_t)blockIdx.y * kTrC, tile);
 }
 
 // Frames of uneven length back to back: frame f holds (C, N_f) coil-major
 // at sample offset off[f] (element offset C * off[f]); its (N_f, C)
 // sample-major rows go to the same offset of yt.  blockIdx.z = frame.
-template <bool S24>
+template <int RF>
 __global__ __launch_bounds__(256) void k_transpose_frames(const float2 *__restrict__ y,
                                                           const int64_t *__restrict__ off,
                                                           int64_t C, void *__restrict__ yt) {
@@ -1243,9 +1273,9 @@ __global__ __launch_bounds__(256) void k_transpose_frames(const float2 *__restri
   const int64_t N = off[blockIdx.z + 1] - base;
   const int64_t n0 = (int64_t)blockIdx.x * kTrS;
   if (n0 >= N) return;
-  void *o = S24 ? (void *)(static_cast<unsigned char *>(yt) + s24_row_bytes(C) * base)
-                : (void *)(static_cast<float2 *>(yt) + C * base);
-  transpose_tile<S24>(y + C * base, C, N, o, n0, (int64_t)blockIdx.y * kTrC, tile);
+  const int64_t rb = RF == kRowS24 ? s24_row_bytes(C) : RF == kRowBfp ? bfp_row_bytes(C) : 8 * C;
+  transpose_tile<RF>(y + C * base, C, N, static_cast<unsigned char *>(yt) + rb * base, n0,
+                     (int64_t)blockIdx.y * kTrC, tile);
 }
 
 // out[b, co, m] = K[idx[m]][co] . y[b, :, m / kw]
@@ -1526,66 +1556,80 @@ int pg_grid_scatter(const pg_frame *frames, const uint32_t *tile_ptr,
 }
 
 static int transpose_cn(const void *y, int64_t C, int64_t N, void *y_t,
-                        bool s24, pg_stream_t stream) {
+                        int rf, pg_stream_t stream) {
   PG_CHECK(C >= 0 && N >= 0 && C < 65535 * 32, "transpose: bad sizes");
-  PG_CHECK(!s24 || ((uintptr_t)y_t & 15) == 0, "transpose_s24: output not 16-byte aligned");
+  PG_CHECK(rf == kRowC64 || ((uintptr_t)y_t & 15) == 0, "transpose: packed output not 16-byte aligned");
   if (C == 0 || N == 0) return 0;
   const dim3 grid((unsigned)ceil_div(N, kTrS), (unsigned)ceil_div(C, kTrC));
-  if (s24)
-    hipLaunchKernelGGL(k_transpose_cn<true>, grid, dim3(256), 0, as_stream(stream),
-                       static_cast<const float2 *>(y), C, N, y_t);
-  else
-    hipLaunchKernelGGL(k_transpose_cn<false>, grid, dim3(256), 0, as_stream(stream),
-                       static_cast<const float2 *>(y), C, N, y_t);
+#define PG_TR_LAUNCH(RF)                                                   \
+  hipLaunchKernelGGL(k_transpose_cn<RF>, grid, dim3(256), 0, as_stream(stream), \
+                     static_cast<const float2 *>(y), C, N, y_t)
+  if (rf == kRowS24) PG_TR_LAUNCH(kRowS24);
+  else if (rf == kRowBfp) PG_TR_LAUNCH(kRowBfp);
+  else PG_TR_LAUNCH(kRowC64);
+#undef PG_TR_LAUNCH
   PG_LAUNCHED();
   return 0;
 }
 
 static int transpose_frames(const void *y, const int64_t *frame_off, int64_t F,
-                            int64_t max_n, int64_t C, void *y_t, bool s24,
+                            int64_t max_n, int64_t C, void *y_t, int rf,
                             pg_stream_t stream) {
   PG_CHECK(F >= 0 && F < 65536 && C >= 0 && C < 65535 * 32 && max_n >= 0,
            "transpose_frames: bad sizes (F=%lld, C=%lld)", (long long)F,
            (long long)C);
   PG_CHECK(F == 0 || frame_off, "transpose_frames: null offsets");
-  PG_CHECK(!s24 || ((uintptr_t)y_t & 15) == 0, "transpose_frames_s24: output not 16-byte aligned");
+  PG_CHECK(rf == kRowC64 || ((uintptr_t)y_t & 15) == 0, "transpose_frames: packed output not 16-byte aligned");
   if (F == 0 || C == 0 || max_n == 0) return 0;
   const dim3 grid((unsigned)ceil_div(max_n, kTrS), (unsigned)ceil_div(C, kTrC),
                   (unsigned)F);
-  if (s24)
-    hipLaunchKernelGGL(k_transpose_frames<true>, grid, dim3(256), 0,
-                       as_stream(stream), static_cast<const float2 *>(y),
-                       frame_off, C, y_t);
-  else
-    hipLaunchKernelGGL(k_transpose_frames<false>, grid, dim3(256), 0,
-                       as_stream(stream), static_cast<const float2 *>(y),
-                       frame_off, C, y_t);
+#define PG_TR_LAUNCH(RF)                                                  \
+  hipLaunchKernelGGL(k_transpose_frames<RF>, grid, dim3(256), 0,         \
+                     as_stream(stream), static_cast<const float2 *>(y),  \
+                     frame_off, C, y_t)
+  if (rf == kRowS24) PG_TR_LAUNCH(kRowS24);
+  else if (rf == kRowBfp) PG_TR_LAUNCH(kRowBfp);
+  else PG_TR_LAUNCH(kRowC64);
+#undef PG_TR_LAUNCH
   PG_LAUNCHED();
   return 0;
 }
 
 int pg_transpose_cn_c64(const void *y, int64_t C, int64_t N, void *y_t,
                         pg_stream_t stream) {
-  return transpose_cn(y, C, N, y_t, false, stream);
+  return transpose_cn(y, C, N, y_t, kRowC64, stream);
 }
 
 int pg_transpose_frames_c64(const void *y, const int64_t *frame_off,
                             int64_t F, int64_t max_n, int64_t C, void *y_t,
                             pg_stream_t stream) {
-  return transpose_frames(y, frame_off, F, max_n, C, y_t, false, stream);
+  return transpose_frames(y, frame_off, F, max_n, C, y_t, kRowC64, stream);
 }
 
 int64_t pg_s24_row_bytes(int64_t C) { return C >= 0 ? s24_row_bytes(C) : -1; }
 
 int pg_transpose_cn_s24(const void *y, int64_t C, int64_t N, void *y_s,
                         pg_stream_t stream) {
-  return transpose_cn(y, C, N, y_s, true, stream);
+  return transpose_cn(y, C, N, y_s, kRowS24, stream);
 }
 
 int pg_transpose_frames_s24(const void *y, const int64_t *frame_off,
                             int64_t F, int64_t max_n, int64_t C, void *y_s,
                             pg_stream_t stream) {
-  return transpose_frames(y, frame_off, F, max_n, C, y_s, true, stream);
+  return transpose_frames(y, frame_off, F, max_n, C, y_s, kRowS24, stream);
+}
+
+int64_t pg_bfp_row_bytes(int64_t C) { return C >= 0 ? bfp_row_bytes(C) : -1; }
+
+int pg_transpose_cn_bfp(const void *y, int64_t C, int64_t N, void *y_b,
+                        pg_stream_t stream) {
+  return transpose_cn(y, C, N, y_b, kRowBfp, stream);
+}
+
+int pg_transpose_frames_bfp(const void *y, const int64_t *frame_off,
+                            int64_t F, int64_t max_n, int64_t C, void *y_b,
+                            pg_stream_t stream) {
+  return transpose_frames(y, frame_off, F, max_n, C, y_b, kRowBfp, stream);
 }
 
 int pg_sparse_apply_c64(const void *y, const int64_t *interp_idx,

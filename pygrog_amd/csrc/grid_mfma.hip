@@ -71,7 +71,7 @@ __global__ void k_table_pack_bf16(const float2 *__restrict__ K, int64_t nK,
 #endif
 constexpr int kTilesPerWave = PG_APPLY_TILES;  // 16-replica N tiles per wavefront
 
-template <int MT, int KS, int NTERMS, bool S24>
+template <int MT, int KS, int NTERMS, int RF>
 __global__ __launch_bounds__(256) void k_apply_grouped(
     const void *__restrict__ y_t, const unsigned short *__restrict__ tbl,
     const uint32_t *__restrict__ ent_ptr, const uint32_t *__restrict__ ent_rep,
@@ -172,6 +172,22 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
       }
     }
   };
+  // block-floating-point rows: 8 values = one 16-byte group at byte
+  // 2 col0; groups past the row's 2C values do not exist (value 0)
+  auto load_bfp = [&](int t, uint32_t (&d)[KS][4]) {
+    const bool v = start + 16 * t + j < stop;
+    const unsigned char *yr = static_cast<const unsigned char *>(y_t) + (int64_t)src[t] * bfp_row_bytes(C);
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int col0 = 32 * ks + 8 * q;
+      if (v && col0 < twoC) {
+        const uint4 a = gld(reinterpret_cast<const uint4 *>(yr + 2 * col0));
+        d[ks][0] = a.x; d[ks][1] = a.y; d[ks][2] = a.z; d[ks][3] = a.w;
+      } else {
+        d[ks][0] = d[ks][1] = d[ks][2] = d[ks][3] = 0u;  // E = 0, m = 0
+      }
+    }
+  };
   auto load_rows = [&](int t, float (&f)[KS][8]) {
     const bool v = start + 16 * t + j < stop;
     const float *yr = static_cast<const float *>(y_t) + (int64_t)src[t] * twoC;
@@ -199,12 +215,14 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
 #ifndef PG_APPLY_DEPTH24
 #define PG_APPLY_DEPTH24 2
 #endif
-  constexpr int DEPTH = KS <= 2 ? (S24 ? PG_APPLY_DEPTH24 : PG_APPLY_DEPTH) : 1;
+  constexpr int DEPTH = KS <= 2 ? (RF != kRowC64 ? PG_APPLY_DEPTH24 : PG_APPLY_DEPTH) : 1;
   constexpr int NBUF = DEPTH + 1;
   float fr_[NBUF][KS][8];     // fp32 rows
   uint32_t fs_[NBUF][KS][6];  // 24-bit rows, packed
+  uint32_t fb_[NBUF][KS][4];  // block-floating-point rows, packed
   auto load = [&](int t, int buf) {
-    if constexpr (S24) load_packed(t, fs_[buf]);
+    if constexpr (RF == kRowS24) load_packed(t, fs_[buf]);
+    else if constexpr (RF == kRowBfp) load_bfp(t, fb_[buf]);
     else load_rows(t, fr_[buf]);
   };
 #pragma unroll
@@ -214,7 +232,13 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
   for (int t = 0; t < kTilesPerWave; ++t) {
     if (t >= n_tiles) break;
     float cur[KS][8];
-    if constexpr (S24) {
+    if constexpr (RF == kRowBfp) {
+#pragma unroll
+      for (int ks = 0; ks < KS; ++ks) {
+        const uint32_t(&d)[4] = fb_[t % NBUF][ks];
+        bfp_decode8(d[0], d[1], d[2], d[3], &cur[ks][0]);
+      }
+    } else if constexpr (RF == kRowS24) {
 #pragma unroll
       for (int ks = 0; ks < KS; ++ks) {
         const uint32_t(&d)[6] = fs_[t % NBUF][ks];
@@ -304,11 +328,11 @@ static int launch_grouped(const void *y_t, const void *tbl,
                           const uint32_t *ent_ptr, const uint32_t *ent_rep,
                           const uint32_t *ent_src, const float *ent_s,
                           const uint32_t *items, int64_t n_items, int64_t K,
-                          int64_t C, int32_t n_terms, bool s24, void *vbuf,
+                          int64_t C, int32_t n_terms, int rf, void *vbuf,
                           hipStream_t st) {
   if (n_items == 0) return 0;
   PG_CHECK(items != nullptr, "grouped apply: null work list");
-  PG_CHECK(!s24 || ((uintptr_t)y_t & 15) == 0, "grouped apply: 24-bit rows not 16-byte aligned");
+  PG_CHECK(rf == kRowC64 || ((uintptr_t)y_t & 15) == 0, "grouped apply: packed rows not 16-byte aligned");
   const dim3 grid((unsigned)ceil_div(n_items, 4));
 #define PG_APPLY_LAUNCH(NT, S)                                                   \
   hipLaunchKernelGGL((k_apply_grouped<MT, KS, NT, S>), grid, dim3(256), 0, st,   \
@@ -316,11 +340,13 @@ static int launch_grouped(const void *y_t, const void *tbl,
                      ent_rep, ent_src, ent_s, items, n_items, K, (int)C,          \
                      static_cast<float *>(vbuf))
   if (n_terms == 3) {
-    if (s24) PG_APPLY_LAUNCH(3, true);
-    else PG_APPLY_LAUNCH(3, false);
+    if (rf == kRowS24) PG_APPLY_LAUNCH(3, kRowS24);
+    else if (rf == kRowBfp) PG_APPLY_LAUNCH(3, kRowBfp);
+    else PG_APPLY_LAUNCH(3, kRowC64);
   } else {
-    if (s24) PG_APPLY_LAUNCH(1, true);
-    else PG_APPLY_LAUNCH(1, false);
+    if (rf == kRowS24) PG_APPLY_LAUNCH(1, kRowS24);
+    else if (rf == kRowBfp) PG_APPLY_LAUNCH(1, kRowBfp);
+    else PG_APPLY_LAUNCH(1, kRowC64);
   }
 #undef PG_APPLY_LAUNCH
   PG_LAUNCHED();
@@ -360,7 +386,7 @@ static int apply_grouped(const void *y_t, const void *table_bf16,
                          const uint32_t *ent_ptr, const uint32_t *ent_rep,
                          const uint32_t *ent_src, const float *ent_s,
                          const uint32_t *items, int64_t n_items, int64_t K,
-                         int64_t C, int32_t n_terms, bool s24, void *vbuf,
+                         int64_t C, int32_t n_terms, int rf, void *vbuf,
                          pg_stream_t stream) {
   PG_CHECK(C >= 1 && C <= 64, "grouped apply: coil count must be in [1, 64]");
   PG_CHECK(n_terms == 1 || n_terms == 3, "grouped apply: n_terms 1 or 3");
@@ -372,7 +398,7 @@ static int apply_grouped(const void *y_t, const void *table_bf16,
   if (MT == MTV && KS == KSV)                                                \
     return launch_grouped<MTV, KSV>(y_t, table_bf16, ent_ptr, ent_rep,       \
                                     ent_src, ent_s, items, n_items, K, C,     \
-                                    n_terms, s24, vbuf, st);
+                                    n_terms, rf, vbuf, st);
   PG_GROUPED_CASE(1, 1)
   PG_GROUPED_CASE(2, 1)
   PG_GROUPED_CASE(3, 2)
@@ -393,7 +419,7 @@ int pg_apply_grouped_bf16(const void *y_t, const void *table_bf16,
                           int64_t K, int64_t C, int32_t n_terms, void *vbuf,
                           pg_stream_t stream) {
   return apply_grouped(y_t, table_bf16, ent_ptr, ent_rep, ent_src, ent_s, items,
-                       n_items, K, C, n_terms, false, vbuf, stream);
+                       n_items, K, C, n_terms, kRowC64, vbuf, stream);
 }
 
 int pg_apply_grouped_bf16_s24(const void *y_s, const void *table_bf16,
@@ -403,7 +429,17 @@ int pg_apply_grouped_bf16_s24(const void *y_s, const void *table_bf16,
                               int64_t K, int64_t C, int32_t n_terms, void *vbuf,
                               pg_stream_t stream) {
   return apply_grouped(y_s, table_bf16, ent_ptr, ent_rep, ent_src, ent_s, items,
-                       n_items, K, C, n_terms, true, vbuf, stream);
+                       n_items, K, C, n_terms, kRowS24, vbuf, stream);
+}
+
+int pg_apply_grouped_bf16_bfp(const void *y_b, const void *table_bf16,
+                              const uint32_t *ent_ptr, const uint32_t *ent_rep,
+                              const uint32_t *ent_src, const float *ent_s,
+                              const uint32_t *items, int64_t n_items,
+                              int64_t K, int64_t C, int32_t n_terms, void *vbuf,
+                              pg_stream_t stream) {
+  return apply_grouped(y_b, table_bf16, ent_ptr, ent_rep, ent_src, ent_s, items,
+                       n_items, K, C, n_terms, kRowBfp, vbuf, stream);
 }
 
 }  // extern "C"

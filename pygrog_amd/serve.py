@@ -51,8 +51,8 @@ import torch
 
 from . import _lib
 from .calib._grappa import build_table
-from .calib._grog import (PLAN_RADIUS, GridProgram, GrogInterpolator, _default_oversamp, _kernel_offsets,
-                          sample_row_bytes, uses_s24)
+from .calib._grog import (GRID_MODES, PLAN_RADIUS, GridProgram, GrogInterpolator, _default_oversamp,
+                          _kernel_offsets, apply_entry, mode_terms, row_format, sample_row_bytes, transpose_entry)
 
 
 @dataclasses.dataclass(eq=False)
@@ -292,7 +292,7 @@ class GrogServer:
 
     def __init__(self, device=None, *, mode: str = "fp32", hbm_budget: int | None = None, slots: int = 2,
                  max_inflight: int = 2, keep_results: bool = False):
-        if mode not in ("fp32", "bf16", "bf16x3"):
+        if mode not in GRID_MODES:
             raise ValueError(f"unknown gridding mode {mode!r}")
         self.device = _lib.hip_device(device)
         self.mode = mode
@@ -347,7 +347,7 @@ class GrogServer:
             g._attach_table(res, job.precision, job.quant)
             if self.mode != "fp32":
                 g._entry_groups()
-                g._table_bf16(2 if self.mode == "bf16x3" else 1)
+                g._table_bf16(2 if mode_terms(self.mode) == 3 else 1)
             ev = torch.cuda.Event()
             ev.record(st)
         job.interp = g
@@ -372,20 +372,19 @@ class GrogServer:
                              f"of {comp.S} trajectory")
         if y.device != self.device or y.dtype != torch.complex64 or not y.is_contiguous():
             y = y.to(self.device, torch.complex64).contiguous()
-        s24 = uses_s24(self.mode, comp.C)
-        yt = self._alloc(max(comp.npts, 1) * sample_row_bytes(comp.C, s24), torch.uint8)
+        fmt = row_format(self.mode, comp.C)
+        yt = self._alloc(max(comp.npts, 1) * sample_row_bytes(comp.C, fmt), torch.uint8)
         out = self._alloc(comp.C * comp.G, torch.complex64).reshape(comp.C, comp.G)
         vbuf = None
         if self.mode != "fp32":
             vbuf = self._alloc(g._rmax() * comp.C, torch.complex64)
-        _lib.call("pg_transpose_cn_s24" if s24 else "pg_transpose_cn_c64", _lib.ptr(y), comp.C, comp.npts,
-                  _lib.ptr(yt), st.cuda_stream)
+        _lib.call(transpose_entry(fmt, False), _lib.ptr(y), comp.C, comp.npts, _lib.ptr(yt), st.cuda_stream)
         if self.mode != "fp32":
-            n_terms = 1 if self.mode == "bf16" else 3
+            n_terms = mode_terms(self.mode)
             tbl = g._table_bf16(2 if n_terms == 3 else 1)
             grp = g._entry_groups()[0]
             if grp.n_items > 0:
-                _lib.call("pg_apply_grouped_bf16_s24" if s24 else "pg_apply_grouped_bf16", _lib.ptr(yt),
+                _lib.call(apply_entry(fmt), _lib.ptr(yt),
                           _lib.ptr(tbl), _lib.ptr(grp.ent_ptr),
                           _lib.ptr(grp.ent_rep), _lib.ptr(grp.ent_src), _lib.ptr(grp.ent_s),
                           _lib.ptr(grp.items), grp.n_items, int(comp.table.shape[0]), comp.C, n_terms,

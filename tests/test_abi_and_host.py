@@ -102,7 +102,19 @@ def test_library_version_and_errors():
 
     assert uses_s24("bf16x3", 32) and uses_s24("bf16", 20)
     assert not uses_s24("bf16x3", 16) and not uses_s24("bf16x3", 8) and not uses_s24("fp32", 32)
-    assert not uses_s24("bf16x3", 33)
+    assert not uses_s24("bf16x3", 33) and not uses_s24("bf16x3c", 32)
+    # block-floating-point rows: 16 bytes per 4 coils, 16-byte aligned buffers
+    from pygrog_amd.calib._grog import row_format, sample_row_bytes
+
+    assert [lib.pg_bfp_row_bytes(c) for c in (0, 1, 4, 5, 16, 32, 33)] == [0, 16, 16, 32, 64, 128, 144]
+    assert lib.pg_bfp_row_bytes(-1) == -1
+    assert [row_format("bf16x3c", c) for c in (1, 16, 32, 64)] == ["bfp"] * 4
+    assert row_format("bf16x3", 32) == "s24" and row_format("fp32", 32) == "c64"
+    assert sample_row_bytes(32, "bfp") == 128 and sample_row_bytes(32, "c64") == 256
+    assert lib.pg_transpose_cn_bfp(4096, 32, 100, 4104, None) != 0 and b"16-byte" in lib.pg_last_error()
+    assert lib.pg_transpose_frames_bfp(4096, 4096, 1, 100, 32, 4104, None) != 0
+    assert lib.pg_apply_grouped_bf16_bfp(4104, 4096, 4096, 4096, 4096, 4096, 4096, 5, 10, 32, 3, 4096, None) != 0
+    assert b"16-byte" in lib.pg_last_error()
 
 
 def test_kernel_offsets_and_quant_steps():
