@@ -143,9 +143,12 @@ def alg_bytes(kernel: str, st: dict) -> float:
         # sample rows + bf16 table (complex C x C per entry and part) + V rows
         # written + records (source, sqrt weight, replica id)
         row = rb24 if kernel.endswith("_s24") else rbfp if kernel.endswith("_bfp") else 8 * C
-        return row * R + 4 * parts * C * C * K + 8 * C * R + 12 * R
+        vrow = rbfp if kernel.endswith("_bfp") else 8 * C  # bf16x3c writes V in the same compact rows
+        return row * R + 4 * parts * C * C * K + vrow * R + 12 * R
     if kernel == "pg_grid_reduce_v":
         return 8 * C * R + 8 * C * G + 8 * G
+    if kernel == "pg_grid_reduce_v_bfp":
+        return rbfp * R + 8 * C * G + 8 * G
     if kernel == "pg_grid_scatter":
         return 8 * C * R + 8 * C * G + 8 * R + 8 * G
     if kernel in ("pg_transpose_cn_c64", "pg_transpose_frames_c64"):
@@ -204,7 +207,8 @@ def step_roofline(stats: dict, mode_res: dict, mode: str, traffic_file) -> dict:
 # the fp32 and reduce entries (same plan, same hot cells)
 ENTRY_KERNELS = {
     "pg_grid_fp32": [r"k_hot_units(?:_par)?<\d+, 0>", r"k_hot_cells<", r"k_tile<\d+, 0, \d+>|k_wtile<\d+, 0>"],
-    "pg_grid_reduce_v": [r"k_hot_units<\d+, 1>", r"k_hot_cells<", r"k_tile<\d+, 1, \d+>|k_wtile<\d+, 1>|k_wrows<\d+>"],
+    "pg_grid_reduce_v": [r"k_hot_units<\d+, 1>", r"k_hot_cells<", r"k_tile<\d+, 1, \d+>|k_wtile<\d+, 1>|k_wrows<\d+, 1>"],
+    "pg_grid_reduce_v_bfp": [r"k_hot_units<\d+, 3>", r"k_hot_cells<", r"k_wtile<\d+, 3>|k_wrows<\d+, 3>"],
     "pg_apply_grouped_bf16:1": [r"k_apply_grouped<\d+, \d+, 1, 0>"],
     "pg_apply_grouped_bf16:3": [r"k_apply_grouped<\d+, \d+, 3, 0>"],
     "pg_apply_grouped_bf16_s24:1": [r"k_apply_grouped<\d+, \d+, 1, 1>"],

@@ -410,6 +410,16 @@ int pg_grid_reduce_v(const pg_frame *frames, const uint32_t *tile_ptr,
                      int64_t tile_cells, int64_t n_blocks,
                      int64_t max_hot_units, int64_t max_hot,
                      pg_stream_t stream);
+/* pg_grid_reduce_v with frame.vbuf holding block-floating-point rows (the
+ * bfp15 layout of pg_bfp_row_bytes(C) bytes per replica, as written by
+ * pg_apply_grouped_bf16_bfp): each value is decoded, then folded in the same
+ * canonical order.  tile_cells must be the wave tile (256); other tile sizes
+ * are an error.  No reference counterpart. */
+int pg_grid_reduce_v_bfp(const pg_frame *frames, const uint32_t *tile_ptr,
+                         int64_t F, int64_t n_tiles, int64_t C,
+                         int64_t tile_cells, int64_t n_blocks,
+                         int64_t max_hot_units, int64_t max_hot,
+                         pg_stream_t stream);
 
 /* Third mode, used by SparseFFT (reference _scatter_ifft_crop,
  * src/pygrog/operator/_sparse_fft.py:396-435, 1154-1165): identity operator,
@@ -543,8 +553,12 @@ int pg_apply_grouped_bf16_s24(const void *y_s, const void *table_bf16,
                               int64_t K, int64_t C, int32_t n_terms, void *vbuf,
                               pg_stream_t stream);
 /* pg_apply_grouped_bf16 reading the bfp15 sample rows of
- * pg_transpose_cn_bfp / pg_transpose_frames_bfp (y_b 16-byte aligned); the
- * products and the V rows are as there.  No reference counterpart. */
+ * pg_transpose_cn_bfp / pg_transpose_frames_bfp (y_b 16-byte aligned) and
+ * writing V in the same bfp15 row layout (vbuf: replica r's row of
+ * pg_bfp_row_bytes(C) bytes at byte r * pg_bfp_row_bytes(C), 16-byte
+ * aligned): each 4-coil group of (K[e] y_b[src]) * s * s is encoded as the
+ * transposes encode k-space.  The products are as there; reduce with
+ * pg_grid_reduce_v_bfp.  No reference counterpart. */
 int pg_apply_grouped_bf16_bfp(const void *y_b, const void *table_bf16,
                               const uint32_t *ent_ptr, const uint32_t *ent_rep,
                               const uint32_t *ent_src, const float *ent_s,

@@ -112,6 +112,8 @@ _SIGS = {
     "pg_table_packed_elems": [c_int64, c_int64],
     "pg_grid_fp32": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
     "pg_grid_reduce_v": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
+    "pg_grid_reduce_v_bfp": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64,
+                             c_void_p],
     "pg_grid_scatter": [c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_int64, c_void_p],
     "pg_grid_tile_cells": [c_int64],
     "pg_grid_tile_options": [c_int64, c_void_p, ctypes.c_int32],

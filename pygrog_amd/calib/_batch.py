@@ -36,8 +36,9 @@ import torch
 
 from .. import _lib
 from ._grappa import build_table
-from ._grog import (GRID_MODES, GridProgram, GrogInterpolator, KernelTimer, apply_entry, group_by_entry, mode_terms,
-                    new_sample_rows, row_format, row_format_of, sample_row_bytes, transpose_entry, tune_program)
+from ._grog import (GRID_MODES, GridProgram, GrogInterpolator, KernelTimer, apply_entry, group_by_entry,
+                    mode_default_tile, mode_terms, mode_tile_options, new_sample_rows, reduce_entry, row_format,
+                    row_format_of, sample_row_bytes, transpose_entry, tune_program, v_format)
 
 _TUNE_CACHE: dict = {}
 
@@ -136,18 +137,20 @@ class GrogFrameBatch:
         C = self.C
         yt = new_sample_rows(int(self.sample_off[-1]), C, row_format(mode, C), self.device)
         rb = sample_row_bytes(C, row_format_of(yt))
-        tc = _lib.load().pg_grid_tile_cells(C)
+        tc = mode_default_tile(mode, C)
         frames = []
         vbuf = None
         if mode != "fp32":
             m = self._merged_groups()
-            vbuf = torch.empty(max(m["R"], 1) * C, dtype=torch.complex64, device=self.device)
+            vbuf = new_sample_rows(max(m["R"], 1), C, v_format(mode), self.device)
+            vrow = sample_row_bytes(C, v_format(mode)) // vbuf.element_size()
         for f, fr in enumerate(self.frames):
             yv = yt[(rb // yt.element_size()) * int(self.sample_off[f]):]
-            vv = None if vbuf is None else vbuf[C * int(self._merged["rep_off"][f]):]
+            vv = None if vbuf is None else vbuf[vrow * int(self._merged["rep_off"][f]):]
             frames += fr._frames(yv, out[f], 1, vv)
         tune = self.tuned.get(mode, {})
-        prog = GridProgram(frames, C, self.G, self.device, tune.get("tile_cells", tc), tune.get("n_blocks"))
+        prog = GridProgram(frames, C, self.G, self.device, tune.get("tile_cells", tc), tune.get("n_blocks"),
+                           tile_opts=mode_tile_options(mode, C))
         self._programs[key] = (yt, vbuf, prog)
         return self._programs[key]
 
@@ -191,7 +194,7 @@ class GrogFrameBatch:
                       grp.n_items, self.K, self.C, n_terms, _lib.ptr(vbuf), st)
         if timer is not None:
             timer.stop(name, e0)
-        prog.launch("pg_grid_reduce_v", timer)
+        prog.launch(reduce_entry(mode), timer)
 
     def new_output(self) -> torch.Tensor:
         return torch.empty(self.F, self.C, *self.grid_shape, dtype=torch.complex64, device=self.device)
@@ -237,8 +240,7 @@ class GrogFrameBatch:
         yt, vbuf, prog = self._program(mode, y, out)
         if key not in _TUNE_CACHE:
             self._run(mode, y, out)
-            entry = "pg_grid_fp32" if mode == "fp32" else "pg_grid_reduce_v"
-            _TUNE_CACHE[key] = tune_program(prog, entry, reps, candidates)
+            _TUNE_CACHE[key] = tune_program(prog, reduce_entry(mode), reps, candidates)
         self.tuned[mode] = dict(_TUNE_CACHE[key])
         for k, (yt_, vbuf_, pr) in self._programs.items():
             if k[0] == mode:

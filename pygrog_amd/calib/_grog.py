@@ -425,6 +425,38 @@ def sample_row_bytes(C: int, fmt: str) -> int:
     return 8 * C
 
 
+def v_format(mode: str) -> str:
+    """Row format of a grouped mode's applied-replica rows V: bf16x3c keeps
+    them in the block-floating-point format too (half the bytes of
+    complex64 rows both ways: written by the apply, read by the reduction),
+    the other modes in complex64."""
+    return "bfp" if mode == "bf16x3c" else "c64"
+
+
+def reduce_entry(mode: str) -> str:
+    """The reduction (or, for fp32, the fused gridding) entry of ``mode``."""
+    if mode == "fp32":
+        return "pg_grid_fp32"
+    return "pg_grid_reduce_v_bfp" if v_format(mode) == "bfp" else "pg_grid_reduce_v"
+
+
+def mode_tile_options(mode: str, C: int) -> list:
+    """Launch tiles of ``mode``'s gridding kernel: block-floating-point V
+    rows are reduced by the wave-tile kernels only."""
+    opts = _lib.tile_options(C)
+    return [t for t in opts if t == WAVE_TILE] if mode is not None and v_format(mode) == "bfp" else opts
+
+
+def mode_default_tile(mode: str, C: int) -> int:
+    opts = mode_tile_options(mode, C)
+    t = int(_lib.load().pg_grid_tile_cells(C))
+    return t if t in opts else opts[0]
+
+
+# cells per wave-tile launch (four wavefronts of 64 cells)
+WAVE_TILE = 256
+
+
 def new_sample_rows(n: int, C: int, fmt: str, device) -> torch.Tensor:
     """Buffer for the sample-major copy of ``n`` samples in format ``fmt``."""
     dt = _ROW_DTYPE[fmt]
@@ -440,9 +472,11 @@ class GridProgram:
     the kernel (no host work beyond the ctypes call).
     """
 
-    def __init__(self, frames: list, C: int, G: int, device, tile_cells: int, n_blocks: int | None = None):
+    def __init__(self, frames: list, C: int, G: int, device, tile_cells: int, n_blocks: int | None = None,
+                 tile_opts: list | None = None):
         self.device = device
         self.C = int(C)
+        self.tile_opts = list(tile_opts) if tile_opts is not None else _lib.tile_options(self.C)
         self.F = len(frames)
         self.tile_cells = int(tile_cells)
         # per-frame scratch of the hot-cell pre-pass
@@ -466,8 +500,8 @@ class GridProgram:
         """Launch shape: cells per tile (one of ``_lib.tile_options(C)``) and
         the persistent grid size (None: one workgroup per tile up to 12 per
         CU).  The shape never changes the result bits."""
-        if int(tile_cells) not in _lib.tile_options(self.C):
-            raise ValueError(f"tile_cells {tile_cells} not compiled for C={self.C}: {_lib.tile_options(self.C)}")
+        if int(tile_cells) not in self.tile_opts:
+            raise ValueError(f"tile_cells {tile_cells} not available for C={self.C}: {self.tile_opts}")
         self.tile_cells = int(tile_cells)
         tiles_per = (self.G + self.tile_cells - 1) // self.tile_cells
         self.n_tiles = tiles_per * self.F
@@ -493,7 +527,7 @@ def tune_program(prog: GridProgram, entry: str, reps: int = 5, candidates=None) 
     leave it at the fastest shape.  The shape never changes the bits."""
     n_cu = torch.cuda.get_device_properties(prog.device).multi_processor_count
     times = {}
-    for tc in _lib.tile_options(prog.C):
+    for tc in prog.tile_opts:
         prog.set_shape(tc, None)
         cands = candidates or [n_cu * k for k in (4, 8, 16, 32, 64)] + [prog.n_tiles]
         for nb in sorted({max(8, min(int(c), prog.n_tiles)) for c in cands}):
@@ -804,7 +838,8 @@ class GrogInterpolator:
         comp = self.plan._compiled
         return data.to(comp.device).to(torch.complex64).reshape(B, S, C, comp.npts).contiguous()
 
-    def _frames(self, y: torch.Tensor, out: torch.Tensor, B: int, vbuf: torch.Tensor | None = None):
+    def _frames(self, y: torch.Tensor, out: torch.Tensor, B: int, vbuf: torch.Tensor | None = None,
+                v_row: int | None = None):
         """Frames of a (B, S) launch; ``y`` holds the per-frame sources
         (sample-major (N, C) rows for the fp32 mode) back to back."""
         comp = self.plan._compiled
@@ -827,7 +862,9 @@ class GrogInterpolator:
                 f.unit_rep = comp.unit_rep.data_ptr() + s * 4
                 f.cell_unit = comp.cell_unit.data_ptr() + s * G * 4
                 if vbuf is not None:
-                    off = (b * S + s) * self._rmax() * C * 8
+                    # a frame's V: rmax rows of v_row bytes (complex64 or
+                    # block-floating-point rows), frames back to back
+                    off = (b * S + s) * self._rmax() * (v_row if v_row is not None else 8 * C)
                     f.vbuf = vbuf.data_ptr() + off
                 hot = comp.hot[s]
                 f.unit_hot = comp.unit_hot.data_ptr()
@@ -850,11 +887,13 @@ class GrogInterpolator:
     def grid_program(self, y: torch.Tensor, out: torch.Tensor, B: int = 1, vbuf=None, n_blocks=None,
                      mode: str | None = None) -> GridProgram:
         comp = self.plan._compiled
-        tc = _lib.load().pg_grid_tile_cells(comp.C)
+        tc = mode_default_tile(mode, comp.C)
         tuned = getattr(self, "_tuned", {}).get(mode)
         if tuned is not None and n_blocks is None:
             tc, n_blocks = tuned
-        return GridProgram(self._frames(y, out, B, vbuf), comp.C, comp.G, comp.device, tc, n_blocks)
+        v_row = sample_row_bytes(comp.C, v_format(mode)) if mode is not None and mode != "fp32" else None
+        return GridProgram(self._frames(y, out, B, vbuf, v_row), comp.C, comp.G, comp.device, tc, n_blocks,
+                           tile_opts=mode_tile_options(mode, comp.C))
 
     # -- grouped (BF16 MFMA) structures ---------------------------------------
     GROUP_CHUNK = 128  # replicas per MFMA work item (8 tiles of 16)
@@ -977,7 +1016,7 @@ class GrogInterpolator:
             if mode == "fp32":
                 cache[key] = (None, yt, self.grid_program(yt, out, B, mode=mode))
             else:
-                vbuf = torch.empty(B * S * self._rmax() * C, dtype=torch.complex64, device=comp.device)
+                vbuf = new_sample_rows(B * S * self._rmax(), C, v_format(mode), comp.device)
                 cache[key] = (vbuf, yt, self.grid_program(yt, out, B, vbuf=vbuf, mode=mode))
         vbuf, yt, prog = cache[key]
         self._transpose_all(y, yt, timer)
@@ -1007,13 +1046,13 @@ class GrogInterpolator:
                     if comp.n_valid[s] == 0 or grp.n_items == 0:
                         continue
                     yf = yt.data_ptr() + (b * S + s) * N * rb
-                    vb = vbuf.data_ptr() + (b * S + s) * self._rmax() * C * 8
+                    vb = vbuf.data_ptr() + (b * S + s) * self._rmax() * sample_row_bytes(C, v_format(mode))
                     _lib.call(name, yf, _lib.ptr(tbl), _lib.ptr(grp.ent_ptr), _lib.ptr(grp.ent_rep),
                               _lib.ptr(grp.ent_src), _lib.ptr(grp.ent_s), _lib.ptr(grp.items), grp.n_items, K, C,
                               n_terms, vb, st)
         if timer is not None:
             timer.stop(name, e0)
-        prog.launch("pg_grid_reduce_v", timer)
+        prog.launch(reduce_entry(mode), timer)
         return out
 
     def autotune(self, y: torch.Tensor, mode: str = "fp32", out: torch.Tensor | None = None,
@@ -1026,7 +1065,7 @@ class GrogInterpolator:
         out = self.grid_device(y, mode, out, program_cache=cache)
         B = y.shape[0]
         _, _, prog = cache[(mode, y.data_ptr(), out.data_ptr(), B)]
-        res = tune_program(prog, "pg_grid_fp32" if mode == "fp32" else "pg_grid_reduce_v", reps)
+        res = tune_program(prog, reduce_entry(mode), reps)
         self._tuned = getattr(self, "_tuned", {})
         self._tuned[mode] = (res["tile_cells"], res["n_blocks"])
         return res

@@ -76,6 +76,14 @@ template <> __device__ __forceinline__ void gst<float4>(float4 *p, float4 v) {
   n.w = v.w;
   *(PG_GAS pg_f4n *)(uintptr_t)p = n;
 }
+template <> __device__ __forceinline__ void gst<uint4>(uint4 *p, uint4 v) {
+  pg_u4n n;
+  n.x = v.x;
+  n.y = v.y;
+  n.z = v.z;
+  n.w = v.w;
+  *(PG_GAS pg_u4n *)(uintptr_t)p = n;
+}
 
 // Streaming (non-temporal) variants for data read or written once: they do
 // not displace what should stay in the caches (the operator table).
@@ -89,6 +97,10 @@ template <> __device__ __forceinline__ float2 gld_nt<float2>(const float2 *p) {
 template <> __device__ __forceinline__ float4 gld_nt<float4>(const float4 *p) {
   const pg_f4n v = __builtin_nontemporal_load((const PG_GAS pg_f4n *)(uintptr_t)p);
   return make_float4(v.x, v.y, v.z, v.w);
+}
+template <> __device__ __forceinline__ uint4 gld_nt<uint4>(const uint4 *p) {
+  const pg_u4n v = __builtin_nontemporal_load((const PG_GAS pg_u4n *)(uintptr_t)p);
+  return make_uint4(v.x, v.y, v.z, v.w);
 }
 __device__ __forceinline__ void gst_nt(float2 *p, float2 v) {
   pg_f2n n;
@@ -189,11 +201,21 @@ __device__ __forceinline__ uint4 bfp_encode8(const float (&x)[8]) {
   }
   return make_uint4(f[0] | (f[1] << 16), f[2] | (f[3] << 16), f[4] | (f[5] << 16), f[6] | (f[7] << 16));
 }
-__device__ __forceinline__ void bfp_decode8(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3, float *o) {
+__device__ __forceinline__ int bfp_scale(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
   const uint32_t t0 = d0 & 0x10001u, t1 = d1 & 0x10001u, t2 = d2 & 0x10001u, t3 = d3 & 0x10001u;
   const uint32_t E = ((t0 | (t0 >> 15)) & 3u) | (((t1 | (t1 >> 15)) & 3u) << 2) |
                      (((t2 | (t2 >> 15)) & 3u) << 4) | (((t3 | (t3 >> 15)) & 3u) << 6);
-  const int s = (int)E - kBfpBias;
+  return (int)E - kBfpBias;
+}
+// (re, im) of coil k (0..3) of a group
+__device__ __forceinline__ float2 bfp_decode_coil(uint4 g, int k) {
+  const int s = bfp_scale(g.x, g.y, g.z, g.w);
+  const uint32_t d = k == 0 ? g.x : k == 1 ? g.y : k == 2 ? g.z : g.w;
+  return make_float2(__builtin_amdgcn_ldexpf((float)((int)(d << 16) >> 17), s),
+                     __builtin_amdgcn_ldexpf((float)((int)d >> 17), s));
+}
+__device__ __forceinline__ void bfp_decode8(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3, float *o) {
+  const int s = bfp_scale(d0, d1, d2, d3);
   const uint32_t d[4] = {d0, d1, d2, d3};
 #pragma unroll
   for (int k = 0; k < 4; ++k) {

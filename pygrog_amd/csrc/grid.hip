@@ -59,6 +59,22 @@ __host__ __device__ constexpr int coil_group(int64_t C) {
 // MODE 1: v = vbuf[r]                          (grouped modes, second pass)
 // MODE 2: v = y[:, src] * s                    (SparseFFT scatter, identity op,
 //         y coil-major (C, n_src))
+// MODE 3: v = vbuf[r] decoded                  (MODE 1 with V in the
+//         block-floating-point row format of common.h: 16 ceil(C/4) bytes a
+//         row; wave-tile launches and the hot pre-pass only)
+template <int MODE> constexpr bool kVMode = MODE == 1 || MODE == 3;
+
+// coil co of V row r: MODE 1 complex64 rows, MODE 3 block-floating-point rows
+template <int MODE>
+__device__ __forceinline__ float2 v_coil(const void *vbuf, uint64_t r, int co, int C) {
+  if constexpr (MODE == 3) {
+    const uint4 g = gld(reinterpret_cast<const uint4 *>(static_cast<const unsigned char *>(vbuf) +
+                                                       r * (uint64_t)bfp_row_bytes(C) + 16 * (co >> 2)));
+    return bfp_decode_coil(g, co & 3);
+  } else {
+    return gld(reinterpret_cast<const float2 *>(vbuf) + r * (uint32_t)C + co);
+  }
+}
 struct Rec {
   uint32_t src, ent;
   float s;
@@ -67,7 +83,7 @@ struct Rec {
 template <int MODE>
 __device__ __forceinline__ Rec load_rec(const pg_frame &fr, uint32_t r) {
   Rec q{0u, 0u, 0.0f};
-  if (MODE != 1) {
+  if (!kVMode<MODE>) {
     q.src = fr.rep_src[r];
     q.s = fr.rep_s[r];
     if (MODE == 0) q.ent = fr.rep_ent[r];
@@ -183,8 +199,7 @@ template <int GS, int MODE>
 __device__ __forceinline__ float2 coil_value(const pg_frame &fr, uint32_t r,
                                              const Rec &q, int co, int C,
                                              int P) {
-  if (MODE == 1)
-    return reinterpret_cast<const float2 *>(fr.vbuf)[(int64_t)r * C + co];
+  if (kVMode<MODE>) return v_coil<MODE>(fr.vbuf, r, co, C);
   if (MODE == 2) {
     const float2 a =
         reinterpret_cast<const float2 *>(fr.y)[(int64_t)co * fr.n_src + q.src];
@@ -205,14 +220,13 @@ __device__ __forceinline__ float2 unit_partial(const pg_frame &fr, uint32_t u,
   const uint32_t r0 = fr.unit_rep[u], r1 = fr.unit_rep[u + 1];
   float px = 0.0f, py = 0.0f;
   if (r0 >= r1 || co >= C) return make_float2(px, py);
-  if (MODE == 1) {
+  if (kVMode<MODE>) {
     // contiguous rows: issue 8 loads, then add them in order
-    const float2 *vb = reinterpret_cast<const float2 *>(fr.vbuf) + co;
     for (uint32_t rb = r0; rb < r1; rb += 8) {
       float2 v[8];
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        v[k] = (rb + k < r1) ? vb[(int64_t)(rb + k) * C] : make_float2(0.f, 0.f);
+        v[k] = (rb + k < r1) ? v_coil<MODE>(fr.vbuf, rb + k, co, C) : make_float2(0.f, 0.f);
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         if (rb + k < r1) {
@@ -280,8 +294,8 @@ __device__ __forceinline__ float2 multi_unit_coil(const pg_frame &fr, uint32_t u
 #pragma unroll 1
     for (uint32_t r = r0; r < r1; ++r) {
       float vx = 0.0f, vy = 0.0f;
-      if (MODE == 1) {
-        const float2 v = gld(reinterpret_cast<const float2 *>(fr.vbuf) + (uint64_t)r * (uint32_t)C + co);
+      if (kVMode<MODE>) {
+        const float2 v = v_coil<MODE>(fr.vbuf, r, co, C);
         vx = v.x;
         vy = v.y;
       } else if (MODE == 2) {
@@ -625,9 +639,9 @@ __global__ __launch_bounds__(kGridThreads) void k_wtile(
   constexpr int PM = GS / 2;  // table pairs per (entry, coil)
   static_assert(GS % 2 == 0 && (RS % 2) == 0 && kWave % CW == 0, "wave tile shape");
   __shared__ __align__(16) float2 s_rows[NW][RB * RS];
-  __shared__ uint32_t s_ra[NW][RB];                 // MODE 1: replica; else source
+  __shared__ uint32_t s_ra[NW][RB];                 // MODE 1/3: replica; else source
   __shared__ uint32_t s_rb[NW][MODE == 0 ? RB : 1];  // MODE 0: entry
-  __shared__ float s_rw[NW][MODE == 1 ? 1 : RB];     // MODE 0/2: weight
+  __shared__ float s_rw[NW][kVMode<MODE> ? 1 : RB];  // MODE 0/2: weight
 
   const int w = threadIdx.x / kWave;
   const int l = threadIdx.x % kWave;
@@ -655,7 +669,7 @@ __global__ __launch_bounds__(kGridThreads) void k_wtile(
     for (int j = l; j < RB; j += kWave) {
       ra[j] = 0u;
       if (MODE == 0) rb[j] = 0u;
-      if (MODE != 1) rw[j] = 0.0f;
+      if (!kVMode<MODE>) rw[j] = 0.0f;
     }
 
     // the cell's row ranges; cells with rows but no ordinary rows are hot
@@ -701,7 +715,7 @@ __global__ __launch_bounds__(kGridThreads) void k_wtile(
           float q2[4];
 #pragma unroll
           for (int i = 0; i < 4; ++i) r[i] = cr0 + ((k1 + i < b ? k1 + i : k1) - lo);
-          if (MODE != 1) {
+          if (!kVMode<MODE>) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
               q0[i] = sld(fr.rep_src + r[i]);
@@ -713,7 +727,7 @@ __global__ __launch_bounds__(kGridThreads) void k_wtile(
           for (int i = 0; i < 4; ++i) {
             if (k1 + i < b) {
               const int j = (int)(k1 + i - k0);
-              if (MODE == 1) {
+              if (kVMode<MODE>) {
                 ra[j] = r[i];
               } else {
                 ra[j] = q0[i];
@@ -744,9 +758,44 @@ __global__ __launch_bounds__(kGridThreads) void k_wtile(
           const int cs = cv ? cg : 0;
           // this lane's first value of every row (MODE 2: its coil's row)
           const float2 *ybase =
-              MODE == 1 ? reinterpret_cast<const float2 *>(fr.vbuf) + cs
+              kVMode<MODE> ? reinterpret_cast<const float2 *>(fr.vbuf) + cs
               : MODE == 0 ? reinterpret_cast<const float2 *>(fr.y) + cs
                           : reinterpret_cast<const float2 *>(fr.y) + (uint64_t)cs * fr.n_src;
+          if constexpr (MODE == 3) {
+            // block-floating-point V rows: lane = (row, 4-coil group of the
+            // pass), one 16-byte load per row, 4 coils decoded into LDS
+            constexpr int GW = CW / 4;
+            constexpr int RPG = kWave / GW;  // rows per load instruction
+            constexpr int NG = 8;            // loads per lane and round
+            const int gl = l % GW, jg = l / GW;
+            const int ga = cc / 4 + gl;
+            const unsigned char *gb = static_cast<const unsigned char *>(fr.vbuf) + 16 * (4 * ga < C ? ga : 0);
+            const uint32_t vrb = (uint32_t)bfp_row_bytes(C);
+            for (int jb = 0; jb < nk; jb += NG * RPG) {
+              uint32_t qg[NG];
+              uint4 gq[NG];
+#pragma unroll
+              for (int i = 0; i < NG; ++i) {
+                const int jj = jb + jg + i * RPG;
+                qg[i] = ra[jj < RB ? jj : RB - 1];
+              }
+#pragma unroll
+              for (int i = 0; i < NG; ++i)
+                if (jb + i * RPG < nk) gq[i] = sld(reinterpret_cast<const uint4 *>(gb + (uint64_t)qg[i] * vrb));
+#pragma unroll
+              for (int i = 0; i < NG; ++i) {
+                if (jb + i * RPG >= nk) break;
+                const int j = jb + jg + i * RPG;
+                if (j < nk) {
+                  float o[8];
+                  bfp_decode8(gq[i].x, gq[i].y, gq[i].z, gq[i].w, o);
+#pragma unroll
+                  for (int k = 0; k < 4; ++k)
+                    if (cc + 4 * gl + k < C) rows[j * RS + 4 * gl + k] = make_float2(o[2 * k], o[2 * k + 1]);
+                }
+              }
+            }
+          } else
           for (int jb = 0; jb < nk; jb += NB) {
             // the round's record indices first (all LDS reads in flight
             // together), then its loads
@@ -890,6 +939,27 @@ __device__ __forceinline__ void load_row(const float2 *vr, int C, int cc, bool e
   }
 }
 
+// the same for block-floating-point V rows (MODE 3): W coils from coil cc
+// (a multiple of 4) of the row at vr, W / 4 groups of 16 bytes
+template <int W>
+__device__ __forceinline__ void load_row_bfp(const unsigned char *vr, int C, int cc,
+                                             float2 (&v)[W]) {
+  static_assert(W % 4 == 0, "whole groups");
+  uint4 g[W / 4];
+#pragma unroll
+  for (int i = 0; i < W / 4; ++i) {
+    const int gi = cc / 4 + i;
+    g[i] = gld(reinterpret_cast<const uint4 *>(vr + 16 * (4 * gi < C ? gi : 0)));
+  }
+#pragma unroll
+  for (int i = 0; i < W / 4; ++i) {
+    float o[8];
+    bfp_decode8(g[i].x, g[i].y, g[i].z, g[i].w, o);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[4 * i + k] = make_float2(o[2 * k], o[2 * k + 1]);
+  }
+}
+
 #ifndef PG_WROWS_CW
 #define PG_WROWS_CW 16
 #endif
@@ -905,7 +975,7 @@ __device__ __forceinline__ void load_row(const float2 *vr, int C, int cc, bool e
 #ifndef PG_WROWS_NTST
 #define PG_WROWS_NTST 1
 #endif
-template <int GS>
+template <int GS, int MODE>
 __global__ __launch_bounds__(kGridThreads) void k_wrows(
     const pg_frame *__restrict__ frames, const uint32_t *__restrict__ tile_ptr,
     int F, int C, uint32_t n_tiles) {
@@ -958,7 +1028,7 @@ __global__ __launch_bounds__(kGridThreads) void k_wrows(
         if (!(fr.n_hot > 0 && gld(fr.unit_hot + ua) >= 0)) {
 #pragma unroll 1
           for (int c = 0; c < C; ++c)
-            gst(g + (int64_t)c * fr.n_cell, multi_unit_coil<GS, 1>(fr, ua, ub, c, C));
+            gst(g + (int64_t)c * fr.n_cell, multi_unit_coil<GS, MODE>(fr, ua, ub, c, C));
         }
       } else {
         const uint32_t n = h.cr1 - h.cr0;
@@ -968,6 +1038,12 @@ __global__ __launch_bounds__(kGridThreads) void k_wrows(
         const uint32_t r1 = n >= 2 ? h.cr0 + 1 : r0;
         const uint32_t rn = TWO ? h.cr0 + 2 : h.cr0 + 1;  // first row of the loop
         const float2 *vb = reinterpret_cast<const float2 *>(fr.vbuf);
+        const unsigned char *vq = static_cast<const unsigned char *>(fr.vbuf);
+        const int64_t vrb = bfp_row_bytes(C);
+        auto ldrow = [&](uint32_t r, int cc, float2 (&v)[CW]) {
+          if constexpr (MODE == 3) load_row_bfp<CW>(vq + (int64_t)r * vrb, C, cc, v);
+          else load_row<CW>(vb + (int64_t)r * C + cc, C, cc, even, v);
+        };
 #pragma unroll 1
         for (int cc = 0; cc < C; cc += CW) {
           float2 a[CW], b[CW];
@@ -975,8 +1051,8 @@ __global__ __launch_bounds__(kGridThreads) void k_wrows(
           for (int c = 0; c < CW; ++c) a[c] = b[c] = make_float2(0.0f, 0.0f);
           // only lanes that own rows issue loads (empty cells -- half the
           // kooshball grid -- would otherwise spend load bandwidth)
-          if (!PG_WROWS_MASKED || n >= 1) load_row<CW>(vb + (int64_t)r0 * C + cc, C, cc, even, a);
-          if (TWO && (!PG_WROWS_MASKED || n >= 2)) load_row<CW>(vb + (int64_t)r1 * C + cc, C, cc, even, b);
+          if (!PG_WROWS_MASKED || n >= 1) ldrow(r0, cc, a);
+          if (TWO && (!PG_WROWS_MASKED || n >= 2)) ldrow(r1, cc, b);
           float2 acc[CW];
 #pragma unroll
           for (int c = 0; c < CW; ++c) {
@@ -992,7 +1068,7 @@ __global__ __launch_bounds__(kGridThreads) void k_wrows(
           }
 #pragma unroll 1
           for (uint32_t r = rn; r < h.cr1; ++r) {
-            load_row<CW>(vb + (int64_t)r * C + cc, C, cc, even, a);
+            ldrow(r, cc, a);
 #pragma unroll
             for (int c = 0; c < CW; ++c) {
               acc[c].x = acc[c].x + a[c].x;
@@ -1016,7 +1092,7 @@ __global__ __launch_bounds__(kGridThreads) void k_wrows(
 }
 
 // Hot pre-pass 1: partial of every unit of every hot cell, in parallel
-// (grid.x over hot units, grid.y over frames).  MODE 1 (values read from V):
+// (grid.x over hot units, grid.y over frames).  MODE 1 / 3 (values read from V):
 // one group of GS lanes per unit folds its rows straight from memory.
 template <int GS, int MODE>
 __global__ __launch_bounds__(kGridThreads) void k_hot_units(
@@ -1361,8 +1437,8 @@ static int launch_tile(int64_t tile_cells, dim3 grid, hipStream_t st,
   if (tile_cells == kWaveTile) {
     // the V reduction: row-direct above 16 coils (rows of 256 bytes and
     // more), LDS-staged wave tiles below (measured on MI355X)
-    if constexpr (MODE == 1 && GS >= 32)
-      hipLaunchKernelGGL((k_wrows<GS>), grid, block, 0, st, frames, tile_ptr,
+    if constexpr (kVMode<MODE> && GS >= 32)
+      hipLaunchKernelGGL((k_wrows<GS, MODE>), grid, block, 0, st, frames, tile_ptr,
                          F, C, n_tiles);
     else
       hipLaunchKernelGGL((k_wtile<GS, MODE>), grid, block, 0, st, frames,
@@ -1370,7 +1446,10 @@ static int launch_tile(int64_t tile_cells, dim3 grid, hipStream_t st,
     PG_LAUNCHED();
     return 0;
   }
-  if constexpr (GS <= 32) {
+  if constexpr (MODE == 3) {
+    PG_CHECK(false, "grid: block-floating-point V rows are reduced by wave tiles only (tile_cells %d), got %lld",
+             kWaveTile, (long long)tile_cells);
+  } else if constexpr (GS <= 32) {
     PG_TILE_LAUNCH(128) PG_TILE_LAUNCH(64)
   } else {
     PG_TILE_LAUNCH(64) PG_TILE_LAUNCH(32)
@@ -1385,7 +1464,7 @@ static int launch_hot(hipStream_t st, const pg_frame *frames, int F, int C,
                       int64_t max_hot_units, int64_t max_hot) {
   constexpr int NG = kGridThreads / GS;
   const dim3 block(kGridThreads);
-  if constexpr (MODE == 1)
+  if constexpr (kVMode<MODE>)
     hipLaunchKernelGGL((k_hot_units<GS, MODE>),
                        dim3((unsigned)ceil_div(max_hot_units, NG), (unsigned)F),
                        block, 0, st, frames, C);
@@ -1544,6 +1623,18 @@ int pg_grid_reduce_v(const pg_frame *frames, const uint32_t *tile_ptr,
                      int64_t max_hot_units, int64_t max_hot,
                      pg_stream_t stream) {
   return launch_grid<1>(frames, tile_ptr, F, n_tiles, C, tile_cells, n_blocks,
+                        max_hot_units, max_hot, stream);
+}
+
+int pg_grid_reduce_v_bfp(const pg_frame *frames, const uint32_t *tile_ptr,
+                         int64_t F, int64_t n_tiles, int64_t C,
+                         int64_t tile_cells, int64_t n_blocks,
+                         int64_t max_hot_units, int64_t max_hot,
+                         pg_stream_t stream) {
+  PG_CHECK(tile_cells == kWaveTile,
+           "grid_reduce_v_bfp: block-floating-point V rows are reduced by wave tiles only (tile_cells %d), got %lld",
+           kWaveTile, (long long)tile_cells);
+  return launch_grid<3>(frames, tile_ptr, F, n_tiles, C, tile_cells, n_blocks,
                         max_hot_units, max_hot, stream);
 }
 

@@ -278,7 +278,40 @@ __global__ __launch_bounds__(256) void k_apply_grouped(
         }
       }
     }
-    if (vec) {
+    if constexpr (RF == kRowBfp) {
+      // block-floating-point V rows (any C): through LDS, then one lane per
+      // (replica, 4-coil group) encodes 8 values into its 16 bytes;
+      // consecutive lanes write consecutive groups of a row
+      const float s = sw[t];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        float4 o;
+        o.x = (acc[mt][0] * s) * s;
+        o.y = (acc[mt][1] * s) * s;
+        o.z = (acc[mt][2] * s) * s;
+        o.w = (acc[mt][3] * s) * s;
+        *reinterpret_cast<float4 *>(&vt[j * RLP + 16 * mt + 4 * q]) = o;
+      }
+      wave_lds_sync();
+      constexpr int NGR = 2 * MT;  // groups per (padded) row
+      const int64_t vrb = bfp_row_bytes(C);
+#pragma unroll
+      for (int k = 0; k < (16 * NGR + 63) / 64; ++k) {
+        const int idx = 64 * k + lane;
+        const int jr = idx / NGR, gi = idx % NGR;
+        const uint32_t rj = __shfl(rr[t], jr < 16 ? jr : 0);
+        if (idx < 16 * NGR && 8 * gi < twoC && start + 16 * t + jr < stop) {
+          // values past 2C are zero (the table's padded rows)
+          const float4 lo4 = *reinterpret_cast<const float4 *>(&vt[jr * RLP + 8 * gi]);
+          const float4 hi4 = *reinterpret_cast<const float4 *>(&vt[jr * RLP + 8 * gi + 4]);
+          const float x[8] = {lo4.x, lo4.y, lo4.z, lo4.w, hi4.x, hi4.y, hi4.z, hi4.w};
+          const uint4 w = bfp_encode8(x);
+          gst(reinterpret_cast<uint4 *>(static_cast<unsigned char *>(static_cast<void *>(vbuf)) +
+                                        (int64_t)rj * vrb + 16 * gi), w);
+        }
+      }
+      wave_lds_sync();
+    } else if (vec) {
       // even C: through LDS, then whole rows (16 lanes x 16 bytes per
       // replica row piece, consecutive lanes on consecutive bytes)
       const float s = sw[t];
@@ -333,6 +366,7 @@ static int launch_grouped(const void *y_t, const void *tbl,
   if (n_items == 0) return 0;
   PG_CHECK(items != nullptr, "grouped apply: null work list");
   PG_CHECK(rf == kRowC64 || ((uintptr_t)y_t & 15) == 0, "grouped apply: packed rows not 16-byte aligned");
+  PG_CHECK(rf != kRowBfp || ((uintptr_t)vbuf & 15) == 0, "grouped apply: packed V rows not 16-byte aligned");
   const dim3 grid((unsigned)ceil_div(n_items, 4));
 #define PG_APPLY_LAUNCH(NT, S)                                                   \
   hipLaunchKernelGGL((k_apply_grouped<MT, KS, NT, S>), grid, dim3(256), 0, st,   \

@@ -52,7 +52,8 @@ import torch
 from . import _lib
 from .calib._grappa import build_table
 from .calib._grog import (GRID_MODES, PLAN_RADIUS, GridProgram, GrogInterpolator, _default_oversamp,
-                          _kernel_offsets, apply_entry, mode_terms, row_format, sample_row_bytes, transpose_entry)
+                          _kernel_offsets, apply_entry, mode_default_tile, mode_terms, mode_tile_options,
+                          reduce_entry, row_format, sample_row_bytes, transpose_entry, v_format)
 
 
 @dataclasses.dataclass(eq=False)
@@ -377,7 +378,7 @@ class GrogServer:
         out = self._alloc(comp.C * comp.G, torch.complex64).reshape(comp.C, comp.G)
         vbuf = None
         if self.mode != "fp32":
-            vbuf = self._alloc(g._rmax() * comp.C, torch.complex64)
+            vbuf = self._alloc(g._rmax() * sample_row_bytes(comp.C, v_format(self.mode)), torch.uint8)
         _lib.call(transpose_entry(fmt, False), _lib.ptr(y), comp.C, comp.npts, _lib.ptr(yt), st.cuda_stream)
         if self.mode != "fp32":
             n_terms = mode_terms(self.mode)
@@ -412,8 +413,9 @@ class GrogServer:
                     bad[job] = f"launch failed: {type(exc).__name__}: {exc}"
                     job.result = None
             for (C, G), frames in groups.items():
-                prog = GridProgram(frames, C, G, self.device, _lib.load().pg_grid_tile_cells(C))
-                prog.launch("pg_grid_fp32" if self.mode == "fp32" else "pg_grid_reduce_v")
+                prog = GridProgram(frames, C, G, self.device, mode_default_tile(self.mode, C),
+                                   tile_opts=mode_tile_options(self.mode, C))
+                prog.launch(reduce_entry(self.mode))
                 keep.append(prog)
             ev = torch.cuda.Event()
             ev.record(st)

@@ -115,6 +115,14 @@ def test_library_version_and_errors():
     assert lib.pg_transpose_frames_bfp(4096, 4096, 1, 100, 32, 4104, None) != 0
     assert lib.pg_apply_grouped_bf16_bfp(4104, 4096, 4096, 4096, 4096, 4096, 4096, 5, 10, 32, 3, 4096, None) != 0
     assert b"16-byte" in lib.pg_last_error()
+    # block-floating-point V rows are reduced by the wave tiles only
+    assert lib.pg_grid_reduce_v_bfp(4096, 4096, 1, 1, 32, 64, 0, 0, 0, None) != 0
+    assert b"wave tiles" in lib.pg_last_error()
+    from pygrog_amd.calib._grog import mode_tile_options, reduce_entry, v_format
+
+    assert v_format("bf16x3c") == "bfp" and v_format("bf16x3") == "c64"
+    assert reduce_entry("bf16x3c") == "pg_grid_reduce_v_bfp" and reduce_entry("bf16") == "pg_grid_reduce_v"
+    assert mode_tile_options("bf16x3c", 32) == [256] and len(mode_tile_options("bf16x3", 32)) == 3
 
 
 def test_kernel_offsets_and_quant_steps():

@@ -48,7 +48,7 @@ def test_radial_256_8coil_full_size(device):
     assert ogrid.rel_l2(got, ref) <= 1e-5
     assert ogrid.rel_l2(got, g64) <= 1e-5
     assert ogrid.rel_l2(g.grid_device(yy, "bf16x3").reshape(C, -1).cpu().numpy(), g64) <= 1e-5
-    assert ogrid.rel_l2(g.grid_device(yy, "bf16x3c").reshape(C, -1).cpu().numpy(), g64) <= 1e-4
+    assert ogrid.rel_l2(g.grid_device(yy, "bf16x3c").reshape(C, -1).cpu().numpy(), g64) <= 2e-4
     assert ogrid.rel_l2(g.grid_device(yy, "bf16").reshape(C, -1).cpu().numpy(), g64) <= 5e-3
 
 
@@ -75,7 +75,7 @@ def test_coil_counts_every_kernel_shape(device, C):
     g64 = ogrid.grid_fp64(y, p, e, g.plan._compiled.table64.cpu().numpy())
     assert ogrid.rel_l2(g.grid_device(yy, "fp32").reshape(C, -1).cpu().numpy(), ref) <= 1e-5
     assert ogrid.rel_l2(g.grid_device(yy, "bf16x3").reshape(C, -1).cpu().numpy(), g64) <= 1e-5
-    assert ogrid.rel_l2(g.grid_device(yy, "bf16x3c").reshape(C, -1).cpu().numpy(), g64) <= 1e-4
+    assert ogrid.rel_l2(g.grid_device(yy, "bf16x3c").reshape(C, -1).cpu().numpy(), g64) <= 2e-4
     assert ogrid.rel_l2(g.grid_device(yy, "bf16").reshape(C, -1).cpu().numpy(), g64) <= 5e-3
 
 

@@ -83,7 +83,7 @@ def test_kooshball_192_full_size_vs_oracle(device, kooshball_plan, C):
     yy = y.reshape(1, 1, C, -1)
     assert uses_s24("bf16x3", C) == (C == 32)
     for mode, against, tol in (("fp32", ref, 1e-5), ("fp32", g64, 1e-5), ("bf16x3", g64, 1e-5),
-                               ("bf16x3c", g64, 1e-4), ("bf16", g64, 5e-3)):
+                               ("bf16x3c", g64, 2e-4), ("bf16", g64, 5e-3)):
         got = g.grid_device(yy, mode).reshape(C, -1)[:, torch.as_tensor(cells, device=device)].cpu().numpy()
         rel = ogrid.rel_l2(got, against)
         print(f"  {mode}: rel-L2 {rel:.2e} (bar {tol:g})", flush=True)

@@ -44,7 +44,7 @@ def _alone(job, mode, device):
     return g.grid_device(job.kspace.reshape(1, 1, job.coils, -1).contiguous(), mode)
 
 
-@pytest.mark.parametrize("mode", ["fp32", "bf16x3", "bf16"])
+@pytest.mark.parametrize("mode", ["fp32", "bf16x3", "bf16", "bf16x3c"])
 def test_serve_mix_equals_alone(device, mode):
     from pygrog_amd.serve import GrogServer
 
