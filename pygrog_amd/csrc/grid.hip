@@ -975,11 +975,16 @@ __device__ __forceinline__ void load_row_bfp(const unsigned char *vr, int C, int
 #ifndef PG_WROWS_NTST
 #define PG_WROWS_NTST 1
 #endif
+// block-floating-point V rows: a whole 32-coil row (8 groups) per pass
+#ifndef PG_WROWS_CWB
+#define PG_WROWS_CWB 32
+#endif
 template <int GS, int MODE>
 __global__ __launch_bounds__(kGridThreads) void k_wrows(
     const pg_frame *__restrict__ frames, const uint32_t *__restrict__ tile_ptr,
     int F, int C, uint32_t n_tiles) {
-  constexpr int CW = GS < PG_WROWS_CW ? GS : PG_WROWS_CW;  // coils per pass
+  constexpr int CWM = MODE == 3 ? PG_WROWS_CWB : PG_WROWS_CW;
+  constexpr int CW = GS < CWM ? GS : CWM;  // coils per pass
   constexpr bool TWO = PG_WROWS_TWO != 0;  // second row loaded with the first
   const int w = threadIdx.x / kWave;
   const int l = threadIdx.x % kWave;
