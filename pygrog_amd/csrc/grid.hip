@@ -975,6 +975,10 @@ __device__ __forceinline__ void load_row_bfp(const unsigned char *vr, int C, int
 #ifndef PG_WROWS_NTST
 #define PG_WROWS_NTST 1
 #endif
+// smallest coil group whose block-floating-point V rows are reduced row-direct
+#ifndef PG_WROWS_MIN_GS_BFP
+#define PG_WROWS_MIN_GS_BFP 32
+#endif
 // block-floating-point V rows: a whole 32-coil row (8 groups) per pass
 #ifndef PG_WROWS_CWB
 #define PG_WROWS_CWB 32
@@ -1442,7 +1446,7 @@ static int launch_tile(int64_t tile_cells, dim3 grid, hipStream_t st,
   if (tile_cells == kWaveTile) {
     // the V reduction: row-direct above 16 coils (rows of 256 bytes and
     // more), LDS-staged wave tiles below (measured on MI355X)
-    if constexpr (kVMode<MODE> && GS >= 32)
+    if constexpr (kVMode<MODE> && GS >= (MODE == 3 ? PG_WROWS_MIN_GS_BFP : 32))
       hipLaunchKernelGGL((k_wrows<GS, MODE>), grid, block, 0, st, frames, tile_ptr,
                          F, C, n_tiles);
     else
