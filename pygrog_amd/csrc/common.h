@@ -174,32 +174,28 @@ constexpr int kRowBfp = 2;  // block floating point (below), 16 * ceil(C / 4) by
 __host__ __device__ constexpr int64_t bfp_row_bytes(int64_t C) { return (C + 3) / 4 * 16; }
 constexpr int kBfpBias = 140;
 __device__ __forceinline__ uint4 bfp_encode8(const float (&x)[8]) {
-  float M = 0.0f;
-  bool finite = true;
+  // M's bit pattern: the largest |x| as an unsigned integer (|x| orders as
+  // its pattern does, and an infinity or a NaN is above every finite value)
+  uint32_t mb = 0u;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) mb = max(mb, __float_as_uint(x[i]) & 0x7fffffffu);
+  // s = floor(log2 M) - 13 from the exponent field (zero and subnormal M:
+  // field 0, s = -140), plus 1 when M * 2^-s rounds past 16383: the
+  // fraction bits reach 16383.5 / 8192 (normal) or 16383.5 / 16384
+  // (subnormal)
+  const uint32_t ef = mb >> 23;
+  const int s = (int)ef - kBfpBias + ((mb & 0x7fffffu) >= (ef != 0u ? 0x7ffe00u : 0x7fff00u) ? 1 : 0);
+  const uint32_t E = (uint32_t)(s + kBfpBias);
+  uint32_t f[8];
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    M = fmaxf(M, fabsf(x[i]));
-    finite = finite && __builtin_isfinite(x[i]);
+    const int m = (int)rintf(__builtin_amdgcn_ldexpf(x[i], -s));
+    f[i] = ((uint32_t)m << 1) | ((E >> i) & 1u);  // bits 16+ dropped when packed
   }
-  int s = -kBfpBias;
-  if (M > 0.0f) {
-    s = __builtin_amdgcn_frexp_expf(M) - 1 - 13;  // M in [2^(s+13), 2^(s+14))
-    if (s < -kBfpBias) s = -kBfpBias;
-    if (__builtin_amdgcn_ldexpf(M, -s) >= 16383.5f) s += 1;
-  }
-  uint32_t f[8];
-  if (finite) {
-    const uint32_t E = (uint32_t)(s + kBfpBias);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int m = (int)rintf(__builtin_amdgcn_ldexpf(x[i], -s));
-      f[i] = (((uint32_t)m & 0x7fffu) << 1) | ((E >> i) & 1u);
-    }
-  } else {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) f[i] = 0x8000u | 1u;  // E = 255, m = -16384
-  }
-  return make_uint4(f[0] | (f[1] << 16), f[2] | (f[3] << 16), f[4] | (f[5] << 16), f[6] | (f[7] << 16));
+  uint4 w = make_uint4((f[0] & 0xffffu) | (f[1] << 16), (f[2] & 0xffffu) | (f[3] << 16),
+                       (f[4] & 0xffffu) | (f[5] << 16), (f[6] & 0xffffu) | (f[7] << 16));
+  if (mb >= 0x7f800000u) w = make_uint4(0x80018001u, 0x80018001u, 0x80018001u, 0x80018001u);  // E = 255, m = -16384
+  return w;
 }
 __device__ __forceinline__ int bfp_scale(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
   const uint32_t t0 = d0 & 0x10001u, t1 = d1 & 0x10001u, t2 = d2 & 0x10001u, t3 = d3 & 0x10001u;
