@@ -307,6 +307,18 @@ int pg_plan_compile(const float *coords, int64_t S, int64_t npts,
                     int64_t min_units, void *out_ws, int64_t out_ws_bytes,
                     void *tmp_ws, int64_t tmp_ws_bytes, float *distances,
                     int64_t *summary, pg_stream_t stream);
+/* pg_plan_compile without the synchronisation: the summary is copied into
+ * summary_pinned (page-locked host memory, e.g. from hipHostMalloc) by a copy
+ * enqueued on `stream`, and is valid once the stream has reached that point
+ * (the caller records an event or synchronises; tmp_ws must stay allocated
+ * until then).  For callers that overlap host work with the plan kernels
+ * (the serving loop fits the job's operator table meanwhile).  No reference
+ * counterpart beyond the steps it runs. */
+int pg_plan_compile_async(const float *coords, int64_t S, int64_t npts,
+                          pg_plan_geom geom, int64_t G, int32_t chunk,
+                          int64_t min_units, void *out_ws, int64_t out_ws_bytes,
+                          void *tmp_ws, int64_t tmp_ws_bytes, float *distances,
+                          int64_t *summary_pinned, pg_stream_t stream);
 
 /* Replicas grouped by table entry for pg_apply_grouped_bf16* (no reference
  * counterpart: the grouped path's preparation).  For the R replicas with

@@ -134,6 +134,8 @@ _SIGS = {
     "pg_plan_compile_layout": [c_int64, c_int64, c_int32, c_int32, c_int64, c_void_p, c_void_p, c_void_p],
     "pg_plan_compile": [c_void_p, c_int64, c_int64, PlanGeom, c_int64, c_int32, c_int64, c_void_p, c_int64, c_void_p,
                         c_int64, c_void_p, c_void_p, c_void_p],
+    "pg_plan_compile_async": [c_void_p, c_int64, c_int64, PlanGeom, c_int64, c_int32, c_int64, c_void_p, c_int64,
+                              c_void_p, c_int64, c_void_p, c_void_p, c_void_p],
     "pg_vdot_workspace_bytes": [],
     "pg_sense_expand_c64": [c_void_p, c_void_p, c_int64, c_int64, ctypes.c_int32, c_void_p, c_void_p,
                             ctypes.c_int32, c_void_p, c_void_p],

@@ -582,6 +582,25 @@ def _geom(shape, grid_shape, offsets) -> _lib.PlanGeom:
 def _create_plan(shape, coords, oversamp, kernel_width, kernel_shape, time_map, stack_shape=(), device=None):
     """Build the plan of every stack element in one set of kernel launches
     (reference _create_plan + _create_plan_single + _attach_fft_plan)."""
+    return _plan_finish(_plan_launch(shape, coords, oversamp, kernel_width, kernel_shape, time_map, stack_shape,
+                                     device))
+
+
+class _PendingPlan(SimpleNamespace):
+    """A plan whose kernels are enqueued but whose host summary may not have
+    arrived yet (``_plan_launch(..., defer=True)``)."""
+
+    def wait(self) -> None:
+        if self.event is not None:
+            self.event.synchronize()
+            self.event = None
+
+
+def _plan_launch(shape, coords, oversamp, kernel_width, kernel_shape, time_map, stack_shape=(), device=None,
+                 defer: bool = False) -> _PendingPlan:
+    """Enqueue the plan compiler on the current stream.  ``defer``: do not
+    wait for it -- the summary goes to page-locked memory behind an event,
+    and ``_plan_finish`` waits (the caller does host work meanwhile)."""
     dev = _lib.hip_device(device)
     st = _lib.stream(dev)
     home = coords.device if isinstance(coords, torch.Tensor) else torch.device("cpu")
@@ -615,10 +634,34 @@ def _create_plan(shape, coords, oversamp, kernel_width, kernel_shape, time_map, 
     ws = torch.empty(int(ob.value), dtype=torch.uint8, device=dev)
     tmp = torch.empty(int(tb.value), dtype=torch.uint8, device=dev)
     distances = torch.empty(S, nper, ndim, dtype=torch.float32, device=dev)
-    summary = np.zeros(3 * S + 3, dtype=np.int64)
-    _lib.call("pg_plan_compile", _lib.ptr(cd), S, npts, geom, G, UNIT_CHUNK, int(HOT_MIN_UNITS), _lib.ptr(ws),
-              int(ob.value), _lib.ptr(tmp), int(tb.value), _lib.ptr(distances), summary.ctypes.data, st)
-    del tmp  # the call synchronised its stream
+    event = None
+    if defer:
+        summary_t = torch.empty(3 * S + 3, dtype=torch.int64, pin_memory=True)
+        _lib.call("pg_plan_compile_async", _lib.ptr(cd), S, npts, geom, G, UNIT_CHUNK, int(HOT_MIN_UNITS),
+                  _lib.ptr(ws), int(ob.value), _lib.ptr(tmp), int(tb.value), _lib.ptr(distances),
+                  summary_t.data_ptr(), st)
+        event = torch.cuda.Event()
+        event.record(torch.cuda.current_stream(dev))
+        summary = summary_t.numpy()
+    else:
+        summary_t = None
+        summary = np.zeros(3 * S + 3, dtype=np.int64)
+        _lib.call("pg_plan_compile", _lib.ptr(cd), S, npts, geom, G, UNIT_CHUNK, int(HOT_MIN_UNITS), _lib.ptr(ws),
+                  int(ob.value), _lib.ptr(tmp), int(tb.value), _lib.ptr(distances), summary.ctypes.data, st)
+    return _PendingPlan(dev=dev, S=S, npts=npts, ndim=ndim, kw=kw, nper=nper, G=G, s_shape=s_shape, spatial=spatial,
+                        shape=shape, oversamp=oversamp, grid_shape=grid_shape, kernel_width=kernel_width,
+                        kernel_shape=kernel_shape, time_map=time_map, home=home, offs=offs, ws=ws, tmp=tmp,
+                        distances=distances, summary=summary, summary_t=summary_t, event=event, coords_dev=cd)
+
+
+def _plan_finish(p: _PendingPlan):
+    """The plan object of a launched plan compiler (waits for its summary)."""
+    p.wait()
+    p.tmp = None
+    dev, S, npts, ndim, kw, nper, G = p.dev, p.S, p.npts, p.ndim, p.kw, p.nper, p.G
+    s_shape, spatial, offs, ws, distances, summary = p.s_shape, p.spatial, p.offs, p.ws, p.distances, p.summary
+    shape, oversamp, grid_shape, time_map = p.shape, p.oversamp, p.grid_shape, p.time_map
+    kernel_width, kernel_shape, home = p.kernel_width, p.kernel_shape, p.home
     n_inv = [int(v) for v in summary[:S]]
     U = int(summary[S])
     hb = [int(v) for v in summary[S + 1:2 * S + 2]]
@@ -722,12 +765,29 @@ class GrogInterpolator:
 
     def __init__(self, shape, coords, oversamp=None, kernel_width: int = 2, kernel_shape: str = "circle",
                  time_map=None, image_shape=None, stack_shape: tuple = (), *, device=None):
+        self._init_plan(_plan_finish(self._launch_plan(shape, coords, oversamp, kernel_width, kernel_shape, time_map,
+                                                       stack_shape, device)), image_shape)
+
+    @staticmethod
+    def _launch_plan(shape, coords, oversamp=None, kernel_width: int = 2, kernel_shape: str = "circle",
+                     time_map=None, stack_shape: tuple = (), device=None, defer: bool = False) -> _PendingPlan:
         coords_arr = coords if isinstance(coords, torch.Tensor) else np.asarray(coords)
         stack_eff = tuple(int(s) for s in (stack_shape or ()))
         if not stack_eff and coords_arr.ndim == 4:
             stack_eff = (int(coords_arr.shape[0]),)
-        self.plan = _create_plan(shape, coords_arr, oversamp, kernel_width, kernel_shape, time_map,
-                                 stack_shape=stack_eff, device=device)
+        return _plan_launch(shape, coords_arr, oversamp, kernel_width, kernel_shape, time_map,
+                            stack_shape=stack_eff, device=device, defer=defer)
+
+    @classmethod
+    def _from_pending(cls, pending: _PendingPlan, image_shape=None) -> "GrogInterpolator":
+        """The interpolator of a plan launched with ``_launch_plan(...,
+        defer=True)`` (waits for the plan compiler's summary)."""
+        obj = cls.__new__(cls)
+        obj._init_plan(_plan_finish(pending), image_shape)
+        return obj
+
+    def _init_plan(self, plan, image_shape) -> None:
+        self.plan = plan
         self.device = self.plan._compiled.device
         _attach_fft_plan(self.plan, image_shape)
         self._programs = {}

@@ -209,11 +209,11 @@ int pg_plan_compile_layout(int64_t S, int64_t npts, int32_t ndim, int32_t kw,
   return 0;
 }
 
-int pg_plan_compile(const float *coords, int64_t S, int64_t npts,
-                    pg_plan_geom geom, int64_t G, int32_t chunk,
-                    int64_t min_units, void *out_ws, int64_t out_ws_bytes,
-                    void *tmp_ws, int64_t tmp_ws_bytes, float *distances,
-                    int64_t *summary, pg_stream_t stream) {
+static int plan_compile(const float *coords, int64_t S, int64_t npts,
+                        pg_plan_geom geom, int64_t G, int32_t chunk,
+                        int64_t min_units, void *out_ws, int64_t out_ws_bytes,
+                        void *tmp_ws, int64_t tmp_ws_bytes, float *distances,
+                        int64_t *summary, bool sync, pg_stream_t stream) {
   PG_CHECK(chunk >= 1, "plan_compile: unit chunk must be positive");
   PG_CHECK(out_ws && tmp_ws && summary, "plan_compile: null workspace");
   int64_t offs[kNumFields], ob = 0, tb = 0;
@@ -319,8 +319,26 @@ int pg_plan_compile(const float *coords, int64_t S, int64_t npts,
   hipLaunchKernelGGL(k_plan_summary, dim3(1), dim3(256), 0, st, n_invalid, cell_unit, HF, HN, S, G, summary_dev);
   PG_LAUNCHED();
   PG_HIP(hipMemcpyAsync(summary, summary_dev, 8 * (3 * S + 3), hipMemcpyDeviceToHost, st));
-  PG_HIP(hipStreamSynchronize(st));
+  if (sync) PG_HIP(hipStreamSynchronize(st));
   return 0;
+}
+
+int pg_plan_compile(const float *coords, int64_t S, int64_t npts,
+                    pg_plan_geom geom, int64_t G, int32_t chunk,
+                    int64_t min_units, void *out_ws, int64_t out_ws_bytes,
+                    void *tmp_ws, int64_t tmp_ws_bytes, float *distances,
+                    int64_t *summary, pg_stream_t stream) {
+  return plan_compile(coords, S, npts, geom, G, chunk, min_units, out_ws, out_ws_bytes, tmp_ws, tmp_ws_bytes,
+                      distances, summary, true, stream);
+}
+
+int pg_plan_compile_async(const float *coords, int64_t S, int64_t npts,
+                          pg_plan_geom geom, int64_t G, int32_t chunk,
+                          int64_t min_units, void *out_ws, int64_t out_ws_bytes,
+                          void *tmp_ws, int64_t tmp_ws_bytes, float *distances,
+                          int64_t *summary_pinned, pg_stream_t stream) {
+  return plan_compile(coords, S, npts, geom, G, chunk, min_units, out_ws, out_ws_bytes, tmp_ws, tmp_ws_bytes,
+                      distances, summary_pinned, false, stream);
 }
 
 // bits of the entries 0 .. K-1 (at least one)

@@ -12,8 +12,9 @@ pass to ``GrogInterpolator(shape, coords, ...)``,
     up-front estimate of every buffer the job will hold) and a preparation
     slot are free;
   * prepares each admitted job -- GPU plan compiler and FP64 table fit -- on
-    its slot's own stream, from a worker thread, so preparations of
-    different jobs overlap each other and the gridding on the device;
+    its slot's own streams, from a worker thread (the table is fitted while
+    the plan kernels run), so preparations of different jobs overlap each
+    other and the gridding on the device;
   * every time the gridding stream has room, gathers all prepared jobs
     (continuous batching: jobs that became ready while earlier batches were
     on the GPU join the next one) and grids them together: one transpose per
@@ -282,11 +283,11 @@ class GrogServer:
     """Serve ``GridJob`` requests on one HIP device (see the module notes).
 
     ``hbm_budget`` bytes bound the admitted jobs' estimated footprint,
-    ``slots`` the jobs in preparation at once (one stream and one worker
-    thread each, plus a helper thread and stream for the table fit; the
-    preparations' host work shares the interpreter lock, and on MI355X two
-    slots gave both higher saturation throughput and a lower tail than
-    four),
+    ``slots`` the jobs in preparation at once (one worker thread and two
+    streams each: the plan compiler is enqueued without waiting for it and
+    the same thread fits the table meanwhile, so a preparation's host work
+    overlaps its plan kernels without a second thread contending for the
+    interpreter lock),
     ``max_inflight`` the gridding batches on the device.  The worker threads
     live as long as the server (``close()`` or a ``with`` block ends them), so
     successive ``run`` calls reuse their per-thread library state."""
@@ -307,13 +308,11 @@ class GrogServer:
         self._streams = [torch.cuda.Stream(device=self.device) for _ in range(self.slots)]
         self._fit_streams = [torch.cuda.Stream(device=self.device) for _ in range(self.slots)]
         self._pool = cf.ThreadPoolExecutor(max_workers=self.slots)
-        self._fit_pool = cf.ThreadPoolExecutor(max_workers=self.slots)
         self._grid_stream = torch.cuda.Stream(device=self.device)
 
     def close(self) -> None:
         """End the worker threads (after the work they hold)."""
         self._pool.shutdown(wait=True)
-        self._fit_pool.shutdown(wait=True)
 
     def __enter__(self) -> "GrogServer":
         return self
@@ -334,15 +333,17 @@ class GrogServer:
 
     def _prepare(self, job: GridJob, slot: int) -> GridJob:
         st = self._streams[slot]
-        # the table fit runs on a helper thread (and its own stream) while this
-        # thread builds the plan; the two meet when the shifts are quantised
-        fit = self._fit_pool.submit(self._fit, job, slot)
+        # the plan compiler is enqueued without waiting for it; this thread
+        # fits the table (host LAPACK work, Gram products on the second
+        # stream) while the plan kernels run, then picks up the plan
         with torch.cuda.device(self.device), torch.cuda.stream(st):
+            pending = GrogInterpolator._launch_plan(job.shape, job.coords, job.oversamp, job.kernel_width,
+                                                    job.kernel_shape, device=self.device, defer=True)
             try:
-                g = GrogInterpolator(job.shape, job.coords, job.oversamp, job.kernel_width, job.kernel_shape,
-                                     device=self.device)
+                res, fev = self._fit(job, slot)
             finally:
-                res, fev = fit.result()
+                pending.wait()  # the plan's buffers and summary stay put until its kernels are done
+            g = GrogInterpolator._from_pending(pending)
             st.wait_event(fev)
             res.table.record_stream(st)
             g._attach_table(res, job.precision, job.quant)
