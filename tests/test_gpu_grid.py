@@ -220,3 +220,29 @@ def test_plan_weights_big_cells_bit_exact(device):
     dens_gpu = g.plan._compiled.density.reshape(-1).cpu().numpy()
     p_gpu = dict(p, sqrt_weights=g.plan.sqrt_weights.cpu().numpy())
     assert np.array_equal(dens_gpu, oplan.density(p_gpu)), "density summation order differs"
+
+
+def test_deferred_plan_equals_blocking_plan(device):
+    """A plan compiled without waiting (pg_plan_compile_async: summary
+    through page-locked memory behind an event; the serving loop fits the
+    table meanwhile) holds every field the blocking call produces, bit for
+    bit, on a 2D radial and a 3D kooshball trajectory."""
+    from pygrog_amd import synth
+
+    cases = [((96, 96), _radial(48, 160, seed=3), None, 3, "circle"),
+             ((32, 32, 32), synth.kooshball_3d(300, 48), (1, 1, 1), 2, "sphere")]
+    for shape, coords, over, kw, ks in cases:
+        a = GrogInterpolator(shape, coords, over, kw, ks, device=device)
+        pending = GrogInterpolator._launch_plan(shape, coords, over, kw, ks, device=device, defer=True)
+        b = GrogInterpolator._from_pending(pending)
+        ca, cb = a.plan._compiled, b.plan._compiled
+        assert (ca.n_inv, ca.n_valid, ca.G, ca.npts) == (cb.n_inv, cb.n_valid, cb.G, cb.npts)
+        for name in ("target_idx", "weights", "indices", "sqrt_weights", "sort_perm", "inv_perm"):
+            assert torch.equal(getattr(a.plan, name), getattr(b.plan, name)), name
+        for name in ("cell_ptr", "cell_unit", "unit_cell", "unit_rep", "unit_hot", "cell_vrow", "density",
+                     "n_invalid", "distances"):
+            assert torch.equal(getattr(ca, name), getattr(cb, name)), name
+        R = ca.n_valid[0]  # the record arrays are sized by the candidates; the valid replicas come first
+        for name in ("rep_src", "rep_s"):
+            assert torch.equal(getattr(ca, name)[:, :R], getattr(cb, name)[:, :R]), name
+        assert [(h["n_hot"], h["n_hot_units"]) for h in ca.hot] == [(h["n_hot"], h["n_hot_units"]) for h in cb.hot]
