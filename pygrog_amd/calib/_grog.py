@@ -464,6 +464,15 @@ def new_sample_rows(n: int, C: int, fmt: str, device) -> torch.Tensor:
     return torch.empty(max(n, 1) * per, dtype=dt, device=device)
 
 
+def _to_device_async(t: torch.Tensor, device) -> torch.Tensor:
+    """Copy a small host tensor to ``device`` on the current stream without
+    waiting (through page-locked memory, which torch keeps until the copy
+    has run)."""
+    if not torch.cuda.is_available():
+        return t.to(device)
+    return t.pin_memory().to(device, non_blocking=True)
+
+
 class GridProgram:
     """A ready-to-launch fused gridding job: device frame table + tile map.
 
@@ -491,8 +500,12 @@ class GridProgram:
                 self.max_hot_units = max(self.max_hot_units, int(f.n_hot_units))
                 self.max_hot = max(self.max_hot, int(f.n_hot))
         arr = (_lib.Frame * self.F)(*frames)
+        # host tables go up from page-locked memory without a blocking copy:
+        # a pageable copy would wait for everything already on the stream
+        # (the serving loop builds one program per batch while the previous
+        # batch runs)
         raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
-        self.frames = raw.to(device)
+        self.frames = _to_device_async(raw, device)
         self.G = int(G)
         self.set_shape(tile_cells, n_blocks)
 
@@ -508,12 +521,28 @@ class GridProgram:
         if getattr(self, "tile_ptr", None) is not None:
             # a graph captured earlier may still name the old map: keep it alive
             self._old_maps = getattr(self, "_old_maps", []) + [self.tile_ptr]
-        self.tile_ptr = (torch.arange(self.F + 1, dtype=torch.int64) * tiles_per).to(torch.int32).to(self.device)
+        self.tile_ptr = _to_device_async((torch.arange(self.F + 1, dtype=torch.int64) * tiles_per).to(torch.int32),
+                                         self.device)
+        self._mark_uploaded()
         if n_blocks is None:
             n_blocks = min(self.n_tiles, 256 * 12)
         self.n_blocks = max(1, int(n_blocks))
 
+    def _mark_uploaded(self) -> None:
+        """Record where the tables' uploads were enqueued; the next launch
+        from another stream waits for them."""
+        self._upload = None
+        if torch.cuda.is_available():
+            self._upload = (torch.cuda.current_stream(self.device), torch.cuda.Event())
+            self._upload[1].record(self._upload[0])
+
     def launch(self, entry: str = "pg_grid_fp32", timer: "KernelTimer | None" = None) -> None:
+        if self._upload is not None:
+            up_stream, up_event = self._upload
+            cur = torch.cuda.current_stream(self.device)
+            if cur != up_stream:
+                cur.wait_event(up_event)
+            self._upload = None
         e0 = timer.start() if timer is not None else None
         _lib.call(entry, _lib.ptr(self.frames), _lib.ptr(self.tile_ptr), self.F, self.n_tiles, self.C,
                   self.tile_cells, self.n_blocks, self.max_hot_units, self.max_hot, _lib.stream(self.device))
