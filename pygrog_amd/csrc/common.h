@@ -198,10 +198,14 @@ __device__ __forceinline__ uint4 bfp_encode8(const float (&x)[8]) {
   return w;
 }
 __device__ __forceinline__ int bfp_scale(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
-  const uint32_t t0 = d0 & 0x10001u, t1 = d1 & 0x10001u, t2 = d2 & 0x10001u, t3 = d3 & 0x10001u;
-  const uint32_t E = ((t0 | (t0 >> 15)) & 3u) | (((t1 | (t1 >> 15)) & 3u) << 2) |
-                     (((t2 | (t2 >> 15)) & 3u) << 4) | (((t3 | (t3 >> 15)) & 3u) << 6);
-  return (int)E - kBfpBias;
+  // the fields' low bytes (bytes 0 and 2 of each dword), four to a dword,
+  // their bit 0 kept: byte k of lo holds bit k of E, byte k of hi bit 4 + k;
+  // E - 140 = sum_k lo_k 2^k + sum_k hi_k 2^(4 + k) - 140 as two byte dot
+  // products, the second accumulating onto the first, the first onto -140
+  const uint32_t lo = __builtin_amdgcn_perm(d1, d0, 0x06040200u) & 0x01010101u;
+  const uint32_t hi = __builtin_amdgcn_perm(d3, d2, 0x06040200u) & 0x01010101u;
+  const uint32_t e = __builtin_amdgcn_udot4(lo, 0x08040201u, (uint32_t)(-kBfpBias), false);
+  return (int)__builtin_amdgcn_udot4(hi, 0x80402010u, e, false);
 }
 // (re, im) of coil k (0..3) of a group
 __device__ __forceinline__ float2 bfp_decode_coil(uint4 g, int k) {
