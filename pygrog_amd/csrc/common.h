@@ -188,12 +188,15 @@ __device__ __forceinline__ uint4 bfp_encode8(const float (&x)[8]) {
   const uint32_t E = (uint32_t)(s + kBfpBias);
   uint32_t f[8];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int m = (int)rintf(__builtin_amdgcn_ldexpf(x[i], -s));
-    f[i] = ((uint32_t)m << 1) | ((E >> i) & 1u);  // bits 16+ dropped when packed
-  }
-  uint4 w = make_uint4((f[0] & 0xffffu) | (f[1] << 16), (f[2] & 0xffffu) | (f[3] << 16),
-                       (f[4] & 0xffffu) | (f[5] << 16), (f[6] & 0xffffu) | (f[7] << 16));
+  for (int i = 0; i < 8; ++i) f[i] = (uint32_t)(int)rintf(__builtin_amdgcn_ldexpf(x[i], -s)) << 1;
+  // dword k: the low halves of f[2k] and f[2k+1] (one byte permute), then
+  // E bit 2k at bit 0 and E bit 2k + 1 at bit 16 (from E | E << 15)
+  const uint32_t eb = E | (E << 15);
+  uint32_t d[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    d[k] = __builtin_amdgcn_perm(f[2 * k + 1], f[2 * k], 0x05040100u) | ((eb >> (2 * k)) & 0x10001u);
+  uint4 w = make_uint4(d[0], d[1], d[2], d[3]);
   if (mb >= 0x7f800000u) w = make_uint4(0x80018001u, 0x80018001u, 0x80018001u, 0x80018001u);  // E = 255, m = -16384
   return w;
 }
