@@ -18,6 +18,11 @@
 // MFMA 16x16x32 bf16 lane layout (64 lanes, q = lane / 16, j = lane % 16):
 //   A operand: row j, k = 8q .. 8q+7;  B operand: column j, k = 8q .. 8q+7;
 //   D: column j, rows 4q .. 4q+3.
+// A lane's B values of one K-step are 8 consecutive values of one sample
+// row, i.e. exactly one 16-byte group of the block-floating-point rows
+// (RF = kRowBfp), so every lane decodes its own group.  Sample rows come as
+// complex64 (kRowC64), 24-bit (kRowS24) or bfp15 (kRowBfp) rows; V rows are
+// written as complex64, or as bfp15 for kRowBfp.
 #include "common.h"
 
 namespace pg {
