@@ -16,7 +16,8 @@ grids with per-coil Python loops over C++ scatter calls, here:
   * ``interpolate(grid=True)`` / ``ret_image=True`` run one fused kernel
     (operator apply + deterministic cell-ordered reduction) for all batch,
     stack and coil entries (``pygrog_amd/csrc/grid.hip``), or the BF16-MFMA
-    grouped-GEMM path (``grid_mfma.hip``) with ``mode="bf16"``/``"bf16x3"``;
+    grouped-GEMM path (``grid_mfma.hip``) with ``mode="bf16"``/``"bf16x3"``/
+    ``"bf16x3c"`` (the last over block-floating-point sample and V rows);
   * ``interpolate()`` (sparse output) runs one fused apply kernel.
 
 Public tensor fields of the plan (``target_idx``, ``indices``, ...) live on
