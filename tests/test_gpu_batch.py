@@ -56,7 +56,7 @@ def test_batch_fp32_vs_oracle(device):
         np.testing.assert_allclose(b.density(f).reshape(-1).cpu().numpy(), dens, rtol=1e-6, atol=1e-7)
 
 
-@pytest.mark.parametrize("mode,coils", [("fp32", C), ("bf16x3", C), ("bf16x3", 20)])
+@pytest.mark.parametrize("mode,coils", [("fp32", C), ("bf16x3", C), ("bf16x3", 20), ("bf16x3c", C)])
 def test_batch_graph_replays_match_eager(device, mode, coils):
     b, _, ys = _batch(device, seed=5, coils=coils)
     y = b.pack(ys)

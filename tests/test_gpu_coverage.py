@@ -359,8 +359,10 @@ def test_grouped_s24_same_bits_on_16bit_inputs(device, monkeypatch, C):
 @pytest.mark.parametrize("C", [8, 20])
 def test_grid_device_batch_of_stack_bf16(device, C):
     """grid_device over a batch of 2 of a stacked plan of 3 elements, BF16
-    modes (20 coils: 24-bit sample rows, 8: complex64 rows): every (batch,
-    element) grid equals that element gridded alone, bit for bit."""
+    modes (20 coils: 24-bit sample rows, 8: complex64 rows; bf16x3c:
+    block-floating-point sample and V rows, whose frames sit at their own
+    row size): every (batch, element) grid equals that element gridded
+    alone, bit for bit."""
     from pygrog_amd.calib import GrogInterpolator
 
     rng = np.random.default_rng(700 + C)
@@ -373,7 +375,7 @@ def test_grid_device_batch_of_stack_bf16(device, C):
     n = coords.shape[1] * coords.shape[2]
     y = (rng.standard_normal((2, 3, C, n)) + 1j * rng.standard_normal((2, 3, C, n))).astype(np.complex64)
     yy = torch.from_numpy(y).to(device).contiguous()
-    for mode in ("bf16x3", "bf16"):
+    for mode in ("bf16x3", "bf16", "bf16x3c"):
         got = g.grid_device(yy, mode, program_cache={})
         for s in range(3):
             gs = GrogInterpolator(shape, coords[s], kernel_width=3, device=device)
