@@ -441,6 +441,10 @@ def reduce_entry(mode: str) -> str:
     return "pg_grid_reduce_v_bfp" if v_format(mode) == "bfp" else "pg_grid_reduce_v"
 
 
+# cells per wave-tile launch (four wavefronts of 64 cells)
+WAVE_TILE = 256
+
+
 def mode_tile_options(mode: str, C: int) -> list:
     """Launch tiles of ``mode``'s gridding kernel: block-floating-point V
     rows are reduced by the wave-tile kernels only."""
@@ -453,9 +457,6 @@ def mode_default_tile(mode: str, C: int) -> int:
     t = int(_lib.load().pg_grid_tile_cells(C))
     return t if t in opts else opts[0]
 
-
-# cells per wave-tile launch (four wavefronts of 64 cells)
-WAVE_TILE = 256
 
 
 def new_sample_rows(n: int, C: int, fmt: str, device) -> torch.Tensor:

@@ -67,8 +67,9 @@ def test_interpolate_complex128_sparse_grid_image(device):
     assert img.dtype == torch.float64
     ref_i = ogrid.image_rss(ref_g, tuple(g.plan.grid_shape), tuple(g.plan.image_shape))
     assert ogrid.rel_l2(img.cpu().numpy(), ref_i) <= 1e-13
-    with pytest.raises(ValueError, match="complex128"):
-        g.interpolate(y, grid=True, mode="bf16x3")
+    for mode in ("bf16x3", "bf16x3c"):
+        with pytest.raises(ValueError, match="complex128"):
+            g.interpolate(y, grid=True, mode=mode)
 
 
 def test_shot_gridding_complex128(device):
