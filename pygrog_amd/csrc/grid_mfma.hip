@@ -30,6 +30,9 @@ namespace pg {
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float float2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ unsigned short f2bf(float f) {
   uint32_t u = __float_as_uint(f);
@@ -273,14 +276,19 @@ __global__ __launch_bounds__(64 * kApplyWaves) void k_apply_grouped(
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
       // hi = bf16(x), lo = bf16(x - hi), both round to nearest even: the
-      // hardware conversion (v_cvt_pk_bf16_f32) gives the bits of f2bf
-      bf16x8_t bhi, blo;
+      // hardware conversion (v_cvt_pk_bf16_f32) gives the bits of f2bf.
+      // One packed conversion per pair for hi; the pair's fp32 hi values
+      // are its two halves moved to the top of a word
+      uint32_t hp[4], lp[4];
 #pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const __bf16 h = (__bf16)cur[ks][i];
-        bhi[i] = h;
-        blo[i] = (__bf16)(cur[ks][i] - (float)h);
+      for (int i = 0; i < 4; ++i) {
+        const float x0 = cur[ks][2 * i], x1 = cur[ks][2 * i + 1];
+        hp[i] = __builtin_bit_cast(uint32_t, __builtin_convertvector((float2_t){x0, x1}, bf16x2_t));
+        const float h0 = __uint_as_float(hp[i] << 16), h1 = __uint_as_float(hp[i] & 0xffff0000u);
+        lp[i] = __builtin_bit_cast(uint32_t, __builtin_convertvector((float2_t){x0 - h0, x1 - h1}, bf16x2_t));
       }
+      const bf16x8_t bhi = __builtin_bit_cast(bf16x8_t, (u32x4_t){hp[0], hp[1], hp[2], hp[3]});
+      const bf16x8_t blo = __builtin_bit_cast(bf16x8_t, (u32x4_t){lp[0], lp[1], lp[2], lp[3]});
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
         acc[mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi[mt][ks], bhi,
