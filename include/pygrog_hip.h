@@ -254,6 +254,19 @@ int pg_plan_units(const uint32_t *cell_ptr, const uint32_t *cell_unit,
 int pg_grappa_gram_f64(const void *calib, int64_t C, int32_t ndim,
                        const int64_t *dims, void *out, pg_stream_t stream);
 
+/* Operator table from the per-axis fractional powers (reference
+ * prepare_grappa_table, _grappa.py:131-181): powers (ndim, nsteps, C, C)
+ * complex128 = Gx, Gy(, Gz) powers for the nsteps exponents;
+ *   2D: T[i nsteps + j] = Gx[j] Gy[i]
+ *   3D: T[i nsteps^2 + j nsteps + k] = (F[k] Gy[j]) L[i], (F, L) = (Gx, Gz)
+ *       (reference pairing) or (Gz, Gx) when `corrected`,
+ * in FP64 (ascending fma chains), written to table_c64 (K, C, C) complex64
+ * rounded to nearest and, when table_c128 is not NULL, to table_c128
+ * (K, C, C) complex128.  C <= 32. */
+int pg_table_compose_f64(const void *powers, int32_t ndim, int64_t nsteps, int64_t C,
+                         int32_t corrected, void *table_c64, void *table_c128,
+                         pg_stream_t stream);
+
 /* Hot cells and the tile kernels' row numbering from the reduction units.
  * cell_unit (S*G + 1) is the exclusive scan of units per cell over all
  * elements.  A cell with nu units is hot when nu >= min_units and special
@@ -337,6 +350,19 @@ int pg_group_by_entry(const uint32_t *rep_ent, const uint32_t *rep_src,
                       float *ent_s, uint32_t *items, int64_t items_max,
                       void *tmp, int64_t tmp_bytes, int64_t *n_items,
                       pg_stream_t stream);
+/* pg_group_by_entry without the synchronisation: the work list's slots from
+ * its length up to items_max hold entry 0xFFFFFFFF (which pg_apply_grouped_bf16*
+ * skips), so the apply can be launched with n_items = items_max before the
+ * length is known on the host; the length itself is copied into
+ * *n_items_pinned when that is not NULL (page-locked host memory that stays
+ * allocated until `stream` has passed this point; zeroed by the call).  For callers that enqueue a job's
+ * preparation and go on (the serving loop).  No reference counterpart. */
+int pg_group_by_entry_async(const uint32_t *rep_ent, const uint32_t *rep_src,
+                            const float *rep_s, int64_t R, int64_t K,
+                            int64_t chunk, uint32_t *ent_ptr, uint32_t *ent_rep,
+                            uint32_t *ent_src, float *ent_s, uint32_t *items,
+                            int64_t items_max, void *tmp, int64_t tmp_bytes,
+                            int64_t *n_items_pinned, pg_stream_t stream);
 
 /* Quantised table entry of every candidate replica (reference
  * calc_interp_table, _grog.py:406-422, generalised from 10**precision to an

@@ -148,6 +148,9 @@ _SIGS = {
     "pg_group_by_entry_layout": [c_int64, c_int64, c_int64, c_void_p, c_void_p],
     "pg_group_by_entry": [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p, c_void_p,
                           c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p],
+    "pg_table_compose_f64": [c_void_p, c_int32, c_int64, c_int64, c_int32, c_void_p, c_void_p, c_void_p],
+    "pg_group_by_entry_async": [c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p, c_void_p,
+                                c_void_p, c_void_p, c_void_p, c_int64, c_void_p, c_int64, c_void_p, c_void_p],
     "pg_apply_grouped_bf16": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_apply_grouped_bf16_s24": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
     "pg_apply_grouped_bf16_bfp": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_int32, c_void_p, c_void_p],
@@ -251,12 +254,26 @@ def require_hip(*tensors: torch.Tensor) -> torch.device:
     return dev
 
 
+_hip_seen = False
+
+
+def hip_available() -> bool:
+    """torch.cuda.is_available(), remembered once it has been true (the
+    device count of a process does not drop)."""
+    global _hip_seen
+    if not _hip_seen:
+        _hip_seen = bool(torch.cuda.is_available())
+    return _hip_seen
+
+
 def hip_device(device=None) -> torch.device:
     """Resolve the compute device: a HIP device, or raise."""
-    if not torch.cuda.is_available():
+    if not hip_available():
         raise RuntimeError(
             "pygrog_amd needs a HIP (ROCm) GPU: torch.cuda.is_available() is False"
         )
+    if isinstance(device, torch.device) and device.type == "cuda" and device.index is not None:
+        return device
     if device is None:
         return torch.device("cuda", torch.cuda.current_device())
     device = torch.device(device)

@@ -30,8 +30,8 @@ the reference table pairs ``Gx`` (axis-2 operator) with ``d0``.  The default
 
 from __future__ import annotations
 
-__all__ = ["KernelTable", "KernelTableResult", "grappa_power", "grappa_powers", "lstsq", "prepare_grappa_table",
-           "train_grappa"]
+__all__ = ["KernelTable", "KernelTableResult", "build_table_finish", "build_table_launch", "compose_table",
+           "grappa_power", "grappa_powers", "lstsq", "prepare_grappa_table", "train_grappa"]
 
 import math
 from dataclasses import dataclass, field
@@ -128,7 +128,16 @@ def _matrix_logm(A: torch.Tensor, host_steps: list | None = None) -> torch.Tenso
     return vecs @ torch.diag_embed(torch.log(vals)) @ torch.linalg.inv(vecs)
 
 
-def grappa_powers(G_units: torch.Tensor, exponents, host_steps: list | None = None,  # noqa: ARG001
+def _powers_host(G: np.ndarray, exps: np.ndarray) -> np.ndarray:
+    """V diag(w^p) V^-1 for a stack of C x C complex128 operators and
+    float64 exponents, on the host: (A, C, C), (n,) -> (A, n, C, C)."""
+    vals, vecs = np.linalg.eig(G)
+    vinv = np.linalg.inv(vecs)
+    wp = np.exp(exps[None, :, None] * np.log(vals)[:, None, :])  # (A, n, C)
+    return (vecs[:, None] * wp[:, :, None, :]) @ vinv[:, None]
+
+
+def grappa_powers(G_units, exponents, host_steps: list | None = None,  # noqa: ARG001
                   device=None) -> torch.Tensor:
     """Fractional powers of a stack of unit operators, (A, C, C) -> (A, n, C, C)
     complex128 on ``device`` (default: the operators' device), one batched
@@ -139,18 +148,23 @@ def grappa_powers(G_units: torch.Tensor, exponents, host_steps: list | None = No
     exponential is V diag(exp(p log w)) V^-1, evaluated here directly (same
     principal logarithm of the eigenvalues, no matrix exponential).  The
     operators are C x C (C coils), so the whole evaluation runs on the host
-    and only the powers go to the device: LAPACK finishes it in a fraction
-    of a millisecond, where the device eigensolver spends milliseconds in
-    launches and host round trips (measured on MI355X: 3 x 16 x 16 with 11
-    exponents, 3.4 ms on the device, 0.3 ms on the host; same values to
-    1e-15)."""
-    G = torch.as_tensor(G_units)
-    Gh = G.to(torch.complex128).cpu()
-    exps = torch.as_tensor(exponents, dtype=torch.float32).cpu().to(torch.float64)
-    vals, vecs = torch.linalg.eig(Gh)
-    vinv = torch.linalg.inv(vecs)
-    wp = torch.exp(exps.view(1, -1, 1).to(vals.dtype) * torch.log(vals).unsqueeze(1))  # (A, n, C)
-    return ((vecs.unsqueeze(1) * wp.unsqueeze(-2)) @ vinv.unsqueeze(1)).to(device if device is not None else G.device)
+    (LAPACK through numpy) and only the powers go to the device: the
+    device eigensolver spends milliseconds in launches and host round
+    trips (measured on MI355X: 3 x 16 x 16 with 11 exponents, 3.4 ms on the
+    device, 0.3 ms on the host; same values to 1e-15).  The upload goes
+    through page-locked memory without waiting for the device's stream."""
+    if isinstance(G_units, torch.Tensor):
+        home = G_units.device
+        Gh = G_units.detach().to(torch.complex128).cpu().numpy()
+    else:
+        home = torch.device("cpu")
+        Gh = np.asarray(G_units, dtype=np.complex128)
+    exps = np.asarray(torch.as_tensor(exponents, dtype=torch.float32).cpu().numpy(), dtype=np.float64)
+    P = torch.from_numpy(_powers_host(Gh, exps))
+    dev = torch.device(device) if device is not None else home
+    if dev.type == "cpu":
+        return P
+    return P.pin_memory().to(dev, non_blocking=True)
 
 
 def grappa_power(G_unit: torch.Tensor, exponents, host_steps: list | None = None) -> torch.Tensor:
@@ -247,18 +261,48 @@ def _gram_fit(calib: torch.Tensor, lamda: float, on_host: bool = False) -> torch
     reference does (_grappa.py:211-287), without forming the source and
     target matrices.  ``on_host``: leave the operators on the host (where the
     table build takes their eigendecompositions next)."""
+    ops = _gram_ops(_gram_launch(calib, lamda))
+    return ops if on_host else ops.to(calib.device)
+
+
+def _gram_launch(calib: torch.Tensor, lamda: float) -> SimpleNamespace:
+    """Enqueue the Gram products of ``_gram_fit`` and their copy into
+    page-locked host memory on the current stream; ``_gram_ops`` waits for
+    them and solves."""
     C = int(calib.shape[0])
     dims = [int(n) for n in calib.shape[1:]]
     nd = len(dims)
     x = calib.contiguous()
     out = torch.empty(2 * nd * C * C + C, dtype=torch.complex128, device=x.device)
     _lib.call("pg_grappa_gram_f64", _lib.ptr(x), C, nd, _lib.dims(dims), _lib.ptr(out), _lib.stream(x.device))
-    h = out.cpu()
+    host = torch.empty(out.shape, dtype=out.dtype, pin_memory=True)
+    host.copy_(out, non_blocking=True)
+    ev = torch.cuda.Event()
+    ev.record()
+    return SimpleNamespace(host=host, event=ev, C=C, nd=nd, lamda=lamda, device=x.device)
+
+
+def _gram_ops(g: SimpleNamespace) -> torch.Tensor:
+    """(nd, C, C) complex128 unit operators on the host from ``_gram_launch``."""
+    g.event.synchronize()
+    C, nd = g.C, g.nd
+    h = g.host.numpy()
     n2 = float(h[2 * nd * C * C:].real.sum())
     M = h[:2 * nd * C * C].reshape(nd, 2, C, C)
-    eye = torch.eye(C, dtype=torch.complex128)
-    ops = torch.linalg.solve(M[:, 0] + lamda * n2 * eye, M[:, 1]).mT
-    return ops if on_host else ops.to(x.device)
+    return torch.from_numpy(np.linalg.solve(M[:, 0] + (g.lamda * n2) * np.eye(C), M[:, 1]).transpose(0, 2, 1))
+
+
+def _gram_applies(calib: torch.Tensor, dev: torch.device, coords, lamda: float) -> bool:
+    """Whether ``train_grappa`` fits ``calib`` by the normal equations."""
+    return (dev.type != "cpu" and coords is None and lamda >= NORMAL_EQ_MIN_LAMDA and calib.ndim in (3, 4)
+            and calib.shape[0] <= 256 and min(calib.shape[1:]) >= 2)
+
+
+def _units_of(ops: torch.Tensor, ndim: int) -> SimpleNamespace:
+    """Axis operators (axis 0 first) as the fit's x / y / z units."""
+    if ndim == 2:
+        return SimpleNamespace(x=ops[0], y=ops[1], z=None)
+    return SimpleNamespace(x=ops[2], y=ops[1], z=ops[0])
 
 
 def train_grappa(train_data, lamda: float | None = None, coords=None, *, device=None, host_steps: list | None = None,
@@ -274,12 +318,8 @@ def train_grappa(train_data, lamda: float | None = None, coords=None, *, device=
         lamda = 0.01 if coords is None else 0.0
     dev = torch.device(device) if device is not None else torch.device("cpu")
     calib = _to_dev(train_data, dev, torch.complex128)
-    if (dev.type != "cpu" and coords is None and lamda >= NORMAL_EQ_MIN_LAMDA and calib.ndim in (3, 4)
-            and calib.shape[0] <= 256 and min(calib.shape[1:]) >= 2):
-        ops = _gram_fit(calib, lamda, on_host=host_units)
-        if calib.ndim == 3:
-            return SimpleNamespace(x=ops[0], y=ops[1], z=None)
-        return SimpleNamespace(x=ops[2], y=ops[1], z=ops[0])
+    if _gram_applies(calib, dev, coords, lamda):
+        return _units_of(_gram_fit(calib, lamda, on_host=host_units), calib.ndim - 1)
     calib = calib / torch.linalg.norm(calib)
     if coords is not None:
         c = torch.as_tensor(np.asarray(coords) if not isinstance(coords, torch.Tensor) else coords)
@@ -323,12 +363,49 @@ def build_table(train_data, radius: float, precision: int | None = 1, lamda: flo
     # the unit operators stay on the host when the fit leaves them there: their
     # eigendecompositions run there, and only the powers go to the device
     kern = train_grappa(train_data, lamda, device=device, host_steps=host_steps, host_units=True)
+    return _table_from_units(kern, radius, precision, quant, device, axis_order, keep_fp64, host_steps)
+
+
+def build_table_launch(train_data, radius: float, precision: int | None = 1, lamda: float = 0.01, *, quant=None,
+                       device=None, axis_order: str = "reference", keep_fp64: bool = False) -> SimpleNamespace:
+    """``build_table`` in two steps: this one enqueues the device part of a
+    Cartesian normal-equation fit (calibration cast and Gram products, then
+    their copy to the host) on the current stream and returns;
+    ``build_table_finish`` waits for them and does the rest, so the caller
+    can enqueue other work (a plan compiler) in between.  Other fits run
+    here in full."""
+    dev = torch.device(device) if device is not None else torch.device("cpu")
+    args = SimpleNamespace(radius=radius, precision=precision, quant=quant, device=device, axis_order=axis_order,
+                           keep_fp64=keep_fp64, gram=None, res=None, ndim=None)
+    calib = _to_dev(train_data, dev, torch.complex128)
+    if _gram_applies(calib, dev, None, lamda):
+        args.gram = _gram_launch(calib, lamda)
+        args.ndim = calib.ndim - 1
+    else:
+        args.res = build_table(train_data, radius, precision, lamda, quant=quant, device=device,
+                               axis_order=axis_order, keep_fp64=keep_fp64)
+    return args
+
+
+def build_table_finish(p: SimpleNamespace) -> KernelTableResult:
+    """The table of a ``build_table_launch``."""
+    if p.res is not None:
+        return p.res
+    kern = _units_of(_gram_ops(p.gram), p.ndim)
+    return _table_from_units(kern, p.radius, p.precision, p.quant, p.device, p.axis_order, p.keep_fp64, [])
+
+
+def _table_from_units(kern, radius, precision, quant, device, axis_order, keep_fp64, host_steps) -> KernelTableResult:
     pfac, stepsize, nsteps = quant_steps(radius, precision, quant)
     deltas = torch.arange(nsteps).float()
     deltas = (deltas - (nsteps - 1) / 2) * stepsize  # float32, as the reference
     units = [kern.x, kern.y] + ([kern.z] if kern.z is not None else [])
     dev = torch.device(device) if device is not None else units[0].device
-    powers = grappa_powers(torch.stack([torch.as_tensor(u) for u in units]), deltas, host_steps, device=dev)
+    if all(isinstance(u, torch.Tensor) and u.device.type == "cpu" for u in units):
+        stacked = np.stack([u.numpy() for u in units])
+    else:
+        stacked = torch.stack([torch.as_tensor(u) for u in units])
+    powers = grappa_powers(stacked, deltas, host_steps, device=dev)
     Gx, Gy = powers[0], powers[1]
     if kern.z is not None:
         Gz = powers[2]
@@ -336,10 +413,29 @@ def build_table(train_data, radius: float, precision: int | None = 1, lamda: flo
     else:
         Gz = None
         ndim = 2
-    t64 = prepare_grappa_table(Gx, Gy, Gz, nsteps, ndim, axis_order=axis_order)
-    t32 = t64.to(torch.complex64).contiguous()
-    return KernelTableResult(t32, nsteps, ndim, stepsize, host_steps, t64 if keep_fp64 else None,
+    t32, t64 = compose_table(powers, ndim, nsteps, axis_order, keep_fp64)
+    return KernelTableResult(t32, nsteps, ndim, stepsize, host_steps, t64,
                              powers64=(Gx, Gy, Gz), axis_order=axis_order)
+
+
+def compose_table(powers: torch.Tensor, ndim: int, nsteps: int, axis_order: str = "reference",
+                  keep_fp64: bool = False):
+    """(complex64 table, complex128 table or None) from the stacked per-axis
+    powers (ndim, nsteps, C, C) complex128: one library call
+    (``pg_table_compose_f64``) on a HIP device for up to 32 coils, else the
+    batched products of ``prepare_grappa_table``."""
+    C = int(powers.shape[-1])
+    if powers.device.type == "cuda" and C <= 32 and ndim in (2, 3):
+        P = powers.contiguous()
+        K = nsteps ** ndim
+        t32 = torch.empty(K, C, C, dtype=torch.complex64, device=P.device)
+        t64 = torch.empty(K, C, C, dtype=torch.complex128, device=P.device) if keep_fp64 else None
+        _lib.call("pg_table_compose_f64", _lib.ptr(P), ndim, nsteps, C, 1 if axis_order == "corrected" else 0,
+                  _lib.ptr(t32), _lib.ptr(t64), _lib.stream(P.device))
+        return t32, t64
+    Gz = powers[2] if ndim == 3 else None
+    t64 = prepare_grappa_table(powers[0], powers[1], Gz, nsteps, ndim, axis_order=axis_order)
+    return t64.to(torch.complex64).contiguous(), (t64 if keep_fp64 else None)
 
 
 def KernelTable(train_data, radius: float, precision: int = 1, lamda: float = 0.01, *, quant=None, device=None,

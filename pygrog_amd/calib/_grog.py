@@ -31,7 +31,9 @@ __all__ = ["GrogInterpolator", "GrogPlan"]
 
 import copy
 import ctypes
+import functools
 import gc
+import math
 from typing import NamedTuple
 from types import SimpleNamespace
 
@@ -149,8 +151,20 @@ def _kernel_offsets(ndim: int, width: int, shape: str) -> np.ndarray:
 
     Axis values ``arange(width) - floor((width - 1) / 2)``, enumerated with the
     last axis fastest; 'circle'/'sphere' keeps radius <= (width-1)/2 + 0.5,
-    'cross' keeps offsets with at most one nonzero axis.
+    'cross' keeps offsets with at most one nonzero axis.  The array is
+    shared between calls with the same arguments (read-only).
     """
+    return _kernel_offsets_of(ndim, width, shape)
+
+
+@functools.lru_cache(maxsize=64)
+def _kernel_offsets_of(ndim: int, width: int, shape: str) -> np.ndarray:
+    offs = _kernel_offsets_new(ndim, width, shape)
+    offs.flags.writeable = False
+    return offs
+
+
+def _kernel_offsets_new(ndim: int, width: int, shape: str) -> np.ndarray:
     half = (width - 1) / 2.0
     ax = np.arange(width) - int(np.floor(half))
     mesh = np.meshgrid(*([ax] * ndim), indexing="ij")
@@ -322,19 +336,25 @@ class EntryGroups(NamedTuple):
 
 
 def group_by_entry(rep_ent: torch.Tensor, R: int, K: int, device, chunk: int, rep_src: torch.Tensor,
-                   rep_s: torch.Tensor) -> EntryGroups:
+                   rep_s: torch.Tensor, defer: bool = False) -> EntryGroups:
     """Replicas of each table entry (a stable radix sort of ``rep_ent[:R]``),
     the work list of chunks of ``chunk`` replicas, and each replica's source
     and weight gathered into that order (so the apply reads its records
     contiguously instead of gathering them): one library call
-    (``pg_group_by_entry``), one small host read."""
+    (``pg_group_by_entry``), one small host read.  ``defer``: no host read
+    (``pg_group_by_entry_async``): ``n_items`` is the work list's capacity,
+    whose unused slots the apply skips."""
     if R <= 0:
         z = torch.zeros(1, dtype=torch.int32, device=device)
         return EntryGroups(torch.zeros(K + 1, dtype=torch.int32, device=device), z, torch.zeros(1, 2, dtype=torch.int32,
                            device=device), 0, z.clone(), torch.zeros(1, dtype=torch.float32, device=device))
-    keys = rep_ent[:R].to(torch.int32).contiguous()
-    src = rep_src[:R].to(torch.int32).contiguous()
-    sw = rep_s[:R].to(torch.float32).contiguous()
+    keys, src, sw = rep_ent[:R], rep_src[:R], rep_s[:R]
+    if keys.dtype != torch.int32 or not keys.is_contiguous():
+        keys = keys.to(torch.int32).contiguous()
+    if src.dtype != torch.int32 or not src.is_contiguous():
+        src = src.to(torch.int32).contiguous()
+    if sw.dtype != torch.float32 or not sw.is_contiguous():
+        sw = sw.to(torch.float32).contiguous()
     im, tb = ctypes.c_int64(0), ctypes.c_int64(0)
     _lib.call("pg_group_by_entry_layout", R, K, chunk, ctypes.addressof(im), ctypes.addressof(tb))
     ent_ptr = torch.empty(K + 1, dtype=torch.int32, device=device)
@@ -343,10 +363,13 @@ def group_by_entry(rep_ent: torch.Tensor, R: int, K: int, device, chunk: int, re
     ent_s = torch.empty(R, dtype=torch.float32, device=device)
     items = torch.empty(int(im.value), 2, dtype=torch.int32, device=device)
     tmp = torch.empty(int(tb.value), dtype=torch.uint8, device=device)
+    args = (_lib.ptr(keys), _lib.ptr(src), _lib.ptr(sw), R, K, chunk, _lib.ptr(ent_ptr), _lib.ptr(ent_rep),
+            _lib.ptr(ent_src), _lib.ptr(ent_s), _lib.ptr(items), int(im.value), _lib.ptr(tmp), int(tb.value))
+    if defer:
+        _lib.call("pg_group_by_entry_async", *args, None, _lib.stream(device))
+        return EntryGroups(ent_ptr, ent_rep, items, int(im.value), ent_src, ent_s)
     n = ctypes.c_int64(0)
-    _lib.call("pg_group_by_entry", _lib.ptr(keys), _lib.ptr(src), _lib.ptr(sw), R, K, chunk, _lib.ptr(ent_ptr),
-              _lib.ptr(ent_rep), _lib.ptr(ent_src), _lib.ptr(ent_s), _lib.ptr(items), int(im.value), _lib.ptr(tmp),
-              int(tb.value), ctypes.addressof(n), _lib.stream(device))
+    _lib.call("pg_group_by_entry", *args, ctypes.addressof(n), _lib.stream(device))
     return EntryGroups(ent_ptr, ent_rep, items, int(n.value), ent_src, ent_s)
 
 
@@ -470,7 +493,7 @@ def _to_device_async(t: torch.Tensor, device) -> torch.Tensor:
     """Copy a small host tensor to ``device`` on the current stream without
     waiting (through page-locked memory, which torch keeps until the copy
     has run)."""
-    if not torch.cuda.is_available():
+    if not _lib.hip_available():
         return t.to(device)
     return t.pin_memory().to(device, non_blocking=True)
 
@@ -501,15 +524,27 @@ class GridProgram:
                 f.p_unit = pu.data_ptr()
                 self.max_hot_units = max(self.max_hot_units, int(f.n_hot_units))
                 self.max_hot = max(self.max_hot, int(f.n_hot))
-        arr = (_lib.Frame * self.F)(*frames)
-        # host tables go up from page-locked memory without a blocking copy:
-        # a pageable copy would wait for everything already on the stream
-        # (the serving loop builds one program per batch while the previous
-        # batch runs)
-        raw = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8)
-        self.frames = _to_device_async(raw, device)
         self.G = int(G)
+        # the frame table and the default tile map go up in one copy from
+        # page-locked memory, without a blocking copy: a pageable copy would
+        # wait for everything already on the stream (the serving loop builds
+        # one program per batch while the previous batch runs)
+        fb = bytes((_lib.Frame * self.F)(*frames))
+        nf = (len(fb) + 15) // 16 * 16
+        tp = self._tile_map(self.tile_cells)
+        host = torch.empty(nf + tp.nbytes, dtype=torch.uint8, pin_memory=_lib.hip_available())
+        hn = host.numpy()
+        hn[:len(fb)] = np.frombuffer(fb, dtype=np.uint8)
+        hn[nf:] = tp.view(np.uint8)
+        both = host.to(device, non_blocking=True)
+        self.frames = both[:nf]
+        self._maps = {self.tile_cells: both[nf:].view(torch.int32)}
         self.set_shape(tile_cells, n_blocks)
+
+    def _tile_map(self, tile_cells: int) -> np.ndarray:
+        """First tile of every frame, (F + 1) int32."""
+        tiles_per = (self.G + tile_cells - 1) // tile_cells
+        return (np.arange(self.F + 1, dtype=np.int64) * tiles_per).astype(np.int32)
 
     def set_shape(self, tile_cells: int, n_blocks: int | None = None) -> None:
         """Launch shape: cells per tile (one of ``_lib.tile_options(C)``) and
@@ -520,11 +555,12 @@ class GridProgram:
         self.tile_cells = int(tile_cells)
         tiles_per = (self.G + self.tile_cells - 1) // self.tile_cells
         self.n_tiles = tiles_per * self.F
-        if getattr(self, "tile_ptr", None) is not None:
-            # a graph captured earlier may still name the old map: keep it alive
-            self._old_maps = getattr(self, "_old_maps", []) + [self.tile_ptr]
-        self.tile_ptr = _to_device_async((torch.arange(self.F + 1, dtype=torch.int64) * tiles_per).to(torch.int32),
-                                         self.device)
+        # every tile map stays alive with the program (a graph captured
+        # earlier may name any of them)
+        if self.tile_cells not in self._maps:
+            self._maps[self.tile_cells] = _to_device_async(torch.from_numpy(self._tile_map(self.tile_cells)),
+                                                           self.device)
+        self.tile_ptr = self._maps[self.tile_cells]
         self._mark_uploaded()
         if n_blocks is None:
             n_blocks = min(self.n_tiles, 256 * 12)
@@ -534,7 +570,7 @@ class GridProgram:
         """Record where the tables' uploads were enqueued; the next launch
         from another stream waits for them."""
         self._upload = None
-        if torch.cuda.is_available():
+        if _lib.hip_available():
             self._upload = (torch.cuda.current_stream(self.device), torch.cuda.Event())
             self._upload[1].record(self._upload[0])
 
@@ -581,6 +617,13 @@ def tune_program(prog: GridProgram, entry: str, reps: int = 5, candidates=None) 
 # ---------------------------------------------------------------------------
 # Plan creation on the GPU
 # ---------------------------------------------------------------------------
+@functools.lru_cache(maxsize=256)
+def _geom_of(shape: tuple, grid_shape: tuple, ndim: int, kernel_width, kernel_shape: str) -> _lib.PlanGeom:
+    """The plan compiler's geometry argument (passed by value; shared
+    between calls with the same arguments, never modified)."""
+    return _geom(shape, grid_shape, _kernel_offsets(ndim, kernel_width, kernel_shape))
+
+
 def _geom(shape, grid_shape, offsets) -> _lib.PlanGeom:
     ndim = len(shape)
     kw = offsets.shape[0]
@@ -643,19 +686,21 @@ def _plan_launch(shape, coords, oversamp, kernel_width, kernel_shape, time_map, 
     if s_shape and tuple(c.shape[: len(s_shape)]) != s_shape:
         raise ValueError(f"coords leading shape {tuple(c.shape[:len(s_shape)])} does not match stack_shape {s_shape}")
     spatial = tuple(int(s) for s in c.shape[len(s_shape):-1])
-    S = int(np.prod(s_shape)) if s_shape else 1
-    npts = int(np.prod(spatial)) if spatial else 1
+    S = math.prod(s_shape)
+    npts = math.prod(spatial)
 
     shape = _default_shape(ndim, shape)
     oversamp = _default_oversamp(ndim, oversamp)
-    grid_shape = tuple(int(np.ceil(s * o)) for s, o in zip(shape, oversamp))
-    G = int(np.prod(grid_shape))
-    offsets = _kernel_offsets(ndim, kernel_width, kernel_shape)
-    kw = int(offsets.shape[0])
+    grid_shape = tuple(int(math.ceil(s * o)) for s, o in zip(shape, oversamp))
+    G = math.prod(grid_shape)
+    kw = int(_kernel_offsets(ndim, kernel_width, kernel_shape).shape[0])
     nper = npts * kw
-    geom = _geom(shape, grid_shape, offsets)
+    geom = _geom_of(shape, grid_shape, ndim, kernel_width, kernel_shape)
 
-    cd = c.reshape(S, npts, ndim).to(dev).contiguous()
+    cd = c.reshape(S, npts, ndim)
+    if cd.device != dev:
+        cd = cd.to(dev)
+    cd = cd.contiguous()
     # the whole plan compiler in one library call (the calling thread holds
     # no interpreter lock meanwhile); every array is a view of one workspace
     offs = (ctypes.c_int64 * PLAN_FIELDS)()
@@ -853,9 +898,12 @@ class GrogInterpolator:
                           axis_order=axis_order, keep_fp64=keep_fp64)
         self._attach_table(res, precision, quant)
 
-    def _attach_table(self, res, precision: int = 1, quant: int | None = None, packed: torch.Tensor | None = None):
+    def _attach_table(self, res, precision: int = 1, quant: int | None = None, packed: torch.Tensor | None = None,
+                      pack: bool = True):
         """Quantise every replica's shift against the fitted table ``res``
-        and keep the table (``packed``: an already packed copy to share)."""
+        and keep the table (``packed``: an already packed copy to share;
+        ``pack=False``: pack the plain kernel's copy only when a plain launch
+        first needs it -- the grouped modes never read it)."""
         comp = self.plan._compiled
         if comp.distances is None:
             raise RuntimeError("calc_interp_table: shifts were already released by a previous call")
@@ -882,14 +930,9 @@ class GrogInterpolator:
         comp.table_nsteps = int(nsteps)
         comp.table_axis_order = getattr(res, "axis_order", "reference")
         comp.table_host_steps = res.host_steps
-        if C <= 64:
-            if packed is None:
-                packed = torch.empty(int(_lib.load().pg_table_packed_elems(K, C)) * 4, dtype=torch.float32,
-                                     device=dev)
-                _lib.call("pg_table_pack_c64", _lib.ptr(res.table), K, C, _lib.ptr(packed), _lib.stream(dev))
-            comp.table_packed = packed
-        else:
-            comp.table_packed = None
+        comp.table_packed = packed if C <= 64 else None
+        if C <= 64 and packed is None and pack:
+            self._pack_table()
         comp.table_bf16 = {}
         # release the shifts, as the reference does
         comp.distances = None
@@ -900,6 +943,15 @@ class GrogInterpolator:
         self._interp_idx = interp_idx.reshape(full)
         self._interpolator_set = True
         self._programs = {}
+
+    def _pack_table(self) -> None:
+        """The plain kernel's copy of the table (coil pairs, zero padded to
+        the coil group)."""
+        comp = self.plan._compiled
+        K, C, dev = int(comp.table.shape[0]), comp.C, comp.device
+        packed = torch.empty(int(_lib.load().pg_table_packed_elems(K, C)) * 4, dtype=torch.float32, device=dev)
+        _lib.call("pg_table_pack_c64", _lib.ptr(comp.table), K, C, _lib.ptr(packed), _lib.stream(dev))
+        comp.table_packed = packed
 
     def set_dataset_shape(self, shape):
         if self._dataset_shape_set:
@@ -938,6 +990,8 @@ class GrogInterpolator:
         # bytes of one frame's sources: complex64 rows, or the 24-bit rows
         # of the BF16 modes (which only the reduction's frame table carries)
         y_frame = (y.numel() * y.element_size()) // max(B * S, 1)
+        if vbuf is None and comp.table_packed is None and C <= 64:
+            self._pack_table()
         frames = []
         for b in range(B):
             for s in range(S):
@@ -989,10 +1043,11 @@ class GrogInterpolator:
     # -- grouped (BF16 MFMA) structures ---------------------------------------
     GROUP_CHUNK = 128  # replicas per MFMA work item (8 tiles of 16)
 
-    def _entry_groups(self):
+    def _entry_groups(self, defer: bool = False):
         """Per stack element: replicas of each table entry (stable radix sort of
         the entries) as (ent_ptr (K+1), ent_rep (R), items (n, 2) work list of
-        (entry, first replica offset) chunks of GROUP_CHUNK)."""
+        (entry, first replica offset) chunks of GROUP_CHUNK).  ``defer``: built
+        without a host read (the work list is launched over its capacity)."""
         comp = self.plan._compiled
         if comp.entry_groups is not None:
             return comp.entry_groups
@@ -1002,7 +1057,7 @@ class GrogInterpolator:
         for s in range(comp.S):
             R = comp.n_valid[s]
             groups.append(group_by_entry(comp.rep_ent[s, :max(R, 1)], R, K, dev, self.GROUP_CHUNK,
-                                         comp.rep_src[s], comp.rep_s[s]))
+                                         comp.rep_src[s], comp.rep_s[s], defer=defer))
         comp.entry_groups = groups
         return groups
 
@@ -1247,16 +1302,15 @@ class GrogInterpolator:
     def _table64(self) -> torch.Tensor:
         """The (K, C, C) complex128 table: kept at fit time or assembled
         (once) from the FP64 per-axis powers of the fit."""
-        from ._grappa import prepare_grappa_table
+        from ._grappa import compose_table
 
         comp = self.plan._compiled
         if getattr(comp, "table64", None) is None:
             if getattr(comp, "powers64", None) is None:
                 raise RuntimeError("complex128 data needs the FP64 table: this table was set without its FP64 "
                                    "powers (fit it with calc_interp_table)")
-            Gx, Gy, Gz = comp.powers64
-            comp.table64 = prepare_grappa_table(Gx, Gy, Gz, comp.table_nsteps, comp.ndim,
-                                                axis_order=comp.table_axis_order).to(comp.device).contiguous()
+            P = torch.stack([g for g in comp.powers64 if g is not None]).to(comp.device)
+            comp.table64 = compose_table(P, comp.ndim, comp.table_nsteps, comp.table_axis_order, keep_fp64=True)[1]
         return comp.table64
 
     def _apply_c128(self, y: torch.Tensor, s: int) -> torch.Tensor:

@@ -1492,7 +1492,13 @@ static int launch_hot(hipStream_t st, const pg_frame *frames, int F, int C,
 // Helper stream per device for the hot pre-pass, forked from and joined back
 // into the caller's stream with two events (created once, outside any graph
 // capture; default priority).  The lock keeps one call's record/wait pairs
-// together.
+// together.  Launches of fewer tiles than kHotForkMinTiles run the pre-pass
+// in line instead: the fork's four stream operations cost more host time
+// than the pre-pass takes on the device there (a 256^2 frame: 8 us of
+// kernels against about 30 us of record/wait calls), and the order of the
+// writes does not change a bit (the pre-pass and the tile kernel write
+// disjoint cells).
+constexpr int64_t kHotForkMinTiles = 4096;
 struct HotFork {
   std::mutex m;
   hipStream_t helper[kMaxDevices] = {};
@@ -1511,6 +1517,10 @@ static int launch_gs(int64_t tile_cells, dim3 grid, hipStream_t st,
                      int64_t max_hot) {
   if (max_hot_units <= 0)
     return launch_tile<GS, MODE>(tile_cells, grid, st, frames, tile_ptr, F, C, n_tiles);
+  if ((int64_t)n_tiles < kHotForkMinTiles) {
+    if (launch_hot<GS, MODE>(st, frames, F, C, max_hot_units, max_hot)) return 1;
+    return launch_tile<GS, MODE>(tile_cells, grid, st, frames, tile_ptr, F, C, n_tiles);
+  }
   int dev = 0;
   PG_HIP(hipGetDevice(&dev));
   HotFork &hf = hot_fork();

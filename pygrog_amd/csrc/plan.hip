@@ -15,6 +15,7 @@
 //   CSR) -> weights (EGROG per cell, f64 sums in reference order) -> scan ->
 //   units; later quantize (table entries) once the table is known.
 #include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "common.h"
 
@@ -507,6 +508,71 @@ __global__ void k_plan_hot_rows(const uint32_t *__restrict__ cell_unit,
   }
 }
 
+// Operator table from the per-axis fractional powers P (ndim, n, C, C)
+// complex128 (P[0] = Gx, P[1] = Gy, P[2] = Gz), one workgroup per entry and
+// one thread per output element (r, c):
+//   2D: T[i n + j]         = Gx[j] Gy[i]
+//   3D: T[i n^2 + j n + k] = (F[k] Gy[j]) L[i],  (F, L) = (Gx, Gz), or (Gz, Gx)
+//       with `corrected`
+// in FP64 (every complex product an fma chain over the inner index in
+// ascending order, the 3D intermediate rounded to FP64 as a separate
+// product would be), written as complex64 (round to nearest) and, when
+// t64 is given, complex128.  The operands of an entry are staged in LDS.
+constexpr int kComposeMaxC = 32;
+__device__ __forceinline__ double2 cmac(double2 acc, double2 a, double2 b) {
+  acc.x = fma(a.x, b.x, acc.x);
+  acc.x = fma(-a.y, b.y, acc.x);
+  acc.y = fma(a.x, b.y, acc.y);
+  acc.y = fma(a.y, b.x, acc.y);
+  return acc;
+}
+__global__ __launch_bounds__(kComposeMaxC * kComposeMaxC) void k_table_compose(
+    const double2 *__restrict__ P, int ndim, int64_t n, int C, int corrected,
+    float2 *__restrict__ t32, double2 *__restrict__ t64) {
+  __shared__ double2 s_a[kComposeMaxC * kComposeMaxC];
+  __shared__ double2 s_b[kComposeMaxC * kComposeMaxC];
+  __shared__ double2 s_l[kComposeMaxC * kComposeMaxC];
+  const int64_t e = blockIdx.x;
+  const int t = threadIdx.x;
+  const int CC = C * C;
+  const int r = t / C, c = t - (t / C) * C;
+  const int64_t mat = (int64_t)CC;
+  const double2 *Gx = P, *Gy = P + n * mat, *Gz = P + 2 * n * mat;
+  double2 acc = make_double2(0.0, 0.0);
+  if (ndim == 2) {
+    const int64_t i = e / n, j = e - (e / n) * n;
+    if (t < CC) {
+      s_a[t] = Gx[j * mat + t];
+      s_b[t] = Gy[i * mat + t];
+    }
+    __syncthreads();
+    if (t < CC)
+      for (int l = 0; l < C; ++l) acc = cmac(acc, s_a[r * C + l], s_b[l * C + c]);
+  } else {
+    const int64_t i = e / (n * n), rem = e - (e / (n * n)) * (n * n);
+    const int64_t j = rem / n, k = rem - (rem / n) * n;
+    const double2 *F = corrected ? Gz : Gx, *L = corrected ? Gx : Gz;
+    if (t < CC) {
+      s_a[t] = F[k * mat + t];
+      s_b[t] = Gy[j * mat + t];
+      s_l[t] = L[i * mat + t];
+    }
+    __syncthreads();
+    double2 m = make_double2(0.0, 0.0);
+    if (t < CC)
+      for (int l = 0; l < C; ++l) m = cmac(m, s_a[r * C + l], s_b[l * C + c]);
+    __syncthreads();
+    if (t < CC) s_a[t] = m;
+    __syncthreads();
+    if (t < CC)
+      for (int l = 0; l < C; ++l) acc = cmac(acc, s_a[r * C + l], s_l[l * C + c]);
+  }
+  if (t < CC) {
+    t32[e * mat + t] = make_float2((float)acc.x, (float)acc.y);
+    if (t64 != nullptr) t64[e * mat + t] = acc;
+  }
+}
+
 // Normal-equation products of the Cartesian GRAPPA fit, in FP64, from the
 // coil-major calibration x (C, P) complex128 over P = n0 n1 n2 positions
 // (C-order, unused dims 1).  Block (k, m): for m < 2 ndim, axis d = m / 2,
@@ -666,6 +732,19 @@ int pg_plan_targets(const float *coords, const float *cmax, int64_t S,
   return 0;
 }
 
+}  // extern "C"
+
+// The library's key-value sorts: rocPRIM's onesweep radix sort for every
+// size above one block (its default switches to a merge sort below 2^20
+// items, about 18 launches for a 2^17-item plan against 8; both are stable,
+// so the permutation is the same).
+namespace {
+using SortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                              rocprim::default_config, 0>;
+}
+
+extern "C" {
+
 int pg_sort_pairs_u64(void *workspace, size_t *ws_bytes,
                       const uint64_t *keys_in, uint64_t *keys_out,
                       const uint32_t *vals_in, uint32_t *vals_out, int64_t n,
@@ -673,9 +752,8 @@ int pg_sort_pairs_u64(void *workspace, size_t *ws_bytes,
   PG_CHECK(ws_bytes != nullptr, "sort: ws_bytes is null");
   PG_CHECK(n >= 0 && n < (int64_t)INT32_MAX, "sort: bad n");
   PG_CHECK(end_bit >= 1 && end_bit <= 64, "sort: bad end_bit");
-  PG_HIP(hipcub::DeviceRadixSort::SortPairs(
-      workspace, *ws_bytes, keys_in, keys_out, vals_in, vals_out, (int)n, 0,
-      end_bit, as_stream(stream)));
+  PG_HIP(rocprim::radix_sort_pairs<SortConfig>(workspace, *ws_bytes, keys_in, keys_out, vals_in, vals_out,
+                                               (unsigned int)n, 0u, (unsigned int)end_bit, as_stream(stream)));
   return 0;
 }
 
@@ -686,9 +764,8 @@ int pg_sort_pairs_u32(void *workspace, size_t *ws_bytes,
   PG_CHECK(ws_bytes != nullptr, "sort: ws_bytes is null");
   PG_CHECK(n >= 0 && n < (int64_t)INT32_MAX, "sort: bad n");
   PG_CHECK(end_bit >= 1 && end_bit <= 32, "sort: bad end_bit");
-  PG_HIP(hipcub::DeviceRadixSort::SortPairs(
-      workspace, *ws_bytes, keys_in, keys_out, vals_in, vals_out, (int)n, 0,
-      end_bit, as_stream(stream)));
+  PG_HIP(rocprim::radix_sort_pairs<SortConfig>(workspace, *ws_bytes, keys_in, keys_out, vals_in, vals_out,
+                                               (unsigned int)n, 0u, (unsigned int)end_bit, as_stream(stream)));
   return 0;
 }
 
@@ -791,6 +868,22 @@ int pg_grappa_gram_f64(const void *calib, int64_t C, int32_t ndim,
   hipLaunchKernelGGL(k_grappa_gram, grid, dim3(kGramThreads), 0, as_stream(stream),
                      static_cast<const double2 *>(calib), C, (int)ndim, n[0], n[1],
                      n[2], static_cast<double2 *>(out));
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_table_compose_f64(const void *powers, int32_t ndim, int64_t nsteps, int64_t C,
+                         int32_t corrected, void *table_c64, void *table_c128,
+                         pg_stream_t stream) {
+  PG_CHECK((ndim == 2 || ndim == 3) && nsteps >= 1 && C >= 1 && C <= kComposeMaxC && powers && table_c64,
+           "table_compose: bad arguments (ndim %d, nsteps %lld, C %lld; at most %d coils)", (int)ndim,
+           (long long)nsteps, (long long)C, kComposeMaxC);
+  const int64_t K = ndim == 2 ? nsteps * nsteps : nsteps * nsteps * nsteps;
+  PG_CHECK(K < (int64_t)INT32_MAX, "table_compose: too many entries");
+  const int threads = (int)((C * C + 63) / 64 * 64);
+  hipLaunchKernelGGL(k_table_compose, dim3((unsigned)K), dim3(threads), 0, as_stream(stream),
+                     static_cast<const double2 *>(powers), (int)ndim, nsteps, (int)C, (int)corrected,
+                     static_cast<float2 *>(table_c64), static_cast<double2 *>(table_c128));
   PG_LAUNCHED();
   return 0;
 }

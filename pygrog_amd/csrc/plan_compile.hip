@@ -159,6 +159,18 @@ __global__ void k_entry_items(const uint32_t *__restrict__ first, const uint32_t
   }
 }
 
+// Work-list slots past its length (first[K]) get entry 0xFFFFFFFF, which the
+// grouped apply skips: a launch over the whole capacity then needs no host
+// copy of the length.
+__global__ void k_items_tail(const uint32_t *__restrict__ total, int64_t items_max, uint32_t *__restrict__ items) {
+  const int64_t n = *total;
+  for (int64_t i = n + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < items_max;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    items[2 * i] = 0xFFFFFFFFu;
+    items[2 * i + 1] = 0u;
+  }
+}
+
 __global__ void k_gather_records(const uint32_t *__restrict__ ent_rep, const uint32_t *__restrict__ rep_src,
                                  const float *__restrict__ rep_s, int64_t R, uint32_t *__restrict__ ent_src,
                                  float *__restrict__ ent_s) {
@@ -167,6 +179,24 @@ __global__ void k_gather_records(const uint32_t *__restrict__ ent_rep, const uin
     ent_src[i] = rep_src[r];
     ent_s[i] = rep_s[r];
   }
+}
+
+// Up to kZeroRanges buffers of 32-bit words cleared in one launch
+// (blockIdx.y = buffer; every buffer starts on a 16-byte boundary).
+constexpr int kZeroRanges = 8;
+struct ZeroRanges {
+  uint32_t *p[kZeroRanges];
+  int64_t n[kZeroRanges];
+};
+__global__ void k_zero_ranges(ZeroRanges z) {
+  uint32_t *p = z.p[blockIdx.y];
+  const int64_t n = z.n[blockIdx.y];
+  const int64_t n4 = n / 4;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  uint4 *q = reinterpret_cast<uint4 *>(p);
+  for (int64_t i = t; i < n4; i += stride) q[i] = make_uint4(0u, 0u, 0u, 0u);
+  for (int64_t i = 4 * n4 + t; i < n; i += stride) p[i] = 0u;
 }
 
 __global__ void k_fill_u32(uint32_t *__restrict__ p, int64_t n, uint32_t v) {
@@ -267,15 +297,29 @@ static int plan_compile(const float *coords, int64_t S, int64_t npts,
   auto *unit_count = static_cast<uint32_t *>(T[kUnitCount]);
   const int64_t nc1 = z.ncell + 1;
 
-  // zero what the kernels accumulate into or only partly write
-  PG_HIP(hipMemsetAsync(cmax, 0, tbytes[kCmax], st));
-  PG_HIP(hipMemsetAsync(n_invalid, 0, 8 * S, st));
-  PG_HIP(hipMemsetAsync(weights, 0, 4 * rep, st));
-  PG_HIP(hipMemsetAsync(sqrt_w, 0, 4 * rep, st));
-  PG_HIP(hipMemsetAsync(rep_src, 0, 4 * S * z.rmax, st));
-  PG_HIP(hipMemsetAsync(rep_s, 0, 4 * S * z.rmax, st));
-  PG_HIP(hipMemsetAsync(unit_count, 0, 4 * nc1, st));
-  if (z.nper == 0) PG_HIP(hipMemsetAsync(cell_ptr, 0, 4 * S * (G + 1), st));
+  // zero what the kernels accumulate into or only partly write (one launch
+  // for all of them; every buffer is a 256-byte aligned slice of a workspace)
+  {
+    ZeroRanges zr{};
+    int nz = 0;
+    auto add = [&](void *ptr, int64_t bytes) {
+      zr.p[nz] = static_cast<uint32_t *>(ptr);
+      zr.n[nz] = bytes / 4;
+      ++nz;
+    };
+    add(cmax, tbytes[kCmax]);
+    add(n_invalid, 8 * S);
+    add(weights, 4 * rep);
+    add(sqrt_w, 4 * rep);
+    add(rep_src, 4 * S * z.rmax);
+    add(rep_s, 4 * S * z.rmax);
+    add(unit_count, 4 * nc1);
+    if (z.nper == 0) add(cell_ptr, 4 * S * (G + 1));
+    int64_t most = 1;
+    for (int i = 0; i < nz; ++i) most = zr.n[i] > most ? zr.n[i] : most;
+    hipLaunchKernelGGL(k_zero_ranges, dim3(grid_for((most + 3) / 4, 256, 2048), nz), dim3(256), 0, st, zr);
+    PG_LAUNCHED();
+  }
 
   if (int rc = pg_plan_absmax(coords, S, npts, geom.ndim, cmax, stream)) return rc;
   if (int rc = pg_plan_targets(coords, cmax, S, npts, geom, target_idx, distances, gauss, keys, vals,
@@ -367,14 +411,14 @@ int pg_group_by_entry_layout(int64_t R, int64_t K, int64_t chunk, int64_t *items
   return 0;
 }
 
-int pg_group_by_entry(const uint32_t *rep_ent, const uint32_t *rep_src, const float *rep_s, int64_t R,
-                      int64_t K, int64_t chunk, uint32_t *ent_ptr, uint32_t *ent_rep, uint32_t *ent_src,
-                      float *ent_s, uint32_t *items, int64_t items_max, void *tmp, int64_t tmp_bytes,
-                      int64_t *n_items, pg_stream_t stream) {
+static int group_by_entry(const uint32_t *rep_ent, const uint32_t *rep_src, const float *rep_s, int64_t R,
+                          int64_t K, int64_t chunk, uint32_t *ent_ptr, uint32_t *ent_rep, uint32_t *ent_src,
+                          float *ent_s, uint32_t *items, int64_t items_max, void *tmp, int64_t tmp_bytes,
+                          int64_t *n_items, bool sync, pg_stream_t stream) {
   int64_t im = 0, tb = 0;
   if (int rc = pg_group_by_entry_layout(R, K, chunk, &im, &tb)) return rc;
   PG_CHECK(items_max >= im && tmp_bytes >= tb, "group_by_entry: work list or workspace too small");
-  PG_CHECK(rep_ent && rep_src && rep_s && ent_ptr && ent_rep && ent_src && ent_s && items && tmp && n_items,
+  PG_CHECK(rep_ent && rep_src && rep_s && ent_ptr && ent_rep && ent_src && ent_s && items && tmp && (n_items || !sync),
            "group_by_entry: null pointer");
   const hipStream_t st = as_stream(stream);
   size_t sort_ws = 0, scan_ws = 0;
@@ -408,11 +452,38 @@ int pg_group_by_entry(const uint32_t *rep_ent, const uint32_t *rep_src, const fl
   hipLaunchKernelGGL(k_gather_records, dim3(grid_for(R, 256)), dim3(256), 0, st, ent_rep, rep_src, rep_s, R,
                      ent_src, ent_s);
   PG_LAUNCHED();
-  uint32_t total = 0;
-  PG_HIP(hipMemcpyAsync(&total, first + K, 4, hipMemcpyDeviceToHost, st));
-  PG_HIP(hipStreamSynchronize(st));
-  *n_items = total;
+  if (sync) {
+    uint32_t total = 0;
+    PG_HIP(hipMemcpyAsync(&total, first + K, 4, hipMemcpyDeviceToHost, st));
+    PG_HIP(hipStreamSynchronize(st));
+    *n_items = total;
+    return 0;
+  }
+  hipLaunchKernelGGL(k_items_tail, dim3(grid_for(items_max, 256, 1024)), dim3(256), 0, st, first + K, items_max,
+                     items);
+  PG_LAUNCHED();
+  // the length, for host readers once the stream has passed this point
+  if (n_items != nullptr) {
+    *n_items = 0;
+    PG_HIP(hipMemcpyAsync(n_items, first + K, 4, hipMemcpyDeviceToHost, st));
+  }
   return 0;
+}
+
+int pg_group_by_entry(const uint32_t *rep_ent, const uint32_t *rep_src, const float *rep_s, int64_t R,
+                      int64_t K, int64_t chunk, uint32_t *ent_ptr, uint32_t *ent_rep, uint32_t *ent_src,
+                      float *ent_s, uint32_t *items, int64_t items_max, void *tmp, int64_t tmp_bytes,
+                      int64_t *n_items, pg_stream_t stream) {
+  return group_by_entry(rep_ent, rep_src, rep_s, R, K, chunk, ent_ptr, ent_rep, ent_src, ent_s, items, items_max,
+                        tmp, tmp_bytes, n_items, true, stream);
+}
+
+int pg_group_by_entry_async(const uint32_t *rep_ent, const uint32_t *rep_src, const float *rep_s, int64_t R,
+                            int64_t K, int64_t chunk, uint32_t *ent_ptr, uint32_t *ent_rep, uint32_t *ent_src,
+                            float *ent_s, uint32_t *items, int64_t items_max, void *tmp, int64_t tmp_bytes,
+                            int64_t *n_items_pinned, pg_stream_t stream) {
+  return group_by_entry(rep_ent, rep_src, rep_s, R, K, chunk, ent_ptr, ent_rep, ent_src, ent_s, items, items_max,
+                        tmp, tmp_bytes, n_items_pinned, false, stream);
 }
 
 }  // extern "C"

@@ -12,9 +12,9 @@ pass to ``GrogInterpolator(shape, coords, ...)``,
     up-front estimate of every buffer the job will hold) and a preparation
     slot are free;
   * prepares each admitted job -- GPU plan compiler and FP64 table fit -- on
-    its slot's own streams, from a worker thread (the table is fitted while
-    the plan kernels run), so preparations of different jobs overlap each
-    other and the gridding on the device;
+    its slot's own stream, from a worker thread (the fit is solved on the
+    host while the plan kernels run), so preparations of different jobs
+    overlap each other and the gridding on the device;
   * every time the gridding stream has room, gathers all prepared jobs
     (continuous batching: jobs that became ready while earlier batches were
     on the GPU join the next one) and grids them together: one transpose per
@@ -51,7 +51,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .calib._grappa import build_table
+from .calib._grappa import build_table_finish, build_table_launch
 from .calib._grog import (GRID_MODES, PLAN_RADIUS, GridProgram, GrogInterpolator, _default_oversamp,
                           _kernel_offsets, apply_entry, mode_default_tile, mode_terms, mode_tile_options,
                           reduce_entry, row_format, sample_row_bytes, transpose_entry, v_format)
@@ -283,11 +283,11 @@ class GrogServer:
     """Serve ``GridJob`` requests on one HIP device (see the module notes).
 
     ``hbm_budget`` bytes bound the admitted jobs' estimated footprint,
-    ``slots`` the jobs in preparation at once (one worker thread and two
-    streams each: the plan compiler is enqueued without waiting for it and
-    the same thread fits the table meanwhile, so a preparation's host work
-    overlaps its plan kernels without a second thread contending for the
-    interpreter lock),
+    ``slots`` the jobs in preparation at once (one worker thread and one
+    stream each: the fit's Gram products and the plan compiler are enqueued
+    without waiting for them and the same thread solves the fit on the host
+    meanwhile, so a preparation's host work overlaps its plan kernels
+    without a second thread contending for the interpreter lock),
     ``max_inflight`` the gridding batches on the device.  The worker threads
     live as long as the server (``close()`` or a ``with`` block ends them), so
     successive ``run`` calls reuse their per-thread library state."""
@@ -306,7 +306,6 @@ class GrogServer:
         self.max_inflight = int(max_inflight)
         self.keep_results = keep_results
         self._streams = [torch.cuda.Stream(device=self.device) for _ in range(self.slots)]
-        self._fit_streams = [torch.cuda.Stream(device=self.device) for _ in range(self.slots)]
         self._pool = cf.ThreadPoolExecutor(max_workers=self.slots)
         self._grid_stream = torch.cuda.Stream(device=self.device)
 
@@ -321,34 +320,32 @@ class GrogServer:
         self.close()
 
     # -- preparation (worker thread, own stream) --------------------------------
-    def _fit(self, job: GridJob, slot: int):
-        """The job's operator table (FP64 fit, mostly host LAPACK work) on the
-        slot's second stream; returns the table and the event after it."""
-        fs = self._fit_streams[slot]
-        with torch.cuda.device(self.device), torch.cuda.stream(fs):
-            res = build_table(job.calib, PLAN_RADIUS, job.precision, job.lamda, quant=job.quant, device=self.device)
-            ev = torch.cuda.Event()
-            ev.record(fs)
-        return res, ev
+    def _fit_finish(self, job: GridJob, fit):
+        """The job's operator table from its launched fit (host LAPACK work,
+        then the table's assembly on the stream)."""
+        return build_table_finish(fit)
 
     def _prepare(self, job: GridJob, slot: int) -> GridJob:
         st = self._streams[slot]
-        # the plan compiler is enqueued without waiting for it; this thread
-        # fits the table (host LAPACK work, Gram products on the second
-        # stream) while the plan kernels run, then picks up the plan
+        # one stream per slot, in this order: the table fit's Gram products
+        # (a small kernel, then their copy to the host), the plan compiler,
+        # and -- once this thread has solved the fit on the host (LAPACK),
+        # which overlaps the plan kernels -- the table, the quantisation and
+        # the entry groups; nothing here waits for the device but the Gram
+        # copy and the plan's summary
         with torch.cuda.device(self.device), torch.cuda.stream(st):
+            fit = build_table_launch(job.calib, PLAN_RADIUS, job.precision, job.lamda, quant=job.quant,
+                                     device=self.device)
             pending = GrogInterpolator._launch_plan(job.shape, job.coords, job.oversamp, job.kernel_width,
                                                     job.kernel_shape, device=self.device, defer=True)
             try:
-                res, fev = self._fit(job, slot)
+                res = self._fit_finish(job, fit)
             finally:
                 pending.wait()  # the plan's buffers and summary stay put until its kernels are done
             g = GrogInterpolator._from_pending(pending)
-            st.wait_event(fev)
-            res.table.record_stream(st)
-            g._attach_table(res, job.precision, job.quant)
+            g._attach_table(res, job.precision, job.quant, pack=self.mode == "fp32")
             if self.mode != "fp32":
-                g._entry_groups()
+                g._entry_groups(defer=True)
                 g._table_bf16(2 if mode_terms(self.mode) == 3 else 1)
             ev = torch.cuda.Event()
             ev.record(st)

@@ -457,3 +457,67 @@ def test_group_by_entry_matches_torch(device, R, K, chunk):
     assert torch.equal(grp.items[:total].cpu().to(torch.int64), torch.stack([e, off], 1))
     assert torch.equal(grp.ent_src.cpu(), src[order])
     assert torch.equal(grp.ent_s.cpu(), sw[order])
+
+
+@pytest.mark.parametrize("R,K", [(1, 1), (250000, 1089), (100000, 68921)])
+def test_group_by_entry_deferred(device, R, K):
+    """pg_group_by_entry_async: the same groups as the synchronising call,
+    the work list's unused capacity marked with entry 0xFFFFFFFF, and a
+    grouped apply launched over the whole capacity writing the same V bits
+    as one launched over the exact length."""
+    from pygrog_amd import _lib
+    from pygrog_amd.calib._grog import group_by_entry
+
+    g = torch.Generator(device="cpu").manual_seed(7 * R + K)
+    ent = torch.randint(0, K, (R,), generator=g, dtype=torch.int32)
+    src = torch.randint(0, 4096, (R,), generator=g, dtype=torch.int32)
+    sw = torch.rand(R, generator=g)
+    a = group_by_entry(ent.to(device), R, K, device, 128, src.to(device), sw.to(device))
+    b = group_by_entry(ent.to(device), R, K, device, 128, src.to(device), sw.to(device), defer=True)
+    torch.cuda.synchronize()
+    assert b.n_items == b.items.shape[0] >= a.n_items
+    for x, y in ((a.ent_ptr, b.ent_ptr), (a.ent_rep, b.ent_rep), (a.ent_src, b.ent_src), (a.ent_s, b.ent_s)):
+        assert torch.equal(x, y)
+    assert torch.equal(a.items[:a.n_items], b.items[:a.n_items])
+    assert bool((b.items[a.n_items:, 0] == -1).all()) and bool((b.items[a.n_items:, 1] == 0).all())
+    # the apply over the capacity: same V as over the exact length
+    C = 8
+    n_src = 4096
+    y = (torch.randn(C, n_src, generator=g) + 1j * torch.randn(C, n_src, generator=g)).to(torch.complex64)
+    yt = y.T.contiguous().to(device)
+    tbl = (torch.randn(K, C, C, generator=g) + 1j * torch.randn(K, C, C, generator=g)).to(torch.complex64).to(device)
+    n = int(_lib.load().pg_table_bf16_elems(K, C, 2))
+    packed = torch.empty(n, dtype=torch.int16, device=device)
+    _lib.call("pg_table_pack_bf16", _lib.ptr(tbl), K, C, 2, _lib.ptr(packed), _lib.stream(device))
+    outs = []
+    for grp in (a, b):
+        v = torch.full((R * C,), complex(float("nan"), 0.0), dtype=torch.complex64, device=device)
+        _lib.call("pg_apply_grouped_bf16", _lib.ptr(yt), _lib.ptr(packed), _lib.ptr(grp.ent_ptr), _lib.ptr(grp.ent_rep),
+                  _lib.ptr(grp.ent_src), _lib.ptr(grp.ent_s), _lib.ptr(grp.items), grp.n_items, K, C, 3,
+                  _lib.ptr(v), _lib.stream(device))
+        outs.append(v)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1])
+    assert not bool(torch.isnan(outs[0].real).any())
+
+
+@pytest.mark.parametrize("ndim,n,C,order", [(2, 33, 8, "reference"), (3, 11, 16, "reference"),
+                                            (3, 5, 16, "corrected"), (3, 3, 32, "reference"), (2, 4, 1, "reference")])
+def test_table_compose_matches_batched_products(device, ndim, n, C, order):
+    """pg_table_compose_f64 against the batched complex128 products of
+    prepare_grappa_table (torch on the device): the FP64 tables within a few
+    units in the last place (different summation order), the complex64
+    tables their rounding."""
+    from pygrog_amd.calib._grappa import compose_table, prepare_grappa_table
+
+    g = torch.Generator(device="cpu").manual_seed(ndim * 100 + n + C)
+    P = (torch.randn(ndim, n, C, C, generator=g, dtype=torch.float64)
+         + 1j * torch.randn(ndim, n, C, C, generator=g, dtype=torch.float64)).to(device)
+    t32, t64 = compose_table(P, ndim, n, order, keep_fp64=True)
+    ref = prepare_grappa_table(P[0], P[1], P[2] if ndim == 3 else None, n, ndim, axis_order=order)
+    torch.cuda.synchronize()
+    assert t64.shape == ref.shape == (n ** ndim, C, C)
+    err = (t64 - ref).abs().max().item() / ref.abs().max().item()
+    assert err < 1e-14, err
+    assert torch.allclose(t32, t64.to(torch.complex64), rtol=0, atol=0)
+    assert (t32 != ref.to(torch.complex64)).float().mean().item() < 1e-3

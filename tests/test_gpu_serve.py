@@ -110,14 +110,14 @@ def test_serve_failures_isolated(device, monkeypatch, mode):
             current["job"] = job.job_id
             return real_stage(job, st, groups, keep)
 
-        real_fit = srv._fit
+        real_fit = srv._fit_finish
 
-        def fit(job, slot):
+        def fit(job, launched):
             if job.job_id == 106:
                 raise RuntimeError("injected: table fit failed")
-            return real_fit(job, slot)
+            return real_fit(job, launched)
 
-        monkeypatch.setattr(srv, "_fit", fit)
+        monkeypatch.setattr(srv, "_fit_finish", fit)
         monkeypatch.setattr(srv, "_alloc", alloc)
         monkeypatch.setattr(srv, "_stage_job", stage)
         stats = srv.run(jobs, timeout=300)
