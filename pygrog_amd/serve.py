@@ -270,7 +270,12 @@ def serve_loop(jobs: list, *, budget: int, slots: int, max_inflight: int, estima
                     still.append((handle, batch, keep))
             inflight = still
             if not progressed:
-                time.sleep(0.0002)
+                # wake as soon as a preparation ends (or after 0.2 ms, for
+                # arrivals and batch completions)
+                if preparing:
+                    cf.wait(list(preparing), timeout=0.0002, return_when=cf.FIRST_COMPLETED)
+                else:
+                    time.sleep(0.0002)
     finally:
         # a failed run leaves no preparation behind
         cf.wait(list(preparing))
