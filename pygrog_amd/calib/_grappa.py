@@ -33,6 +33,7 @@ from __future__ import annotations
 __all__ = ["KernelTable", "KernelTableResult", "build_table_finish", "build_table_launch", "compose_table",
            "grappa_power", "grappa_powers", "lstsq", "prepare_grappa_table", "train_grappa"]
 
+import functools
 import math
 from dataclasses import dataclass, field
 from types import SimpleNamespace
@@ -159,7 +160,11 @@ def grappa_powers(G_units, exponents, host_steps: list | None = None,  # noqa: A
     else:
         home = torch.device("cpu")
         Gh = np.asarray(G_units, dtype=np.complex128)
-    exps = np.asarray(torch.as_tensor(exponents, dtype=torch.float32).cpu().numpy(), dtype=np.float64)
+    # exponents in float32, as the reference computes them
+    if isinstance(exponents, torch.Tensor):
+        exps = exponents.detach().to(torch.float32).cpu().numpy().astype(np.float64)
+    else:
+        exps = np.asarray(exponents, dtype=np.float32).astype(np.float64)
     P = torch.from_numpy(_powers_host(Gh, exps))
     dev = torch.device(device) if device is not None else home
     if dev.type == "cpu":
@@ -395,10 +400,20 @@ def build_table_finish(p: SimpleNamespace) -> KernelTableResult:
     return _table_from_units(kern, p.radius, p.precision, p.quant, p.device, p.axis_order, p.keep_fp64, [])
 
 
+@functools.lru_cache(maxsize=64)
+def _exponents(nsteps: int, stepsize: float) -> np.ndarray:
+    """The power exponents (i - (n-1)/2) * stepsize, computed in float32 as
+    the reference does (_grappa.py:100-103), as float64 (read-only)."""
+    deltas = torch.arange(nsteps).float()
+    deltas = (deltas - (nsteps - 1) / 2) * stepsize
+    out = deltas.numpy().astype(np.float64)
+    out.flags.writeable = False
+    return out
+
+
 def _table_from_units(kern, radius, precision, quant, device, axis_order, keep_fp64, host_steps) -> KernelTableResult:
     pfac, stepsize, nsteps = quant_steps(radius, precision, quant)
-    deltas = torch.arange(nsteps).float()
-    deltas = (deltas - (nsteps - 1) / 2) * stepsize  # float32, as the reference
+    deltas = _exponents(nsteps, stepsize)
     units = [kern.x, kern.y] + ([kern.z] if kern.z is not None else [])
     dev = torch.device(device) if device is not None else units[0].device
     if all(isinstance(u, torch.Tensor) and u.device.type == "cpu" for u in units):

@@ -386,6 +386,7 @@ def group_by_entry(rep_ent: torch.Tensor, R: int, K: int, device, chunk: int, re
 S24_MAX_COILS = 32
 
 
+@functools.lru_cache(maxsize=None)
 def _lines_per_row(row_bytes: int) -> float:
     """128-byte lines a row of ``row_bytes`` touches, averaged over the
     offsets rows at a stride of ``row_bytes`` take."""
@@ -440,6 +441,7 @@ def apply_entry(fmt: str) -> str:
     return "pg_apply_grouped_bf16" + _ROW_APPLY[fmt]
 
 
+@functools.lru_cache(maxsize=None)
 def sample_row_bytes(C: int, fmt: str) -> int:
     """Bytes of one sample's row of the sample-major copy."""
     if fmt == "s24":
@@ -471,14 +473,22 @@ WAVE_TILE = 256
 def mode_tile_options(mode: str, C: int) -> list:
     """Launch tiles of ``mode``'s gridding kernel: block-floating-point V
     rows are reduced by the wave-tile kernels only."""
-    opts = _lib.tile_options(C)
-    return [t for t in opts if t == WAVE_TILE] if mode is not None and v_format(mode) == "bfp" else opts
+    return list(_mode_tiles(mode, int(C))[0])
 
 
 def mode_default_tile(mode: str, C: int) -> int:
-    opts = mode_tile_options(mode, C)
+    return _mode_tiles(mode, int(C))[1]
+
+
+@functools.lru_cache(maxsize=None)
+def _mode_tiles(mode, C: int) -> tuple:
+    """(tile options, default tile) of a mode and coil count (library
+    constants: asked once)."""
+    opts = _lib.tile_options(C)
+    if mode is not None and v_format(mode) == "bfp":
+        opts = [t for t in opts if t == WAVE_TILE]
     t = int(_lib.load().pg_grid_tile_cells(C))
-    return t if t in opts else opts[0]
+    return tuple(opts), (t if t in opts else opts[0])
 
 
 
@@ -743,12 +753,20 @@ def _plan_finish(p: _PendingPlan):
     hb = [int(v) for v in summary[S + 1:2 * S + 2]]
     ub = [int(v) for v in summary[2 * S + 2:3 * S + 3]]
 
+    # one view of the workspace per element type (every field starts on a
+    # 256-byte boundary), then a slice per field
+    views = {}
+
     def field(i, dtype, *shape):
         n = 1
         for v in shape:
             n *= int(v)
-        start = int(offs[i])
-        return ws[start:start + n * _ITEMSIZE[dtype]].view(dtype).reshape(shape)
+        isz = _ITEMSIZE[dtype]
+        start = int(offs[i]) // isz
+        base = views.get(dtype)
+        if base is None:
+            base = views[dtype] = ws.view(dtype)
+        return base[start:start + n].view(shape)
 
     i32 = torch.int32
     target_idx = field(0, torch.int64, S, nper)
