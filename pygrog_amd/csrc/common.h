@@ -200,30 +200,35 @@ __device__ __forceinline__ uint4 bfp_encode8(const float (&x)[8]) {
   if (mb >= 0x7f800000u) w = make_uint4(0x80018001u, 0x80018001u, 0x80018001u, 0x80018001u);  // E = 255, m = -16384
   return w;
 }
-__device__ __forceinline__ int bfp_scale(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3) {
+// E - 140 + adj of a group (adj: a constant folded into the sum for free)
+__device__ __forceinline__ int bfp_scale(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3, int adj = 0) {
   // the fields' low bytes (bytes 0 and 2 of each dword), four to a dword,
   // their bit 0 kept: byte k of lo holds bit k of E, byte k of hi bit 4 + k;
   // E - 140 = sum_k lo_k 2^k + sum_k hi_k 2^(4 + k) - 140 as two byte dot
   // products, the second accumulating onto the first, the first onto -140
   const uint32_t lo = __builtin_amdgcn_perm(d1, d0, 0x06040200u) & 0x01010101u;
   const uint32_t hi = __builtin_amdgcn_perm(d3, d2, 0x06040200u) & 0x01010101u;
-  const uint32_t e = __builtin_amdgcn_udot4(lo, 0x08040201u, (uint32_t)(-kBfpBias), false);
+  const uint32_t e = __builtin_amdgcn_udot4(lo, 0x08040201u, (uint32_t)(adj - kBfpBias), false);
   return (int)__builtin_amdgcn_udot4(hi, 0x80402010u, e, false);
 }
+// Decoding: with the exponent bits (bit 0 of each field) cleared, a field
+// read as a signed 16-bit integer is 2 m exactly, so a value is
+// ldexp(2 m, s - 1) = m 2^s -- one mask per two fields, then a convert of
+// each 16-bit half (sub-dword operand select, no shifts) and an ldexp.
 // (re, im) of coil k (0..3) of a group
 __device__ __forceinline__ float2 bfp_decode_coil(uint4 g, int k) {
-  const int s = bfp_scale(g.x, g.y, g.z, g.w);
-  const uint32_t d = k == 0 ? g.x : k == 1 ? g.y : k == 2 ? g.z : g.w;
-  return make_float2(__builtin_amdgcn_ldexpf((float)((int)(d << 16) >> 17), s),
-                     __builtin_amdgcn_ldexpf((float)((int)d >> 17), s));
+  const int s = bfp_scale(g.x, g.y, g.z, g.w, -1);
+  const uint32_t d = (k == 0 ? g.x : k == 1 ? g.y : k == 2 ? g.z : g.w) & 0xFFFEFFFEu;
+  return make_float2(__builtin_amdgcn_ldexpf((float)(short)(d & 0xFFFFu), s),
+                     __builtin_amdgcn_ldexpf((float)(short)(d >> 16), s));
 }
 __device__ __forceinline__ void bfp_decode8(uint32_t d0, uint32_t d1, uint32_t d2, uint32_t d3, float *o) {
-  const int s = bfp_scale(d0, d1, d2, d3);
-  const uint32_t d[4] = {d0, d1, d2, d3};
+  const int s = bfp_scale(d0, d1, d2, d3, -1);
+  const uint32_t d[4] = {d0 & 0xFFFEFFFEu, d1 & 0xFFFEFFFEu, d2 & 0xFFFEFFFEu, d3 & 0xFFFEFFFEu};
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
-    o[2 * k] = __builtin_amdgcn_ldexpf((float)((int)(d[k] << 16) >> 17), s);
-    o[2 * k + 1] = __builtin_amdgcn_ldexpf((float)((int)d[k] >> 17), s);
+    o[2 * k] = __builtin_amdgcn_ldexpf((float)(short)(d[k] & 0xFFFFu), s);
+    o[2 * k + 1] = __builtin_amdgcn_ldexpf((float)(short)(d[k] >> 16), s);
   }
 }
 
