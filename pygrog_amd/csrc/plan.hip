@@ -69,13 +69,17 @@ struct GeomDev {
   int off[128][3];
 };
 
+// KT: the sort key's type -- 32 bits whenever S (G + 1) fits (every
+// single-trajectory plan up to 2^32 cells), halving the key bytes the
+// radix sort moves; the keys' values are the same either way.
+template <class KT>
 __global__ void k_plan_targets(const float *__restrict__ coords,
                                const float *__restrict__ cmax, int64_t S,
                                int64_t npts, GeomDev g,
                                int64_t *__restrict__ target_idx,
                                float *__restrict__ distances,
                                float *__restrict__ gauss,
-                               uint64_t *__restrict__ sort_key,
+                               KT *__restrict__ sort_key,
                                uint32_t *__restrict__ sort_val,
                                unsigned long long *__restrict__ n_invalid) {
   const int64_t nper = npts * g.kw;
@@ -125,8 +129,8 @@ __global__ void k_plan_targets(const float *__restrict__ coords,
       for (int d = 0; d < g.ndim; ++d) distances[gi * g.ndim + d] = dist[d];
       target_idx[gi] = valid ? flat : (int64_t)-1;
       gauss[gi] = valid ? expf(-0.5f * dsq) : 0.0f;
-      sort_key[gi] = (uint64_t)s * (uint64_t)(G + 1) +
-                     (valid ? (uint64_t)(flat + 1) : 0ull);
+      sort_key[gi] = (KT)((uint64_t)s * (uint64_t)(G + 1) +
+                          (valid ? (uint64_t)(flat + 1) : 0ull));
       sort_val[gi] = (uint32_t)gi;
     }
     // lanes in lane 0's element count into the wave's run; the others (a
@@ -167,7 +171,8 @@ __device__ __forceinline__ void wave_fill(uint32_t *cp, int64_t lo, int64_t hi, 
   }
 }
 
-__global__ void k_plan_post(const uint64_t *__restrict__ keys,
+template <class KT>
+__global__ void k_plan_post(const KT *__restrict__ keys,
                             const uint32_t *__restrict__ vals,
                             const float *__restrict__ gauss,
                             const int64_t *__restrict__ n_invalid, int64_t S,
@@ -185,7 +190,7 @@ __global__ void k_plan_post(const uint64_t *__restrict__ keys,
     int64_t lo = 0, hi = -1, lo2 = 0, hi2 = -1;  // cp[lo..hi] = v, cp[lo2..hi2] = v2
     uint32_t v = 0, v2 = 0;
     if (gi < total) {
-      const uint64_t key = keys[gi];
+      const uint64_t key = (uint64_t)keys[gi];
       const int64_t s = gi / nper;
       const int64_t iloc = gi - s * nper;
       const int64_t kloc = (int64_t)(key - (uint64_t)s * (uint64_t)(G + 1));
@@ -203,7 +208,7 @@ __global__ void k_plan_post(const uint64_t *__restrict__ keys,
         const int64_t cell = kloc - 1;
         int64_t prev = -1;
         if (r > 0) {
-          const int64_t pk = (int64_t)(keys[gi - 1] - (uint64_t)s * (uint64_t)(G + 1));
+          const int64_t pk = (int64_t)((uint64_t)keys[gi - 1] - (uint64_t)s * (uint64_t)(G + 1));
           prev = pk - 1;
         }
         lo = prev + 1;
@@ -685,25 +690,12 @@ __global__ void k_csr_from_sorted(const I *__restrict__ idx, int64_t n,
 
 using namespace pg;
 
-extern "C" {
+namespace pg {
 
-int pg_plan_absmax(const float *coords, int64_t S, int64_t npts, int32_t ndim,
-                   float *cmax, pg_stream_t stream) {
-  PG_CHECK(ndim >= 1 && ndim <= 3, "plan: ndim must be 1..3");
-  PG_CHECK(S >= 1 && npts >= 0 && S <= 65535, "plan: bad stack/points");
-  if (npts == 0) return 0;
-  dim3 grid(grid_for(npts, kAbsmaxThreads, 512), (unsigned)S);
-  hipLaunchKernelGGL(k_absmax, grid, dim3(kAbsmaxThreads), 0, as_stream(stream),
-                     coords, S, npts, ndim, reinterpret_cast<unsigned *>(cmax));
-  PG_LAUNCHED();
-  return 0;
-}
-
-int pg_plan_targets(const float *coords, const float *cmax, int64_t S,
-                    int64_t npts, pg_plan_geom geom, int64_t *target_idx,
-                    float *distances, float *gauss, uint64_t *sort_key,
-                    uint32_t *sort_val, int64_t *n_invalid,
-                    pg_stream_t stream) {
+template <class KT>
+int plan_targets_keyed(const float *coords, const float *cmax, int64_t S, int64_t npts, pg_plan_geom geom,
+                       int64_t *target_idx, float *distances, float *gauss, KT *sort_key, uint32_t *sort_val,
+                       int64_t *n_invalid, pg_stream_t stream) {
   PG_CHECK(geom.ndim >= 1 && geom.ndim <= 3, "plan: ndim must be 1..3");
   PG_CHECK(geom.kw >= 1 && geom.kw <= 128, "plan: kernel size must be 1..128");
   const int64_t total = S * npts * geom.kw;
@@ -724,12 +716,66 @@ int pg_plan_targets(const float *coords, const float *cmax, int64_t S,
   const int threads = 256;
   // a persistent grid of at most 2048 workgroups (8 per CU): each wavefront
   // walks many replicas and adds its invalid count once
-  hipLaunchKernelGGL(k_plan_targets, dim3(grid_for(total, threads, 2048)),
+  hipLaunchKernelGGL(k_plan_targets<KT>, dim3(grid_for(total, threads, 2048)),
                      dim3(threads), 0, as_stream(stream), coords, cmax, S,
                      npts, g, target_idx, distances, gauss, sort_key, sort_val,
                      reinterpret_cast<unsigned long long *>(n_invalid));
   PG_LAUNCHED();
   return 0;
+}
+
+template <class KT>
+int plan_post_keyed(const KT *keys_sorted, const uint32_t *vals_sorted, const float *gauss, const int64_t *n_invalid,
+                    int64_t S, int64_t nper, int64_t G, int64_t *indices, int64_t *sort_perm, int64_t *inv_perm,
+                    float *gauss_sorted, uint32_t *cell_ptr, pg_stream_t stream) {
+  const int64_t total = S * nper;
+  if (total == 0) return 0;
+  const int threads = 256;
+  hipLaunchKernelGGL(k_plan_post<KT>, dim3(grid_for(total, threads)),
+                     dim3(threads), 0, as_stream(stream), keys_sorted,
+                     vals_sorted, gauss, n_invalid, S, nper, G, indices,
+                     sort_perm, inv_perm, gauss_sorted, cell_ptr);
+  PG_LAUNCHED();
+  return 0;
+}
+
+// 32-bit-key plan steps for the one-call compiler (plan_compile.hip)
+int plan_targets_k32(const float *coords, const float *cmax, int64_t S, int64_t npts, pg_plan_geom geom,
+                     int64_t *target_idx, float *distances, float *gauss, uint32_t *sort_key, uint32_t *sort_val,
+                     int64_t *n_invalid, pg_stream_t stream) {
+  return plan_targets_keyed(coords, cmax, S, npts, geom, target_idx, distances, gauss, sort_key, sort_val,
+                            n_invalid, stream);
+}
+int plan_post_k32(const uint32_t *keys_sorted, const uint32_t *vals_sorted, const float *gauss,
+                  const int64_t *n_invalid, int64_t S, int64_t nper, int64_t G, int64_t *indices, int64_t *sort_perm,
+                  int64_t *inv_perm, float *gauss_sorted, uint32_t *cell_ptr, pg_stream_t stream) {
+  return plan_post_keyed(keys_sorted, vals_sorted, gauss, n_invalid, S, nper, G, indices, sort_perm, inv_perm,
+                         gauss_sorted, cell_ptr, stream);
+}
+
+}  // namespace pg
+
+extern "C" {
+
+int pg_plan_absmax(const float *coords, int64_t S, int64_t npts, int32_t ndim,
+                   float *cmax, pg_stream_t stream) {
+  PG_CHECK(ndim >= 1 && ndim <= 3, "plan: ndim must be 1..3");
+  PG_CHECK(S >= 1 && npts >= 0 && S <= 65535, "plan: bad stack/points");
+  if (npts == 0) return 0;
+  dim3 grid(grid_for(npts, kAbsmaxThreads, 512), (unsigned)S);
+  hipLaunchKernelGGL(k_absmax, grid, dim3(kAbsmaxThreads), 0, as_stream(stream),
+                     coords, S, npts, ndim, reinterpret_cast<unsigned *>(cmax));
+  PG_LAUNCHED();
+  return 0;
+}
+
+int pg_plan_targets(const float *coords, const float *cmax, int64_t S,
+                    int64_t npts, pg_plan_geom geom, int64_t *target_idx,
+                    float *distances, float *gauss, uint64_t *sort_key,
+                    uint32_t *sort_val, int64_t *n_invalid,
+                    pg_stream_t stream) {
+  return pg::plan_targets_keyed(coords, cmax, S, npts, geom, target_idx, distances, gauss, sort_key, sort_val,
+                                n_invalid, stream);
 }
 
 }  // extern "C"
@@ -786,15 +832,8 @@ int pg_plan_post(const uint64_t *keys_sorted, const uint32_t *vals_sorted,
                  int64_t nper, int64_t G, int64_t *indices, int64_t *sort_perm,
                  int64_t *inv_perm, float *gauss_sorted, uint32_t *cell_ptr,
                  pg_stream_t stream) {
-  const int64_t total = S * nper;
-  if (total == 0) return 0;
-  const int threads = 256;
-  hipLaunchKernelGGL(k_plan_post, dim3(grid_for(total, threads)),
-                     dim3(threads), 0, as_stream(stream), keys_sorted,
-                     vals_sorted, gauss, n_invalid, S, nper, G, indices,
-                     sort_perm, inv_perm, gauss_sorted, cell_ptr);
-  PG_LAUNCHED();
-  return 0;
+  return pg::plan_post_keyed(keys_sorted, vals_sorted, gauss, n_invalid, S, nper, G, indices, sort_perm, inv_perm,
+                             gauss_sorted, cell_ptr, stream);
 }
 
 int pg_plan_weights(const uint32_t *vals_sorted, const float *gauss_sorted,

@@ -19,6 +19,17 @@
 
 namespace pg {
 
+// 32-bit-key variants of pg_plan_targets / pg_plan_post (plan.hip)
+int plan_targets_k32(const float *coords, const float *cmax, int64_t S, int64_t npts, pg_plan_geom geom,
+                     int64_t *target_idx, float *distances, float *gauss, uint32_t *sort_key, uint32_t *sort_val,
+                     int64_t *n_invalid, pg_stream_t stream);
+int plan_post_k32(const uint32_t *keys_sorted, const uint32_t *vals_sorted, const float *gauss,
+                  const int64_t *n_invalid, int64_t S, int64_t nper, int64_t G, int64_t *indices, int64_t *sort_perm,
+                  int64_t *inv_perm, float *gauss_sorted, uint32_t *cell_ptr, pg_stream_t stream);
+
+// Sort keys are S (G + 1) values at most: 32 bits wide when that fits.
+inline bool keys32(int64_t S, int64_t G) { return S * (G + 1) - 1 <= (int64_t)UINT32_MAX; }
+
 namespace {
 
 // Field ids of the output workspace (offsets written by the layout query).
@@ -109,8 +120,9 @@ int tmp_bytes(const Sizes &z, int64_t *b) {
   size_t sort_ws = 0, scan_ws = 0;
   if (rep > 0) {
     const int end_bit = 64 - __builtin_clzll((unsigned long long)(z.S * (z.G + 1) - 1) | 1ull);
-    if (pg_sort_pairs_u64(nullptr, &sort_ws, nullptr, nullptr, nullptr, nullptr, rep,
-                          end_bit > 0 ? end_bit : 1, nullptr))
+    const int eb = end_bit > 0 ? end_bit : 1;
+    if (keys32(z.S, z.G) ? pg_sort_pairs_u32(nullptr, &sort_ws, nullptr, nullptr, nullptr, nullptr, rep, eb, nullptr)
+                         : pg_sort_pairs_u64(nullptr, &sort_ws, nullptr, nullptr, nullptr, nullptr, rep, eb, nullptr))
       return 1;
   }
   if (pg_exclusive_scan_u32(nullptr, &scan_ws, nullptr, nullptr, z.ncell, nullptr)) return 1;
@@ -322,18 +334,26 @@ static int plan_compile(const float *coords, int64_t S, int64_t npts,
   }
 
   if (int rc = pg_plan_absmax(coords, S, npts, geom.ndim, cmax, stream)) return rc;
-  if (int rc = pg_plan_targets(coords, cmax, S, npts, geom, target_idx, distances, gauss, keys, vals,
-                               n_invalid, stream))
+  const bool k32 = keys32(S, G);
+  auto *keys32_in = reinterpret_cast<uint32_t *>(keys);
+  auto *keys32_s = reinterpret_cast<uint32_t *>(keys_s);
+  if (int rc = k32 ? plan_targets_k32(coords, cmax, S, npts, geom, target_idx, distances, gauss, keys32_in, vals,
+                                      n_invalid, stream)
+                   : pg_plan_targets(coords, cmax, S, npts, geom, target_idx, distances, gauss, keys, vals,
+                                     n_invalid, stream))
     return rc;
   if (rep > 0) {
     const int end_bit = 64 - __builtin_clzll((unsigned long long)(S * (G + 1) - 1) | 1ull);
+    const int eb = end_bit > 0 ? end_bit : 1;
     size_t ws = (size_t)tbytes[kSortWs];
-    if (int rc = pg_sort_pairs_u64(T[kSortWs], &ws, keys, keys_s, vals, vals_s, rep, end_bit > 0 ? end_bit : 1,
-                                   stream))
+    if (int rc = k32 ? pg_sort_pairs_u32(T[kSortWs], &ws, keys32_in, keys32_s, vals, vals_s, rep, eb, stream)
+                     : pg_sort_pairs_u64(T[kSortWs], &ws, keys, keys_s, vals, vals_s, rep, eb, stream))
       return rc;
   }
-  if (int rc = pg_plan_post(keys_s, vals_s, gauss, n_invalid, S, z.nper, G, indices, sort_perm, inv_perm,
-                            gauss_sorted, cell_ptr, stream))
+  if (int rc = k32 ? plan_post_k32(keys32_s, vals_s, gauss, n_invalid, S, z.nper, G, indices, sort_perm, inv_perm,
+                                   gauss_sorted, cell_ptr, stream)
+                   : pg_plan_post(keys_s, vals_s, gauss, n_invalid, S, z.nper, G, indices, sort_perm, inv_perm,
+                                  gauss_sorted, cell_ptr, stream))
     return rc;
   if (int rc = pg_plan_weights(vals_s, gauss_sorted, n_invalid, cell_ptr, S, z.nper, G, geom.kw, chunk, weights,
                                sqrt_w, rep_src, rep_s, density, unit_count, z.rmax, stream))
