@@ -186,16 +186,23 @@ __device__ __forceinline__ uint4 bfp_encode8(const float (&x)[8]) {
   const uint32_t ef = mb >> 23;
   const int s = (int)ef - kBfpBias + ((mb & 0x7fffffu) >= (ef != 0u ? 0x7ffe00u : 0x7fff00u) ? 1 : 0);
   const uint32_t E = (uint32_t)(s + kBfpBias);
-  uint32_t f[8];
+  // m_i: x_i 2^-s (exact) plus 1.5 2^23 rounds to an integer, ties to even
+  // (the sum lies in [2^23, 2^24), whose spacing is 1, for |m| < 2^22), and
+  // the sum's low 16 bits are then m_i in two's complement (1.5 2^23 has
+  // none set)
+  uint32_t b[8];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) f[i] = (uint32_t)(int)rintf(__builtin_amdgcn_ldexpf(x[i], -s)) << 1;
-  // dword k: the low halves of f[2k] and f[2k+1] (one byte permute), then
-  // E bit 2k at bit 0 and E bit 2k + 1 at bit 16 (from E | E << 15)
+  for (int i = 0; i < 8; ++i) b[i] = __float_as_uint(__builtin_amdgcn_ldexpf(x[i], -s) + 12582912.0f);
+  // dword k: the low halves of b[2k] and b[2k+1] (one byte permute), each
+  // moved up one bit within its half, then E bit 2k at bit 0 and E bit
+  // 2k + 1 at bit 16 (from E | E << 15)
   const uint32_t eb = E | (E << 15);
   uint32_t d[4];
 #pragma unroll
-  for (int k = 0; k < 4; ++k)
-    d[k] = __builtin_amdgcn_perm(f[2 * k + 1], f[2 * k], 0x05040100u) | ((eb >> (2 * k)) & 0x10001u);
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t pk = __builtin_amdgcn_perm(b[2 * k + 1], b[2 * k], 0x05040100u);
+    d[k] = ((pk << 1) & 0xFFFEFFFEu) | ((eb >> (2 * k)) & 0x10001u);
+  }
   uint4 w = make_uint4(d[0], d[1], d[2], d[3]);
   if (mb >= 0x7f800000u) w = make_uint4(0x80018001u, 0x80018001u, 0x80018001u, 0x80018001u);  // E = 255, m = -16384
   return w;
