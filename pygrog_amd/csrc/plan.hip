@@ -245,6 +245,132 @@ constexpr int kBigCell = 256;
 constexpr int kPwThreads = 256;
 constexpr int kPwChunk = 2048;
 
+// One big cell (more than kBigCell replicas) by the whole workgroup: the
+// float64 Gaussian sum and the float32 density sum in replica order, the
+// per-replica writes in parallel.  Every thread of the workgroup calls it.
+struct PwShared {
+  float val[2][kPwChunk];
+  double rd[kPwThreads];
+  int ri[kPwThreads];
+  float denom;
+};
+__device__ void big_cell_weights(int64_t tb, PwShared &sh, const uint32_t *__restrict__ vals,
+                                 const float *__restrict__ gauss_sorted, const int64_t *__restrict__ n_invalid,
+                                 const uint32_t *__restrict__ cell_ptr, int64_t nper, int64_t G, int kw,
+                                 float *__restrict__ weights, float *__restrict__ sqrt_weights,
+                                 uint32_t *__restrict__ rep_src, float *__restrict__ rep_s,
+                                 float *__restrict__ density, int64_t rep_stride) {
+  constexpr int KB = 16;
+  const int tid = threadIdx.x;
+  float *s_val = sh.val[0];
+  double *s_rd = sh.rd;
+  int *s_ri = sh.ri;
+  const int64_t s = tb / G;
+  const int64_t c = tb - s * G;
+  const uint32_t *cp = cell_ptr + s * (G + 1);
+  const int64_t r0 = cp[c], r1 = cp[c + 1];
+  const int64_t base = s * nper + n_invalid[s];
+  const float *gs = gauss_sorted + base;
+  // The float64 sum of the cell's values in replica order.  Every value
+  // is a non-negative float32, an integer multiple of 2^L (L: the
+  // smallest exponent of a least significant bit among them); while a
+  // sum stays below 2^(53+L) every partial sum, in any order, is exact.
+  // So the sum is first formed in parallel; if it is below that bound it
+  // is the ordered sum's exact value (a rounded partial sum would have
+  // been at or above the bound, and so the total too), otherwise the
+  // ordered sum is formed one value after another as before.
+  double part = 0.0;
+  int lsb = INT_MAX;
+  for (int64_t r = r0 + tid; r < r1; r += kPwThreads) {
+    const float g = gs[r];
+    part += (double)g;
+    const uint32_t e = (__float_as_uint(g) >> 23) & 0xffu;
+    if (g > 0.0f) lsb = min(lsb, (int)(e == 0u ? 1u : e) - 150);
+  }
+  s_rd[tid] = part;
+  s_ri[tid] = lsb;
+  __syncthreads();
+  for (int h = kPwThreads / 2; h > 0; h >>= 1) {
+    if (tid < h) {
+      s_rd[tid] += s_rd[tid + h];
+      s_ri[tid] = min(s_ri[tid], s_ri[tid + h]);
+    }
+    __syncthreads();
+  }
+  const double psum = s_rd[0];
+  const int plsb = s_ri[0];
+  const bool exact = plsb == INT_MAX || psum < ldexp(1.0, 53 + plsb);
+  __syncthreads();
+  double gsum = exact ? psum : 0.0;
+  for (int64_t cb = r0; !exact && cb < r1; cb += kPwChunk) {
+    const int n = (int)(r1 - cb < kPwChunk ? r1 - cb : kPwChunk);
+    for (int k = tid; k < n; k += kPwThreads) s_val[k] = gs[cb + k];
+    __syncthreads();
+    if (tid == 0) {
+      for (int k0 = 0; k0 < n; k0 += KB) {
+        float g[KB];
+#pragma unroll
+        for (int k = 0; k < KB; ++k) g[k] = (k0 + k < n) ? s_val[k0 + k] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < KB; ++k)
+          if (k0 + k < n) gsum += (double)g[k];
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) sh.denom = (float)(gsum > 1e-12 ? gsum : 1e-12);
+  __syncthreads();
+  const float denom = sh.denom;
+  // per-replica writes in parallel, sw^2 staged in LDS for the ordered
+  // density sum, double-buffered: while thread 0 adds chunk i in order, the
+  // other three wavefronts write chunk i + 1 (all four write chunk 0)
+  float dens = 0.0f;
+  auto write_chunk = [&](int64_t cb, float *buf, int t0, int nt) {
+    const int n = (int)(r1 - cb < kPwChunk ? r1 - cb : kPwChunk);
+    for (int k = tid - t0; k < n; k += nt) {
+      const int64_t r = cb + k;
+      const uint32_t v = vals[base + r];
+      const float w = gs[r] / denom;
+      const float sw = sqrtf(w > 0.0f ? w : 0.0f);
+      weights[v] = w;
+      sqrt_weights[base + r] = sw;
+      rep_s[s * rep_stride + r] = sw;
+      rep_src[s * rep_stride + r] = (uint32_t)(((int64_t)v - s * nper) / kw);
+      buf[k] = sw * sw;
+    }
+  };
+  write_chunk(r0, s_val, 0, kPwThreads);
+  __syncthreads();
+  int ib = 0;
+  for (int64_t cb = r0; cb < r1; cb += kPwChunk, ib ^= 1) {
+    const int n = (int)(r1 - cb < kPwChunk ? r1 - cb : kPwChunk);
+    if (tid < kWave) {
+      if (tid == 0) {
+        // the ordered float32 sum: 64 values per round of 16-byte reads
+        const float *q4 = sh.val[ib];
+        int k0 = 0;
+        for (; k0 + 64 <= n; k0 += 64) {
+          float4 q[16];
+#pragma unroll
+          for (int k = 0; k < 16; ++k) q[k] = reinterpret_cast<const float4 *>(q4 + k0)[k];
+#pragma unroll
+          for (int k = 0; k < 16; ++k) {
+            dens = dens + q[k].x;
+            dens = dens + q[k].y;
+            dens = dens + q[k].z;
+            dens = dens + q[k].w;
+          }
+        }
+        for (; k0 < n; ++k0) dens = dens + q4[k0];
+      }
+    } else if (cb + kPwChunk < r1) {
+      write_chunk(cb + kPwChunk, sh.val[ib ^ 1], kWave, kPwThreads - kWave);
+    }
+    __syncthreads();
+  }
+  if (tid == 0) density[tb] = dens;
+}
+
 __global__ __launch_bounds__(kPwThreads) void k_plan_weights(
     const uint32_t *__restrict__ vals, const float *__restrict__ gauss_sorted,
     const int64_t *__restrict__ n_invalid, const uint32_t *__restrict__ cell_ptr,
@@ -252,13 +378,10 @@ __global__ __launch_bounds__(kPwThreads) void k_plan_weights(
     float *__restrict__ weights, float *__restrict__ sqrt_weights,
     uint32_t *__restrict__ rep_src, float *__restrict__ rep_s,
     float *__restrict__ density, uint32_t *__restrict__ unit_count,
-    int64_t rep_stride) {
+    int64_t rep_stride, int64_t big_min, int64_t *__restrict__ big_list, uint32_t *__restrict__ big_count) {
   __shared__ int64_t s_big[kPwThreads];
   __shared__ int s_nbig;
-  __shared__ __align__(16) float s_val[kPwChunk];
-  __shared__ float s_denom;
-  __shared__ double s_rd[kPwThreads];
-  __shared__ int s_ri[kPwThreads];
+  __shared__ __align__(16) PwShared sh;
   const int tid = threadIdx.x;
   const int64_t total = S * G;
   constexpr int KB = 16;
@@ -274,8 +397,11 @@ __global__ __launch_bounds__(kPwThreads) void k_plan_weights(
       const int64_t r0 = cp[c], r1 = cp[c + 1];
       const int64_t base = s * nper + n_invalid[s];
       unit_count[t] = (uint32_t)((r1 - r0 + chunk - 1) / chunk);
-      if (r1 - r0 > kBigCell) {
-        s_big[atomicAdd(&s_nbig, 1)] = t;
+      if (r1 - r0 > big_min) {
+        // big cells: listed for k_plan_weights_big when a list is given,
+        // else done by this workgroup below
+        if (big_list != nullptr) big_list[atomicAdd(big_count, 1u)] = t;
+        else s_big[atomicAdd(&s_nbig, 1)] = t;
       } else {
         // loads go 16 at a time (independent, in flight together) and are
         // added in order
@@ -319,102 +445,26 @@ __global__ __launch_bounds__(kPwThreads) void k_plan_weights(
     }
     __syncthreads();
     const int nbig = s_nbig;
-    for (int b = 0; b < nbig; ++b) {
-      const int64_t tb = s_big[b];
-      const int64_t s = tb / G;
-      const int64_t c = tb - s * G;
-      const uint32_t *cp = cell_ptr + s * (G + 1);
-      const int64_t r0 = cp[c], r1 = cp[c + 1];
-      const int64_t base = s * nper + n_invalid[s];
-      const float *gs = gauss_sorted + base;
-      // The float64 sum of the cell's values in replica order.  Every value
-      // is a non-negative float32, an integer multiple of 2^L (L: the
-      // smallest exponent of a least significant bit among them); while a
-      // sum stays below 2^(53+L) every partial sum, in any order, is exact.
-      // So the sum is first formed in parallel; if it is below that bound it
-      // is the ordered sum's exact value (a rounded partial sum would have
-      // been at or above the bound, and so the total too), otherwise the
-      // ordered sum is formed one value after another as before.
-      double part = 0.0;
-      int lsb = INT_MAX;
-      for (int64_t r = r0 + tid; r < r1; r += kPwThreads) {
-        const float g = gs[r];
-        part += (double)g;
-        const uint32_t e = (__float_as_uint(g) >> 23) & 0xffu;
-        if (g > 0.0f) lsb = min(lsb, (int)(e == 0u ? 1u : e) - 150);
-      }
-      s_rd[tid] = part;
-      s_ri[tid] = lsb;
-      __syncthreads();
-      for (int h = kPwThreads / 2; h > 0; h >>= 1) {
-        if (tid < h) {
-          s_rd[tid] += s_rd[tid + h];
-          s_ri[tid] = min(s_ri[tid], s_ri[tid + h]);
-        }
-        __syncthreads();
-      }
-      const double psum = s_rd[0];
-      const int plsb = s_ri[0];
-      const bool exact = plsb == INT_MAX || psum < ldexp(1.0, 53 + plsb);
-      __syncthreads();
-      double gsum = exact ? psum : 0.0;
-      for (int64_t cb = r0; !exact && cb < r1; cb += kPwChunk) {
-        const int n = (int)(r1 - cb < kPwChunk ? r1 - cb : kPwChunk);
-        for (int k = tid; k < n; k += kPwThreads) s_val[k] = gs[cb + k];
-        __syncthreads();
-        if (tid == 0) {
-          for (int k0 = 0; k0 < n; k0 += KB) {
-            float g[KB];
-#pragma unroll
-            for (int k = 0; k < KB; ++k) g[k] = (k0 + k < n) ? s_val[k0 + k] : 0.0f;
-#pragma unroll
-            for (int k = 0; k < KB; ++k)
-              if (k0 + k < n) gsum += (double)g[k];
-          }
-        }
-        __syncthreads();
-      }
-      if (tid == 0) s_denom = (float)(gsum > 1e-12 ? gsum : 1e-12);
-      __syncthreads();
-      const float denom = s_denom;
-      // per-replica writes in parallel; sw^2 staged for the ordered density sum
-      float dens = 0.0f;
-      for (int64_t cb = r0; cb < r1; cb += kPwChunk) {
-        const int n = (int)(r1 - cb < kPwChunk ? r1 - cb : kPwChunk);
-        for (int k = tid; k < n; k += kPwThreads) {
-          const int64_t r = cb + k;
-          const uint32_t v = vals[base + r];
-          const float w = gs[r] / denom;
-          const float sw = sqrtf(w > 0.0f ? w : 0.0f);
-          weights[v] = w;
-          sqrt_weights[base + r] = sw;
-          rep_s[s * rep_stride + r] = sw;
-          rep_src[s * rep_stride + r] = (uint32_t)(((int64_t)v - s * nper) / kw);
-          s_val[k] = sw * sw;
-        }
-        __syncthreads();
-        if (tid == 0) {
-          // the ordered float32 sum: 64 values per round of 16-byte reads
-          int k0 = 0;
-          for (; k0 + 64 <= n; k0 += 64) {
-            float4 q[16];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) q[k] = reinterpret_cast<const float4 *>(s_val + k0)[k];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-              dens = dens + q[k].x;
-              dens = dens + q[k].y;
-              dens = dens + q[k].z;
-              dens = dens + q[k].w;
-            }
-          }
-          for (; k0 < n; ++k0) dens = dens + s_val[k0];
-        }
-        __syncthreads();
-      }
-      if (tid == 0) density[tb] = dens;
-    }
+    for (int b = 0; b < nbig; ++b)
+      big_cell_weights(s_big[b], sh, vals, gauss_sorted, n_invalid, cell_ptr, nper, G, kw, weights, sqrt_weights,
+                       rep_src, rep_s, density, rep_stride);
   }
+}
+
+// The big cells a k_plan_weights launch listed (big_list), one workgroup at a
+// time each, in parallel across the grid.
+__global__ __launch_bounds__(kPwThreads) void k_plan_weights_big(
+    const int64_t *__restrict__ big_list, const uint32_t *__restrict__ big_count,
+    const uint32_t *__restrict__ vals, const float *__restrict__ gauss_sorted,
+    const int64_t *__restrict__ n_invalid, const uint32_t *__restrict__ cell_ptr,
+    int64_t nper, int64_t G, int kw, float *__restrict__ weights, float *__restrict__ sqrt_weights,
+    uint32_t *__restrict__ rep_src, float *__restrict__ rep_s, float *__restrict__ density,
+    int64_t rep_stride) {
+  __shared__ __align__(16) PwShared sh;
+  const uint32_t n = *big_count;
+  for (uint32_t b = blockIdx.x; b < n; b += gridDim.x)
+    big_cell_weights(big_list[b], sh, vals, gauss_sorted, n_invalid, cell_ptr, nper, G, kw, weights, sqrt_weights,
+                     rep_src, rep_s, density, rep_stride);
 }
 
 __global__ void k_plan_units(const uint32_t *__restrict__ cell_ptr,
@@ -739,6 +789,38 @@ int plan_post_keyed(const KT *keys_sorted, const uint32_t *vals_sorted, const fl
   return 0;
 }
 
+// Cells above this many replicas go to k_plan_weights_big when the caller
+// lists them: one workgroup per cell, in parallel, so a thread walks at most
+// this many replicas in order.
+constexpr int64_t kBigCellListed = 64;
+int64_t plan_big_cells_max(int64_t S, int64_t nper) { return S * nper / (kBigCellListed + 1) + 1; }
+
+// pg_plan_weights with the big cells listed (big_list: room for
+// plan_big_cells_max cells; *big_count zero on entry) and done by
+// k_plan_weights_big, one workgroup per cell across the grid, instead of
+// one after another by the workgroup that holds them (the cells round the
+// k-space centre of a 3D kooshball sit in a few workgroups)
+int plan_weights_listed(const uint32_t *vals_sorted, const float *gauss_sorted, const int64_t *n_invalid,
+                        const uint32_t *cell_ptr, int64_t S, int64_t nper, int64_t G, int32_t kw, int32_t chunk,
+                        float *weights, float *sqrt_weights, uint32_t *rep_src, float *rep_s, float *density,
+                        uint32_t *unit_count, int64_t rep_stride, int64_t *big_list, uint32_t *big_count,
+                        pg_stream_t stream) {
+  PG_CHECK(chunk >= 1, "plan: chunk must be positive");
+  const int64_t total = S * G;
+  if (total == 0) return 0;
+  const hipStream_t st = as_stream(stream);
+  hipLaunchKernelGGL(k_plan_weights, dim3(grid_for(total, kPwThreads)), dim3(kPwThreads), 0, st, vals_sorted,
+                     gauss_sorted, n_invalid, cell_ptr, S, nper, G, kw, chunk, weights, sqrt_weights, rep_src,
+                     rep_s, density, unit_count, rep_stride, (int64_t)kBigCellListed, big_list, big_count);
+  PG_LAUNCHED();
+  const int64_t most = plan_big_cells_max(S, nper);
+  hipLaunchKernelGGL(k_plan_weights_big, dim3((unsigned)(most < 2048 ? most : 2048)), dim3(kPwThreads), 0, st,
+                     big_list, big_count, vals_sorted, gauss_sorted, n_invalid, cell_ptr, nper, G, kw, weights,
+                     sqrt_weights, rep_src, rep_s, density, rep_stride);
+  PG_LAUNCHED();
+  return 0;
+}
+
 // 32-bit-key plan steps for the one-call compiler (plan_compile.hip)
 int plan_targets_k32(const float *coords, const float *cmax, int64_t S, int64_t npts, pg_plan_geom geom,
                      int64_t *target_idx, float *distances, float *gauss, uint32_t *sort_key, uint32_t *sort_val,
@@ -850,7 +932,7 @@ int pg_plan_weights(const uint32_t *vals_sorted, const float *gauss_sorted,
                      dim3(kPwThreads), 0, as_stream(stream), vals_sorted,
                      gauss_sorted, n_invalid, cell_ptr, S, nper, G, kw, chunk,
                      weights, sqrt_weights, rep_src, rep_s, density,
-                     unit_count, rep_stride);
+                     unit_count, rep_stride, (int64_t)kBigCell, (int64_t *)nullptr, (uint32_t *)nullptr);
   PG_LAUNCHED();
   return 0;
 }

@@ -27,6 +27,13 @@ int plan_post_k32(const uint32_t *keys_sorted, const uint32_t *vals_sorted, cons
                   const int64_t *n_invalid, int64_t S, int64_t nper, int64_t G, int64_t *indices, int64_t *sort_perm,
                   int64_t *inv_perm, float *gauss_sorted, uint32_t *cell_ptr, pg_stream_t stream);
 
+int plan_weights_listed(const uint32_t *vals_sorted, const float *gauss_sorted, const int64_t *n_invalid,
+                        const uint32_t *cell_ptr, int64_t S, int64_t nper, int64_t G, int32_t kw, int32_t chunk,
+                        float *weights, float *sqrt_weights, uint32_t *rep_src, float *rep_s, float *density,
+                        uint32_t *unit_count, int64_t rep_stride, int64_t *big_list, uint32_t *big_count,
+                        pg_stream_t stream);
+int64_t plan_big_cells_max(int64_t S, int64_t nper);
+
 // Sort keys are S (G + 1) values at most: 32 bits wide when that fits.
 inline bool keys32(int64_t S, int64_t G) { return S * (G + 1) - 1 <= (int64_t)UINT32_MAX; }
 
@@ -104,7 +111,7 @@ void field_bytes(const Sizes &z, int64_t *b) {
 // Scratch fields, in carve order.
 enum TmpField {
   kCmax = 0, kGauss, kKeys, kVals, kKeysS, kValsS, kGaussSorted, kUnitCount,
-  kA, kHF, kHN, kAs, kHFs, kHNs, kSortWs, kScanWs, kNumTmp
+  kA, kHF, kHN, kAs, kHFs, kHNs, kSortWs, kScanWs, kBigList, kBigCount, kNumTmp
 };
 
 int tmp_bytes(const Sizes &z, int64_t *b) {
@@ -128,6 +135,8 @@ int tmp_bytes(const Sizes &z, int64_t *b) {
   if (pg_exclusive_scan_u32(nullptr, &scan_ws, nullptr, nullptr, z.ncell, nullptr)) return 1;
   b[kSortWs] = (int64_t)sort_ws;
   b[kScanWs] = (int64_t)scan_ws;
+  b[kBigList] = 8 * plan_big_cells_max(z.S, z.nper);
+  b[kBigCount] = 4;
   return 0;
 }
 
@@ -195,7 +204,7 @@ __global__ void k_gather_records(const uint32_t *__restrict__ ent_rep, const uin
 
 // Up to kZeroRanges buffers of 32-bit words cleared in one launch
 // (blockIdx.y = buffer; every buffer starts on a 16-byte boundary).
-constexpr int kZeroRanges = 8;
+constexpr int kZeroRanges = 12;
 struct ZeroRanges {
   uint32_t *p[kZeroRanges];
   int64_t n[kZeroRanges];
@@ -326,6 +335,7 @@ static int plan_compile(const float *coords, int64_t S, int64_t npts,
     add(rep_src, 4 * S * z.rmax);
     add(rep_s, 4 * S * z.rmax);
     add(unit_count, 4 * nc1);
+    add(T[kBigCount], 4);
     if (z.nper == 0) add(cell_ptr, 4 * S * (G + 1));
     int64_t most = 1;
     for (int i = 0; i < nz; ++i) most = zr.n[i] > most ? zr.n[i] : most;
@@ -355,8 +365,10 @@ static int plan_compile(const float *coords, int64_t S, int64_t npts,
                    : pg_plan_post(keys_s, vals_s, gauss, n_invalid, S, z.nper, G, indices, sort_perm, inv_perm,
                                   gauss_sorted, cell_ptr, stream))
     return rc;
-  if (int rc = pg_plan_weights(vals_s, gauss_sorted, n_invalid, cell_ptr, S, z.nper, G, geom.kw, chunk, weights,
-                               sqrt_w, rep_src, rep_s, density, unit_count, z.rmax, stream))
+  if (int rc = plan_weights_listed(vals_s, gauss_sorted, n_invalid, cell_ptr, S, z.nper, G, geom.kw, chunk, weights,
+                                   sqrt_w, rep_src, rep_s, density, unit_count, z.rmax,
+                                   static_cast<int64_t *>(T[kBigList]), static_cast<uint32_t *>(T[kBigCount]),
+                                   stream))
     return rc;
   size_t scan_ws = (size_t)tbytes[kScanWs];
   if (int rc = pg_exclusive_scan_u32(T[kScanWs], &scan_ws, unit_count, cell_unit, z.ncell, stream)) return rc;
