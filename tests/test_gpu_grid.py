@@ -206,8 +206,11 @@ def test_hot_prepass_and_schedule_do_not_change_bits(device, monkeypatch):
 def test_plan_weights_big_cells_bit_exact(device):
     """Cells with hundreds to thousands of replicas (800 spokes through a
     16 x 16 grid: 21 cells above 256, one above 2048) take the plan's
-    workgroup path for the per-cell sums: weights and sqrt weights match the
-    oracle and the density is bitwise the reference's ordered float32 sum."""
+    workgroup path for the per-cell sums (the one-call compiler lists every
+    cell above 64 replicas for a kernel of one workgroup per cell; a cell
+    above 2048 crosses the double-buffered chunks of its ordered density
+    sum): weights and sqrt weights match the oracle and the density is
+    bitwise the reference's ordered float32 sum."""
     coords = _radial(800, 32, 3)
     shape = (16, 16)
     g = GrogInterpolator(shape, coords, kernel_width=3, device=device)
