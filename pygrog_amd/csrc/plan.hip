@@ -16,6 +16,7 @@
 //   units; later quantize (table entries) once the table is known.
 #include <hipcub/hipcub.hpp>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include "common.h"
 
@@ -131,7 +132,7 @@ __global__ void k_plan_targets(const float *__restrict__ coords,
       gauss[gi] = valid ? expf(-0.5f * dsq) : 0.0f;
       sort_key[gi] = (KT)((uint64_t)s * (uint64_t)(G + 1) +
                           (valid ? (uint64_t)(flat + 1) : 0ull));
-      sort_val[gi] = (uint32_t)gi;
+      if (sort_val != nullptr) sort_val[gi] = (uint32_t)gi;
     }
     // lanes in lane 0's element count into the wave's run; the others (a
     // wave straddling two elements) add directly
@@ -896,6 +897,26 @@ int pg_sort_pairs_u32(void *workspace, size_t *ws_bytes,
                                                (unsigned int)n, 0u, (unsigned int)end_bit, as_stream(stream)));
   return 0;
 }
+
+}  // extern "C"
+
+namespace pg {
+// Key-value sort of (keys, 0 .. n-1): the values come from a counting
+// iterator instead of a materialised index array (same sort, one launch and
+// 8 n bytes less).
+int sort_pairs_u32_iota(void *workspace, size_t *ws_bytes, const uint32_t *keys_in, uint32_t *keys_out,
+                        uint32_t *vals_out, int64_t n, int32_t end_bit, pg_stream_t stream) {
+  PG_CHECK(ws_bytes != nullptr, "sort: ws_bytes is null");
+  PG_CHECK(n >= 0 && n < (int64_t)INT32_MAX, "sort: bad n");
+  PG_CHECK(end_bit >= 1 && end_bit <= 32, "sort: bad end_bit");
+  PG_HIP(rocprim::radix_sort_pairs<SortConfig>(workspace, *ws_bytes, keys_in, keys_out,
+                                               rocprim::counting_iterator<uint32_t>(0u), vals_out, (unsigned int)n,
+                                               0u, (unsigned int)end_bit, as_stream(stream)));
+  return 0;
+}
+}  // namespace pg
+
+extern "C" {
 
 int pg_exclusive_scan_u32(void *workspace, size_t *ws_bytes,
                           const uint32_t *in, uint32_t *out, int64_t n,

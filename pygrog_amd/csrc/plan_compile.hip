@@ -33,6 +33,8 @@ int plan_weights_listed(const uint32_t *vals_sorted, const float *gauss_sorted, 
                         uint32_t *unit_count, int64_t rep_stride, int64_t *big_list, uint32_t *big_count,
                         pg_stream_t stream);
 int64_t plan_big_cells_max(int64_t S, int64_t nper);
+int sort_pairs_u32_iota(void *workspace, size_t *ws_bytes, const uint32_t *keys_in, uint32_t *keys_out,
+                        uint32_t *vals_out, int64_t n, int32_t end_bit, pg_stream_t stream);
 
 // Sort keys are S (G + 1) values at most: 32 bits wide when that fits.
 inline bool keys32(int64_t S, int64_t G) { return S * (G + 1) - 1 <= (int64_t)UINT32_MAX; }
@@ -128,7 +130,7 @@ int tmp_bytes(const Sizes &z, int64_t *b) {
   if (rep > 0) {
     const int end_bit = 64 - __builtin_clzll((unsigned long long)(z.S * (z.G + 1) - 1) | 1ull);
     const int eb = end_bit > 0 ? end_bit : 1;
-    if (keys32(z.S, z.G) ? pg_sort_pairs_u32(nullptr, &sort_ws, nullptr, nullptr, nullptr, nullptr, rep, eb, nullptr)
+    if (keys32(z.S, z.G) ? sort_pairs_u32_iota(nullptr, &sort_ws, nullptr, nullptr, nullptr, rep, eb, nullptr)
                          : pg_sort_pairs_u64(nullptr, &sort_ws, nullptr, nullptr, nullptr, nullptr, rep, eb, nullptr))
       return 1;
   }
@@ -154,11 +156,6 @@ __global__ void k_plan_summary(const int64_t *__restrict__ n_invalid,
     else v = HN[(i - 2 * S - 2) * G];
     out[i] = v;
   }
-}
-
-__global__ void k_iota_u32(uint32_t *__restrict__ p, int64_t n) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x)
-    p[t] = (uint32_t)t;
 }
 
 // chunks of `chunk` replicas of each entry; cnt[K] = 0 (the scan's last input)
@@ -347,7 +344,9 @@ static int plan_compile(const float *coords, int64_t S, int64_t npts,
   const bool k32 = keys32(S, G);
   auto *keys32_in = reinterpret_cast<uint32_t *>(keys);
   auto *keys32_s = reinterpret_cast<uint32_t *>(keys_s);
-  if (int rc = k32 ? plan_targets_k32(coords, cmax, S, npts, geom, target_idx, distances, gauss, keys32_in, vals,
+  // (32-bit keys: the sort's values are the candidate indices themselves,
+  // from a counting iterator, so the targets kernel writes none)
+  if (int rc = k32 ? plan_targets_k32(coords, cmax, S, npts, geom, target_idx, distances, gauss, keys32_in, nullptr,
                                       n_invalid, stream)
                    : pg_plan_targets(coords, cmax, S, npts, geom, target_idx, distances, gauss, keys, vals,
                                      n_invalid, stream))
@@ -356,7 +355,7 @@ static int plan_compile(const float *coords, int64_t S, int64_t npts,
     const int end_bit = 64 - __builtin_clzll((unsigned long long)(S * (G + 1) - 1) | 1ull);
     const int eb = end_bit > 0 ? end_bit : 1;
     size_t ws = (size_t)tbytes[kSortWs];
-    if (int rc = k32 ? pg_sort_pairs_u32(T[kSortWs], &ws, keys32_in, keys32_s, vals, vals_s, rep, eb, stream)
+    if (int rc = k32 ? sort_pairs_u32_iota(T[kSortWs], &ws, keys32_in, keys32_s, vals_s, rep, eb, stream)
                      : pg_sort_pairs_u64(T[kSortWs], &ws, keys, keys_s, vals, vals_s, rep, eb, stream))
       return rc;
   }
@@ -434,7 +433,7 @@ int pg_group_by_entry_layout(int64_t R, int64_t K, int64_t chunk, int64_t *items
            (long long)chunk);
   PG_CHECK(items_max && tmp_bytes, "group_by_entry: null output");
   size_t sort_ws = 0, scan_ws = 0;
-  if (int rc = pg_sort_pairs_u32(nullptr, &sort_ws, nullptr, nullptr, nullptr, nullptr, R, entry_bits(K), nullptr))
+  if (int rc = sort_pairs_u32_iota(nullptr, &sort_ws, nullptr, nullptr, nullptr, R, entry_bits(K), nullptr))
     return rc;
   if (int rc = pg_exclusive_scan_u32(nullptr, &scan_ws, nullptr, nullptr, K, nullptr)) return rc;
   auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
@@ -455,7 +454,7 @@ static int group_by_entry(const uint32_t *rep_ent, const uint32_t *rep_src, cons
   const hipStream_t st = as_stream(stream);
   size_t sort_ws = 0, scan_ws = 0;
   const int end_bit = entry_bits(K);
-  if (int rc = pg_sort_pairs_u32(nullptr, &sort_ws, nullptr, nullptr, nullptr, nullptr, R, end_bit, nullptr))
+  if (int rc = sort_pairs_u32_iota(nullptr, &sort_ws, nullptr, nullptr, nullptr, R, end_bit, nullptr))
     return rc;
   if (int rc = pg_exclusive_scan_u32(nullptr, &scan_ws, nullptr, nullptr, K, nullptr)) return rc;
   auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
@@ -472,9 +471,8 @@ static int group_by_entry(const uint32_t *rep_ent, const uint32_t *rep_src, cons
   base += up(4 * (K + 1));
   auto *first = reinterpret_cast<uint32_t *>(base);
 
-  hipLaunchKernelGGL(k_iota_u32, dim3(grid_for(R, 256)), dim3(256), 0, st, iota, R);
-  PG_LAUNCHED();
-  if (int rc = pg_sort_pairs_u32(sort_t, &sort_ws, rep_ent, keys_o, iota, ent_rep, R, end_bit, stream)) return rc;
+  (void)iota;
+  if (int rc = sort_pairs_u32_iota(sort_t, &sort_ws, rep_ent, keys_o, ent_rep, R, end_bit, stream)) return rc;
   if (int rc = pg_csr_from_sorted_u32(keys_o, R, K, ent_ptr, stream)) return rc;
   hipLaunchKernelGGL(k_entry_chunks, dim3(grid_for(K + 1, 256)), dim3(256), 0, st, ent_ptr, K, chunk, cnt);
   PG_LAUNCHED();
