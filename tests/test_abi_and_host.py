@@ -73,6 +73,17 @@ def test_library_version_and_errors():
     assert lib.pg_scatter_add_rows_c64(4096, 16, 4096, 4096, 4096, 10, 0, None) == 0
     assert lib.pg_grid_tile_cells(65) == 0
     assert _lib.tile_options(65) == [] and _lib.tile_options(0) == []
+    # the table assembly takes up to 32 coils and 2 or 3 axes
+    assert lib.pg_table_compose_f64(4096, 3, 11, 33, 0, 4096, None, None) != 0
+    assert b"32 coils" in lib.pg_last_error()
+    assert lib.pg_table_compose_f64(4096, 4, 11, 8, 0, 4096, None, None) != 0
+    # the deferred grouping checks its sizes and pointers before any launch
+    # (its host length pointer may be NULL; the synchronising call's may not)
+    assert lib.pg_group_by_entry_async(4096, 4096, 4096, 0, 5, 128, 4096, 4096, 4096, 4096, 4096, 10, 4096, 1 << 20,
+                                       None, None) != 0
+    assert b"bad sizes" in lib.pg_last_error()
+    assert lib.pg_group_by_entry(4096, 4096, 4096, 100, 5, 128, 4096, 4096, 4096, 4096, 4096, 10, 4096, 1 << 20,
+                                 None, None) != 0
     for c, gs in ((3, 4), (8, 8), (16, 16), (32, 32), (64, 64)):
         tc = lib.pg_grid_tile_cells(c)
         opts = _lib.tile_options(c)
