@@ -960,6 +960,31 @@ __device__ __forceinline__ void load_row_bfp(const unsigned char *vr, int C, int
   }
 }
 
+// One block-floating-point V row added into acc, group by group as it is
+// decoded (acc[c] + v_c, the fold's next addition; no decoded copy of the
+// row is held, which keeps the row-direct reduction at four wavefronts per
+// SIMD at 32 coils).
+template <int W>
+__device__ __forceinline__ void add_row_bfp(const unsigned char *vr, int C, int cc, float2 (&acc)[W]) {
+  static_assert(W % 4 == 0, "whole groups");
+  uint4 g[W / 4];
+#pragma unroll
+  for (int i = 0; i < W / 4; ++i) {
+    const int gi = cc / 4 + i;
+    g[i] = gld(reinterpret_cast<const uint4 *>(vr + 16 * (4 * gi < C ? gi : 0)));
+  }
+#pragma unroll
+  for (int i = 0; i < W / 4; ++i) {
+    float o[8];
+    bfp_decode8(g[i].x, g[i].y, g[i].z, g[i].w, o);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      acc[4 * i + k].x = acc[4 * i + k].x + o[2 * k];
+      acc[4 * i + k].y = acc[4 * i + k].y + o[2 * k + 1];
+    }
+  }
+}
+
 #ifndef PG_WROWS_CW
 #define PG_WROWS_CW 16
 #endif
@@ -1055,6 +1080,23 @@ __global__ __launch_bounds__(kGridThreads) void k_wrows(
         };
 #pragma unroll 1
         for (int cc = 0; cc < C; cc += CW) {
+          if constexpr (MODE == 3 && !TWO) {
+            // rows decoded straight into the sums: ((0 + v_r0) + v_r1) + ...
+            float2 acc[CW];
+#pragma unroll
+            for (int c = 0; c < CW; ++c) acc[c] = make_float2(0.0f, 0.0f);
+            if (n >= 1) add_row_bfp<CW>(vq + (int64_t)r0 * vrb, C, cc, acc);
+#pragma unroll 1
+            for (uint32_t r = rn; r < h.cr1; ++r) add_row_bfp<CW>(vq + (int64_t)r * vrb, C, cc, acc);
+#pragma unroll
+            for (int c = 0; c < CW; ++c)
+              if (cc + c < C) {
+                const float2 o = make_float2(0.0f + acc[c].x, 0.0f + acc[c].y);
+                if (PG_WROWS_NTST) gst_nt(g + (int64_t)(cc + c) * fr.n_cell, o);
+                else gst(g + (int64_t)(cc + c) * fr.n_cell, o);
+              }
+            continue;
+          }
           float2 a[CW], b[CW];
 #pragma unroll
           for (int c = 0; c < CW; ++c) a[c] = b[c] = make_float2(0.0f, 0.0f);
